@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+    config.addinivalue_line("markers", "slow: longer CPU statistical test")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    """The CPU oracle (ctypes binding), built on first use."""
+    from oracle import oracle
+    oracle.lib()
+    return oracle
