@@ -1,0 +1,155 @@
+"""ctypes declaration of the C ABI in include/cudapic_amd.h.
+
+The shared library is the product: if it is missing this module raises
+instead of falling back to anything else.  Build it with
+``python -m cudapic_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcudapic_amd.so")
+
+PIC_OK, PIC_ERR_HIP, PIC_ERR_ARG, PIC_ERR_CAPACITY, PIC_ERR_UNSUPPORTED, PIC_ERR_IO = range(6)
+PIC_PREC_FP32, PIC_PREC_COMP, PIC_PREC_FP64 = 0, 1, 2
+PIC_ELECTRONS, PIC_IONS = 0, 1
+PIC_PATH_REFERENCE, PIC_PATH_FUSED = 0, 1
+PIC_CS_RANGES = 1_000_000
+PIC_N_CS = 5
+PIC_N_EEPF = 2000
+PIC_N_IFED = 200
+PIC_MAX_HARMONICS = 16
+
+
+class PicParams(C.Structure):
+    _fields_ = (
+        [(n, C.c_int32) for n in ("n_g", "n_t", "n_sub", "precision")]
+        + [(n, C.c_double) for n in ("freq", "v0", "gap", "pressure", "t_gas", "weight", "area",
+                                      "dt_e", "dt_i", "dx", "inv_dx", "gas_density")]
+        + [(n, C.c_float) for n in (
+            "inv_dx_f", "factor_e", "factor_i", "weight_factor", "dt_e_f", "dt_i_f", "gap_f",
+            "normal_range", "f1", "f2", "half_me_over_e", "half_mu_over_e", "half_mi_over_e",
+            "e_charge_f", "alpha_f", "half_dx_over_eps_f", "half_inv_dx_f",
+            "min_x_f", "max_x_f", "de_eepf_f", "de_cs_f", "me_f", "exc_th_f", "ion_th_f",
+            "eepf_kick_f")]
+        + [("wave_cos", C.c_double * PIC_MAX_HARMONICS), ("wave_sin", C.c_double * PIC_MAX_HARMONICS)]
+    )
+
+
+class PicSpecies(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("x", "vx", "vy", "vz", "rng0", "rng1", "rng2", "rng3")] + [
+        ("capacity", C.c_uint32)]
+
+
+class PicSimConfig(C.Structure):
+    _fields_ = [
+        ("params", PicParams),
+        ("capacity", C.c_uint32), ("spawn_capacity", C.c_uint32),
+        ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
+        *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
+                                    "replicas", "measure")],
+    ]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+U64 = C.c_uint64
+I32 = C.c_int
+F32 = C.c_float
+F64 = C.c_double
+PP = C.POINTER(PicParams)
+PS = C.POINTER(PicSpecies)
+
+# name -> (restype, argtypes); every function in include/cudapic_amd.h
+SIGNATURES = {
+    "pic_params_default": (None, [PP]),
+    "pic_params_derive": (I32, [PP]),
+    "pic_cross_sections": (I32, [F64, P, P, P]),
+    "pic_get_density": (I32, [P, P, P, I32, PP, P]),
+    "pic_solve_poisson": (I32, [P, P, F64, I32, P, P, PP, P]),
+    "pic_solve_poisson_flexible": (I32, [P, I32, F32, F32, F32, I32, P, P]),
+    "pic_move_and_compact": (I32, [P, P, P, P, P, P, P, P, P, P, P, F32, F32, F32, P, P, PP, P]),
+    "pic_check_collisions_e": (I32, [P, P, P, PS, PS, P, P, P, P, PP, P]),
+    "pic_check_collisions_i": (I32, [P, P, P, PS, P, PP, P]),
+    "pic_accumulate_density": (I32, [P, I32, P, I32, P]),
+    "pic_accumulate_eepf": (I32, [P, P, P, P, P, P, P, PP, P]),
+    "pic_xoshiro_next_n": (I32, [C.POINTER(U32), U32, P, P, P]),
+    "pic_seed_streams": (I32, [C.POINTER(U32), U32, P, P, P, P, P]),
+    "pic_sim_config_default": (None, [C.POINTER(PicSimConfig)]),
+    "pic_sim_create": (I32, [C.POINTER(PicSimConfig), C.POINTER(P)]),
+    "pic_sim_destroy": (I32, [P]),
+    "pic_sim_upload": (I32, [P, I32, P, P, P, P, U32]),
+    "pic_sim_download": (I32, [P, I32, P, P, P, P, U32, C.POINTER(U32)]),
+    "pic_sim_rng_upload": (I32, [P, I32, P, P, P, P, U32]),
+    "pic_sim_rng_download": (I32, [P, I32, P, P, P, P, U32]),
+    "pic_sim_set_tables": (I32, [P, P, P, P]),
+    "pic_sim_counts": (I32, [P, C.POINTER(U32), C.POINTER(U32)]),
+    "pic_sim_time": (I32, [P, C.POINTER(U32), C.POINTER(C.c_ulonglong)]),
+    "pic_sim_set_time": (I32, [P, U32]),
+    "pic_sim_run": (I32, [P, U32, I32]),
+    "pic_sim_synchronize": (I32, [P]),
+    "pic_sim_check": (I32, [P]),
+    "pic_sim_read_grid": (I32, [P, P, P, P, P]),
+    "pic_sim_read_measurements": (I32, [P, P, P, P, P, P]),
+    "pic_sim_reset_measurements": (I32, [P]),
+    "pic_sim_time_steps": (I32, [P, U32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(U32),
+                                 C.POINTER(C.c_ulonglong)]),
+    "pic_sim_save": (I32, [P, C.c_char_p]),
+    "pic_sim_load": (I32, [P, C.c_char_p]),
+    "pic_malloc": (I32, [C.POINTER(P), U64]),
+    "pic_free": (I32, [P]),
+    "pic_memcpy_htod": (I32, [P, P, U64]),
+    "pic_memcpy_dtoh": (I32, [P, P, U64]),
+    "pic_memset": (I32, [P, I32, U64]),
+    "pic_device_synchronize": (I32, []),
+    "pic_last_error": (C.c_char_p, []),
+    "pic_device_info": (I32, [C.POINTER(I32), C.POINTER(I32), C.c_char_p, I32]),
+    "pic_abi_layout": (I32, [C.POINTER(U64)]),
+}
+
+
+class PicError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    """Load the engine library (once).  Raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: the HIP engine must be built first "
+                "(python -m cudapic_amd.build)")
+        lib_ = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib_, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib_
+    return _LIB
+
+
+def check(rc: int, where: str) -> None:
+    if rc != PIC_OK:
+        msg = lib().pic_last_error()
+        raise PicError(rc, where, msg.decode() if msg else "")
+
+
+def call(name: str, *args):
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def layout_matches() -> bool:
+    """True if the ctypes structures have the C sizes the library reports."""
+    out = (U64 * 3)()
+    lib().pic_abi_layout(out)
+    return (out[0], out[1], out[2]) == (C.sizeof(PicParams), C.sizeof(PicSpecies), C.sizeof(PicSimConfig))

@@ -1,0 +1,275 @@
+// pic_device.h -- device-side building blocks shared by the gfx950 kernels:
+// xoshiro128+ in registers, wave64 ballot compaction, CIC weights, and the
+// electron / ion Monte Carlo collision bodies.
+//
+// All floating point here is compiled with -ffp-contract=off: the only fused
+// multiply-adds are the __builtin_fmaf calls written out below, so the push
+// arithmetic is reproducible operation for operation on the CPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pic {
+
+constexpr int kWave = 64;
+constexpr int kCsRanges = 1000000;
+constexpr int kEepfBins = 2000;
+constexpr int kIfedBins = 200;
+constexpr float kPiF = 3.14159265358979f;
+constexpr float kTwoPiF = 6.28318530717959f;
+constexpr float kHalfPiF = 1.57079632679490f;
+constexpr float kLn2F = 0.693147180559945f;
+
+// Physics factors the particle kernels need, all in f32 as the reference
+// casts them (Rust-CUDA/src/main.rs:80-109).
+struct KPhys {
+    int n_g;
+    float inv_dx, gap, wf;
+    float factor_e, factor_i, dt_e, dt_i;
+    float normal_range, f1, f2, half_me_over_e, half_mu_over_e, half_mi_over_e;
+    float e_charge, me, exc_th, ion_th, de_cs;
+    float min_x, max_x, de_eepf, eepf_kick;
+};
+
+// ---------------------------------------------------------------- RNG
+// xoshiro128+ 1.0 (main.rs:860-881); the state lives in VGPRs for the whole
+// particle body and is loaded / stored once per particle.
+struct Xo {
+    uint32_t s0, s1, s2, s3;
+};
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+__device__ __forceinline__ uint32_t xo_next(Xo &s) {
+    const uint32_t result = s.s0 + s.s3;
+    const uint32_t t = s.s1 << 9;
+    s.s2 ^= s.s0;
+    s.s3 ^= s.s1;
+    s.s1 ^= s.s2;
+    s.s0 ^= s.s3;
+    s.s2 ^= t;
+    s.s3 = rotl32(s.s3, 11);
+    return result;
+}
+
+// main.rs:883-885: the top 23 bits as the mantissa of [1,2), minus 1.
+__device__ __forceinline__ float u32_to_real(uint32_t x) {
+    return __uint_as_float(0x3F800000u | (x >> 9)) - 1.0f;
+}
+
+__device__ __forceinline__ float draw(Xo &s) { return u32_to_real(xo_next(s)); }
+
+__device__ __forceinline__ Xo rng_load(const uint32_t *__restrict__ r0, const uint32_t *__restrict__ r1,
+                                       const uint32_t *__restrict__ r2, const uint32_t *__restrict__ r3,
+                                       uint32_t i) {
+    return Xo{r0[i], r1[i], r2[i], r3[i]};
+}
+
+__device__ __forceinline__ void rng_store(uint32_t *__restrict__ r0, uint32_t *__restrict__ r1,
+                                          uint32_t *__restrict__ r2, uint32_t *__restrict__ r3,
+                                          uint32_t i, const Xo &s) {
+    r0[i] = s.s0;
+    r1[i] = s.s1;
+    r2[i] = s.s2;
+    r3[i] = s.s3;
+}
+
+// Three N(0, sigma^2) samples from four uniforms (main.rs:908-969): two
+// Box-Muller pairs, the fourth normal discarded.  The angle goes straight to
+// the hardware sine/cosine, which take their argument in revolutions, so
+// cos(2 pi u) needs no range reduction; log uses the hardware log2.
+__device__ __forceinline__ void three_normal(Xo &s, float sigma, float &a, float &b, float &c) {
+    const float u1 = fmaxf(draw(s), 1e-9f);
+    const float u2 = fmaxf(draw(s), 1e-9f);
+    const float u3 = fmaxf(draw(s), 1e-9f);
+    const float u4 = fmaxf(draw(s), 1e-9f);
+    const float r1 = __builtin_sqrtf(-2.0f * kLn2F * __builtin_amdgcn_logf(u1));
+    const float r2 = __builtin_sqrtf(-2.0f * kLn2F * __builtin_amdgcn_logf(u3));
+    a = r1 * __builtin_amdgcn_cosf(u2) * sigma;
+    b = r1 * __builtin_amdgcn_sinf(u2) * sigma;
+    c = r2 * __builtin_amdgcn_cosf(u4) * sigma;
+}
+
+// ---------------------------------------------------------------- wave64
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Number of set bits of `mask` below this lane (v_mbcnt_lo/hi).
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// ---------------------------------------------------------------- grid
+// CIC cell of a position, clamped to [0, n_g-2] so that x == L lands on the
+// last two nodes (main.rs:385-386 and :600 would index N_G there).
+__device__ __forceinline__ int cell_of(float pos, int n_g) {
+    int q = (int)pos;
+    q = q < 0 ? 0 : q;
+    return q > n_g - 2 ? n_g - 2 : q;
+}
+
+// Field at x by linear interpolation and the leapfrog update (main.rs:597-603),
+// spelled with explicit fmaf.
+__device__ __forceinline__ float field_at(const float *ef, float x, float inv_dx, int n_g) {
+    const float pos = x * inv_dx;
+    const int q = cell_of(pos, n_g);
+    const float c = pos - (float)q;
+    return __builtin_fmaf(c, ef[q + 1], (1.0f - c) * ef[q]);
+}
+
+// ---------------------------------------------------------------- collisions
+__device__ __forceinline__ int energy_index(float energy, float de_cs) {
+    const float c = energy / de_cs + 0.5f;
+    const uint32_t idx = (uint32_t)c;
+    return idx > (uint32_t)(kCsRanges - 1) ? kCsRanges - 1 : (int)idx;
+}
+
+struct Frame {
+    float st, ct, sp, cp;
+};
+
+// Euler angles of the relative velocity (main.rs:1068-1093).
+__device__ __forceinline__ Frame frame_of(float gx, float gy, float gz) {
+    float theta, phi;
+    if (gx == 0.0f) theta = kHalfPiF;
+    else theta = atan2f(__builtin_sqrtf(gy * gy + gz * gz), gx);
+    if (gy == 0.0f) phi = gz > 0.0f ? kHalfPiF : -kHalfPiF;
+    else phi = atan2f(gz, gy);
+    Frame f;
+    sincosf(theta, &f.st, &f.ct);
+    sincosf(phi, &f.sp, &f.cp);
+    return f;
+}
+
+// Rotate a relative speed g by scattering angle chi and azimuth eta in the
+// frame f (main.rs:1152-1158).
+__device__ __forceinline__ void rotate(float g, float chi, float eta, const Frame &f, float &ox,
+                                       float &oy, float &oz) {
+    float sc, cc, se, ce;
+    sincosf(chi, &sc, &cc);
+    sincosf(eta, &se, &ce);
+    ox = g * (f.ct * cc - f.st * sc * ce);
+    oy = g * (f.st * f.cp * cc + f.ct * f.cp * sc * ce - f.sp * sc * se);
+    oz = g * (f.st * f.sp * cc + f.ct * f.sp * sc * ce + f.cp * sc * se);
+}
+
+enum CollKind { kNone = 0, kElastic = 1, kExcitation = 2, kIonisation = 3 };
+
+// Collision test and scattering of one electron (check_collisions_e +
+// collision_e, main.rs:996-1165).  v is updated in place; on ionisation the
+// ejected electron and the new ion velocities are returned.  RNG draw order:
+// test, process, then (elastic/excitation) chi, eta or (ionisation) energy
+// share, eta, and four draws for the ion's thermal velocity.
+__device__ __forceinline__ int collide_electron(const float *__restrict__ stot_e,
+                                                const float *__restrict__ cs, const KPhys &k,
+                                                float &vx, float &vy, float &vz, Xo &s,
+                                                float *se, float *si) {
+    const float v2 = vx * vx + vy * vy + vz * vz;
+    const float energy = k.half_me_over_e * v2;
+    const int idx = energy_index(energy, k.de_cs);
+    const float velocity = __builtin_sqrtf(v2);
+    const float nu = stot_e[idx] * velocity;
+    const float rv = draw(s);
+    const float pc = 1.0f - expf(-nu * k.dt_e);
+    if (!(rv < pc)) return kNone;
+
+    float gx = vx, gy = vy, gz = vz;
+    float g = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
+    const float wx = k.f1 * vx, wy = k.f1 * vy, wz = k.f1 * vz;
+    const float t0 = cs[idx];
+    const float t1 = t0 + cs[kCsRanges + idx];
+    const float t2 = t1 + cs[2 * kCsRanges + idx];
+    const Frame f = frame_of(gx, gy, gz);
+    float chi, eta;
+    int kind;
+    const float rnd = draw(s);
+    if (rnd < t0 / t2) {
+        chi = acosf(1.0f - 2.0f * draw(s));
+        eta = kTwoPiF * draw(s);
+        kind = kElastic;
+    } else if (rnd < t1 / t2) {
+        float en = 0.5f * k.me * g * g;
+        en = fabsf(en - k.exc_th * k.e_charge);
+        g = __builtin_sqrtf(2.0f * en / k.me);
+        chi = acosf(1.0f - 2.0f * draw(s));
+        eta = kTwoPiF * draw(s);
+        kind = kExcitation;
+    } else {
+        float en = 0.5f * k.me * g * g;
+        en = fabsf(en - k.ion_th * k.e_charge);
+        const float r = draw(s);
+        const float e_new = 10.0f * tanf(r * atanf(en / k.e_charge / 20.0f)) * k.e_charge;
+        const float e_orig = fabsf(en - e_new);
+        g = __builtin_sqrtf(2.0f * e_orig / k.me);
+        const float g_new = __builtin_sqrtf(2.0f * e_new / k.me);
+        chi = acosf(__builtin_sqrtf(e_orig / en));
+        const float chi_new = acosf(__builtin_sqrtf(e_new / en));
+        eta = kTwoPiF * draw(s);
+        const float eta_new = eta + kPiF;
+        float nx, ny, nz;
+        rotate(g_new, chi_new, eta_new, f, nx, ny, nz);
+        se[0] = wx + k.f2 * nx;
+        se[1] = wy + k.f2 * ny;
+        se[2] = wz + k.f2 * nz;
+        three_normal(s, k.normal_range, si[0], si[1], si[2]);
+        kind = kIonisation;
+    }
+    float ox, oy, oz;
+    rotate(g, chi, eta, f, ox, oy, oz);
+    vx = wx + k.f2 * ox;
+    vy = wy + k.f2 * oy;
+    vz = wz + k.f2 * oz;
+    return kind;
+}
+
+// Collision test and scattering of one ion against a Maxwellian atom
+// (check_collisions_i + collision_i, main.rs:1185-1292).
+__device__ __forceinline__ int collide_ion(const float *__restrict__ stot_i,
+                                           const float *__restrict__ cs, const KPhys &k,
+                                           float &vx, float &vy, float &vz, Xo &s) {
+    float ax, ay, az;
+    three_normal(s, k.normal_range, ax, ay, az);
+    const float gx = vx - ax, gy = vy - ay, gz = vz - az;
+    const float g2 = gx * gx + gy * gy + gz * gz;
+    const float energy = k.half_mu_over_e * g2;
+    const int idx = energy_index(energy, k.de_cs);
+    const float g = __builtin_sqrtf(g2);
+    const float nu = stot_i[idx] * g;
+    const float rv = draw(s);
+    const float pc = 1.0f - expf(-nu * k.dt_i);
+    if (!(rv < pc)) return kNone;
+
+    const float t0 = cs[3 * kCsRanges + idx];
+    const float t1 = t0 + cs[4 * kCsRanges + idx];
+    const float wx = 0.5f * (vx + ax), wy = 0.5f * (vy + ay), wz = 0.5f * (vz + az);
+    const Frame f = frame_of(gx, gy, gz);
+    float chi;
+    int kind;
+    const float rnd = draw(s);
+    if (rnd < t0 / t1) {
+        chi = acosf(1.0f - 2.0f * draw(s));
+        kind = kElastic;
+    } else {
+        chi = kPiF;
+        kind = kExcitation;  // backward (charge exchange)
+    }
+    const float eta = kTwoPiF * draw(s);
+    float ox, oy, oz;
+    rotate(g, chi, eta, f, ox, oy, oz);
+    vx = wx + 0.5f * ox;
+    vy = wy + 0.5f * oy;
+    vz = wz + 0.5f * oz;
+    return kind;
+}
+
+}  // namespace pic

@@ -1,0 +1,297 @@
+// pic_fused.hip -- the fast path: one persistent kernel per species per RF
+// step that does everything the reference does in move_and_compact,
+// check_collisions_e/_i, get_density (of the next step) and accumulate_eepf
+// (Rust-CUDA/src/main.rs:566-678, 971-1292, 356-406, 1301-1339) in a single
+// pass over the particles.
+//
+// Per particle-step the HBM traffic is the compulsory 64 B: read x,vx,vy,vz
+// (16 B), write them (16 B), read and write the four xoshiro words of the
+// destination slot (32 B).  The field, the histograms and the cross-section
+// gathers stay in LDS / L2.
+//
+// Semantics kept from the reference: survivors are compacted first, then the
+// collision of a survivor uses the RNG stream of the slot it lands in
+// (main.rs:1785-1791 runs check_collisions_e over the compacted array with
+// per-slot streams); electrons born in a step are not collision-tested in
+// that step (the collision bound is the pre-spawn count, main.rs:992); the
+// next step's density includes every survivor and every newborn particle.
+// The fused kernel stages newborn electrons in a side array that the next
+// step reads after the main array, since the main array's survivor count is
+// only final when the kernel ends.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pic_internal.h"
+
+namespace pic {
+
+struct FusedArgs {
+    FusedJob j;
+    KPhys k;
+    int lds_reps;
+    int reps, rep_stride;
+};
+
+template <typename Acc>
+__device__ __forceinline__ void deposit_lds(float *h, float x, const KPhys &k) {
+    const float pos = x * k.inv_dx;
+    const int q = cell_of(pos, k.n_g);
+    const float rem = pos - (float)q;
+    atomicAdd(&h[q], (1.0f - rem) * k.wf);
+    atomicAdd(&h[q + 1], rem * k.wf);
+}
+
+template <typename Acc>
+__device__ __forceinline__ void deposit_global(Acc *d, float x, const KPhys &k) {
+    const float pos = x * k.inv_dx;
+    const int q = cell_of(pos, k.n_g);
+    const float rem = pos - (float)q;
+    Acc c1 = (Acc)((1.0f - rem) * k.wf), c2 = (Acc)(rem * k.wf);
+    if (q == 0) c1 *= (Acc)2;
+    if (q + 1 == k.n_g - 1) c2 *= (Acc)2;
+    unsafeAtomicAdd(&d[q], c1);
+    unsafeAtomicAdd(&d[q + 1], c2);
+}
+
+__device__ __forceinline__ void eepf_lds(uint32_t *h, const float *ef, float x, float vx, float vy,
+                                         float vz, const KPhys &k) {
+    if (x <= k.min_x || x >= k.max_x) return;
+    const float e = field_at(ef, x, k.inv_dx, k.n_g);
+    const float mvx = vx - 0.5f * e * k.eepf_kick;
+    const float en = k.half_me_over_e * (mvx * mvx + vy * vy + vz * vz);
+    const uint32_t b = (uint32_t)(en / k.de_eepf);
+    if (b < (uint32_t)kEepfBins) atomicAdd(&h[b], 1u);
+}
+
+__device__ uint32_t tile_reserve_f(uint32_t wave_count, uint32_t *s_wave, uint32_t *s_base, int buf,
+                                   uint32_t *counter) {
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane_id() == 0) s_wave[buf * 16 + wave] = wave_count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            uint32_t c = s_wave[buf * 16 + w];
+            s_wave[buf * 16 + w] = acc;
+            acc += c;
+        }
+        s_base[buf] = acc ? atomicAdd(counter, acc) : 0u;
+    }
+    __syncthreads();
+    return s_base[buf] + s_wave[buf * 16 + wave];
+}
+
+template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
+__global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
+    const FusedJob &j = A.j;
+    const KPhys &k = A.k;
+    extern __shared__ __align__(16) float sh[];
+    __shared__ uint32_t s_wave[32], s_base[2];
+    const int n_g = k.n_g;
+    float *ef = sh;
+    float *hist = sh + n_g;
+    uint32_t *aux = (uint32_t *)(hist + A.lds_reps * n_g);  // EEPF (electrons) or IFED (ions)
+    const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
+    for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
+    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
+    __syncthreads();
+    const int wave = threadIdx.x / kWave;
+    float *h = hist + (wave % A.lds_reps) * n_g;
+
+    // counters that overflowed their arrays (appends are dropped and flagged)
+    // are clamped here, so no read goes past the end of an array
+    const uint32_t n_main = min(*j.n_main, j.capacity);
+    const uint32_t n_sp = ELECTRON ? min(*j.n_spawn, j.spawn_capacity) : 0u;
+    const uint32_t n = n_main + n_sp;
+    const float factor = ELECTRON ? k.factor_e : k.factor_i;
+    const float dt = ELECTRON ? k.dt_e : k.dt_i;
+    const uint32_t tile_sz = blockDim.x * PPT;
+    const uint32_t n_tiles = (n + tile_sz - 1) / tile_sz;
+    uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
+    bool overflow = false;
+    int buf = 0;
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x, buf ^= 1) {
+        float nx[PPT], nvx[PPT], vy[PPT], vz[PPT];
+        bool keep[PPT];
+        uint32_t off[PPT];
+        const uint32_t base = tile * tile_sz + threadIdx.x;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint32_t v = base + u * blockDim.x;
+            keep[u] = false;
+            if (v < n) {
+                const float *X = j.sx, *VX = j.svx, *VY = j.svy, *VZ = j.svz;
+                uint32_t i = v;
+                if (ELECTRON && v >= n_main) {
+                    X = j.px;
+                    VX = j.pvx;
+                    VY = j.pvy;
+                    VZ = j.pvz;
+                    i = v - n_main;
+                }
+                const float x = X[i];
+                const float e = field_at(ef, x, k.inv_dx, n_g);
+                nvx[u] = __builtin_fmaf(factor, e, VX[i]);
+                nx[u] = __builtin_fmaf(nvx[u], dt, x);
+                keep[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
+                if (keep[u]) {
+                    vy[u] = VY[i];
+                    vz[u] = VZ[i];
+                } else {
+                    const bool pw = nx[u] < 0.0f;
+                    lost_pow += pw;
+                    lost_gnd += !pw;
+                    if (!ELECTRON) {
+                        const float y = VY[i], z = VZ[i];
+                        const float en = k.half_mi_over_e * (nvx[u] * nvx[u] + y * y + z * z);
+                        const int b = (int)en;
+                        if (en >= 0.0f && b < kIfedBins) atomicAdd(&aux[(pw ? 0 : 1) * kIfedBins + b], 1u);
+                    }
+                }
+            }
+        }
+        uint32_t wcount = 0;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint64_t m = ballot64(keep[u]);
+            off[u] = wcount + mbcnt64(m);
+            wcount += (uint32_t)__popcll(m);
+        }
+        const uint32_t wbase = tile_reserve_f(wcount, s_wave, s_base, buf, j.n_out);
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            int kind = kNone;
+            float se[3], si[3];
+            if (keep[u]) {
+                const uint32_t slot = wbase + off[u];
+                if (slot < j.capacity) {
+                    Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot);
+                    float vx = nvx[u], y = vy[u], z = vz[u];
+                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
+                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s);
+                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+                    j.dx[slot] = nx[u];
+                    j.dvx[slot] = vx;
+                    j.dvy[slot] = y;
+                    j.dvz[slot] = z;
+                    deposit_lds<Acc>(h, nx[u], k);
+                    if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], vx, y, z, k);
+                    ncoll += kind != kNone;
+                } else {
+                    overflow = true;
+                }
+            }
+            if (ELECTRON) {
+                const uint64_t sp = ballot64(kind == kIonisation);
+                if (sp) {
+                    const int leader = __builtin_ctzll(sp);
+                    uint32_t be = 0, bi = 0;
+                    if ((int)lane_id() == leader) {
+                        const uint32_t cnt = (uint32_t)__popcll(sp);
+                        be = atomicAdd(j.n_spawn_out, cnt);
+                        bi = atomicAdd(j.n_ions, cnt);
+                    }
+                    be = __shfl(be, leader, kWave);
+                    bi = __shfl(bi, leader, kWave);
+                    if (kind == kIonisation) {
+                        const uint32_t o = mbcnt64(sp);
+                        const uint32_t es = be + o, is = bi + o;
+                        const float x = nx[u];
+                        if (es < j.spawn_capacity) {
+                            j.ox[es] = x;
+                            j.ovx[es] = se[0];
+                            j.ovy[es] = se[1];
+                            j.ovz[es] = se[2];
+                            deposit_lds<Acc>(h, x, k);
+                            if (MEASURE) eepf_lds(aux, ef, x, se[0], se[1], se[2], k);
+                        } else {
+                            overflow = true;
+                        }
+                        if (is < j.ion_capacity) {
+                            j.ix[is] = x;
+                            j.ivx[is] = si[0];
+                            j.ivy[is] = si[1];
+                            j.ivz[is] = si[2];
+                            deposit_global<Acc>((Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride, x, k);
+                        } else {
+                            overflow = true;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // per-workgroup diagnostics: one atomic per counter
+    {
+        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
+        if (lane_id() == 0 && j.counters) {
+            const int b = ELECTRON ? 0 : 2;
+            if (wp) atomicAdd(&j.counters[b], (unsigned long long)wp);
+            if (wg) atomicAdd(&j.counters[b + 1], (unsigned long long)wg);
+            if (wc) atomicAdd(&j.counters[ELECTRON ? 4 : 5], (unsigned long long)wc);
+        }
+        if (overflow) *j.overflow = 1u;
+    }
+    if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
+    // one flush of the workgroup histogram into the next step's density
+    Acc *out = (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride;
+    for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
+        float s = 0.0f;
+        for (int r = 0; r < A.lds_reps; ++r) s += hist[r * n_g + q];
+        if (s != 0.0f) {
+            Acc v = (Acc)s;
+            if (q == 0 || q == n_g - 1) v *= (Acc)2;
+            unsafeAtomicAdd(&out[q], v);
+        }
+    }
+    uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;
+    if (aux_out)
+        for (int q = threadIdx.x; q < aux_n; q += blockDim.x)
+            if (aux[q]) atomicAdd(&aux_out[q], aux[q]);
+}
+
+template <bool E, bool M, int PPT>
+static void launch_t(const FusedArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
+    if (f64) k_fused<E, M, PPT, double><<<grid, block, lds, s>>>(a);
+    else k_fused<E, M, PPT, float><<<grid, block, lds, s>>>(a);
+}
+
+template <bool E, bool M>
+static void launch_ppt(const FusedArgs &a, bool f64, int ppt, int grid, int block, size_t lds,
+                       hipStream_t s) {
+    switch (ppt) {
+    case 1: launch_t<E, M, 1>(a, f64, grid, block, lds, s); break;
+    case 2: launch_t<E, M, 2>(a, f64, grid, block, lds, s); break;
+    default: launch_t<E, M, 4>(a, f64, grid, block, lds, s); break;
+    }
+}
+
+int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
+    Shape sh = resolve(sh0);
+    FusedArgs a;
+    a.j = j;
+    a.k = k;
+    const int block = 256;
+    const bool measure = j.electrons && j.eepf != nullptr;
+    const int aux_words = j.electrons ? (measure ? kEepfBins : 0) : 2 * kIfedBins;
+    const int fixed = (k.n_g + aux_words) * 4;
+    const int budget = 40 * 1024;
+    a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (k.n_g * 4)));
+    a.reps = sh.replicas;
+    a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
+    const size_t lds = (size_t)fixed + (size_t)a.lds_reps * k.n_g * 4;
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    if (j.electrons) {
+        if (measure) launch_ppt<true, true>(a, j.dens_f64, sh.ppt, grid, block, lds, s);
+        else launch_ppt<true, false>(a, j.dens_f64, sh.ppt, grid, block, lds, s);
+    } else {
+        launch_ppt<false, false>(a, j.dens_f64, sh.ppt, grid, block, lds, s);
+    }
+    PIC_LAUNCHED("k_fused");
+    return PIC_OK;
+}
+
+}  // namespace pic

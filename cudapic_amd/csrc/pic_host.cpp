@@ -1,0 +1,69 @@
+// pic_host.cpp -- host-side tables of the engine.
+//
+// Cross sections: the analytic argon fits eduPIC tabulates on 1e6 bins of
+// 1 meV (Rust-CUDA/src/main.rs:1378-1444 == C/eduPIC.cc:173-229):
+// electrons from Phelps & Petrovic, PSST 8 R21 (1999); Ar+/Ar isotropic and
+// backward (charge exchange) from Phelps, J. Appl. Phys. 76, 747 (1994),
+// evaluated at E_lab = 2 E_com.  The tables are computed in f64 and stored as
+// f32; the totals are summed in f64 from the stored f32 values and scaled by
+// the gas density, as the reference does.
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/cudapic_amd.h"
+
+namespace {
+double q_elastic(double e) {
+    return 1.0e-20 * (std::fabs(6.0 / std::pow(1.0 + e / 0.1 + std::pow(e / 0.6, 2.0), 3.3) -
+                                1.1 * std::pow(e, 1.4) / (1.0 + std::pow(e / 15.0, 1.2)) /
+                                    std::sqrt(1.0 + std::pow(e / 5.5, 2.5) + std::pow(e / 60.0, 4.1))) +
+                      0.05 / std::pow(1.0 + e / 10.0, 2.0) +
+                      0.01 * std::pow(e, 3.0) / (1.0 + std::pow(e / 12.0, 6.0)));
+}
+double q_excitation(double e) {
+    if (e <= 11.5) return 0.0;
+    return (0.034 * std::pow(e - 11.5, 1.1) * (1.0 + std::pow(e / 15.0, 2.8)) / (1.0 + std::pow(e / 23.0, 5.5)) +
+            0.023 * (e - 11.5) / std::pow(1.0 + e / 80.0, 1.9)) *
+           1.0e-20;
+}
+double q_ionisation(double e) {
+    if (e <= 15.8) return 0.0;
+    return (970.0 * (e - 15.8) / std::pow(70.0 + e, 2.0) + 0.06 * std::pow(e - 15.8, 2.0) * std::exp(-e / 9.0)) *
+           1.0e-20;
+}
+double q_ion_momentum(double el) { return 1.15e-18 * std::pow(el, -0.1) * std::pow(1.0 + 0.015 / el, 0.6); }
+double q_ion_isotropic(double el) {
+    return 2.0e-19 * std::pow(el, -0.5) / (1.0 + el) + 3.0e-19 * el / std::pow(1.0 + el / 3.0, 2.0);
+}
+}  // namespace
+
+extern "C" int pic_cross_sections(double gas_density, float *cs, float *stot_e, float *stot_i) {
+    if (!cs || !stot_e || !stot_i || !(gas_density > 0.0)) return PIC_ERR_ARG;
+    const int n = PIC_CS_RANGES;
+    const double de = 0.001;
+    for (int i = 0; i < n; ++i) {
+        const double e = i == 0 ? de : (double)i * de;
+        const double el = 2.0 * e;
+        const double iso = q_ion_isotropic(el);
+        cs[0 * n + i] = (float)q_elastic(e);
+        cs[1 * n + i] = (float)q_excitation(e);
+        cs[2 * n + i] = (float)q_ionisation(e);
+        cs[3 * n + i] = (float)iso;
+        cs[4 * n + i] = (float)(0.5 * (q_ion_momentum(el) - iso));
+    }
+    for (int i = 0; i < n; ++i) {
+        const double se = (double)cs[i] + (double)cs[n + i] + (double)cs[2 * n + i];
+        const double si = (double)cs[3 * n + i] + (double)cs[4 * n + i];
+        stot_e[i] = (float)(se * gas_density);
+        stot_i[i] = (float)(si * gas_density);
+    }
+    return PIC_OK;
+}
+
+extern "C" int pic_abi_layout(uint64_t out[3]) {
+    if (!out) return PIC_ERR_ARG;
+    out[0] = sizeof(pic_params_t);
+    out[1] = sizeof(pic_species_t);
+    out[2] = sizeof(pic_sim_config_t);
+    return PIC_OK;
+}

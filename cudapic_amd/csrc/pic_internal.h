@@ -1,0 +1,147 @@
+// pic_internal.h -- launchers shared between the stage entry points
+// (pic_kernels.hip) and the simulation object (pic_sim.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/cudapic_amd.h"
+#include "pic_device.h"
+
+namespace pic {
+
+// thread-local error text behind pic_last_error()
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define PIC_HIP(call)                                      \
+    do {                                                   \
+        hipError_t e_ = (call);                            \
+        if (e_ != hipSuccess) return ::pic::hip_fail(e_, #call); \
+    } while (0)
+#define PIC_LAUNCHED(what)                                         \
+    do {                                                           \
+        hipError_t e_ = hipGetLastError();                         \
+        if (e_ != hipSuccess) return ::pic::hip_fail(e_, what);    \
+    } while (0)
+
+int device_cus();  // compute units of the current device (cached)
+
+KPhys make_kphys(const pic_params_t &p);
+
+// Launch shape of the persistent particle kernels.
+struct Shape {
+    int block = 256;   // threads per workgroup (multiple of 64, <= 1024)
+    int grid = 0;      // persistent workgroups (0 = 8 per compute unit)
+    int ppt = 4;       // particles per thread per tile
+    int replicas = 1;  // global density replicas the flush spreads over
+    int rep_stride = 0;
+};
+Shape resolve(Shape s);
+
+// Poisson solve request (one workgroup of 1024 lanes).
+struct PoissonJob {
+    int precision = PIC_PREC_FP64;
+    int n_g = 0;
+    const void *e_dens = nullptr;  // float* (FP32/COMP) or double* (FP64)
+    const void *i_dens = nullptr;
+    int replicas = 1, rep_stride = 0;
+    double pot0 = 0.0;                // used when time == nullptr
+    const uint32_t *time = nullptr;   // device step index within the cycle
+    int n_t = 4000, n_sub = 20;
+    double wave_cos[PIC_MAX_HARMONICS] = {}, wave_sin[PIC_MAX_HARMONICS] = {};
+    double e_charge = 0, alpha = 0, inv_dx = 0, half_inv_dx = 0, half_dx_over_eps = 0;
+    float *pot = nullptr, *efield = nullptr;
+    // fused-path extras, all optional
+    double *cumul_e = nullptr, *cumul_i = nullptr;  // measurement accumulation
+    unsigned long long *meas_steps = nullptr;       // [2] electron / ion steps measured
+    void *zero_a = nullptr; int zero_a_words = 0;   // 32-bit words to clear
+    void *zero_b = nullptr; int zero_b_words = 0;   // cleared on ion steps only
+    uint32_t *zero_cnt[4] = {nullptr, nullptr, nullptr, nullptr};  // cleared every step
+    uint32_t *zero_cnt_ion = nullptr;                               // cleared on ion steps
+    uint32_t *advance_time = nullptr;                               // t <- (t+1) % n_t
+};
+int launch_poisson(const PoissonJob &j, hipStream_t s);
+
+// n_cap bounds the live count read from n_active (an overflowed counter can
+// exceed the array it counts)
+int launch_deposit(const float *x, const uint32_t *n_active, void *dens, bool f64,
+                   const KPhys &k, const Shape &sh, hipStream_t s, uint32_t n_cap = 0xffffffffu);
+int launch_fill(void *ptr, uint32_t value, uint64_t words, hipStream_t s);
+// *dst = src ? min(*src, clamp) : 0
+int launch_copy_u32(uint32_t *dst, const uint32_t *src, hipStream_t s, uint32_t clamp = 0xffffffffu);
+
+struct MoveJob {
+    const float *efield;
+    const float *sx, *svx, *svy, *svz;
+    float *dx, *dvx, *dvy, *dvz;
+    uint32_t *alive;
+    const uint32_t *n_active;
+    float factor, dt, half_m_over_e;
+    unsigned long long *abs_counts;  // [2] or null
+    uint32_t *ifed;                  // [2*200] or null
+    uint32_t n_cap;                  // bound on *n_active
+};
+int launch_move_compact(const MoveJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+
+struct CollideEJob {
+    const float *stot_e, *cs;
+    const uint32_t *n_active;
+    pic_species_t e, i;
+    uint32_t *alive_e, *alive_i, *overflow;
+    unsigned long long *n_coll;
+};
+int launch_collide_e(const CollideEJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+
+struct CollideIJob {
+    const float *stot_i, *cs;
+    const uint32_t *n_active;
+    pic_species_t i;
+    unsigned long long *n_coll;
+};
+int launch_collide_i(const CollideIJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+
+int launch_accum_density(const void *dens, bool f64, double *cumul, int n_g, hipStream_t s);
+int launch_eepf(const float *x, const float *vx, const float *vy, const float *vz,
+                const float *efield, const uint32_t *n_active, uint32_t *eepf, const KPhys &k,
+                const Shape &sh, hipStream_t s, uint32_t n_cap = 0xffffffffu);
+
+// Fused particle step of the fast path (one kernel per species per step):
+// push, survivor filter, wave64 compaction, MCC collision with the
+// destination slot's RNG stream, and CIC deposit of the new positions into
+// the next step's density (plus EEPF in measurement steps).
+struct FusedJob {
+    bool electrons;
+    const float *efield;
+    // input: main array [0, *n_main) followed by the spawn staging [0, *n_spawn)
+    const float *sx, *svx, *svy, *svz;
+    const uint32_t *n_main;
+    const float *px, *pvx, *pvy, *pvz;   // previous step's spawns (electrons only)
+    const uint32_t *n_spawn;
+    // output
+    float *dx, *dvx, *dvy, *dvz;
+    uint32_t *n_out;
+    uint32_t capacity;
+    uint32_t *rng0, *rng1, *rng2, *rng3;  // per destination slot
+    // electron ionisation products
+    float *ox, *ovx, *ovy, *ovz;          // spawned electrons (staging)
+    uint32_t *n_spawn_out;
+    uint32_t spawn_capacity;
+    float *ix, *ivx, *ivy, *ivz;          // ion array (append)
+    uint32_t *n_ions;
+    uint32_t ion_capacity;
+    void *ion_dens;                        // next ion density (spawned ions deposit here)
+    // densities
+    void *dens;                            // next density of this species
+    bool dens_f64;
+    // tables and diagnostics
+    const float *stot, *cs;
+    uint32_t *overflow;
+    unsigned long long *counters;          // [8] as pic_sim_read_measurements
+    uint32_t *eepf;                        // measurement steps only
+    uint32_t *ifed;                        // ions only
+};
+int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+
+}  // namespace pic

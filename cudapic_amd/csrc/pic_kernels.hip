@@ -1,0 +1,1354 @@
+// pic_kernels.hip -- gfx950 kernels of the RF step and the stage entry
+// points of the C ABI (one per kernel of the reference's `mod kernels`,
+// Rust-CUDA/src/main.rs:352-1358).
+//
+// Launch model: every particle kernel is a persistent grid (a fixed number of
+// workgroups, by default 8 per compute unit) that reads the live count from
+// device memory and strides over tiles, so no host round trip is needed and
+// the work scales with the live count, not with MAX_PARTICLES.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pic_internal.h"
+
+namespace pic {
+
+// ============================================================ errors / device
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+int hip_fail(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return PIC_ERR_HIP;
+}
+
+int device_cus() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cache[dev] = cus;
+    }
+    return cache[dev];
+}
+
+Shape resolve(Shape s) {
+    if (s.block <= 0) s.block = 256;
+    s.block = std::min(1024, std::max(64, (s.block / 64) * 64));
+    if (s.grid <= 0) s.grid = device_cus() * std::max(1, 2048 / s.block);
+    if (s.ppt <= 0) s.ppt = 4;
+    if (s.replicas <= 0) s.replicas = 1;
+    return s;
+}
+
+KPhys make_kphys(const pic_params_t &p) {
+    KPhys k;
+    k.n_g = p.n_g;
+    k.inv_dx = p.inv_dx_f;
+    k.gap = p.gap_f;
+    k.wf = p.weight_factor;
+    k.factor_e = p.factor_e;
+    k.factor_i = p.factor_i;
+    k.dt_e = p.dt_e_f;
+    k.dt_i = p.dt_i_f;
+    k.normal_range = p.normal_range;
+    k.f1 = p.f1;
+    k.f2 = p.f2;
+    k.half_me_over_e = p.half_me_over_e;
+    k.half_mu_over_e = p.half_mu_over_e;
+    k.half_mi_over_e = p.half_mi_over_e;
+    k.e_charge = p.e_charge_f;
+    k.me = p.me_f;
+    k.exc_th = p.exc_th_f;
+    k.ion_th = p.ion_th_f;
+    k.de_cs = p.de_cs_f;
+    k.min_x = p.min_x_f;
+    k.max_x = p.max_x_f;
+    k.de_eepf = p.de_eepf_f;
+    k.eepf_kick = p.eepf_kick_f;
+    return k;
+}
+
+// LDS budget (bytes) one particle workgroup may take for its histograms
+// so that several workgroups stay resident per compute unit.
+static constexpr int kLdsBudget = 48 * 1024;
+
+static inline int lds_replicas(int block, int n_g, int fixed_bytes) {
+    int waves = block / kWave;
+    int fit = (kLdsBudget - fixed_bytes) / (n_g * 4);
+    return std::max(1, std::min(waves, fit));
+}
+
+// ============================================================ small kernels
+__global__ void k_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int launch_fill(void *ptr, uint32_t value, uint64_t words, hipStream_t s) {
+    if (words == 0) return PIC_OK;
+    int grid = (int)std::min<uint64_t>((words + 255) / 256, (uint64_t)device_cus() * 4);
+    k_fill<<<grid, 256, 0, s>>>((uint32_t *)ptr, value, words);
+    PIC_LAUNCHED("k_fill");
+    return PIC_OK;
+}
+
+__global__ void k_copy_u32(uint32_t *dst, const uint32_t *src, uint32_t clamp) {
+    if (threadIdx.x == 0) *dst = src ? min(*src, clamp) : 0u;
+}
+
+int launch_copy_u32(uint32_t *dst, const uint32_t *src, hipStream_t s, uint32_t clamp) {
+    k_copy_u32<<<1, 64, 0, s>>>(dst, src, clamp);
+    PIC_LAUNCHED("k_copy_u32");
+    return PIC_OK;
+}
+
+// ============================================================ deposition
+// get_density (main.rs:356-406).  Each wave owns an LDS histogram replica
+// (ds_add_f32 atomics; replicas cut same-bin conflicts between waves), the
+// workgroup sums its replicas once and flushes each non-zero node with one
+// global atomic (f32 or f64), spread over `replicas` global copies.
+template <typename Acc>
+__global__ __launch_bounds__(256) void k_deposit(const float *__restrict__ x,
+                                                 const uint32_t *__restrict__ n_active,
+                                                 uint32_t n_cap, Acc *__restrict__ dens, int reps,
+                                                 int rep_stride, float inv_dx, float wf, int n_g,
+                                                 int lds_reps) {
+    extern __shared__ float sh[];
+    const int wave = threadIdx.x / kWave;
+    for (int k = threadIdx.x; k < lds_reps * n_g; k += blockDim.x) sh[k] = 0.0f;
+    __syncthreads();
+    float *h = sh + (wave % lds_reps) * n_g;
+    const uint32_t n = min(*n_active, n_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    bool any = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 4 * stride) {
+        float xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint32_t idx = i + u * stride;
+            xv[u] = idx < n ? x[idx] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i + u * stride < n) {
+                const float pos = xv[u] * inv_dx;
+                const int q = cell_of(pos, n_g);
+                const float rem = pos - (float)q;
+                atomicAdd(&h[q], (1.0f - rem) * wf);
+                atomicAdd(&h[q + 1], rem * wf);
+            }
+        }
+        any = true;
+    }
+    if (!__syncthreads_or(any)) return;
+    Acc *out = dens + (blockIdx.x % reps) * rep_stride;
+    for (int k = threadIdx.x; k < n_g; k += blockDim.x) {
+        float s = 0.0f;
+        for (int r = 0; r < lds_reps; ++r) s += sh[r * n_g + k];
+        if (s != 0.0f) {
+            Acc v = (Acc)s;
+            if (k == 0 || k == n_g - 1) v *= (Acc)2;
+            unsafeAtomicAdd(&out[k], v);
+        }
+    }
+}
+
+int launch_deposit(const float *x, const uint32_t *n_active, void *dens, bool f64, const KPhys &k,
+                   const Shape &sh0, hipStream_t s, uint32_t n_cap) {
+    Shape sh = resolve(sh0);
+    const int block = 256;
+    const int lr = lds_replicas(block, k.n_g, 0);
+    const size_t lds = (size_t)lr * k.n_g * sizeof(float);
+    const int stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
+    const int grid = std::max(1, sh.grid * sh.block / block / 2);
+    if (f64)
+        k_deposit<double><<<grid, block, lds, s>>>(x, n_active, n_cap, (double *)dens, sh.replicas, stride,
+                                                   k.inv_dx, k.wf, k.n_g, lr);
+    else
+        k_deposit<float><<<grid, block, lds, s>>>(x, n_active, n_cap, (float *)dens, sh.replicas, stride,
+                                                  k.inv_dx, k.wf, k.n_g, lr);
+    PIC_LAUNCHED("k_deposit");
+    return PIC_OK;
+}
+
+// ============================================================ Poisson
+// Scan arithmetic for the three precisions: float (the reference), double,
+// and float-float ("compensated FP32": an unevaluated sum hi + lo carried
+// through error-free transformations, Dekker/Knuth TwoSum and fma TwoProd).
+struct ff {
+    float hi, lo;
+};
+__device__ __forceinline__ ff ff_fast2(float a, float b) {
+    float s = a + b;
+    return ff{s, b - (s - a)};
+}
+__device__ __forceinline__ ff ff_two_sum(float a, float b) {
+    float s = a + b;
+    float bb = s - a;
+    return ff{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ ff operator+(ff a, ff b) {
+    ff s = ff_two_sum(a.hi, b.hi);
+    ff t = ff_two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = ff_fast2(s.hi, s.lo);
+    s.lo += t.lo;
+    return ff_fast2(s.hi, s.lo);
+}
+__device__ __forceinline__ ff operator-(ff a) { return ff{-a.hi, -a.lo}; }
+__device__ __forceinline__ ff operator-(ff a, ff b) { return a + (-b); }
+__device__ __forceinline__ ff operator*(ff a, ff b) {
+    float p = a.hi * b.hi;
+    float e = __builtin_fmaf(a.hi, b.hi, -p);
+    e += a.hi * b.lo + a.lo * b.hi;
+    return ff_fast2(p, e);
+}
+__device__ __forceinline__ ff operator/(ff a, ff b) {
+    float q1 = a.hi / b.hi;
+    ff r = a - b * ff{q1, 0.0f};
+    float q2 = r.hi / b.hi;
+    r = r - b * ff{q2, 0.0f};
+    float q3 = r.hi / b.hi;
+    ff q = ff_fast2(q1, q2);
+    return q + ff{q3, 0.0f};
+}
+
+template <typename T> struct Num;
+template <> struct Num<float> {
+    __device__ static float from(double d) { return (float)d; }
+    __device__ static float fromf(float f) { return f; }
+    __device__ static float zero() { return 0.0f; }
+    __device__ static float to_f(float v) { return v; }
+    __device__ static float shfl_up(float v, int d) { return __shfl_up(v, d, kWave); }
+};
+template <> struct Num<double> {
+    __device__ static double from(double d) { return d; }
+    __device__ static double fromf(float f) { return (double)f; }
+    __device__ static double zero() { return 0.0; }
+    __device__ static float to_f(double v) { return (float)v; }
+    __device__ static double shfl_up(double v, int d) { return __shfl_up(v, d, kWave); }
+};
+template <> struct Num<ff> {
+    __device__ static ff from(double d) {
+        float h = (float)d;
+        return ff{h, (float)(d - (double)h)};
+    }
+    __device__ static ff fromf(float f) { return ff{f, 0.0f}; }
+    __device__ static ff zero() { return ff{0.0f, 0.0f}; }
+    __device__ static float to_f(ff v) { return v.hi + v.lo; }
+    __device__ static ff shfl_up(ff v, int d) {
+        return ff{__shfl_up(v.hi, d, kWave), __shfl_up(v.lo, d, kWave)};
+    }
+};
+
+// Inclusive scan over the 1024 lanes of the workgroup (16 waves): shuffle
+// scan inside each wave, then the 16 wave totals through LDS.
+template <typename T>
+__device__ __forceinline__ T block_scan_incl(T v, T *wtot) {
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        T o = Num<T>::shfl_up(v, off);
+        if (lane >= off) v = o + v;
+    }
+    if (lane == kWave - 1) wtot[wave] = v;
+    __syncthreads();
+    if (wave == 0) {
+        T w = lane < nw ? wtot[lane] : Num<T>::zero();
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            T o = Num<T>::shfl_up(w, off);
+            if (lane >= off) w = o + w;
+        }
+        if (lane < nw) wtot[lane] = w;
+    }
+    __syncthreads();
+    if (wave > 0) v = wtot[wave - 1] + v;
+    __syncthreads();
+    return v;
+}
+
+constexpr int kPoissonThreads = 1024;
+constexpr int kPoissonMaxPer = 4;  // n_g <= 4096
+
+struct PoissonArgs {
+    const void *e_dens, *i_dens;
+    int reps, rep_stride, n_g, per;
+    double pot0;
+    const uint32_t *time;
+    int n_t, n_sub, n_harm;
+    double wave_cos[PIC_MAX_HARMONICS], wave_sin[PIC_MAX_HARMONICS];
+    double e_charge, alpha, inv_dx, half_inv_dx, half_dx_over_eps;
+    float *pot, *efield;
+    double *cumul_e, *cumul_i;
+    unsigned long long *meas_steps;
+    uint32_t *zero_a;
+    int zero_a_words;
+    uint32_t *zero_b;
+    int zero_b_words;
+    uint32_t *zero_cnt[4];
+    uint32_t *zero_cnt_ion;
+    uint32_t *advance_time;
+};
+
+template <typename D>
+__device__ __forceinline__ double load_dens(const D *p, int reps, int stride, int k) {
+    double s = 0.0;
+    for (int r = 0; r < reps; ++r) s += (double)p[r * stride + k];
+    return s;
+}
+
+// solve_poisson_scan_f32 (main.rs:408-494) generalised to three
+// precisions and n_g <= 4096 (up to four nodes per lane).  The tridiagonal
+// Dirichlet system with A = C = 1, B = -2 is solved as two inclusive scans:
+//   G_i = sum_{j<=i} j f_j,  r_i = -G_i / ((i+1) i),  R = scan(r),
+//   pot_i = i (R_{N-2} - R_i + r_i),
+// then E by central differences with the electrode formulas of
+// C/eduPIC.cc:491-493.
+template <typename T, typename D>
+__global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *pot_s = reinterpret_cast<T *>(smem);               // n_g
+    T *wtot = pot_s + a.n_g;                              // 16
+    __shared__ double s_pot0;
+    __shared__ int s_ion;
+    const int tid = threadIdx.x;
+    const int n_g = a.n_g;
+    const int per = a.per;
+    if (tid == 0) {
+        double p0 = a.pot0;
+        int ion = 1;
+        if (a.time) {
+            const uint32_t t = *a.time;
+            ion = (t % (uint32_t)a.n_sub) == 0;
+            p0 = 0.0;
+            for (int h = 0; h < a.n_harm; ++h) {
+                // 2 pi (h+1) t / N_T reduced exactly in integers first
+                const uint64_t m = (2ull * (uint64_t)(h + 1) * t) % (2ull * (uint64_t)a.n_t);
+                double sv, cv;
+                sincospi((double)m / (double)a.n_t, &sv, &cv);
+                p0 += a.wave_cos[h] * cv + a.wave_sin[h] * sv;
+            }
+        }
+        s_pot0 = p0;
+        s_ion = ion;
+    }
+    __syncthreads();
+    const T pot0 = Num<T>::from(s_pot0);
+    const T alpha = Num<T>::from(a.alpha);
+    const T echarge = Num<T>::from(a.e_charge);
+
+    const D *ed = (const D *)a.e_dens;
+    const D *id = (const D *)a.i_dens;
+    T rho[kPoissonMaxPer], g[kPoissonMaxPer];
+    const int k0 = tid * per;
+    // densities (summed over the global replicas), charge density, g_i = i f_i
+    T run = Num<T>::zero();
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        rho[u] = Num<T>::zero();
+        g[u] = Num<T>::zero();
+        if (u < per && k < n_g) {
+            T ne, ni;
+            if (a.reps == 1) {
+                ne = Num<T>::from((double)ed[k]);
+                ni = Num<T>::from((double)id[k]);
+            } else {
+                ne = Num<T>::from(load_dens(ed, a.reps, a.rep_stride, k));
+                ni = Num<T>::from(load_dens(id, a.reps, a.rep_stride, k));
+            }
+            rho[u] = echarge * (ni - ne);
+            if (k >= 1 && k <= n_g - 2) {
+                T f = alpha * rho[u];
+                if (k == 1) f = f - pot0;
+                g[u] = Num<T>::from((double)k) * f;
+            }
+            run = run + g[u];
+            g[u] = run;  // chunk-local inclusive prefix
+        }
+    }
+    T excl = block_scan_incl<T>(run, wtot) - run;
+    T r[kPoissonMaxPer], R[kPoissonMaxPer];
+    run = Num<T>::zero();
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        r[u] = Num<T>::zero();
+        if (u < per && k >= 1 && k <= n_g - 2) {
+            const T G = excl + g[u];
+            const T h = -G / Num<T>::from((double)k + 1.0);
+            r[u] = h / Num<T>::from((double)k);
+        }
+        if (u < per && k < n_g) {
+            run = run + r[u];
+            R[u] = run;
+        }
+    }
+    excl = block_scan_incl<T>(run, wtot) - run;
+    // R_{N-2}
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u)
+        if (u < per && k0 + u == n_g - 2) wtot[16] = excl + R[u];
+    __syncthreads();
+    const T r_total = wtot[16];
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        if (u < per && k < n_g) {
+            T pk;
+            if (k == 0) pk = pot0;
+            else if (k < n_g - 1) pk = Num<T>::from((double)k) * (r_total - (excl + R[u]) + r[u]);
+            else pk = Num<T>::zero();
+            pot_s[k] = pk;
+        }
+    }
+    __syncthreads();
+    const T inv_dx = Num<T>::from(a.inv_dx);
+    const T half_inv_dx = Num<T>::from(a.half_inv_dx);
+    const T hdx_eps = Num<T>::from(a.half_dx_over_eps);
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        if (u < per && k < n_g) {
+            const T pk = pot_s[k];
+            T e;
+            if (k == 0) e = (pk - pot_s[1]) * inv_dx - rho[u] * hdx_eps;
+            else if (k == n_g - 1) e = (pot_s[n_g - 2] - pk) * inv_dx + rho[u] * hdx_eps;
+            else e = (pot_s[k - 1] - pot_s[k + 1]) * half_inv_dx;
+            a.pot[k] = Num<T>::to_f(pk);
+            a.efield[k] = Num<T>::to_f(e);
+            if (a.cumul_e) {
+                a.cumul_e[k] += a.reps == 1 ? (double)ed[k] : load_dens(ed, a.reps, a.rep_stride, k);
+                if (s_ion) a.cumul_i[k] += a.reps == 1 ? (double)id[k] : load_dens(id, a.reps, a.rep_stride, k);
+            }
+        }
+    }
+    // fused-path bookkeeping for the particle kernels that follow
+    for (int w = tid; w < a.zero_a_words; w += blockDim.x) a.zero_a[w] = 0u;
+    if (s_ion)
+        for (int w = tid; w < a.zero_b_words; w += blockDim.x) a.zero_b[w] = 0u;
+    if (tid < 4 && a.zero_cnt[tid]) *a.zero_cnt[tid] = 0u;
+    if (tid == 4 && s_ion && a.zero_cnt_ion) *a.zero_cnt_ion = 0u;
+    if (tid == 5 && a.meas_steps && a.cumul_e) {
+        a.meas_steps[0] += 1ull;
+        if (s_ion) a.meas_steps[1] += 1ull;
+    }
+    __syncthreads();
+    if (tid == 0 && a.advance_time) *a.advance_time = (*a.advance_time + 1u) % (uint32_t)a.n_t;
+}
+
+int launch_poisson(const PoissonJob &j, hipStream_t s) {
+    if (j.n_g < 3 || j.n_g > kPoissonThreads * kPoissonMaxPer) {
+        set_error("poisson: n_g must be in [3, 4096]");
+        return PIC_ERR_ARG;
+    }
+    PoissonArgs a;
+    memset(&a, 0, sizeof(a));
+    a.e_dens = j.e_dens;
+    a.i_dens = j.i_dens;
+    a.reps = std::max(1, j.replicas);
+    a.rep_stride = j.rep_stride > 0 ? j.rep_stride : j.n_g;
+    a.n_g = j.n_g;
+    a.per = (j.n_g + kPoissonThreads - 1) / kPoissonThreads;
+    a.pot0 = j.pot0;
+    a.time = j.time;
+    a.n_t = j.n_t;
+    a.n_sub = j.n_sub;
+    a.n_harm = 0;
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) {
+        a.wave_cos[h] = j.wave_cos[h];
+        a.wave_sin[h] = j.wave_sin[h];
+        if (j.wave_cos[h] != 0.0 || j.wave_sin[h] != 0.0) a.n_harm = h + 1;
+    }
+    a.e_charge = j.e_charge;
+    a.alpha = j.alpha;
+    a.inv_dx = j.inv_dx;
+    a.half_inv_dx = j.half_inv_dx;
+    a.half_dx_over_eps = j.half_dx_over_eps;
+    a.pot = j.pot;
+    a.efield = j.efield;
+    a.cumul_e = j.cumul_e;
+    a.cumul_i = j.cumul_i;
+    a.meas_steps = j.meas_steps;
+    a.zero_a = (uint32_t *)j.zero_a;
+    a.zero_a_words = j.zero_a_words;
+    a.zero_b = (uint32_t *)j.zero_b;
+    a.zero_b_words = j.zero_b_words;
+    for (int c = 0; c < 4; ++c) a.zero_cnt[c] = j.zero_cnt[c];
+    a.zero_cnt_ion = j.zero_cnt_ion;
+    a.advance_time = j.advance_time;
+    switch (j.precision) {
+    case PIC_PREC_FP32: {
+        size_t lds = (j.n_g + 17) * sizeof(float);
+        k_poisson<float, float><<<1, kPoissonThreads, lds, s>>>(a);
+        break;
+    }
+    case PIC_PREC_COMP: {
+        size_t lds = (j.n_g + 17) * sizeof(ff);
+        k_poisson<ff, float><<<1, kPoissonThreads, lds, s>>>(a);
+        break;
+    }
+    case PIC_PREC_FP64: {
+        size_t lds = (j.n_g + 17) * sizeof(double);
+        k_poisson<double, double><<<1, kPoissonThreads, lds, s>>>(a);
+        break;
+    }
+    default:
+        set_error("poisson: unknown precision");
+        return PIC_ERR_ARG;
+    }
+    PIC_LAUNCHED("k_poisson");
+    return PIC_OK;
+}
+
+// solve_poisson_dps_flexible (main.rs:496-564): psi'' = src on n nodes.
+template <typename T>
+__global__ __launch_bounds__(kPoissonThreads) void k_poisson_flexible(const float *src, int n, int per,
+                                                                        float psi_l, float psi_r,
+                                                                        float dx, float *pot) {
+    __shared__ T wtot[17];
+    const int tid = threadIdx.x;
+    const int k0 = tid * per;
+    const T dx2 = Num<T>::fromf(dx) * Num<T>::fromf(dx);
+    T g[kPoissonMaxPer];
+    T run = Num<T>::zero();
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        g[u] = Num<T>::zero();
+        if (u < per && k >= 1 && k <= n - 2) {
+            T f = dx2 * Num<T>::fromf(src[k]);
+            if (k == 1) f = f - Num<T>::fromf(psi_l);
+            if (k == n - 2) f = f - Num<T>::fromf(psi_r);
+            g[u] = Num<T>::from((double)k) * f;
+        }
+        run = run + g[u];
+        g[u] = run;
+    }
+    T excl = block_scan_incl<T>(run, wtot) - run;
+    T r[kPoissonMaxPer], R[kPoissonMaxPer];
+    run = Num<T>::zero();
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        r[u] = Num<T>::zero();
+        if (u < per && k >= 1 && k <= n - 2) {
+            const T h = -(excl + g[u]) / Num<T>::from((double)k + 1.0);
+            r[u] = h / Num<T>::from((double)k);
+        }
+        run = run + r[u];
+        R[u] = run;
+    }
+    excl = block_scan_incl<T>(run, wtot) - run;
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u)
+        if (u < per && k0 + u == n - 2) wtot[16] = excl + R[u];
+    __syncthreads();
+    const T r_total = wtot[16];
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        const int k = k0 + u;
+        if (u < per && k < n) {
+            T pk;
+            if (k == 0) pk = Num<T>::fromf(psi_l);
+            else if (k < n - 1) pk = Num<T>::from((double)k) * (r_total - (excl + R[u]) + r[u]);
+            else pk = Num<T>::fromf(psi_r);
+            pot[k] = Num<T>::to_f(pk);
+        }
+    }
+}
+
+// ============================================================ push + compaction
+struct MoveArgs {
+    const float *efield;
+    const float *sx, *svx, *svy, *svz;
+    float *dx, *dvx, *dvy, *dvz;
+    uint32_t *alive;
+    const uint32_t *n_active;
+    float factor, dt, half_m_over_e;
+    unsigned long long *abs_counts;
+    uint32_t *ifed;
+    uint32_t n_cap;
+    int n_g;
+    float inv_dx, gap;
+};
+
+// Workgroup-wide slot reservation for one tile: every wave reports its
+// survivor count, thread 0 prefix-sums the (<= 16) wave counts and takes the
+// tile's range with a single global atomic, and every wave reads back its
+// base.  `buf` alternates between two LDS slots so that one tile costs two
+// barriers.
+__device__ __forceinline__ uint32_t tile_reserve(uint32_t wave_count, uint32_t *s_wave,
+                                                 uint32_t *s_base, int buf, uint32_t *counter) {
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane_id() == 0) s_wave[buf * 16 + wave] = wave_count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            uint32_t c = s_wave[buf * 16 + w];
+            s_wave[buf * 16 + w] = acc;
+            acc += c;
+        }
+        s_base[buf] = acc ? atomicAdd(counter, acc) : 0u;
+    }
+    __syncthreads();
+    return s_base[buf] + s_wave[buf * 16 + wave];
+}
+
+// move_and_compact (main.rs:566-678) on wave64: each thread takes PPT
+// particles of a tile (strided by the workgroup size, so every load is
+// coalesced), survivors are ranked with 64-bit ballots and v_mbcnt, and the
+// tile makes one global atomic.  The efield array is staged in LDS.
+// Lost particles are counted per electrode (one atomic per workgroup) and,
+// for ions, binned by energy (IFED, C/eduPIC.cc:625-667).
+template <int PPT>
+__global__ __launch_bounds__(256) void k_move_compact(MoveArgs a) {
+    extern __shared__ float ef[];  // n_g
+    __shared__ uint32_t s_wave[32], s_base[2];
+    __shared__ uint32_t s_ifed[2 * kIfedBins];
+    const bool do_ifed = a.ifed != nullptr;
+    for (int k = threadIdx.x; k < a.n_g; k += blockDim.x) ef[k] = a.efield[k];
+    if (do_ifed)
+        for (int k = threadIdx.x; k < 2 * kIfedBins; k += blockDim.x) s_ifed[k] = 0u;
+    __syncthreads();
+    const uint32_t n = min(*a.n_active, a.n_cap);
+    const uint32_t tile_sz = blockDim.x * PPT;
+    const uint32_t n_tiles = (n + tile_sz - 1) / tile_sz;
+    uint32_t lost_pow = 0, lost_gnd = 0;
+    int buf = 0;
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x, buf ^= 1) {
+        float nx[PPT], nvx[PPT], vy[PPT], vz[PPT];
+        bool keep[PPT];
+        uint32_t off[PPT];
+        const uint32_t base = tile * tile_sz + threadIdx.x;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const uint32_t i = base + j * blockDim.x;
+            keep[j] = false;
+            if (i < n) {
+                const float x = a.sx[i];
+                const float e = field_at(ef, x, a.inv_dx, a.n_g);
+                nvx[j] = __builtin_fmaf(a.factor, e, a.svx[i]);
+                nx[j] = __builtin_fmaf(nvx[j], a.dt, x);
+                keep[j] = nx[j] >= 0.0f && nx[j] <= a.gap;
+                if (keep[j]) {
+                    vy[j] = a.svy[i];
+                    vz[j] = a.svz[i];
+                } else {
+                    const bool pw = nx[j] < 0.0f;
+                    lost_pow += pw;
+                    lost_gnd += !pw;
+                    if (do_ifed) {
+                        const float y = a.svy[i], z = a.svz[i];
+                        const float en = a.half_m_over_e * (nvx[j] * nvx[j] + y * y + z * z);
+                        const int b = (int)en;
+                        if (en >= 0.0f && b < kIfedBins) atomicAdd(&s_ifed[(pw ? 0 : 1) * kIfedBins + b], 1u);
+                    }
+                }
+            }
+        }
+        uint32_t wcount = 0;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const uint64_t m = ballot64(keep[j]);
+            off[j] = wcount + mbcnt64(m);
+            wcount += (uint32_t)__popcll(m);
+        }
+        const uint32_t wbase = tile_reserve(wcount, s_wave, s_base, buf, a.alive);
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            if (keep[j]) {
+                const uint32_t slot = wbase + off[j];
+                a.dx[slot] = nx[j];
+                a.dvx[slot] = nvx[j];
+                a.dvy[slot] = vy[j];
+                a.dvz[slot] = vz[j];
+            }
+        }
+    }
+    if (a.abs_counts) {
+        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd);
+        if (lane_id() == 0) {
+            if (wp) atomicAdd(&a.abs_counts[0], (unsigned long long)wp);
+            if (wg) atomicAdd(&a.abs_counts[1], (unsigned long long)wg);
+        }
+    }
+    if (do_ifed) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < 2 * kIfedBins; k += blockDim.x)
+            if (s_ifed[k]) atomicAdd(&a.ifed[k], s_ifed[k]);
+    }
+}
+
+int launch_move_compact(const MoveJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
+    Shape sh = resolve(sh0);
+    MoveArgs a;
+    a.efield = j.efield;
+    a.sx = j.sx;
+    a.svx = j.svx;
+    a.svy = j.svy;
+    a.svz = j.svz;
+    a.dx = j.dx;
+    a.dvx = j.dvx;
+    a.dvy = j.dvy;
+    a.dvz = j.dvz;
+    a.alive = j.alive;
+    a.n_active = j.n_active;
+    a.factor = j.factor;
+    a.dt = j.dt;
+    a.half_m_over_e = j.half_m_over_e;
+    a.abs_counts = j.abs_counts;
+    a.ifed = j.ifed;
+    a.n_cap = j.n_cap;
+    a.n_g = k.n_g;
+    a.inv_dx = k.inv_dx;
+    a.gap = k.gap;
+    const size_t lds = (size_t)k.n_g * sizeof(float);
+    const int block = 256;
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    switch (sh.ppt) {
+    case 1: k_move_compact<1><<<grid, block, lds, s>>>(a); break;
+    case 2: k_move_compact<2><<<grid, block, lds, s>>>(a); break;
+    case 8: k_move_compact<8><<<grid, block, lds, s>>>(a); break;
+    default: k_move_compact<4><<<grid, block, lds, s>>>(a); break;
+    }
+    PIC_LAUNCHED("k_move_compact");
+    return PIC_OK;
+}
+
+// ============================================================ collisions
+struct CollEArgs {
+    const float *stot_e, *cs;
+    const uint32_t *n_active;
+    float *x, *vx, *vy, *vz;
+    uint32_t *r0, *r1, *r2, *r3;
+    float *ix, *ivx, *ivy, *ivz;
+    uint32_t *alive_e, *alive_i;
+    uint32_t cap_e, cap_i;
+    uint32_t *overflow;
+    unsigned long long *n_coll;
+    KPhys k;
+};
+
+// check_collisions_e (main.rs:971-1165).  The loop is wave-uniform so that
+// the ionisation appends can be aggregated: one atomic per species per wave,
+// lanes ranked by v_mbcnt.
+__global__ __launch_bounds__(256) void k_collide_e(CollEArgs a) {
+    const uint32_t n = min(*a.n_active, a.cap_e);
+    const uint32_t lane = lane_id();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t ncoll = 0;
+    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(kWave - 1)); i0 < n;
+         i0 += stride) {
+        const uint32_t i = i0 + lane;
+        int kind = kNone;
+        float se[3], si[3];
+        if (i < n) {
+            float vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+            Xo s = rng_load(a.r0, a.r1, a.r2, a.r3, i);
+            kind = collide_electron(a.stot_e, a.cs, a.k, vx, vy, vz, s, se, si);
+            rng_store(a.r0, a.r1, a.r2, a.r3, i, s);
+            if (kind != kNone) {
+                a.vx[i] = vx;
+                a.vy[i] = vy;
+                a.vz[i] = vz;
+                ++ncoll;
+            }
+        }
+        const uint64_t sp = ballot64(kind == kIonisation);
+        if (sp) {
+            const int leader = __builtin_ctzll(sp);
+            uint32_t be = 0, bi = 0;
+            if ((int)lane == leader) {
+                const uint32_t cnt = (uint32_t)__popcll(sp);
+                be = atomicAdd(a.alive_e, cnt);
+                bi = atomicAdd(a.alive_i, cnt);
+            }
+            be = __shfl(be, leader, kWave);
+            bi = __shfl(bi, leader, kWave);
+            if (kind == kIonisation) {
+                const uint32_t o = mbcnt64(sp);
+                const float xi = a.x[i];
+                const uint32_t je = be + o, ji = bi + o;
+                if (je < a.cap_e) {
+                    a.x[je] = xi;
+                    a.vx[je] = se[0];
+                    a.vy[je] = se[1];
+                    a.vz[je] = se[2];
+                } else {
+                    *a.overflow = 1u;
+                }
+                if (ji < a.cap_i) {
+                    a.ix[ji] = xi;
+                    a.ivx[ji] = si[0];
+                    a.ivy[ji] = si[1];
+                    a.ivz[ji] = si[2];
+                } else {
+                    *a.overflow = 1u;
+                }
+            }
+        }
+    }
+    if (a.n_coll) {
+        const uint32_t c = wave_sum(ncoll);
+        if (lane == 0 && c) atomicAdd(a.n_coll, (unsigned long long)c);
+    }
+}
+
+int launch_collide_e(const CollideEJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
+    Shape sh = resolve(sh0);
+    CollEArgs a;
+    a.stot_e = j.stot_e;
+    a.cs = j.cs;
+    a.n_active = j.n_active;
+    a.x = j.e.x;
+    a.vx = j.e.vx;
+    a.vy = j.e.vy;
+    a.vz = j.e.vz;
+    a.r0 = j.e.rng0;
+    a.r1 = j.e.rng1;
+    a.r2 = j.e.rng2;
+    a.r3 = j.e.rng3;
+    a.ix = j.i.x;
+    a.ivx = j.i.vx;
+    a.ivy = j.i.vy;
+    a.ivz = j.i.vz;
+    a.alive_e = j.alive_e;
+    a.alive_i = j.alive_i;
+    a.cap_e = j.e.capacity;
+    a.cap_i = j.i.capacity;
+    a.overflow = j.overflow;
+    a.n_coll = j.n_coll;
+    a.k = k;
+    const int block = 256;
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    k_collide_e<<<grid, block, 0, s>>>(a);
+    PIC_LAUNCHED("k_collide_e");
+    return PIC_OK;
+}
+
+struct CollIArgs {
+    const float *stot_i, *cs;
+    const uint32_t *n_active;
+    uint32_t n_cap;
+    float *vx, *vy, *vz;
+    uint32_t *r0, *r1, *r2, *r3;
+    unsigned long long *n_coll;
+    KPhys k;
+};
+
+// check_collisions_i (main.rs:1167-1292)
+__global__ __launch_bounds__(256) void k_collide_i(CollIArgs a) {
+    const uint32_t n = min(*a.n_active, a.n_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t ncoll = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+        Xo s = rng_load(a.r0, a.r1, a.r2, a.r3, i);
+        const int kind = collide_ion(a.stot_i, a.cs, a.k, vx, vy, vz, s);
+        rng_store(a.r0, a.r1, a.r2, a.r3, i, s);
+        if (kind != kNone) {
+            a.vx[i] = vx;
+            a.vy[i] = vy;
+            a.vz[i] = vz;
+            ++ncoll;
+        }
+    }
+    if (a.n_coll) {
+        const uint32_t c = wave_sum(ncoll);
+        if (lane_id() == 0 && c) atomicAdd(a.n_coll, (unsigned long long)c);
+    }
+}
+
+int launch_collide_i(const CollideIJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
+    Shape sh = resolve(sh0);
+    CollIArgs a;
+    a.stot_i = j.stot_i;
+    a.cs = j.cs;
+    a.n_active = j.n_active;
+    a.n_cap = j.i.capacity;
+    a.vx = j.i.vx;
+    a.vy = j.i.vy;
+    a.vz = j.i.vz;
+    a.r0 = j.i.rng0;
+    a.r1 = j.i.rng1;
+    a.r2 = j.i.rng2;
+    a.r3 = j.i.rng3;
+    a.n_coll = j.n_coll;
+    a.k = k;
+    const int block = 256;
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    k_collide_i<<<grid, block, 0, s>>>(a);
+    PIC_LAUNCHED("k_collide_i");
+    return PIC_OK;
+}
+
+// ============================================================ measurement
+// accumulate_density (main.rs:1294-1299)
+template <typename D>
+__global__ void k_accum_density(const D *__restrict__ d, double *__restrict__ c, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) c[k] += (double)d[k];
+}
+
+int launch_accum_density(const void *dens, bool f64, double *cumul, int n_g, hipStream_t s) {
+    int grid = (n_g + 255) / 256;
+    if (f64) k_accum_density<double><<<grid, 256, 0, s>>>((const double *)dens, cumul, n_g);
+    else k_accum_density<float><<<grid, 256, 0, s>>>((const float *)dens, cumul, n_g);
+    PIC_LAUNCHED("k_accum_density");
+    return PIC_OK;
+}
+
+// EEPF bin of one electron: time-centred v_x at its (post-push) position
+// (main.rs:1316-1334).  Returns -1 outside the centre window or the range.
+__device__ __forceinline__ int eepf_bin(const float *ef, float x, float vx, float vy, float vz,
+                                        const KPhys &k) {
+    if (x <= k.min_x || x >= k.max_x) return -1;
+    const float e = field_at(ef, x, k.inv_dx, k.n_g);
+    const float mvx = vx - 0.5f * e * k.eepf_kick;
+    const float v2 = mvx * mvx + vy * vy + vz * vz;
+    const float en = k.half_me_over_e * v2;
+    const uint32_t b = (uint32_t)(en / k.de_eepf);
+    return b < (uint32_t)kEepfBins ? (int)b : -1;
+}
+
+// accumulate_eepf (main.rs:1301-1339) with an LDS histogram per workgroup.
+__global__ __launch_bounds__(256) void k_eepf(const float *__restrict__ x, const float *__restrict__ vx,
+                                              const float *__restrict__ vy, const float *__restrict__ vz,
+                                              const float *__restrict__ efield,
+                                              const uint32_t *__restrict__ n_active, uint32_t n_cap,
+                                              uint32_t *__restrict__ eepf, KPhys k) {
+    __shared__ uint32_t h[kEepfBins];
+    for (int b = threadIdx.x; b < kEepfBins; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    const uint32_t n = min(*n_active, n_cap);
+    bool any = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int b = eepf_bin(efield, x[i], vx[i], vy[i], vz[i], k);
+        if (b >= 0) atomicAdd(&h[b], 1u);
+        any = true;
+    }
+    if (!__syncthreads_or(any)) return;
+    for (int b = threadIdx.x; b < kEepfBins; b += blockDim.x)
+        if (h[b]) atomicAdd(&eepf[b], h[b]);
+}
+
+int launch_eepf(const float *x, const float *vx, const float *vy, const float *vz,
+                const float *efield, const uint32_t *n_active, uint32_t *eepf, const KPhys &k,
+                const Shape &sh0, hipStream_t s, uint32_t n_cap) {
+    Shape sh = resolve(sh0);
+    const int block = 256;
+    const int grid = std::max(1, sh.grid * sh.block / block / 4);
+    k_eepf<<<grid, block, 0, s>>>(x, vx, vy, vz, efield, n_active, n_cap, eepf, k);
+    PIC_LAUNCHED("k_eepf");
+    return PIC_OK;
+}
+
+// ============================================================ RNG utilities
+// xoshiro128p_test_wrapper (main.rs:1341-1357), n outputs in one launch.
+__global__ void k_xo_next_n(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, uint32_t n,
+                            uint32_t *out, uint32_t *state_out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    Xo s{s0, s1, s2, s3};
+    for (uint32_t i = 0; i < n; ++i) out[i] = xo_next(s);
+    if (state_out) {
+        state_out[0] = s.s0;
+        state_out[1] = s.s1;
+        state_out[2] = s.s2;
+        state_out[3] = s.s3;
+    }
+}
+
+// Stream k = J^k * master over GF(2), J = jump() as a 128x128 bit matrix.
+// mats holds J^(2^m) for m = 0..31 column-major: mats[(m*128 + c)*4 + w] is
+// word w of column c.  A lane applies the matrices of the set bits of k.
+__global__ __launch_bounds__(256) void k_seed_streams(const uint32_t *__restrict__ mats, uint32_t m0,
+                                                      uint32_t m1, uint32_t m2, uint32_t m3,
+                                                      uint32_t n, uint32_t *r0, uint32_t *r1,
+                                                      uint32_t *r2, uint32_t *r3) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint32_t v[4] = {m0, m1, m2, m3};
+        for (int m = 0; m < 32; ++m) {
+            if (!((k >> m) & 1u)) continue;
+            const uint32_t *M = mats + (size_t)m * 128 * 4;
+            uint32_t o[4] = {0u, 0u, 0u, 0u};
+            for (int c = 0; c < 128; ++c) {
+                const uint32_t bit = (v[c >> 5] >> (c & 31)) & 1u;
+                const uint32_t msk = 0u - bit;
+                o[0] ^= M[c * 4 + 0] & msk;
+                o[1] ^= M[c * 4 + 1] & msk;
+                o[2] ^= M[c * 4 + 2] & msk;
+                o[3] ^= M[c * 4 + 3] & msk;
+            }
+            v[0] = o[0];
+            v[1] = o[1];
+            v[2] = o[2];
+            v[3] = o[3];
+        }
+        r0[k] = v[0];
+        r1[k] = v[1];
+        r2[k] = v[2];
+        r3[k] = v[3];
+    }
+}
+
+// Host side: the jump of main.rs:1473-1487 as a linear map, and its powers.
+static void host_xo_next(uint32_t s[4]) {
+    const uint32_t t = s[1] << 9;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = (s[3] << 11) | (s[3] >> 21);
+}
+static void host_xo_jump(uint32_t s[4]) {
+    static const uint32_t JUMP[4] = {0x8764000b, 0xf542d2d3, 0x6fa035c3, 0x77f2db5b};
+    uint32_t acc[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 4; ++w)
+        for (int b = 0; b < 32; ++b) {
+            if (JUMP[w] & (1u << b))
+                for (int q = 0; q < 4; ++q) acc[q] ^= s[q];
+            host_xo_next(s);
+        }
+    for (int q = 0; q < 4; ++q) s[q] = acc[q];
+}
+// C = A * B over GF(2), column-major 128x128 with 4 words per column.
+static void gf2_mul(const uint32_t *A, const uint32_t *B, uint32_t *C) {
+    for (int c = 0; c < 128; ++c) {
+        uint32_t o[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 128; ++k)
+            if ((B[c * 4 + (k >> 5)] >> (k & 31)) & 1u)
+                for (int w = 0; w < 4; ++w) o[w] ^= A[k * 4 + w];
+        for (int w = 0; w < 4; ++w) C[c * 4 + w] = o[w];
+    }
+}
+
+static const std::vector<uint32_t> &jump_powers() {
+    static std::vector<uint32_t> mats;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        mats.assign(32 * 128 * 4, 0u);
+        for (int c = 0; c < 128; ++c) {
+            uint32_t s[4] = {0, 0, 0, 0};
+            s[c >> 5] = 1u << (c & 31);
+            host_xo_jump(s);
+            for (int w = 0; w < 4; ++w) mats[c * 4 + w] = s[w];
+        }
+        for (int m = 1; m < 32; ++m)
+            gf2_mul(&mats[(m - 1) * 512], &mats[(m - 1) * 512], &mats[m * 512]);
+    });
+    return mats;
+}
+
+}  // namespace pic
+
+// ============================================================ C ABI
+using namespace pic;
+
+static inline hipStream_t S(pic_stream_t s) { return (hipStream_t)s; }
+
+extern "C" {
+
+const char *pic_last_error(void) { return pic::g_err.c_str(); }
+
+void pic_params_default(pic_params_t *p) {
+    memset(p, 0, sizeof(*p));
+    p->n_g = 400;
+    p->n_t = 4000;
+    p->n_sub = 20;
+    p->precision = PIC_PREC_FP32;
+    p->freq = 13.56e6;
+    p->v0 = 250.0;
+    p->gap = 0.025;
+    p->pressure = 10.0;
+    p->t_gas = 350.0;
+    p->weight = 7.0e4;
+    p->area = 1.0e-4;
+}
+
+int pic_params_derive(pic_params_t *p) {
+    if (!p || p->n_g < 3 || p->n_g > 4096 || p->n_t <= 0 || p->n_sub <= 0 || p->freq <= 0 ||
+        p->gap <= 0 || p->pressure <= 0 || p->t_gas <= 0 || p->weight <= 0 || p->area <= 0) {
+        set_error("pic_params_derive: invalid parameters");
+        return PIC_ERR_ARG;
+    }
+    const double E = 1.60217662e-19, ME = 9.10938356e-31, MAR = 6.63352090e-26;
+    const double KB = 1.38064852e-23, EPS0 = 8.85418781e-12;
+    const double period = 1.0 / p->freq;
+    p->dt_e = period / (double)p->n_t;
+    p->dt_i = (double)p->n_sub * p->dt_e;
+    p->dx = p->gap / (double)(p->n_g - 1);
+    p->inv_dx = 1.0 / p->dx;
+    p->gas_density = p->pressure / (KB * p->t_gas);
+    p->inv_dx_f = (float)p->inv_dx;
+    p->factor_e = (float)(p->dt_e / ME * (-E));
+    p->factor_i = (float)(p->dt_i / MAR * E);
+    p->weight_factor = (float)(p->weight / (p->area * p->dx));
+    p->dt_e_f = (float)p->dt_e;
+    p->dt_i_f = (float)p->dt_i;
+    p->gap_f = (float)p->gap;
+    p->normal_range = (float)std::sqrt(KB * p->t_gas / MAR);
+    p->f1 = (float)(ME / (ME + MAR));
+    p->f2 = (float)(MAR / (ME + MAR));
+    p->half_me_over_e = (float)(0.5 * ME / E);
+    p->half_mu_over_e = (float)(0.5 * (MAR / 2.0) / E);
+    p->half_mi_over_e = (float)(0.5 * MAR / E);
+    p->e_charge_f = (float)E;
+    p->alpha_f = (float)(-p->dx * p->dx / EPS0);
+    p->half_dx_over_eps_f = (float)(p->dx / (2.0 * EPS0));
+    p->half_inv_dx_f = (float)(0.5 * p->inv_dx);
+    p->min_x_f = (float)(0.45 * p->gap);
+    p->max_x_f = (float)(0.55 * p->gap);
+    p->de_eepf_f = (float)0.05;
+    p->de_cs_f = (float)0.001;
+    p->me_f = (float)ME;
+    p->exc_th_f = (float)11.5;
+    p->ion_th_f = (float)15.8;
+    p->eepf_kick_f = (float)(p->dt_e * E / ME);
+    bool any = false;
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) any |= p->wave_cos[h] != 0.0 || p->wave_sin[h] != 0.0;
+    if (!any) p->wave_cos[0] = p->v0;
+    return PIC_OK;
+}
+
+int pic_get_density(const float *x, const uint32_t *n_active, void *density, int density_f64,
+                    const pic_params_t *p, pic_stream_t stream) {
+    if (!x || !n_active || !density || !p) {
+        set_error("pic_get_density: null argument");
+        return PIC_ERR_ARG;
+    }
+    return launch_deposit(x, n_active, density, density_f64 != 0, make_kphys(*p), Shape{}, S(stream));
+}
+
+static PoissonJob poisson_job_from(const pic_params_t &p, int precision) {
+    PoissonJob j;
+    j.precision = precision;
+    j.n_g = p.n_g;
+    j.n_t = p.n_t;
+    j.n_sub = p.n_sub;
+    const double E = 1.60217662e-19, EPS0 = 8.85418781e-12;
+    j.e_charge = E;
+    j.alpha = -p.dx * p.dx / EPS0;
+    j.inv_dx = p.inv_dx;
+    j.half_inv_dx = 0.5 * p.inv_dx;
+    j.half_dx_over_eps = p.dx / (2.0 * EPS0);
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) {
+        j.wave_cos[h] = p.wave_cos[h];
+        j.wave_sin[h] = p.wave_sin[h];
+    }
+    return j;
+}
+
+int pic_solve_poisson(const void *e_density, const void *i_density, double pot0, int precision,
+                      float *pot, float *efield, const pic_params_t *p, pic_stream_t stream) {
+    if (!e_density || !i_density || !pot || !efield || !p) {
+        set_error("pic_solve_poisson: null argument");
+        return PIC_ERR_ARG;
+    }
+    PoissonJob j = poisson_job_from(*p, precision);
+    j.e_dens = e_density;
+    j.i_dens = i_density;
+    j.pot0 = precision == PIC_PREC_FP64 ? pot0 : (double)(float)pot0;
+    j.pot = pot;
+    j.efield = efield;
+    return launch_poisson(j, S(stream));
+}
+
+int pic_solve_poisson_flexible(const float *source, int n, float psi_left, float psi_right, float dx,
+                               int precision, float *pot, pic_stream_t stream) {
+    if (!source || !pot || n < 3 || n > kPoissonThreads * kPoissonMaxPer) {
+        set_error("pic_solve_poisson_flexible: bad argument");
+        return PIC_ERR_ARG;
+    }
+    const int per = (n + kPoissonThreads - 1) / kPoissonThreads;
+    hipStream_t s = S(stream);
+    switch (precision) {
+    case PIC_PREC_FP32:
+        k_poisson_flexible<float><<<1, kPoissonThreads, 0, s>>>(source, n, per, psi_left, psi_right, dx, pot);
+        break;
+    case PIC_PREC_COMP:
+        k_poisson_flexible<ff><<<1, kPoissonThreads, 0, s>>>(source, n, per, psi_left, psi_right, dx, pot);
+        break;
+    case PIC_PREC_FP64:
+        k_poisson_flexible<double><<<1, kPoissonThreads, 0, s>>>(source, n, per, psi_left, psi_right, dx, pot);
+        break;
+    default:
+        set_error("pic_solve_poisson_flexible: unknown precision");
+        return PIC_ERR_ARG;
+    }
+    PIC_LAUNCHED("k_poisson_flexible");
+    return PIC_OK;
+}
+
+int pic_move_and_compact(const float *efield, const float *src_x, const float *src_vx,
+                         const float *src_vy, const float *src_vz, float *dst_x, float *dst_vx,
+                         float *dst_vy, float *dst_vz, uint32_t *alive_counter,
+                         const uint32_t *n_active, float factor, float dt, float half_m_over_e,
+                         unsigned long long *abs_counts, uint32_t *ifed, const pic_params_t *p,
+                         pic_stream_t stream) {
+    if (!efield || !src_x || !src_vx || !src_vy || !src_vz || !dst_x || !dst_vx || !dst_vy ||
+        !dst_vz || !alive_counter || !n_active || !p) {
+        set_error("pic_move_and_compact: null argument");
+        return PIC_ERR_ARG;
+    }
+    MoveJob j{efield, src_x, src_vx, src_vy, src_vz, dst_x, dst_vx, dst_vy, dst_vz,
+              alive_counter, n_active, factor, dt, half_m_over_e, abs_counts, ifed, 0xffffffffu};
+    return launch_move_compact(j, make_kphys(*p), Shape{}, S(stream));
+}
+
+static bool species_ok(const pic_species_t *s, bool need_rng) {
+    return s && s->x && s->vx && s->vy && s->vz &&
+           (!need_rng || (s->rng0 && s->rng1 && s->rng2 && s->rng3));
+}
+
+int pic_check_collisions_e(const float *stot_e, const float *cs, const uint32_t *n_active,
+                           const pic_species_t *electrons, const pic_species_t *ions,
+                           uint32_t *alive_e, uint32_t *alive_i, uint32_t *overflow,
+                           unsigned long long *n_coll, const pic_params_t *p, pic_stream_t stream) {
+    if (!stot_e || !cs || !n_active || !species_ok(electrons, true) || !species_ok(ions, false) ||
+        !alive_e || !alive_i || !overflow || !p) {
+        set_error("pic_check_collisions_e: null argument");
+        return PIC_ERR_ARG;
+    }
+    CollideEJob j{stot_e, cs, n_active, *electrons, *ions, alive_e, alive_i, overflow, n_coll};
+    return launch_collide_e(j, make_kphys(*p), Shape{}, S(stream));
+}
+
+int pic_check_collisions_i(const float *stot_i, const float *cs, const uint32_t *n_active,
+                           const pic_species_t *ions, unsigned long long *n_coll,
+                           const pic_params_t *p, pic_stream_t stream) {
+    if (!stot_i || !cs || !n_active || !species_ok(ions, true) || !p) {
+        set_error("pic_check_collisions_i: null argument");
+        return PIC_ERR_ARG;
+    }
+    CollideIJob j{stot_i, cs, n_active, *ions, n_coll};
+    return launch_collide_i(j, make_kphys(*p), Shape{}, S(stream));
+}
+
+int pic_accumulate_density(const void *density, int density_f64, double *cumul, int n_g,
+                           pic_stream_t stream) {
+    if (!density || !cumul || n_g <= 0) {
+        set_error("pic_accumulate_density: bad argument");
+        return PIC_ERR_ARG;
+    }
+    return launch_accum_density(density, density_f64 != 0, cumul, n_g, S(stream));
+}
+
+int pic_accumulate_eepf(const float *x, const float *vx, const float *vy, const float *vz,
+                        const float *efield, const uint32_t *n_active, uint32_t *eepf,
+                        const pic_params_t *p, pic_stream_t stream) {
+    if (!x || !vx || !vy || !vz || !efield || !n_active || !eepf || !p) {
+        set_error("pic_accumulate_eepf: null argument");
+        return PIC_ERR_ARG;
+    }
+    return launch_eepf(x, vx, vy, vz, efield, n_active, eepf, make_kphys(*p), Shape{}, S(stream));
+}
+
+int pic_xoshiro_next_n(const uint32_t state[4], uint32_t n, uint32_t *out, uint32_t *state_out,
+                       pic_stream_t stream) {
+    if (!state || (n && !out)) {
+        set_error("pic_xoshiro_next_n: null argument");
+        return PIC_ERR_ARG;
+    }
+    k_xo_next_n<<<1, 64, 0, S(stream)>>>(state[0], state[1], state[2], state[3], n, out, state_out);
+    PIC_LAUNCHED("k_xo_next_n");
+    return PIC_OK;
+}
+
+int pic_seed_streams(const uint32_t master[4], uint32_t n, uint32_t *rng0, uint32_t *rng1,
+                     uint32_t *rng2, uint32_t *rng3, pic_stream_t stream) {
+    if (!master || !rng0 || !rng1 || !rng2 || !rng3) {
+        set_error("pic_seed_streams: null argument");
+        return PIC_ERR_ARG;
+    }
+    if (!(master[0] | master[1] | master[2] | master[3])) {
+        set_error("pic_seed_streams: the master state must not be all zero");
+        return PIC_ERR_ARG;
+    }
+    if (n == 0) return PIC_OK;
+    // The 64 KiB of jump matrices live in one device buffer per device,
+    // uploaded on first use and kept for the life of the process.
+    static std::mutex mu;
+    static std::vector<uint32_t *> per_dev;
+    int dev = 0;
+    PIC_HIP(hipGetDevice(&dev));
+    uint32_t *mats = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+        if (!per_dev[dev]) {
+            const std::vector<uint32_t> &h = jump_powers();
+            uint32_t *d = nullptr;
+            PIC_HIP(hipMalloc(&d, h.size() * sizeof(uint32_t)));
+            PIC_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            per_dev[dev] = d;
+        }
+        mats = per_dev[dev];
+    }
+    const int grid = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)device_cus() * 8);
+    k_seed_streams<<<grid, 256, 0, S(stream)>>>(mats, master[0], master[1], master[2], master[3], n,
+                                                rng0, rng1, rng2, rng3);
+    PIC_LAUNCHED("k_seed_streams");
+    return PIC_OK;
+}
+
+int pic_malloc(void **ptr, uint64_t bytes) {
+    if (!ptr) return PIC_ERR_ARG;
+    PIC_HIP(hipMalloc(ptr, bytes ? bytes : 4));
+    return PIC_OK;
+}
+int pic_free(void *ptr) {
+    if (ptr) PIC_HIP(hipFree(ptr));
+    return PIC_OK;
+}
+int pic_memcpy_htod(void *dst, const void *src, uint64_t bytes) {
+    if (bytes) PIC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PIC_OK;
+}
+int pic_memcpy_dtoh(void *dst, const void *src, uint64_t bytes) {
+    if (bytes) PIC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+int pic_memset(void *dst, int value, uint64_t bytes) {
+    if (bytes) PIC_HIP(hipMemset(dst, value, bytes));
+    return PIC_OK;
+}
+int pic_device_synchronize(void) {
+    PIC_HIP(hipDeviceSynchronize());
+    return PIC_OK;
+}
+int pic_device_info(int *compute_units, int *wave_size, char *arch, int arch_len) {
+    int dev = 0;
+    PIC_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    PIC_HIP(hipGetDeviceProperties(&prop, dev));
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (wave_size) *wave_size = prop.warpSize;
+    if (arch && arch_len > 0) {
+        strncpy(arch, prop.gcnArchName, arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,959 @@
+// pic_sim.hip -- the simulation object: the device state the reference keeps
+// in GpuSimState (Rust-CUDA/src/main.rs:132-350) and the RF step loop of
+// main.rs:1723-1828, on one HIP stream, with no host synchronisation inside
+// a step.
+//
+// Two step paths share the state layout:
+//  * PIC_PATH_REFERENCE runs the reference's stage order and counter
+//    protocol with one kernel per stage (deposit e, [deposit i], Poisson,
+//    push+compact e, [push+compact i], collisions e, [collisions i]).
+//  * PIC_PATH_FUSED runs Poisson (which also clears the next step's
+//    accumulators and counters and advances the device clock) and one fused
+//    particle kernel per species.  Its launch sequence repeats every
+//    2*N_SUB steps with identical arguments, so it is captured once into a
+//    hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pic_internal.h"
+
+using namespace pic;
+
+namespace {
+// device counter block (uint32 words)
+enum Cnt {
+    C_E0 = 0, C_E1 = 1,     // electrons in main array [parity]
+    C_S0 = 2, C_S1 = 3,     // electrons in spawn staging [parity] (fused)
+    C_I0 = 4, C_I1 = 5,     // ions [parity]
+    C_ALIVE = 6,            // reference path scratch counter
+    C_OVERFLOW = 7,
+    C_TIME = 8,             // step index within the cycle (fused path clock)
+    C_WORDS = 16
+};
+constexpr int kMeasCounters = 8;
+constexpr uint32_t kCkptMagic = 0x43504943u;  // "CIPC"
+constexpr uint32_t kCkptVersion = 1;
+}  // namespace
+
+struct pic_sim {
+    pic_sim_config_t cfg;
+    pic_params_t p;
+    KPhys k;
+    Shape sh;
+    hipStream_t stream = nullptr;
+    int prec = PIC_PREC_FP32;
+    bool f64 = false;
+    uint32_t cap = 0, spawn_cap = 0;
+    // particles: [parity][x, vx, vy, vz]
+    float *e[2][4] = {}, *es[2][4] = {}, *ion[2][4] = {};
+    uint32_t *e_rng[4] = {}, *i_rng[4] = {};
+    uint32_t *cnt = nullptr;
+    // grid
+    float *pot = nullptr, *efield = nullptr;
+    void *e_dens[2] = {}, *i_dens[2] = {};  // replicas * stride entries of 8 bytes
+    int dens_words = 0;                      // entries per buffer (replicas * stride)
+    // measurement
+    double *cumul_e = nullptr, *cumul_i = nullptr;
+    uint32_t *eepf = nullptr, *ifed = nullptr;
+    unsigned long long *counters = nullptr;
+    // tables
+    float *cs = nullptr, *stot_e = nullptr, *stot_i = nullptr;
+    // bookkeeping
+    int ce = 0, ci = 0;      // current particle parity
+    int cd = 0, hi = 0;      // fused: current e density, held ion density
+    int last_e = 0, last_i = 0;  // densities used by the last Poisson solve
+    uint32_t t = 0;          // next step's index in the cycle (host mirror)
+    unsigned long long steps = 0;
+    bool need_prologue = true;
+    unsigned long long ref_meas[2] = {0, 0};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_roles[2][4] = {};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    PIC_HIP(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+    PIC_HIP(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return PIC_OK;
+}
+
+#define TRY(x)                     \
+    do {                           \
+        int rc_ = (x);             \
+        if (rc_ != PIC_OK) return rc_; \
+    } while (0)
+
+pic_species_t species(pic_sim *s, int which, int parity) {
+    pic_species_t sp;
+    float **a = which == PIC_ELECTRONS ? s->e[parity] : s->ion[parity];
+    uint32_t **r = which == PIC_ELECTRONS ? s->e_rng : s->i_rng;
+    sp.x = a[0];
+    sp.vx = a[1];
+    sp.vy = a[2];
+    sp.vz = a[3];
+    sp.rng0 = r[0];
+    sp.rng1 = r[1];
+    sp.rng2 = r[2];
+    sp.rng3 = r[3];
+    sp.capacity = s->cap;
+    return sp;
+}
+
+PoissonJob base_poisson(pic_sim *s) {
+    PoissonJob j;
+    const pic_params_t &p = s->p;
+    const double E = 1.60217662e-19, EPS0 = 8.85418781e-12;
+    j.precision = s->prec;
+    j.n_g = p.n_g;
+    j.n_t = p.n_t;
+    j.n_sub = p.n_sub;
+    j.e_charge = E;
+    j.alpha = -p.dx * p.dx / EPS0;
+    j.inv_dx = p.inv_dx;
+    j.half_inv_dx = 0.5 * p.inv_dx;
+    j.half_dx_over_eps = p.dx / (2.0 * EPS0);
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) {
+        j.wave_cos[h] = p.wave_cos[h];
+        j.wave_sin[h] = p.wave_sin[h];
+    }
+    j.pot = s->pot;
+    j.efield = s->efield;
+    j.replicas = s->sh.replicas;
+    j.rep_stride = s->sh.rep_stride;
+    return j;
+}
+
+// pot0 of step t as the host computes it in main.rs:1741, for any waveform.
+double host_pot0(const pic_params_t &p, uint32_t t) {
+    double v = 0.0;
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) {
+        if (p.wave_cos[h] == 0.0 && p.wave_sin[h] == 0.0) continue;
+        const double ph = ((double)t / (double)p.n_t) * 2.0 * 3.141592653589793 * (double)(h + 1);
+        v += p.wave_cos[h] * std::cos(ph) + p.wave_sin[h] * std::sin(ph);
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- reference path
+int step_reference(pic_sim *s, bool measure) {
+    hipStream_t st = s->stream;
+    const bool ion = (s->t % (uint32_t)s->p.n_sub) == 0;
+    uint32_t *ne = s->cnt + C_E0 + s->ce;
+    uint32_t *ni = s->cnt + C_I0 + s->ci;
+    uint32_t *alive = s->cnt + C_ALIVE;
+    Shape one = s->sh;
+    one.replicas = 1;
+    // main.rs:1729-1739
+    TRY(launch_fill(s->e_dens[0], 0u, (uint64_t)s->p.n_g * (s->f64 ? 2 : 1), st));
+    TRY(launch_deposit(s->e[s->ce][0], ne, s->e_dens[0], s->f64, s->k, one, st, s->cap));
+    if (ion) {
+        TRY(launch_fill(s->i_dens[0], 0u, (uint64_t)s->p.n_g * (s->f64 ? 2 : 1), st));
+        TRY(launch_deposit(s->ion[s->ci][0], ni, s->i_dens[0], s->f64, s->k, one, st, s->cap));
+    }
+    // main.rs:1741-1744
+    PoissonJob pj = base_poisson(s);
+    pj.replicas = 1;
+    pj.rep_stride = s->p.n_g;
+    pj.e_dens = s->e_dens[0];
+    pj.i_dens = s->i_dens[0];
+    const double p0 = host_pot0(s->p, s->t);
+    pj.pot0 = s->prec == PIC_PREC_FP64 ? p0 : (double)(float)p0;
+    TRY(launch_poisson(pj, st));
+    s->last_e = 0;
+    s->last_i = 0;
+    // main.rs:1746-1756
+    if (measure) {
+        TRY(launch_accum_density(s->e_dens[0], s->f64, s->cumul_e, s->p.n_g, st));
+        s->ref_meas[0]++;
+        if (ion) {
+            TRY(launch_accum_density(s->i_dens[0], s->f64, s->cumul_i, s->p.n_g, st));
+            s->ref_meas[1]++;
+        }
+    }
+    // main.rs:1758-1768: electrons
+    TRY(launch_copy_u32(alive, nullptr, st));
+    MoveJob mj{s->efield, s->e[s->ce][0], s->e[s->ce][1], s->e[s->ce][2], s->e[s->ce][3],
+               s->e[1 - s->ce][0], s->e[1 - s->ce][1], s->e[1 - s->ce][2], s->e[1 - s->ce][3],
+               alive, ne, s->k.factor_e, s->k.dt_e, 0.0f, s->counters + 0, nullptr, s->cap};
+    TRY(launch_move_compact(mj, s->k, s->sh, st));
+    s->ce ^= 1;
+    ne = s->cnt + C_E0 + s->ce;
+    TRY(launch_copy_u32(ne, alive, st));
+    // main.rs:1770-1783: ions
+    if (ion) {
+        TRY(launch_copy_u32(alive, nullptr, st));
+        MoveJob mi{s->efield, s->ion[s->ci][0], s->ion[s->ci][1], s->ion[s->ci][2], s->ion[s->ci][3],
+                   s->ion[1 - s->ci][0], s->ion[1 - s->ci][1], s->ion[1 - s->ci][2],
+                   s->ion[1 - s->ci][3], alive, ni, s->k.factor_i, s->k.dt_i, s->k.half_mi_over_e,
+                   s->counters + 2, s->ifed, s->cap};
+        TRY(launch_move_compact(mi, s->k, s->sh, st));
+        s->ci ^= 1;
+        ni = s->cnt + C_I0 + s->ci;
+        TRY(launch_copy_u32(ni, alive, st, s->cap));
+        TRY(launch_copy_u32(alive, ne, st));
+    }
+    // main.rs:1785-1792
+    CollideEJob cj{s->stot_e, s->cs, ne, species(s, PIC_ELECTRONS, s->ce), species(s, PIC_IONS, s->ci),
+                   alive, ni, s->cnt + C_OVERFLOW, s->counters + 4};
+    TRY(launch_collide_e(cj, s->k, s->sh, st));
+    // appends past the capacity were dropped (and flagged): clamp the counts
+    TRY(launch_copy_u32(ne, alive, st, s->cap));
+    TRY(launch_copy_u32(ni, ni, st, s->cap));
+    // main.rs:1794-1800
+    if (ion) {
+        CollideIJob ij{s->stot_i, s->cs, ni, species(s, PIC_IONS, s->ci), s->counters + 5};
+        TRY(launch_collide_i(ij, s->k, s->sh, st));
+    }
+    // main.rs:1802-1807
+    if (measure) {
+        float **a = s->e[s->ce];
+        TRY(launch_eepf(a[0], a[1], a[2], a[3], s->efield, ne, s->eepf, s->k, s->sh, st, s->cap));
+    }
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------- fused path
+uint64_t dens_words32(pic_sim *s) { return (uint64_t)s->dens_words * (s->f64 ? 2 : 1); }
+
+// Deposit the current particles into the accumulators the next Poisson
+// solve reads (after an upload, a checkpoint load or a path start).
+int fused_prologue(pic_sim *s) {
+    hipStream_t st = s->stream;
+    Shape sh = s->sh;
+    TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
+    TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
+    TRY(launch_fill(s->i_dens[1], 0u, dens_words32(s), st));
+    uint32_t *ne = s->cnt + C_E0 + s->ce;
+    uint32_t *ns = s->cnt + C_S0 + s->ce;
+    uint32_t *ni = s->cnt + C_I0 + s->ci;
+    TRY(launch_deposit(s->e[s->ce][0], ne, s->e_dens[s->cd], s->f64, s->k, sh, st, s->cap));
+    TRY(launch_deposit(s->es[s->ce][0], ns, s->e_dens[s->cd], s->f64, s->k, sh, st, s->spawn_cap));
+    TRY(launch_deposit(s->ion[s->ci][0], ni, s->i_dens[0], s->f64, s->k, sh, st, s->cap));
+    TRY(launch_deposit(s->ion[s->ci][0], ni, s->i_dens[1], s->f64, s->k, sh, st, s->cap));
+    // the device clock follows the host's
+    PIC_HIP(hipMemcpyAsync(s->cnt + C_TIME, &s->t, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    PIC_HIP(hipStreamSynchronize(st));
+    s->need_prologue = false;
+    return PIC_OK;
+}
+
+int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEvent_t e1) {
+    hipStream_t st = s->stream;
+    const bool ion = (s->t % (uint32_t)s->p.n_sub) == 0;
+    // Poisson of this step: e density accumulated by the previous step,
+    // ion density = the held one, or on ion steps the accumulated one
+    PoissonJob pj = base_poisson(s);
+    pj.e_dens = s->e_dens[s->cd];
+    const int i_read = ion ? 1 - s->hi : s->hi;
+    pj.i_dens = s->i_dens[i_read];
+    pj.time = s->cnt + C_TIME;
+    pj.advance_time = s->cnt + C_TIME;
+    pj.zero_a = s->e_dens[1 - s->cd];
+    pj.zero_a_words = (int)dens_words32(s);
+    if (ion) {
+        pj.zero_b = s->i_dens[s->hi];  // old held buffer becomes the accumulator
+        pj.zero_b_words = (int)dens_words32(s);
+    }
+    pj.zero_cnt[0] = s->cnt + C_E0 + (1 - s->ce);
+    pj.zero_cnt[1] = s->cnt + C_S0 + (1 - s->ce);
+    if (ion) pj.zero_cnt_ion = s->cnt + C_I0 + (1 - s->ci);
+    if (measure) {
+        pj.cumul_e = s->cumul_e;
+        pj.cumul_i = s->cumul_i;
+        pj.meas_steps = s->counters + 6;
+    }
+    TRY(launch_poisson(pj, st));
+    s->last_e = s->cd;
+    s->last_i = i_read;
+    const int i_acc = ion ? s->hi : 1 - s->hi;  // accumulator for the next ion step
+    if (ion) {
+        FusedJob fi;
+        memset(&fi, 0, sizeof(fi));
+        fi.electrons = false;
+        fi.efield = s->efield;
+        fi.sx = s->ion[s->ci][0];
+        fi.svx = s->ion[s->ci][1];
+        fi.svy = s->ion[s->ci][2];
+        fi.svz = s->ion[s->ci][3];
+        fi.n_main = s->cnt + C_I0 + s->ci;
+        fi.dx = s->ion[1 - s->ci][0];
+        fi.dvx = s->ion[1 - s->ci][1];
+        fi.dvy = s->ion[1 - s->ci][2];
+        fi.dvz = s->ion[1 - s->ci][3];
+        fi.n_out = s->cnt + C_I0 + (1 - s->ci);
+        fi.capacity = s->cap;
+        fi.rng0 = s->i_rng[0];
+        fi.rng1 = s->i_rng[1];
+        fi.rng2 = s->i_rng[2];
+        fi.rng3 = s->i_rng[3];
+        fi.dens = s->i_dens[i_acc];
+        fi.dens_f64 = s->f64;
+        fi.stot = s->stot_i;
+        fi.cs = s->cs;
+        fi.overflow = s->cnt + C_OVERFLOW;
+        fi.counters = s->counters;
+        fi.ifed = s->ifed;
+        TRY(launch_fused(fi, s->k, s->sh, st));
+        s->ci ^= 1;
+    }
+    FusedJob fe;
+    memset(&fe, 0, sizeof(fe));
+    fe.electrons = true;
+    fe.efield = s->efield;
+    fe.sx = s->e[s->ce][0];
+    fe.svx = s->e[s->ce][1];
+    fe.svy = s->e[s->ce][2];
+    fe.svz = s->e[s->ce][3];
+    fe.n_main = s->cnt + C_E0 + s->ce;
+    fe.px = s->es[s->ce][0];
+    fe.pvx = s->es[s->ce][1];
+    fe.pvy = s->es[s->ce][2];
+    fe.pvz = s->es[s->ce][3];
+    fe.n_spawn = s->cnt + C_S0 + s->ce;
+    fe.dx = s->e[1 - s->ce][0];
+    fe.dvx = s->e[1 - s->ce][1];
+    fe.dvy = s->e[1 - s->ce][2];
+    fe.dvz = s->e[1 - s->ce][3];
+    fe.n_out = s->cnt + C_E0 + (1 - s->ce);
+    fe.capacity = s->cap;
+    fe.rng0 = s->e_rng[0];
+    fe.rng1 = s->e_rng[1];
+    fe.rng2 = s->e_rng[2];
+    fe.rng3 = s->e_rng[3];
+    fe.ox = s->es[1 - s->ce][0];
+    fe.ovx = s->es[1 - s->ce][1];
+    fe.ovy = s->es[1 - s->ce][2];
+    fe.ovz = s->es[1 - s->ce][3];
+    fe.n_spawn_out = s->cnt + C_S0 + (1 - s->ce);
+    fe.spawn_capacity = s->spawn_cap;
+    fe.ix = s->ion[s->ci][0];
+    fe.ivx = s->ion[s->ci][1];
+    fe.ivy = s->ion[s->ci][2];
+    fe.ivz = s->ion[s->ci][3];
+    fe.n_ions = s->cnt + C_I0 + s->ci;
+    fe.ion_capacity = s->cap;
+    fe.ion_dens = s->i_dens[i_acc];
+    fe.dens = s->e_dens[1 - s->cd];
+    fe.dens_f64 = s->f64;
+    fe.stot = s->stot_e;
+    fe.cs = s->cs;
+    fe.overflow = s->cnt + C_OVERFLOW;
+    fe.counters = s->counters;
+    fe.eepf = measure ? s->eepf : nullptr;
+    if (time_kernel) PIC_HIP(hipEventRecord(e0, st));
+    TRY(launch_fused(fe, s->k, s->sh, st));
+    if (time_kernel) PIC_HIP(hipEventRecord(e1, st));
+    s->ce ^= 1;
+    s->cd ^= 1;
+    if (ion) s->hi ^= 1;
+    return PIC_OK;
+}
+
+void invalidate_graphs(pic_sim *s) {
+    for (int m = 0; m < 2; ++m)
+        if (s->graph[m]) {
+            (void)hipGraphExecDestroy(s->graph[m]);
+            s->graph[m] = nullptr;
+        }
+}
+
+int one_step(pic_sim *s, bool measure) {
+    int rc = s->cfg.path == PIC_PATH_FUSED ? step_fused(s, measure, false, nullptr, nullptr)
+                                           : step_reference(s, measure);
+    if (rc != PIC_OK) return rc;
+    s->t = (s->t + 1) % (uint32_t)s->p.n_t;
+    s->steps++;
+    return PIC_OK;
+}
+
+// Capture 2*N_SUB fused steps (every role flag returns to its start value
+// after that many steps) into a graph for this measure flag.
+int capture_graph(pic_sim *s, bool measure) {
+    const int m = measure ? 1 : 0;
+    const int period = 2 * s->p.n_sub;
+    int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+    uint32_t t0 = s->t;
+    unsigned long long steps0 = s->steps;
+    hipGraph_t g = nullptr;
+    PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PIC_OK;
+    for (int i = 0; i < period && rc == PIC_OK; ++i) rc = one_step(s, measure);
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    // capture does not execute: restore the host mirror
+    s->t = t0;
+    s->steps = steps0;
+    s->ce = roles[0];
+    s->ci = roles[1];
+    s->cd = roles[2];
+    s->hi = roles[3];
+    if (rc != PIC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    s->graph[m] = ex;
+    memcpy(s->graph_roles[m], roles, sizeof(roles));
+    return PIC_OK;
+}
+
+int run_steps(pic_sim *s, uint32_t n, bool measure) {
+    if (s->cfg.path == PIC_PATH_FUSED && s->need_prologue) TRY(fused_prologue(s));
+    const uint32_t period = 2u * (uint32_t)s->p.n_sub;
+    const bool graphs = s->cfg.use_graph && s->cfg.path == PIC_PATH_FUSED &&
+                        (uint32_t)s->p.n_t % period == 0;
+    while (n > 0) {
+        if (graphs && s->t % period == 0 && n >= period) {
+            const int m = measure ? 1 : 0;
+            int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+            if (s->graph[m] && memcmp(roles, s->graph_roles[m], sizeof(roles)) != 0) {
+                (void)hipGraphExecDestroy(s->graph[m]);
+                s->graph[m] = nullptr;
+            }
+            if (!s->graph[m]) TRY(capture_graph(s, measure));
+            while (n >= period) {
+                PIC_HIP(hipGraphLaunch(s->graph[m], s->stream));
+                s->t = (s->t + period) % (uint32_t)s->p.n_t;
+                s->steps += period;
+                n -= period;
+            }
+            continue;
+        }
+        TRY(one_step(s, measure));
+        --n;
+    }
+    return PIC_OK;
+}
+
+int free_all(pic_sim *s) {
+    invalidate_graphs(s);
+    for (int q = 0; q < 2; ++q)
+        for (int c = 0; c < 4; ++c) {
+            (void)hipFree(s->e[q][c]);
+            (void)hipFree(s->es[q][c]);
+            (void)hipFree(s->ion[q][c]);
+        }
+    for (int c = 0; c < 4; ++c) {
+        (void)hipFree(s->e_rng[c]);
+        (void)hipFree(s->i_rng[c]);
+    }
+    void *bufs[] = {s->cnt, s->pot, s->efield, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
+                    s->cumul_e, s->cumul_i, s->eepf, s->ifed, s->counters, s->cs, s->stot_e, s->stot_i};
+    for (void *b : bufs) (void)hipFree(b);
+    for (int i = 0; i < 2; ++i)
+        if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pic_sim_config_default(pic_sim_config_t *c) {
+    memset(c, 0, sizeof(*c));
+    pic_params_default(&c->params);
+    pic_params_derive(&c->params);
+    c->capacity = 120000;  // main.rs:79
+    const uint32_t se[4] = {0x12345678u, 0x11112222u, 0x22223333u, 0x33334444u};  // main.rs:1667
+    const uint32_t si[4] = {0x44445555u, 0x55556666u, 0x66667777u, 0x77778888u};  // main.rs:1671
+    memcpy(c->seed_e, se, sizeof(se));
+    memcpy(c->seed_i, si, sizeof(si));
+    c->path = PIC_PATH_FUSED;
+    c->use_graph = 1;
+    c->measure = 1;
+}
+
+int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
+    if (!c || !out || c->capacity == 0) {
+        set_error("pic_sim_create: bad argument");
+        return PIC_ERR_ARG;
+    }
+    if (c->params.dt_e <= 0.0 || c->params.n_g < 3 || c->params.n_g > 4096) {
+        set_error("pic_sim_create: params not derived (call pic_params_derive) or n_g out of [3, 4096]");
+        return PIC_ERR_ARG;
+    }
+    pic_sim *s = new pic_sim();
+    s->cfg = *c;
+    s->p = c->params;
+    s->k = make_kphys(s->p);
+    s->prec = s->p.precision;
+    if (s->prec < PIC_PREC_FP32 || s->prec > PIC_PREC_FP64) {
+        delete s;
+        set_error("pic_sim_create: unknown precision");
+        return PIC_ERR_ARG;
+    }
+    s->f64 = s->prec == PIC_PREC_FP64;
+    s->cap = c->capacity;
+    s->spawn_cap = c->spawn_capacity ? c->spawn_capacity : std::max<uint32_t>(4096u, c->capacity / 8u);
+    Shape sh;
+    sh.block = c->block_size > 0 ? c->block_size : 256;
+    // the fused kernels hold ~110 VGPRs (4 waves per SIMD): a persistent grid
+    // of 4 workgroups of 256 lanes per compute unit is fully resident
+    sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
+    sh.ppt = c->ppt > 0 ? c->ppt : 4;
+    sh.replicas = c->replicas > 0 ? c->replicas : (c->path == PIC_PATH_FUSED ? 16 : 1);
+    // replica r of a density buffer starts at element r * rep_stride (n_g
+    // rounded up to 32 elements), so the same node of different replicas
+    // never shares a cache line
+    sh.rep_stride = ((s->p.n_g + 31) / 32) * 32;
+    s->sh = resolve(sh);
+    s->dens_words = s->sh.replicas * s->sh.rep_stride;
+    int rc = PIC_OK;
+    auto fail = [&](int r) {
+        free_all(s);
+        delete s;
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "hipStreamCreate"));
+    for (int q = 0; q < 2; ++q)
+        for (int k = 0; k < 4; ++k) {
+            if ((rc = dalloc(&s->e[q][k], s->cap))) return fail(rc);
+            if ((rc = dalloc(&s->ion[q][k], s->cap))) return fail(rc);
+            if ((rc = dalloc(&s->es[q][k], c->path == PIC_PATH_FUSED ? s->spawn_cap : 1))) return fail(rc);
+        }
+    for (int k = 0; k < 4; ++k) {
+        if ((rc = dalloc(&s->e_rng[k], s->cap))) return fail(rc);
+        if ((rc = dalloc(&s->i_rng[k], s->cap))) return fail(rc);
+    }
+    if ((rc = dalloc(&s->cnt, C_WORDS))) return fail(rc);
+    if ((rc = dalloc(&s->pot, s->p.n_g))) return fail(rc);
+    if ((rc = dalloc(&s->efield, s->p.n_g))) return fail(rc);
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = dalloc((double **)&s->e_dens[q], s->dens_words))) return fail(rc);
+        if ((rc = dalloc((double **)&s->i_dens[q], s->dens_words))) return fail(rc);
+    }
+    if ((rc = dalloc(&s->cumul_e, s->p.n_g))) return fail(rc);
+    if ((rc = dalloc(&s->cumul_i, s->p.n_g))) return fail(rc);
+    if ((rc = dalloc(&s->eepf, PIC_N_EEPF))) return fail(rc);
+    if ((rc = dalloc(&s->ifed, 2 * PIC_N_IFED))) return fail(rc);
+    if ((rc = dalloc(&s->counters, kMeasCounters))) return fail(rc);
+    if ((rc = dalloc(&s->cs, (size_t)PIC_N_CS * PIC_CS_RANGES))) return fail(rc);
+    if ((rc = dalloc(&s->stot_e, PIC_CS_RANGES))) return fail(rc);
+    if ((rc = dalloc(&s->stot_i, PIC_CS_RANGES))) return fail(rc);
+    for (int i = 0; i < 2; ++i)
+        if (hipEventCreate(&s->ev[i]) != hipSuccess) return fail(hip_fail(hipGetLastError(), "hipEventCreate"));
+    // tables (main.rs:1649, 1663) and RNG streams (main.rs:1666-1675)
+    {
+        std::vector<float> cs((size_t)PIC_N_CS * PIC_CS_RANGES), se(PIC_CS_RANGES), si(PIC_CS_RANGES);
+        if ((rc = pic_cross_sections(s->p.gas_density, cs.data(), se.data(), si.data()))) return fail(rc);
+        if ((rc = pic_sim_set_tables(s, cs.data(), se.data(), si.data()))) return fail(rc);
+    }
+    if ((rc = pic_seed_streams(c->seed_e, s->cap, s->e_rng[0], s->e_rng[1], s->e_rng[2], s->e_rng[3], s->stream)))
+        return fail(rc);
+    if ((rc = pic_seed_streams(c->seed_i, s->cap, s->i_rng[0], s->i_rng[1], s->i_rng[2], s->i_rng[3], s->stream)))
+        return fail(rc);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "seed"));
+    *out = s;
+    return PIC_OK;
+}
+
+int pic_sim_destroy(pic_sim_t *s) {
+    if (!s) return PIC_OK;
+    (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return PIC_OK;
+}
+
+int pic_sim_set_tables(pic_sim_t *s, const float *cs, const float *stot_e, const float *stot_i) {
+    if (!s || !cs || !stot_e || !stot_i) return PIC_ERR_ARG;
+    PIC_HIP(hipMemcpy(s->cs, cs, sizeof(float) * PIC_N_CS * PIC_CS_RANGES, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(s->stot_e, stot_e, sizeof(float) * PIC_CS_RANGES, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(s->stot_i, stot_i, sizeof(float) * PIC_CS_RANGES, hipMemcpyHostToDevice));
+    return PIC_OK;
+}
+
+int pic_sim_upload(pic_sim_t *s, int species_id, const float *x, const float *vx, const float *vy,
+                   const float *vz, uint32_t n) {
+    if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS) || n > s->cap ||
+        (n && (!x || !vx || !vy || !vz))) {
+        set_error("pic_sim_upload: bad argument (n above capacity?)");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    float **a = species_id == PIC_ELECTRONS ? s->e[s->ce] : s->ion[s->ci];
+    const float *src[4] = {x, vx, vy, vz};
+    for (int c = 0; c < 4; ++c)
+        if (n) PIC_HIP(hipMemcpy(a[c], src[c], sizeof(float) * n, hipMemcpyHostToDevice));
+    uint32_t cnt[2] = {n, 0u};
+    if (species_id == PIC_ELECTRONS) {
+        PIC_HIP(hipMemcpy(s->cnt + C_E0 + s->ce, &cnt[0], 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(s->cnt + C_S0 + s->ce, &cnt[1], 4, hipMemcpyHostToDevice));
+    } else {
+        PIC_HIP(hipMemcpy(s->cnt + C_I0 + s->ci, &cnt[0], 4, hipMemcpyHostToDevice));
+    }
+    s->need_prologue = true;
+    invalidate_graphs(s);
+    return PIC_OK;
+}
+
+int pic_sim_download(pic_sim_t *s, int species_id, float *x, float *vx, float *vy, float *vz,
+                     uint32_t cap, uint32_t *n) {
+    if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS)) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t c[16];
+    PIC_HIP(hipMemcpy(c, s->cnt, sizeof(c), hipMemcpyDeviceToHost));
+    float *dst[4] = {x, vx, vy, vz};
+    if (species_id == PIC_ELECTRONS) {
+        uint32_t nm = std::min(c[C_E0 + s->ce], s->cap);
+        uint32_t ns = s->cfg.path == PIC_PATH_FUSED ? std::min(c[C_S0 + s->ce], s->spawn_cap) : 0u;
+        if (n) *n = nm + ns;
+        uint32_t cm = std::min(nm, cap), cs2 = std::min(ns, cap - cm);
+        for (int q = 0; q < 4; ++q) {
+            if (!dst[q]) continue;
+            if (cm) PIC_HIP(hipMemcpy(dst[q], s->e[s->ce][q], sizeof(float) * cm, hipMemcpyDeviceToHost));
+            if (cs2) PIC_HIP(hipMemcpy(dst[q] + cm, s->es[s->ce][q], sizeof(float) * cs2, hipMemcpyDeviceToHost));
+        }
+    } else {
+        uint32_t ni = std::min(c[C_I0 + s->ci], s->cap);
+        if (n) *n = ni;
+        uint32_t cm = std::min(ni, cap);
+        for (int q = 0; q < 4; ++q)
+            if (dst[q] && cm) PIC_HIP(hipMemcpy(dst[q], s->ion[s->ci][q], sizeof(float) * cm, hipMemcpyDeviceToHost));
+    }
+    return PIC_OK;
+}
+
+int pic_sim_rng_upload(pic_sim_t *s, int species_id, const uint32_t *w0, const uint32_t *w1,
+                       const uint32_t *w2, const uint32_t *w3, uint32_t n) {
+    if (!s || n > s->cap || !w0 || !w1 || !w2 || !w3) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t **r = species_id == PIC_ELECTRONS ? s->e_rng : s->i_rng;
+    const uint32_t *src[4] = {w0, w1, w2, w3};
+    for (int q = 0; q < 4; ++q) PIC_HIP(hipMemcpy(r[q], src[q], 4ull * n, hipMemcpyHostToDevice));
+    return PIC_OK;
+}
+
+int pic_sim_rng_download(pic_sim_t *s, int species_id, uint32_t *w0, uint32_t *w1, uint32_t *w2,
+                         uint32_t *w3, uint32_t n) {
+    if (!s || n > s->cap) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t **r = species_id == PIC_ELECTRONS ? s->e_rng : s->i_rng;
+    uint32_t *dst[4] = {w0, w1, w2, w3};
+    for (int q = 0; q < 4; ++q)
+        if (dst[q]) PIC_HIP(hipMemcpy(dst[q], r[q], 4ull * n, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+int pic_sim_counts(pic_sim_t *s, uint32_t *n_e, uint32_t *n_i) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t c[16];
+    PIC_HIP(hipMemcpy(c, s->cnt, sizeof(c), hipMemcpyDeviceToHost));
+    uint32_t ne = c[C_E0 + s->ce];
+    if (s->cfg.path == PIC_PATH_FUSED) ne += c[C_S0 + s->ce];
+    if (n_e) *n_e = ne;
+    if (n_i) *n_i = c[C_I0 + s->ci];
+    return PIC_OK;
+}
+
+int pic_sim_time(pic_sim_t *s, uint32_t *t_in_cycle, unsigned long long *steps_done) {
+    if (!s) return PIC_ERR_ARG;
+    if (t_in_cycle) *t_in_cycle = s->t;
+    if (steps_done) *steps_done = s->steps;
+    return PIC_OK;
+}
+
+int pic_sim_set_time(pic_sim_t *s, uint32_t t) {
+    if (!s || t >= (uint32_t)s->p.n_t) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    s->t = t;
+    s->need_prologue = true;
+    invalidate_graphs(s);
+    return PIC_OK;
+}
+
+int pic_sim_run(pic_sim_t *s, uint32_t n_steps, int measure) {
+    if (!s) return PIC_ERR_ARG;
+    return run_steps(s, n_steps, measure != 0);
+}
+
+int pic_sim_synchronize(pic_sim_t *s) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    return PIC_OK;
+}
+
+int pic_sim_check(pic_sim_t *s) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t ov = 0;
+    PIC_HIP(hipMemcpy(&ov, s->cnt + C_OVERFLOW, 4, hipMemcpyDeviceToHost));
+    if (ov) {
+        set_error("particle array capacity exceeded (particles were dropped)");
+        return PIC_ERR_CAPACITY;
+    }
+    return PIC_OK;
+}
+
+int pic_sim_read_grid(pic_sim_t *s, float *pot, float *efield, double *e_dens, double *i_dens) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    const int n_g = s->p.n_g;
+    if (pot) PIC_HIP(hipMemcpy(pot, s->pot, sizeof(float) * n_g, hipMemcpyDeviceToHost));
+    if (efield) PIC_HIP(hipMemcpy(efield, s->efield, sizeof(float) * n_g, hipMemcpyDeviceToHost));
+    void *src[2] = {s->cfg.path == PIC_PATH_FUSED ? s->e_dens[s->last_e] : s->e_dens[0],
+                    s->cfg.path == PIC_PATH_FUSED ? s->i_dens[s->last_i] : s->i_dens[0]};
+    double *dst[2] = {e_dens, i_dens};
+    const int reps = s->cfg.path == PIC_PATH_FUSED ? s->sh.replicas : 1;
+    const int stride = s->cfg.path == PIC_PATH_FUSED ? s->sh.rep_stride : n_g;
+    for (int q = 0; q < 2; ++q) {
+        if (!dst[q]) continue;
+        std::vector<double> h((size_t)reps * stride);
+        if (s->f64) {
+            PIC_HIP(hipMemcpy(h.data(), src[q], sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(h.size());
+            PIC_HIP(hipMemcpy(f.data(), src[q], sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < f.size(); ++i) h[i] = f[i];
+        }
+        for (int k = 0; k < n_g; ++k) {
+            double acc = 0.0;
+            for (int r = 0; r < reps; ++r) acc += h[(size_t)r * stride + k];
+            dst[q][k] = acc;
+        }
+    }
+    return PIC_OK;
+}
+
+int pic_sim_read_measurements(pic_sim_t *s, double *cumul_e, double *cumul_i, uint32_t *eepf,
+                              uint32_t *ifed, unsigned long long *counters) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    const int n_g = s->p.n_g;
+    if (cumul_e) PIC_HIP(hipMemcpy(cumul_e, s->cumul_e, sizeof(double) * n_g, hipMemcpyDeviceToHost));
+    if (cumul_i) PIC_HIP(hipMemcpy(cumul_i, s->cumul_i, sizeof(double) * n_g, hipMemcpyDeviceToHost));
+    if (eepf) PIC_HIP(hipMemcpy(eepf, s->eepf, sizeof(uint32_t) * PIC_N_EEPF, hipMemcpyDeviceToHost));
+    if (ifed) PIC_HIP(hipMemcpy(ifed, s->ifed, sizeof(uint32_t) * 2 * PIC_N_IFED, hipMemcpyDeviceToHost));
+    if (counters) {
+        PIC_HIP(hipMemcpy(counters, s->counters, sizeof(unsigned long long) * kMeasCounters,
+                          hipMemcpyDeviceToHost));
+        counters[6] += s->ref_meas[0];
+        counters[7] += s->ref_meas[1];
+    }
+    return PIC_OK;
+}
+
+int pic_sim_reset_measurements(pic_sim_t *s) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    PIC_HIP(hipMemset(s->cumul_e, 0, sizeof(double) * s->p.n_g));
+    PIC_HIP(hipMemset(s->cumul_i, 0, sizeof(double) * s->p.n_g));
+    PIC_HIP(hipMemset(s->eepf, 0, sizeof(uint32_t) * PIC_N_EEPF));
+    PIC_HIP(hipMemset(s->ifed, 0, sizeof(uint32_t) * 2 * PIC_N_IFED));
+    PIC_HIP(hipMemset(s->counters, 0, sizeof(unsigned long long) * kMeasCounters));
+    s->ref_meas[0] = s->ref_meas[1] = 0;
+    return PIC_OK;
+}
+
+int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_total, float *ms_kernel,
+                       uint32_t *kernel_launches, unsigned long long *particle_steps) {
+    if (!s) return PIC_ERR_ARG;
+    const bool fused = s->cfg.path == PIC_PATH_FUSED;
+    if (fused && s->need_prologue) TRY(fused_prologue(s));
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    // particle-steps: electrons every step, ions on ion steps (counted from
+    // the live counts before each step, on the device-side counters)
+    uint32_t ne0 = 0, ni0 = 0;
+    TRY(pic_sim_counts(s, &ne0, &ni0));
+    uint32_t ion_steps = 0;
+    for (uint32_t i = 0, tt = s->t; i < n_steps; ++i, tt = (tt + 1) % (uint32_t)s->p.n_t)
+        ion_steps += (tt % (uint32_t)s->p.n_sub) == 0;
+    float total = 0.0f, kern = 0.0f;
+    uint32_t launches = 0;
+    unsigned long long psteps = 0;
+    if (ms_kernel && fused) {
+        // eager steps, the electron particle kernel bracketed by events; the
+        // chunks are synchronised to read the events back
+        const uint32_t chunk = 64;
+        std::vector<hipEvent_t> ev(2 * chunk);
+        for (auto &e : ev) PIC_HIP(hipEventCreate(&e));
+        hipEvent_t w0, w1;
+        PIC_HIP(hipEventCreate(&w0));
+        PIC_HIP(hipEventCreate(&w1));
+        uint32_t done = 0;
+        while (done < n_steps) {
+            const uint32_t c = std::min(chunk, n_steps - done);
+            PIC_HIP(hipEventRecord(w0, s->stream));
+            for (uint32_t i = 0; i < c; ++i) {
+                TRY(step_fused(s, measure != 0, true, ev[2 * i], ev[2 * i + 1]));
+                s->t = (s->t + 1) % (uint32_t)s->p.n_t;
+                s->steps++;
+            }
+            PIC_HIP(hipEventRecord(w1, s->stream));
+            PIC_HIP(hipEventSynchronize(w1));
+            float ms = 0.0f;
+            PIC_HIP(hipEventElapsedTime(&ms, w0, w1));
+            total += ms;
+            for (uint32_t i = 0; i < c; ++i) {
+                PIC_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+                kern += ms;
+            }
+            launches += c;
+            done += c;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(w0);
+        (void)hipEventDestroy(w1);
+    } else {
+        PIC_HIP(hipEventRecord(s->ev[0], s->stream));
+        TRY(run_steps(s, n_steps, measure != 0));
+        PIC_HIP(hipEventRecord(s->ev[1], s->stream));
+        PIC_HIP(hipEventSynchronize(s->ev[1]));
+        PIC_HIP(hipEventElapsedTime(&total, s->ev[0], s->ev[1]));
+    }
+    // particle-steps over the window from the counts at both ends (the
+    // live count changes by well under a percent per RF cycle at steady state)
+    uint32_t ne1 = 0, ni1 = 0;
+    TRY(pic_sim_counts(s, &ne1, &ni1));
+    const double ne_avg = 0.5 * ((double)ne0 + (double)ne1);
+    const double ni_avg = 0.5 * ((double)ni0 + (double)ni1);
+    psteps = (unsigned long long)(ne_avg * n_steps + ni_avg * ion_steps);
+    if (ms_total) *ms_total = total;
+    if (ms_kernel) *ms_kernel = kern;
+    if (kernel_launches) *kernel_launches = launches;
+    if (particle_steps) *particle_steps = psteps;
+    return PIC_OK;
+}
+
+int pic_sim_save(pic_sim_t *s, const char *path) {
+    if (!s || !path) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("cannot open ") + path);
+        return PIC_ERR_IO;
+    }
+    uint32_t ne = 0, ni = 0;
+    int rc = pic_sim_counts(s, &ne, &ni);
+    if (rc) {
+        fclose(f);
+        return rc;
+    }
+    uint32_t hdr[8] = {kCkptMagic, kCkptVersion, (uint32_t)s->p.n_g, s->t, ne, ni, s->cap, 0};
+    fwrite(hdr, sizeof(hdr), 1, f);
+    fwrite(&s->steps, sizeof(s->steps), 1, f);
+    std::vector<float> buf(std::max<uint32_t>(std::max(ne, ni), 1));
+    std::vector<uint32_t> w(s->cap);
+    for (int sp = 0; sp < 2; ++sp) {
+        const uint32_t n = sp == 0 ? ne : ni;
+        std::vector<float> a[4];
+        for (auto &v : a) v.resize(std::max<uint32_t>(n, 1));
+        uint32_t got = 0;
+        rc = pic_sim_download(s, sp, a[0].data(), a[1].data(), a[2].data(), a[3].data(), n, &got);
+        if (rc) {
+            fclose(f);
+            return rc;
+        }
+        for (auto &v : a) fwrite(v.data(), sizeof(float), n, f);
+        for (int q = 0; q < 4; ++q) {
+            uint32_t *src = (sp == 0 ? s->e_rng : s->i_rng)[q];
+            if (hipMemcpy(w.data(), src, 4ull * s->cap, hipMemcpyDeviceToHost) != hipSuccess) {
+                fclose(f);
+                return hip_fail(hipGetLastError(), "ckpt rng");
+            }
+            fwrite(w.data(), 4, s->cap, f);
+        }
+    }
+    std::vector<double> ce(s->p.n_g), cii(s->p.n_g);
+    std::vector<uint32_t> ee(PIC_N_EEPF), ifd(2 * PIC_N_IFED);
+    unsigned long long cnt[8];
+    rc = pic_sim_read_measurements(s, ce.data(), cii.data(), ee.data(), ifd.data(), cnt);
+    if (rc) {
+        fclose(f);
+        return rc;
+    }
+    fwrite(ce.data(), 8, s->p.n_g, f);
+    fwrite(cii.data(), 8, s->p.n_g, f);
+    fwrite(ee.data(), 4, PIC_N_EEPF, f);
+    fwrite(ifd.data(), 4, 2 * PIC_N_IFED, f);
+    fwrite(cnt, 8, 8, f);
+    bool ok = ferror(f) == 0;
+    fclose(f);
+    if (!ok) {
+        set_error("checkpoint write failed");
+        return PIC_ERR_IO;
+    }
+    return PIC_OK;
+}
+
+int pic_sim_load(pic_sim_t *s, const char *path) {
+    if (!s || !path) return PIC_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("cannot open ") + path);
+        return PIC_ERR_IO;
+    }
+    auto bad = [&](const char *why) {
+        fclose(f);
+        set_error(std::string("checkpoint ") + path + ": " + why);
+        return PIC_ERR_IO;
+    };
+    uint32_t hdr[8];
+    unsigned long long steps = 0;
+    if (fread(hdr, sizeof(hdr), 1, f) != 1 || hdr[0] != kCkptMagic || hdr[1] != kCkptVersion)
+        return bad("not a checkpoint of this version");
+    if ((int)hdr[2] != s->p.n_g) return bad("grid size differs");
+    if (hdr[4] > s->cap || hdr[5] > s->cap) return bad("particle count above capacity");
+    if (fread(&steps, sizeof(steps), 1, f) != 1) return bad("truncated");
+    const uint32_t ckcap = hdr[6];
+    for (int sp = 0; sp < 2; ++sp) {
+        const uint32_t n = sp == 0 ? hdr[4] : hdr[5];
+        std::vector<float> a[4];
+        for (auto &v : a) {
+            v.resize(std::max<uint32_t>(n, 1));
+            if (fread(v.data(), sizeof(float), n, f) != n) return bad("truncated particles");
+        }
+        int rc = pic_sim_upload(s, sp, a[0].data(), a[1].data(), a[2].data(), a[3].data(), n);
+        if (rc) {
+            fclose(f);
+            return rc;
+        }
+        std::vector<uint32_t> w[4];
+        for (auto &v : w) {
+            v.resize(ckcap);
+            if (fread(v.data(), 4, ckcap, f) != ckcap) return bad("truncated rng");
+        }
+        const uint32_t nr = std::min(ckcap, s->cap);
+        rc = pic_sim_rng_upload(s, sp, w[0].data(), w[1].data(), w[2].data(), w[3].data(), nr);
+        if (rc) {
+            fclose(f);
+            return rc;
+        }
+    }
+    std::vector<double> ce(s->p.n_g), cii(s->p.n_g);
+    std::vector<uint32_t> ee(PIC_N_EEPF), ifd(2 * PIC_N_IFED);
+    unsigned long long cnt[8];
+    if (fread(ce.data(), 8, s->p.n_g, f) != (size_t)s->p.n_g || fread(cii.data(), 8, s->p.n_g, f) != (size_t)s->p.n_g ||
+        fread(ee.data(), 4, PIC_N_EEPF, f) != PIC_N_EEPF || fread(ifd.data(), 4, 2 * PIC_N_IFED, f) != 2 * PIC_N_IFED ||
+        fread(cnt, 8, 8, f) != 8)
+        return bad("truncated measurements");
+    fclose(f);
+    PIC_HIP(hipMemcpy(s->cumul_e, ce.data(), 8ull * s->p.n_g, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(s->cumul_i, cii.data(), 8ull * s->p.n_g, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(s->eepf, ee.data(), 4ull * PIC_N_EEPF, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(s->ifed, ifd.data(), 4ull * 2 * PIC_N_IFED, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(s->counters, cnt, 8ull * 8, hipMemcpyHostToDevice));
+    s->ref_meas[0] = s->ref_meas[1] = 0;
+    s->t = hdr[3] % (uint32_t)s->p.n_t;
+    s->steps = steps;
+    s->need_prologue = true;
+    invalidate_graphs(s);
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+"""Simulation object: the reference's GpuSimState + RF loop
+(Rust-CUDA/src/main.rs:132-350, 1611-1878) behind pic_sim_* of the C ABI."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (PIC_ELECTRONS, PIC_IONS, PIC_N_EEPF, PIC_N_IFED, PIC_PATH_FUSED, PicSimConfig,
+                   call, lib)
+
+
+class Sim:
+    def __init__(self, p, capacity=120_000, path=PIC_PATH_FUSED, use_graph=True, seed_e=None,
+                 seed_i=None, spawn_capacity=0, block_size=0, grid_blocks=0, ppt=0, replicas=0):
+        cfg = PicSimConfig()
+        lib().pic_sim_config_default(C.byref(cfg))
+        cfg.params = p
+        cfg.capacity = int(capacity)
+        cfg.spawn_capacity = int(spawn_capacity)
+        if seed_e is not None:
+            cfg.seed_e[:] = [int(v) for v in seed_e]
+        if seed_i is not None:
+            cfg.seed_i[:] = [int(v) for v in seed_i]
+        cfg.path = int(path)
+        cfg.use_graph = int(bool(use_graph))
+        cfg.block_size, cfg.grid_blocks, cfg.ppt, cfg.replicas = block_size, grid_blocks, ppt, replicas
+        self.p = p
+        self.cfg = cfg
+        self.capacity = int(capacity)
+        h = C.c_void_p()
+        call("pic_sim_create", C.byref(cfg), C.byref(h))
+        self.h = h.value
+
+    # -------------------------------------------------------------- state
+    def upload(self, species, x, vx, vy, vz):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (x, vx, vy, vz)]
+        call("pic_sim_upload", self.h, species, *(a.ctypes.data for a in arrs), len(arrs[0]))
+
+    def download(self, species):
+        n = C.c_uint32()
+        call("pic_sim_download", self.h, species, None, None, None, None, 0, C.byref(n))
+        out = [np.zeros(max(1, n.value), np.float32) for _ in range(4)]
+        call("pic_sim_download", self.h, species, *(a.ctypes.data for a in out), n.value, C.byref(n))
+        return [a[:n.value] for a in out]
+
+    def rng_upload(self, species, words):
+        w = [np.ascontiguousarray(a, np.uint32) for a in words]
+        call("pic_sim_rng_upload", self.h, species, *(a.ctypes.data for a in w), len(w[0]))
+
+    def rng_download(self, species, n=None):
+        n = self.capacity if n is None else int(n)
+        out = [np.zeros(max(1, n), np.uint32) for _ in range(4)]
+        call("pic_sim_rng_download", self.h, species, *(a.ctypes.data for a in out), n)
+        return [a[:n] for a in out]
+
+    def set_tables(self, cs, stot_e, stot_i):
+        a = [np.ascontiguousarray(v, np.float32) for v in (cs, stot_e, stot_i)]
+        call("pic_sim_set_tables", self.h, *(v.ctypes.data for v in a))
+
+    def counts(self):
+        ne, ni = C.c_uint32(), C.c_uint32()
+        call("pic_sim_counts", self.h, C.byref(ne), C.byref(ni))
+        return ne.value, ni.value
+
+    def time(self):
+        t, s = C.c_uint32(), C.c_ulonglong()
+        call("pic_sim_time", self.h, C.byref(t), C.byref(s))
+        return t.value, s.value
+
+    def set_time(self, t):
+        call("pic_sim_set_time", self.h, int(t))
+
+    # -------------------------------------------------------------- stepping
+    def run(self, n_steps, measure=False):
+        call("pic_sim_run", self.h, int(n_steps), int(bool(measure)))
+
+    def synchronize(self):
+        call("pic_sim_synchronize", self.h)
+
+    def check(self):
+        call("pic_sim_check", self.h)
+
+    def overflowed(self) -> bool:
+        return lib().pic_sim_check(self.h) != 0
+
+    def time_steps(self, n_steps, measure=False, kernel_events=False):
+        tot, ker = C.c_float(), C.c_float()
+        nl, ps = C.c_uint32(), C.c_ulonglong()
+        call("pic_sim_time_steps", self.h, int(n_steps), int(bool(measure)), C.byref(tot),
+             C.byref(ker) if kernel_events else None, C.byref(nl), C.byref(ps))
+        return {"ms_total": tot.value, "ms_kernel": ker.value if kernel_events else None,
+                "kernel_launches": nl.value, "particle_steps": ps.value}
+
+    # -------------------------------------------------------------- readback
+    def grid(self):
+        n = self.p.n_g
+        pot = np.zeros(n, np.float32)
+        ef = np.zeros(n, np.float32)
+        ed = np.zeros(n, np.float64)
+        idn = np.zeros(n, np.float64)
+        call("pic_sim_read_grid", self.h, pot.ctypes.data, ef.ctypes.data, ed.ctypes.data, idn.ctypes.data)
+        return pot, ef, ed, idn
+
+    def measurements(self):
+        n = self.p.n_g
+        ce = np.zeros(n, np.float64)
+        ci = np.zeros(n, np.float64)
+        eepf = np.zeros(PIC_N_EEPF, np.uint32)
+        ifed = np.zeros(2 * PIC_N_IFED, np.uint32)
+        cnt = np.zeros(8, np.uint64)
+        call("pic_sim_read_measurements", self.h, ce.ctypes.data, ci.ctypes.data, eepf.ctypes.data,
+             ifed.ctypes.data, cnt.ctypes.data)
+        keys = ("e_lost_powered", "e_lost_grounded", "i_lost_powered", "i_lost_grounded",
+                "e_collisions", "i_collisions", "e_meas_steps", "i_meas_steps")
+        return {"cumul_e": ce, "cumul_i": ci, "eepf": eepf, "ifed": ifed.reshape(2, PIC_N_IFED),
+                "counters": dict(zip(keys, (int(v) for v in cnt)))}
+
+    def reset_measurements(self):
+        call("pic_sim_reset_measurements", self.h)
+
+    def save(self, path):
+        call("pic_sim_save", self.h, str(path).encode())
+
+    def load(self, path):
+        call("pic_sim_load", self.h, str(path).encode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pic_sim_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+ELECTRONS, IONS = PIC_ELECTRONS, PIC_IONS

@@ -1,0 +1,269 @@
+/*
+ * cudapic_amd.h -- C ABI of the MI355X (gfx950) PIC/MCC engine for the
+ * 1d3v argon RF discharge of sekulas/cudaPIC.
+ *
+ * Two layers, both plain C (pointers, sizes, status codes; no HIP or torch
+ * types -- a stream is passed as void*, i.e. a hipStream_t):
+ *
+ *  1. Stage entry points, one per kernel of the reference's
+ *     `#[cuda_module] mod kernels` (Rust-CUDA/src/main.rs:352-1358).  They take
+ *     raw device pointers and a stream, read live counts from device memory
+ *     (never from the host), and are asynchronous and graph-capturable.
+ *
+ *  2. A simulation object that owns the device state the reference keeps in
+ *     `GpuSimState` (main.rs:132-350) and runs the RF step loop of main.rs:
+ *     1723-1828, either in the reference's stage order or on the fused
+ *     MI355X path (one particle kernel per species per step).
+ *
+ * Every function returns PIC_OK (0) or an error code; none aborts.
+ */
+#ifndef CUDAPIC_AMD_H
+#define CUDAPIC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *pic_stream_t; /* hipStream_t; NULL = the null stream */
+
+enum pic_status {
+    PIC_OK = 0,
+    PIC_ERR_HIP = 1,       /* a HIP runtime call failed (see pic_last_error) */
+    PIC_ERR_ARG = 2,       /* invalid argument (null pointer, size, mode) */
+    PIC_ERR_CAPACITY = 3,  /* a particle array overflowed (device flag set) */
+    PIC_ERR_UNSUPPORTED = 4,
+    PIC_ERR_IO = 5
+};
+
+/* Precision of the density flush and of the Poisson prefix scans. */
+enum pic_precision {
+    PIC_PREC_FP32 = 0,  /* f32 density atomics, f32 scans (the reference) */
+    PIC_PREC_COMP = 1,  /* f32 density atomics, compensated (float-float) scans */
+    PIC_PREC_FP64 = 2   /* f64 density atomics, f64 scans */
+};
+
+enum pic_species_id { PIC_ELECTRONS = 0, PIC_IONS = 1 };
+
+#define PIC_CS_RANGES 1000000 /* main.rs:69 */
+#define PIC_N_CS 5            /* main.rs:61 */
+#define PIC_N_EEPF 2000       /* main.rs:75 */
+#define PIC_N_IFED 200        /* C/eduPIC.cc:124 */
+#define PIC_MAX_HARMONICS 16
+
+/* Discharge parameters and the f32 factors the reference pre-folds
+ * (main.rs:27-110).  Set the first block, then call pic_params_derive(). */
+typedef struct pic_params {
+    int32_t n_g, n_t, n_sub, precision;
+    double freq, v0, gap, pressure, t_gas, weight, area;
+    /* derived by pic_params_derive() */
+    double dt_e, dt_i, dx, inv_dx, gas_density;
+    float inv_dx_f, factor_e, factor_i, weight_factor, dt_e_f, dt_i_f, gap_f;
+    float normal_range, f1, f2, half_me_over_e, half_mu_over_e, half_mi_over_e;
+    float e_charge_f, alpha_f, half_dx_over_eps_f, half_inv_dx_f;
+    float min_x_f, max_x_f, de_eepf_f, de_cs_f, me_f, exc_th_f, ion_th_f, eepf_kick_f;
+    /* driving waveform pot0(t) = sum_k a_k cos(2 pi k t/N_T) + b_k sin(2 pi k t/N_T);
+     * pic_params_derive() sets a_1 = v0 and the rest to 0 (main.rs:1741). */
+    double wave_cos[PIC_MAX_HARMONICS], wave_sin[PIC_MAX_HARMONICS];
+} pic_params_t;
+
+/* The reference case (main.rs:39-49): N_G=400, N_T=4000, N_SUB=20,
+ * 13.56 MHz, 250 V, 25 mm, 10 Pa, 350 K, W=7e4, A=1e-4 m^2, FP32. */
+void pic_params_default(pic_params_t *p);
+/* Fills the derived block exactly as main.rs:52-110 computes it. */
+int pic_params_derive(pic_params_t *p);
+
+/* SoA particle arrays of one species (the reference's e_x/e_vx/e_vy/e_vz or
+ * i_*, main.rs:134-143) with the per-slot xoshiro128+ state words
+ * (rng_e0..rng_e3 / rng_i0..rng_i3, main.rs:163-170). */
+typedef struct pic_species {
+    float *x, *vx, *vy, *vz;
+    uint32_t *rng0, *rng1, *rng2, *rng3;
+    uint32_t capacity; /* MAX_PARTICLES */
+} pic_species_t;
+
+/* ------------------------------------------------------------------------
+ * Host-side tables (main.rs:1378-1496)
+ * ---------------------------------------------------------------------- */
+
+/* Cross-section tables on 1e6 energy bins of 1 meV into host buffers:
+ * cs[5*PIC_CS_RANGES] (e elastic, e excitation, e ionisation, i isotropic,
+ * i backward) and the total macroscopic cross sections (main.rs:1378-1444). */
+int pic_cross_sections(double gas_density, float *cs, float *stot_e, float *stot_i);
+
+/* ------------------------------------------------------------------------
+ * Stage entry points (one per reference kernel)
+ * ---------------------------------------------------------------------- */
+
+/* get_density (main.rs:356-406): CIC deposit of x[0..*n_active) into
+ * `density` (n_g values, f32 or f64 as `density_f64` says), which the
+ * caller zeroes first (main.rs:1729).  Nodes 0 and n_g-1 are doubled. */
+int pic_get_density(const float *x, const uint32_t *n_active, void *density,
+                    int density_f64, const pic_params_t *p, pic_stream_t stream);
+
+/* solve_poisson_scan_f32 (main.rs:408-494): potential and central-difference
+ * field from the two densities (f32 arrays for PIC_PREC_FP32/COMP, f64 for
+ * PIC_PREC_FP64) with pot[0] = pot0, pot[n_g-1] = 0.  One workgroup. */
+int pic_solve_poisson(const void *e_density, const void *i_density, double pot0,
+                      int precision, float *pot, float *efield, const pic_params_t *p,
+                      pic_stream_t stream);
+
+/* solve_poisson_dps_flexible (main.rs:496-564): psi'' = source with
+ * Dirichlet ends, 3 <= n <= 4096 nodes of spacing dx. */
+int pic_solve_poisson_flexible(const float *source, int n, float psi_left, float psi_right,
+                               float dx, int precision, float *pot, pic_stream_t stream);
+
+/* move_and_compact (main.rs:566-678): leapfrog push of src[0..*n_active)
+ * in `efield`, keep 0 <= x' <= L, compact the survivors into dst and add
+ * their number to *alive_counter (caller zeroes it, main.rs:1758).
+ * Optional: abs_counts[2] += particles lost at x<0 / x>L; ifed[2*200] +=
+ * energy histogram (1 eV bins) of the lost particles with mass factor
+ * half_m_over_e (pass 0 and NULL to skip). */
+int pic_move_and_compact(const float *efield, const float *src_x, const float *src_vx,
+                         const float *src_vy, const float *src_vz, float *dst_x,
+                         float *dst_vx, float *dst_vy, float *dst_vz,
+                         uint32_t *alive_counter, const uint32_t *n_active, float factor,
+                         float dt, float half_m_over_e, unsigned long long *abs_counts,
+                         uint32_t *ifed, const pic_params_t *p, pic_stream_t stream);
+
+/* check_collisions_e + collision_e (main.rs:971-1165): electron/Ar MCC over
+ * slots [0, *n_active).  Ionisations append an electron at (*alive_e)++ and
+ * an ion at (*alive_i)++; *alive_e must equal *n_active on entry
+ * (main.rs:1782).  Appends beyond the capacities are dropped and set
+ * *overflow = 1.  n_coll (optional) += number of collisions. */
+int pic_check_collisions_e(const float *stot_e, const float *cs, const uint32_t *n_active,
+                           const pic_species_t *electrons, const pic_species_t *ions,
+                           uint32_t *alive_e, uint32_t *alive_i, uint32_t *overflow,
+                           unsigned long long *n_coll, const pic_params_t *p,
+                           pic_stream_t stream);
+
+/* check_collisions_i + collision_i (main.rs:1167-1292): Ar+/Ar MCC with a
+ * Maxwellian target atom over slots [0, *n_active). */
+int pic_check_collisions_i(const float *stot_i, const float *cs, const uint32_t *n_active,
+                           const pic_species_t *ions, unsigned long long *n_coll,
+                           const pic_params_t *p, pic_stream_t stream);
+
+/* accumulate_density (main.rs:1294-1299): cumul[k] += (f64) density[k]. */
+int pic_accumulate_density(const void *density, int density_f64, double *cumul, int n_g,
+                           pic_stream_t stream);
+
+/* accumulate_eepf (main.rs:1301-1339): time-centred energy histogram
+ * (2000 bins of 0.05 eV) of electrons in 0.45 L < x < 0.55 L. */
+int pic_accumulate_eepf(const float *x, const float *vx, const float *vy, const float *vz,
+                        const float *efield, const uint32_t *n_active, uint32_t *eepf,
+                        const pic_params_t *p, pic_stream_t stream);
+
+/* xoshiro128p_test_wrapper (main.rs:1341-1357), generalised: n successive
+ * next() outputs from state[4] (host values) into out[n] (device), the final
+ * state into state_out[4] (device). */
+int pic_xoshiro_next_n(const uint32_t state[4], uint32_t n, uint32_t *out,
+                       uint32_t *state_out, pic_stream_t stream);
+
+/* xoshiro128_seed_streams (main.rs:1446-1496) on the device: slot k of the
+ * four planar word arrays gets the master state jumped k times (k in
+ * [0, n)); bit-identical to the sequential host seeding. */
+int pic_seed_streams(const uint32_t master[4], uint32_t n, uint32_t *rng0, uint32_t *rng1,
+                     uint32_t *rng2, uint32_t *rng3, pic_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Simulation object (GpuSimState + the RF loop, main.rs:112-350, 1611-1878)
+ * ---------------------------------------------------------------------- */
+
+enum pic_path {
+    PIC_PATH_REFERENCE = 0, /* the reference's stage order and counter protocol */
+    PIC_PATH_FUSED = 1      /* push+compact+collide+deposit fused per species */
+};
+
+typedef struct pic_sim_config {
+    pic_params_t params;       /* derived */
+    uint32_t capacity;         /* MAX_PARTICLES per species (main.rs:79) */
+    uint32_t spawn_capacity;   /* fused path: electrons created in one step (0 = auto) */
+    uint32_t seed_e[4];        /* master xoshiro states (main.rs:1667, 1671) */
+    uint32_t seed_i[4];
+    int32_t path;              /* enum pic_path */
+    int32_t use_graph;         /* capture 2*N_SUB steps into a hipGraph */
+    int32_t block_size;        /* particle kernels: 0 = tuned default */
+    int32_t grid_blocks;       /* persistent grid: 0 = tuned default */
+    int32_t ppt;               /* particles per thread per tile: 0 = default */
+    int32_t replicas;          /* global density replicas: 0 = default */
+    int32_t measure;           /* allocate measurement buffers */
+} pic_sim_config_t;
+
+typedef struct pic_sim pic_sim_t;
+
+void pic_sim_config_default(pic_sim_config_t *c);
+int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out);
+int pic_sim_destroy(pic_sim_t *s);
+
+/* Host SoA upload of n particles into a species (main.rs:250-278); resets
+ * the step parity bookkeeping.  Download copies the live particles out
+ * (main.rs:315-339); *n receives the live count (at most cap are copied). */
+int pic_sim_upload(pic_sim_t *s, int species, const float *x, const float *vx,
+                   const float *vy, const float *vz, uint32_t n);
+int pic_sim_download(pic_sim_t *s, int species, float *x, float *vx, float *vy, float *vz,
+                     uint32_t cap, uint32_t *n);
+/* Per-slot RNG words, planar, n slots (upload replaces, download copies). */
+int pic_sim_rng_upload(pic_sim_t *s, int species, const uint32_t *w0, const uint32_t *w1,
+                       const uint32_t *w2, const uint32_t *w3, uint32_t n);
+int pic_sim_rng_download(pic_sim_t *s, int species, uint32_t *w0, uint32_t *w1,
+                         uint32_t *w2, uint32_t *w3, uint32_t n);
+/* Replace the cross-section tables (host arrays as pic_cross_sections). */
+int pic_sim_set_tables(pic_sim_t *s, const float *cs, const float *stot_e, const float *stot_i);
+
+/* Live counts (synchronises the sim stream; the reference's checkpoint read,
+ * main.rs:1810-1816). */
+int pic_sim_counts(pic_sim_t *s, uint32_t *n_e, uint32_t *n_i);
+/* Step index within the RF cycle (0..N_T-1) of the next step, and total steps run. */
+int pic_sim_time(pic_sim_t *s, uint32_t *t_in_cycle, unsigned long long *steps_done);
+int pic_sim_set_time(pic_sim_t *s, uint32_t t_in_cycle);
+
+/* Run n_steps RF time steps asynchronously on the sim stream.  measure != 0
+ * accumulates densities and the EEPF (main.rs:1746-1756, 1802-1807). */
+int pic_sim_run(pic_sim_t *s, uint32_t n_steps, int measure);
+int pic_sim_synchronize(pic_sim_t *s);
+/* PIC_ERR_CAPACITY if any particle array overflowed since creation. */
+int pic_sim_check(pic_sim_t *s);
+
+/* Grid state after the last Poisson solve: pot, efield (f32), and the
+ * electron / ion densities that solve used (f64). Any pointer may be NULL. */
+int pic_sim_read_grid(pic_sim_t *s, float *pot, float *efield, double *e_dens, double *i_dens);
+/* Measurement accumulators: cumul_e/i (f64 n_g), eepf (u32 2000),
+ * ifed (u32 2*200: powered, grounded), counters[8] = electrons absorbed at
+ * the powered / grounded electrode, ions ditto, electron collisions, ion
+ * collisions, electron and ion measurement steps.  NULL skips. */
+int pic_sim_read_measurements(pic_sim_t *s, double *cumul_e, double *cumul_i, uint32_t *eepf,
+                              uint32_t *ifed, unsigned long long *counters);
+int pic_sim_reset_measurements(pic_sim_t *s);
+
+/* Timing: run n_steps and report wall milliseconds (HIP events on the sim
+ * stream) and the summed duration of the dominant particle kernel
+ * (electron particle kernel) with its launch count. */
+int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_total,
+                       float *ms_kernel, uint32_t *kernel_launches,
+                       unsigned long long *particle_steps);
+
+/* Binary checkpoint of the whole device state (particles, RNG words,
+ * counters, held densities, time, measurements). */
+int pic_sim_save(pic_sim_t *s, const char *path);
+int pic_sim_load(pic_sim_t *s, const char *path);
+
+/* Device memory helpers for callers without their own allocator. */
+int pic_malloc(void **ptr, uint64_t bytes);
+int pic_free(void *ptr);
+int pic_memcpy_htod(void *dst, const void *src, uint64_t bytes);
+int pic_memcpy_dtoh(void *dst, const void *src, uint64_t bytes);
+int pic_memset(void *dst, int value, uint64_t bytes);
+int pic_device_synchronize(void);
+/* Text of the last error on this thread. */
+const char *pic_last_error(void);
+/* Device properties used for launch shapes: compute units, wave size. */
+int pic_device_info(int *compute_units, int *wave_size, char *arch, int arch_len);
+/* sizeof(pic_params_t), sizeof(pic_species_t), sizeof(pic_sim_config_t), so
+ * that a binding can verify its struct layout (no reference counterpart). */
+int pic_abi_layout(uint64_t out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -182,11 +182,12 @@ void or_cross_sections(double gas_density, float *cs, float *stot_e, float *stot
 }
 
 /* ---- deposition (main.rs:356-406) -------------------------------------- */
-/* Cell index is clamped to n_g-2 so that x == L deposits onto the last two
- * nodes instead of past the end of the grid (main.rs:385-386 would index
+/* Cell index is clamped to [0, n_g-2] so that x == L deposits onto the last
+ * two nodes instead of past the end of the grid (main.rs:385-386 would index
  * N_G there). */
 static inline int cell_of(float pos, int n_g) {
     int q = (int)pos;
+    q = q < 0 ? 0 : q;
     return q > n_g - 2 ? n_g - 2 : q;
 }
 
