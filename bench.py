@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the RF step on one MI355X.
+
+Workload (the plain method at scale): the eduPIC argon CCP (25 mm, 10 Pa,
+350 K, 13.56 MHz, 250 V, N_G = 400, N_T = 4000, N_SUB = 20) with the
+superparticle weight lowered to 2e3 so that ~3.74 M electrons and ~3.93 M ions
+are live (MAX_PARTICLES = 8 M per species, FP32 SoA), started from a
+plasma-like state drawn from the published converged density profiles.
+
+A step is one RF time step (electrons every step, ions every N_SUB steps);
+a particle-step is one particle advanced through one of its own steps.
+The timed region runs on the fast path (fused particle kernels, captured
+into a hipGraph) with every input resident in HBM.  The dominant kernel (the
+fused electron kernel) is then timed live with HIP events on the sim stream
+over a second window, for the roofline.  The CPU baseline times the
+reference's own eduPIC (compiled from source into oracle/_ref) on one core
+over one RF cycle of the reference case (~1e5 particles per species).
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+ALG_BYTES = 80                 # per particle-step: deposit 4 + push/compact 32 + collisions 44
+METRIC = "particle-steps/s at 4M particles, % HBM roofline; batched and adjoint/fwd ratio"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000, help="timed RF time steps (4000 = one RF cycle)")
+    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--kernel-steps", type=int, default=400, help="steps of the event-timed window")
+    ap.add_argument("--n-e", type=int, default=3_740_000)
+    ap.add_argument("--n-i", type=int, default=3_930_000)
+    ap.add_argument("--capacity", type=int, default=8_000_000)
+    ap.add_argument("--weight", type=float, default=2.0e3)
+    ap.add_argument("--path", choices=["fused", "reference"], default="fused")
+    ap.add_argument("--precision", choices=["fp32", "comp", "fp64"], default="fp64")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--ppt", type=int, default=0)
+    ap.add_argument("--replicas", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=100_000, help="particles per species of the CPU sample")
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the fused electron kernel, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None if it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "fused_electron_4M_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n_sample: int):
+    """eduPIC (C/eduPIC.cc compiled into oracle/_ref/edupic) on one core: one
+    RF cycle from n_sample electrons + ions of the reference case (weight 7e4
+    as compiled, i.e. the same physical densities as the 4 M-particle run)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "edupic")
+    if not os.path.exists(exe):
+        return port_baseline()
+    from cudapic_amd import synthetic
+    E, I = synthetic.state(n_sample, int(n_sample * 1.05), seed=7)
+    tmp = tempfile.mkdtemp(prefix="edupic_")
+    try:
+        with open(os.path.join(tmp, "picdata.bin"), "wb") as f:
+            # C/eduPIC.cc:726-749 layout: Time, cycles_done, N_e (f64), e arrays, N_i, i arrays
+            np.array([0.0, 1.0, float(len(E[0]))], np.float64).tofile(f)
+            for a in E:
+                a.astype(np.float64).tofile(f)
+            np.array([float(len(I[0]))], np.float64).tofile(f)
+            for a in I:
+                a.astype(np.float64).tofile(f)
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "1"], cwd=tmp, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return port_baseline()
+        conv = open(os.path.join(tmp, "conv.dat")).read().split()
+        ne1, ni1 = int(conv[-2]), int(conv[-1])
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    ne0, ni0 = len(E[0]), len(I[0])
+    ps = 0.5 * (ne0 + ne1) * 4000 + 0.5 * (ni0 + ni1) * 200
+    return {"value": ps / wall, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"eduPIC C (g++ -O3 -march=x86-64-v3), 1 RF cycle (4000 steps) from "
+                      f"{ne0} e + {ni0} i of the reference case at W=7e4 ({wall:.1f} s wall)"}
+
+
+def port_baseline():
+    """The oracle's serial step (C, one core) when the reference build is absent."""
+    from cudapic_amd import synthetic
+    from oracle import oracle as orc
+    p = orc.params(weight=7.0e4)
+    E, I = synthetic.state(50_000, 52_500, seed=7)
+    cap = 200_000
+    er = orc.seed_streams([0x12345678, 0x11112222, 0x22223333, 0x33334444], cap)
+    ir = orc.seed_streams([0x44445555, 0x55556666, 0x66667777, 0x77778888], cap)
+    sim = orc.Sim(p, cap, *E, *I, er, ir)
+    ps = 0.0
+    t0 = time.perf_counter()
+    for t in range(400):
+        ps += sim.n_e + (sim.n_i if t % 20 == 0 else 0)
+        sim.step(t)
+    wall = time.perf_counter() - t0
+    return {"value": ps / wall, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle step, 400 steps from 50000 e + 52500 i ({wall:.1f} s wall)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the RF step on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import cudapic_amd as ca
+    from cudapic_amd import synthetic
+
+    prec = {"fp32": ca.PIC_PREC_FP32, "comp": ca.PIC_PREC_COMP, "fp64": ca.PIC_PREC_FP64}[args.precision]
+    path = ca.PIC_PATH_FUSED if args.path == "fused" else ca.PIC_PATH_REFERENCE
+    p = ca.params(weight=args.weight, precision=prec)
+    cu, ws, arch = ca.device_info()
+    print(f"bench: device {arch}, {cu} CUs, wave{ws}; building state "
+          f"{args.n_e} e + {args.n_i} i", file=sys.stderr, flush=True)
+    E, I = synthetic.state(args.n_e, args.n_i, seed=1)
+    sim = ca.Sim(p, capacity=args.capacity, path=path, use_graph=not args.no_graph,
+                 grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas)
+    sim.upload(ca.PIC_ELECTRONS, *E)
+    sim.upload(ca.PIC_IONS, *I)
+    del E, I
+    sim.run(args.warmup)
+    sim.synchronize()
+    sim.check()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    # timed region (one GPU: the barrier is the stream synchronisation)
+    sim.synchronize()
+    t = sim.time_steps(args.steps)
+    sim.synchronize()
+    sim.check()
+    n_e, n_i = sim.counts()
+    value = t["particle_steps"] / (t["ms_total"] * 1e-3)
+    ms_per_step = t["ms_total"] / args.steps
+    print(f"bench: timed {args.steps} steps in {t['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    roofline = None
+    if path == ca.PIC_PATH_FUSED and args.kernel_steps > 0:
+        ne0, _ = sim.counts()
+        k = sim.time_steps(args.kernel_steps, kernel_events=True)
+        ne1, _ = sim.counts()
+        avg_ms = k["ms_kernel"] / max(1, k["kernel_launches"])
+        units = 0.5 * (ne0 + ne1)
+        achieved = ALG_BYTES * units / (avg_ms * 1e-3) / 1e9
+        traffic = traffic_from_profiles()
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": "k_fused<electrons> (push+compact+collide+deposit)",
+                    "kernel_avg_us": round(avg_ms * 1e3, 2), "electrons_per_launch": int(units),
+                    "bytes_per_unit": ALG_BYTES}
+    sim.close()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
+    step_frac = value * ALG_BYTES / (HBM_PEAK_GBS * 1e9)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "argon CCP at scale: W=2e3, ~3.74M e + ~3.93M i, MAX=8M/species, "
+                               "N_G=400, N_T=4000, N_SUB=20 (plain method at scale)",
+                   "n_electrons_end": n_e, "n_ions_end": n_i, "path": args.path,
+                   "graph": not args.no_graph, "poisson_and_density_flush": args.precision,
+                   "step_hbm_frac_80B": round(step_frac, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -21,35 +21,11 @@ def refcase_density():
     return d[:, 0], d[:, 1], d[:, 2]
 
 
-def sample_profile(rng, n, gap, profile, n_g=400):
-    """Positions by inverse-CDF sampling of a nodal density profile (linear
-    interpolation between the nodes of the published 400-node grid)."""
-    xs = np.linspace(0.0, gap, len(profile))
-    w = np.maximum(profile, 0.0)
-    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (w[1:] + w[:-1]) * np.diff(xs))])
-    cdf /= cdf[-1]
-    u = rng.random(n)
-    x = np.interp(u, cdf, xs)
-    return np.clip(x, 0.0, gap).astype(np.float32)
-
-
 def synthetic_state(n_e, n_i, gap=0.025, seed=0, te_ev=2.1, ion_beam_frac=0.1):
-    """Plasma-like state: x from the published profiles, Maxwellian electrons
-    (kT_e), thermal ions at 350 K with a fraction carrying sheath energies."""
-    rng = np.random.default_rng(seed)
-    _, ne, ni = refcase_density()
-    ex = sample_profile(rng, n_e, gap, ne)
-    ix = sample_profile(rng, n_i, gap, ni)
-    sig_e = np.sqrt(te_ev * E_CHARGE / E_MASS)
-    ev = (rng.standard_normal((3, n_e)) * sig_e).astype(np.float32)
-    sig_i = np.sqrt(K_B * 350.0 / AR_MASS)
-    iv = (rng.standard_normal((3, n_i)) * sig_i).astype(np.float32)
-    nb = int(ion_beam_frac * n_i)
-    if nb:
-        e_b = rng.uniform(1.0, 60.0, nb)
-        vb = np.sqrt(2 * e_b * E_CHARGE / AR_MASS) * np.sign(ix[:nb] - 0.5 * gap)
-        iv[0, :nb] = vb.astype(np.float32)
-    return (ex, ev[0], ev[1], ev[2]), (ix, iv[0], iv[1], iv[2])
+    """Plasma-like state drawn from the published profiles (the generator the
+    benchmark uses, cudapic_amd/synthetic.py)."""
+    from cudapic_amd import synthetic
+    return synthetic.state(n_e, n_i, gap=gap, seed=seed, te_ev=te_ev, ion_beam_frac=ion_beam_frac)
 
 
 def sort_rows(*cols):

@@ -1,0 +1,50 @@
+"""The C++ host CLI: the reference's run interface and result files
+(Rust-CUDA/src/main.rs:1498-1627)."""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "cudapic_amd", "cudapic")
+
+
+@pytest.mark.parametrize("path", ["fused", "reference"])
+def test_cli_run_writes_reference_outputs(tmp_path, path):
+    out = tmp_path / "results"
+    ck = tmp_path / "state.ckpt"
+    r = subprocess.run([CLI, "3", "--measure", "2", "--path", path, "--out", str(out),
+                        "--save", str(ck)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "final particles" in r.stdout
+    d = glob.glob(str(out / "*"))
+    assert len(d) == 1
+    files = {os.path.basename(f).split("_")[0] for f in glob.glob(d[0] + "/*")}
+    names = sorted(os.path.basename(f) for f in glob.glob(d[0] + "/*"))
+    assert any(n.startswith("0003_") and n.endswith("_electrons.csv") for n in names)
+    assert any(n.startswith("0003_") and n.endswith("_ions.csv") for n in names)
+    for prefix in ("particle_growth_", "density_avg_", "eepf_", "ifed_", "info_"):
+        assert any(n.startswith(prefix) for n in names), (prefix, names)
+    dens = [f for f in glob.glob(d[0] + "/density_avg_*")][0]
+    head = open(dens).readline().strip()
+    assert head == "x,n_e,n_i"
+    a = np.loadtxt(dens, delimiter=",", skiprows=1)
+    assert a.shape == (400, 3) and (a[:, 1] >= 0).all() and a[:, 2].sum() > 0
+    eepf = np.loadtxt(glob.glob(d[0] + "/eepf_*")[0], delimiter=",", skiprows=1)
+    assert eepf.shape == (2000, 2)
+    e = glob.glob(d[0] + "/*_electrons.csv")[0]
+    assert open(e).readline().strip() == "x,vx,vy,vz"
+    assert ck.exists()
+    # resume from the checkpoint
+    r2 = subprocess.run([CLI, "1", "--path", path, "--out", str(tmp_path / "r2"), "--load", str(ck)],
+                        capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert "state loaded" in r2.stdout
+
+
+def test_cli_usage_error():
+    r = subprocess.run([CLI], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr
