@@ -59,7 +59,30 @@ def parse():
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="particles per species of the CPU sample")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the streaming-copy calibration kernel on MIB MiB, 20 times")
     return ap.parse_args()
+
+
+def copy_bandwidth(mib: int, reps: int = 20) -> float:
+    """GB/s (read + write) of pic_stream_copy over mib MiB: the achievable
+    HBM rate on this device, and the calibration kernel of the traffic
+    counters."""
+    import cudapic_amd as ca
+    from cudapic_amd._abi import call
+    n = mib * (1 << 20)
+    a = ca.DeviceBuffer.zeroed(n // 4, np.uint32)
+    b = ca.DeviceBuffer.zeroed(n // 4, np.uint32)
+    call("pic_stream_copy", b.ptr, a.ptr, n, None)
+    ca.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call("pic_stream_copy", b.ptr, a.ptr, n, None)
+    ca.synchronize()
+    dt = time.perf_counter() - t0
+    a.free()
+    b.free()
+    return 2.0 * n * reps / dt / 1e9
 
 
 def traffic_from_profiles():
@@ -137,6 +160,9 @@ def main():
     import cudapic_amd as ca
     from cudapic_amd import synthetic
 
+    if args.copy_probe:
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "GB/s": round(copy_bandwidth(args.copy_probe), 1)}))
+        return
     prec = {"fp32": ca.PIC_PREC_FP32, "comp": ca.PIC_PREC_COMP, "fp64": ca.PIC_PREC_FP64}[args.precision]
     path = ca.PIC_PATH_FUSED if args.path == "fused" else ca.PIC_PATH_REFERENCE
     p = ca.params(weight=args.weight, precision=prec)
@@ -177,6 +203,8 @@ def main():
                     "kernel_avg_us": round(avg_ms * 1e3, 2), "electrons_per_launch": int(units),
                     "bytes_per_unit": ALG_BYTES}
     sim.close()
+    if roofline is not None:
+        roofline["copy_kernel_GBs"] = round(copy_bandwidth(1024), 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
     step_frac = value * ALG_BYTES / (HBM_PEAK_GBS * 1e9)
     out = {

@@ -53,6 +53,16 @@ class PicSimConfig(C.Structure):
     ]
 
 
+class PicEnsembleConfig(C.Structure):
+    _fields_ = [
+        ("n_instances", C.c_int32),
+        ("params", C.POINTER(PicParams)),
+        ("capacity", C.POINTER(C.c_uint32)),
+        ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
+        *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas")],
+    ]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 U64 = C.c_uint64
@@ -98,12 +108,25 @@ SIGNATURES = {
                                  C.POINTER(C.c_ulonglong)]),
     "pic_sim_save": (I32, [P, C.c_char_p]),
     "pic_sim_load": (I32, [P, C.c_char_p]),
+    "pic_ens_create": (I32, [C.POINTER(PicEnsembleConfig), C.POINTER(P)]),
+    "pic_ens_destroy": (I32, [P]),
+    "pic_ens_upload": (I32, [P, I32, I32, P, P, P, P, U32]),
+    "pic_ens_download": (I32, [P, I32, I32, P, P, P, P, U32, C.POINTER(U32)]),
+    "pic_ens_counts": (I32, [P, P, P]),
+    "pic_ens_rng_download": (I32, [P, I32, I32, P, P, P, P, U32]),
+    "pic_ens_run": (I32, [P, U32, I32]),
+    "pic_ens_synchronize": (I32, [P]),
+    "pic_ens_check": (I32, [P]),
+    "pic_ens_time_steps": (I32, [P, U32, I32, C.POINTER(F32), C.POINTER(C.c_ulonglong)]),
+    "pic_ens_read_grid": (I32, [P, I32, P, P]),
+    "pic_ens_read_measurements": (I32, [P, I32, P, P, P]),
     "pic_malloc": (I32, [C.POINTER(P), U64]),
     "pic_free": (I32, [P]),
     "pic_memcpy_htod": (I32, [P, P, U64]),
     "pic_memcpy_dtoh": (I32, [P, P, U64]),
     "pic_memset": (I32, [P, I32, U64]),
     "pic_device_synchronize": (I32, []),
+    "pic_stream_copy": (I32, [P, P, U64, P]),
     "pic_last_error": (C.c_char_p, []),
     "pic_device_info": (I32, [C.POINTER(I32), C.POINTER(I32), C.c_char_p, I32]),
     "pic_abi_layout": (I32, [C.POINTER(U64)]),
@@ -150,6 +173,7 @@ def call(name: str, *args):
 
 def layout_matches() -> bool:
     """True if the ctypes structures have the C sizes the library reports."""
-    out = (U64 * 3)()
+    out = (U64 * 4)()
     lib().pic_abi_layout(out)
-    return (out[0], out[1], out[2]) == (C.sizeof(PicParams), C.sizeof(PicSpecies), C.sizeof(PicSimConfig))
+    return tuple(out) == (C.sizeof(PicParams), C.sizeof(PicSpecies), C.sizeof(PicSimConfig),
+                          C.sizeof(PicEnsembleConfig))

@@ -30,6 +30,11 @@ struct KPhys {
     float normal_range, f1, f2, half_me_over_e, half_mu_over_e, half_mi_over_e;
     float e_charge, me, exc_th, ion_th, de_cs;
     float min_x, max_x, de_eepf, eepf_kick;
+    // multiplies the total cross-section table: 1 for the per-discharge
+    // tables (sigma_tot * n_gas precomputed), the gas density for a shared
+    // table of summed cross sections (ensembles of discharges at different
+    // pressures)
+    float stot_scale;
 };
 
 // ---------------------------------------------------------------- RNG
@@ -178,7 +183,7 @@ __device__ __forceinline__ int collide_electron(const float *__restrict__ stot_e
     const float energy = k.half_me_over_e * v2;
     const int idx = energy_index(energy, k.de_cs);
     const float velocity = __builtin_sqrtf(v2);
-    const float nu = stot_e[idx] * velocity;
+    const float nu = stot_e[idx] * k.stot_scale * velocity;
     const float rv = draw(s);
     const float pc = 1.0f - expf(-nu * k.dt_e);
     if (!(rv < pc)) return kNone;
@@ -244,7 +249,7 @@ __device__ __forceinline__ int collide_ion(const float *__restrict__ stot_i,
     const float energy = k.half_mu_over_e * g2;
     const int idx = energy_index(energy, k.de_cs);
     const float g = __builtin_sqrtf(g2);
-    const float nu = stot_i[idx] * g;
+    const float nu = stot_i[idx] * k.stot_scale * g;
     const float rv = draw(s);
     const float pc = 1.0f - expf(-nu * k.dt_i);
     if (!(rv < pc)) return kNone;

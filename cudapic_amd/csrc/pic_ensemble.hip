@@ -1,0 +1,582 @@
+// pic_ensemble.hip -- ensembles of independent discharges on one GPU: every
+// stage of every instance in one launch (one Poisson workgroup per
+// instance, one persistent particle kernel per species over all instances),
+// the 2*N_SUB-step launch pattern captured into a hipGraph.
+//
+// Instances share the grid (N_G, gap), the time step (frequency, N_T,
+// N_SUB) and the precision; each has its own driving amplitude, gas
+// pressure and temperature, superparticle weight, capacity, particles,
+// RNG streams, counters, densities and measurements.  The per-instance
+// total cross section is the shared summed table times the instance's gas
+// density (the reference tabulates sigma_tot * n_gas, main.rs:1439-1440).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pic_internal.h"
+
+using namespace pic;
+
+struct pic_ensemble {
+    int E = 0;
+    pic_params_t p0;  // shared geometry / time step / precision
+    std::vector<pic_params_t> params;
+    std::vector<EnsInst> inst_h;
+    EnsInst *inst_d = nullptr;
+    uint32_t e_total = 0, s_total = 0, i_total = 0;
+    float *e[2][4] = {}, *es[2][4] = {}, *ion[2][4] = {};
+    uint32_t *e_rng[4] = {}, *i_rng[4] = {};
+    uint32_t *cnt = nullptr;   // [E][EC_WORDS]
+    uint32_t *clock = nullptr; // device step index within the cycle
+    void *e_dens[2] = {}, *i_dens[2] = {};
+    int inst_stride = 0;       // density entries per instance
+    float *pot = nullptr, *efield = nullptr;
+    double *cumul_e = nullptr, *cumul_i = nullptr;
+    unsigned long long *counters = nullptr;  // [E][8]
+    float *cs = nullptr, *stot_e = nullptr, *stot_i = nullptr;
+    Shape sh;
+    bool f64 = true;
+    int prec = PIC_PREC_FP64;
+    hipStream_t stream = nullptr;
+    int ce = 0, ci = 0, cd = 0, hi = 0;
+    uint32_t t = 0;
+    unsigned long long steps = 0;
+    bool need_prologue = true;
+    bool use_graph = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_roles[2][4] = {};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+#define TRY(x)                         \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_ != PIC_OK) return rc_; \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    PIC_HIP(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+    PIC_HIP(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return PIC_OK;
+}
+
+uint64_t dens_words32(pic_ensemble *s) { return (uint64_t)s->E * s->inst_stride * (s->f64 ? 2 : 1); }
+
+EnsPoissonJob poisson_job(pic_ensemble *s) {
+    const double E = 1.60217662e-19, EPS0 = 8.85418781e-12;
+    EnsPoissonJob j;
+    memset(&j, 0, sizeof(j));
+    j.precision = s->prec;
+    j.g.n_g = s->p0.n_g;
+    j.g.reps = s->sh.replicas;
+    j.g.rep_stride = s->sh.rep_stride;
+    j.g.e_charge = E;
+    j.g.alpha = -s->p0.dx * s->p0.dx / EPS0;
+    j.g.inv_dx = s->p0.inv_dx;
+    j.g.half_inv_dx = 0.5 * s->p0.inv_dx;
+    j.g.half_dx_over_eps = s->p0.dx / (2.0 * EPS0);
+    j.inst = s->inst_d;
+    j.n_inst = s->E;
+    j.inst_stride = s->inst_stride;
+    j.time = s->clock;
+    j.n_t = s->p0.n_t;
+    j.n_sub = s->p0.n_sub;
+    j.pot = s->pot;
+    j.efield = s->efield;
+    j.counters = s->counters;
+    j.cnt = s->cnt;
+    j.zc0 = j.zc1 = j.zci = -1;
+    return j;
+}
+
+int prologue(pic_ensemble *s) {
+    hipStream_t st = s->stream;
+    TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
+    TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
+    TRY(launch_fill(s->i_dens[1], 0u, dens_words32(s), st));
+    Shape sh = s->sh;
+    const size_t es = s->f64 ? 8 : 4;
+    for (int i = 0; i < s->E; ++i) {
+        const EnsInst &in = s->inst_h[i];
+        uint32_t *c = s->cnt + (size_t)i * EC_WORDS;
+        char *ed = (char *)s->e_dens[s->cd] + (size_t)i * s->inst_stride * es;
+        TRY(launch_deposit(s->e[s->ce][0] + in.e_off, c + EC_E0 + s->ce, ed, s->f64, in.k, sh, st, in.e_cap));
+        TRY(launch_deposit(s->es[s->ce][0] + in.s_off, c + EC_S0 + s->ce, ed, s->f64, in.k, sh, st, in.s_cap));
+        for (int b = 0; b < 2; ++b) {
+            char *id = (char *)s->i_dens[b] + (size_t)i * s->inst_stride * es;
+            TRY(launch_deposit(s->ion[s->ci][0] + in.i_off, c + EC_I0 + s->ci, id, s->f64, in.k, sh, st, in.i_cap));
+        }
+    }
+    PIC_HIP(hipMemcpyAsync(s->clock, &s->t, 4, hipMemcpyHostToDevice, st));
+    PIC_HIP(hipStreamSynchronize(st));
+    s->need_prologue = false;
+    return PIC_OK;
+}
+
+void fill_species(EnsFusedJob &f, float *const *src, float *const *dst, pic_ensemble *s) {
+    f.sx = src[0];
+    f.svx = src[1];
+    f.svy = src[2];
+    f.svz = src[3];
+    f.dx = dst[0];
+    f.dvx = dst[1];
+    f.dvy = dst[2];
+    f.dvz = dst[3];
+    (void)s;
+}
+
+int step(pic_ensemble *s, bool measure) {
+    hipStream_t st = s->stream;
+    const bool ion = (s->t % (uint32_t)s->p0.n_sub) == 0;
+    EnsPoissonJob pj = poisson_job(s);
+    pj.e_dens = s->e_dens[s->cd];
+    pj.i_dens = s->i_dens[ion ? 1 - s->hi : s->hi];
+    pj.zero_a = s->e_dens[1 - s->cd];
+    pj.zero_b = s->i_dens[s->hi];
+    pj.zc0 = EC_E0 + (1 - s->ce);
+    pj.zc1 = EC_S0 + (1 - s->ce);
+    pj.zci = EC_I0 + (1 - s->ci);
+    if (measure) {
+        pj.cumul_e = s->cumul_e;
+        pj.cumul_i = s->cumul_i;
+    }
+    TRY(launch_poisson_ens(pj, st));
+    const int i_acc = ion ? s->hi : 1 - s->hi;
+    EnsFusedJob base;
+    memset(&base, 0, sizeof(base));
+    base.inst = s->inst_d;
+    base.n_inst = s->E;
+    base.efield = s->efield;
+    base.cnt = s->cnt;
+    base.inst_stride = s->inst_stride;
+    base.dens_f64 = s->f64;
+    base.cs = s->cs;
+    base.counters = s->counters;
+    base.n_t = s->p0.n_t;
+    if (ion) {
+        EnsFusedJob fi = base;
+        fi.electrons = false;
+        fill_species(fi, s->ion[s->ci], s->ion[1 - s->ci], s);
+        fi.rng0 = s->i_rng[0];
+        fi.rng1 = s->i_rng[1];
+        fi.rng2 = s->i_rng[2];
+        fi.rng3 = s->i_rng[3];
+        fi.c_in = EC_I0 + s->ci;
+        fi.c_out = EC_I0 + (1 - s->ci);
+        fi.dens = s->i_dens[i_acc];
+        fi.stot = s->stot_i;
+        TRY(launch_fused_ens(fi, s->sh, s->p0.n_g, st));
+        s->ci ^= 1;
+    }
+    EnsFusedJob fe = base;
+    fe.electrons = true;
+    fill_species(fe, s->e[s->ce], s->e[1 - s->ce], s);
+    fe.px = s->es[s->ce][0];
+    fe.pvx = s->es[s->ce][1];
+    fe.pvy = s->es[s->ce][2];
+    fe.pvz = s->es[s->ce][3];
+    fe.ox = s->es[1 - s->ce][0];
+    fe.ovx = s->es[1 - s->ce][1];
+    fe.ovy = s->es[1 - s->ce][2];
+    fe.ovz = s->es[1 - s->ce][3];
+    fe.ix = s->ion[s->ci][0];
+    fe.ivx = s->ion[s->ci][1];
+    fe.ivy = s->ion[s->ci][2];
+    fe.ivz = s->ion[s->ci][3];
+    fe.rng0 = s->e_rng[0];
+    fe.rng1 = s->e_rng[1];
+    fe.rng2 = s->e_rng[2];
+    fe.rng3 = s->e_rng[3];
+    fe.c_in = EC_E0 + s->ce;
+    fe.c_out = EC_E0 + (1 - s->ce);
+    fe.s_in = EC_S0 + s->ce;
+    fe.s_out = EC_S0 + (1 - s->ce);
+    fe.c_ion = EC_I0 + s->ci;
+    fe.dens = s->e_dens[1 - s->cd];
+    fe.ion_dens = s->i_dens[i_acc];
+    fe.stot = s->stot_e;
+    fe.time = s->clock;
+    TRY(launch_fused_ens(fe, s->sh, s->p0.n_g, st));
+    s->ce ^= 1;
+    s->cd ^= 1;
+    if (ion) s->hi ^= 1;
+    s->t = (s->t + 1) % (uint32_t)s->p0.n_t;
+    s->steps++;
+    return PIC_OK;
+}
+
+void invalidate_graphs(pic_ensemble *s) {
+    for (int m = 0; m < 2; ++m)
+        if (s->graph[m]) {
+            (void)hipGraphExecDestroy(s->graph[m]);
+            s->graph[m] = nullptr;
+        }
+}
+
+int capture(pic_ensemble *s, bool measure) {
+    const int m = measure ? 1 : 0;
+    const int period = 2 * s->p0.n_sub;
+    int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+    const uint32_t t0 = s->t;
+    const unsigned long long st0 = s->steps;
+    hipGraph_t g = nullptr;
+    PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PIC_OK;
+    for (int i = 0; i < period && rc == PIC_OK; ++i) rc = step(s, measure);
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    s->t = t0;
+    s->steps = st0;
+    s->ce = roles[0];
+    s->ci = roles[1];
+    s->cd = roles[2];
+    s->hi = roles[3];
+    if (rc != PIC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    s->graph[m] = ex;
+    memcpy(s->graph_roles[m], roles, sizeof(roles));
+    return PIC_OK;
+}
+
+int run(pic_ensemble *s, uint32_t n, bool measure) {
+    if (s->need_prologue) TRY(prologue(s));
+    const uint32_t period = 2u * (uint32_t)s->p0.n_sub;
+    const bool graphs = s->use_graph && (uint32_t)s->p0.n_t % period == 0;
+    while (n > 0) {
+        if (graphs && s->t % period == 0 && n >= period) {
+            const int m = measure ? 1 : 0;
+            int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+            if (s->graph[m] && memcmp(roles, s->graph_roles[m], sizeof(roles)) != 0) {
+                (void)hipGraphExecDestroy(s->graph[m]);
+                s->graph[m] = nullptr;
+            }
+            if (!s->graph[m]) TRY(capture(s, measure));
+            while (n >= period) {
+                PIC_HIP(hipGraphLaunch(s->graph[m], s->stream));
+                s->t = (s->t + period) % (uint32_t)s->p0.n_t;
+                s->steps += period;
+                n -= period;
+            }
+            continue;
+        }
+        TRY(step(s, measure));
+        --n;
+    }
+    return PIC_OK;
+}
+
+void free_all(pic_ensemble *s) {
+    invalidate_graphs(s);
+    for (int q = 0; q < 2; ++q)
+        for (int c = 0; c < 4; ++c) {
+            (void)hipFree(s->e[q][c]);
+            (void)hipFree(s->es[q][c]);
+            (void)hipFree(s->ion[q][c]);
+        }
+    for (int c = 0; c < 4; ++c) {
+        (void)hipFree(s->e_rng[c]);
+        (void)hipFree(s->i_rng[c]);
+    }
+    void *bufs[] = {s->inst_d, s->cnt, s->clock, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
+                    s->pot, s->efield, s->cumul_e, s->cumul_i, s->counters, s->cs, s->stot_e, s->stot_i};
+    for (void *b : bufs) (void)hipFree(b);
+    for (int i = 0; i < 2; ++i)
+        if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+bool same_shared(const pic_params_t &a, const pic_params_t &b) {
+    return a.n_g == b.n_g && a.n_t == b.n_t && a.n_sub == b.n_sub && a.freq == b.freq && a.gap == b.gap &&
+           a.precision == b.precision;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
+    if (!c || !out || !c->params || !c->capacity || c->n_instances <= 0 || c->n_instances > 256) {
+        set_error("pic_ens_create: 1..256 instances with params and capacities are required");
+        return PIC_ERR_ARG;
+    }
+    for (int i = 0; i < c->n_instances; ++i)
+        if (c->params[i].dt_e <= 0.0 || !same_shared(c->params[0], c->params[i]) || c->capacity[i] == 0) {
+            set_error("pic_ens_create: instances must share n_g, n_t, n_sub, frequency, gap and precision, "
+                      "have derived params and a capacity");
+            return PIC_ERR_ARG;
+        }
+    pic_ensemble *s = new pic_ensemble();
+    s->E = c->n_instances;
+    s->p0 = c->params[0];
+    s->params.assign(c->params, c->params + s->E);
+    s->prec = s->p0.precision;
+    s->f64 = s->prec == PIC_PREC_FP64;
+    s->use_graph = c->use_graph != 0;
+    Shape sh;
+    sh.block = 256;
+    sh.grid = c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
+    sh.ppt = c->ppt > 0 ? c->ppt : 4;
+    sh.replicas = c->replicas > 0 ? c->replicas : 4;
+    sh.rep_stride = ((s->p0.n_g + 31) / 32) * 32;
+    s->sh = resolve(sh);
+    s->inst_stride = s->sh.replicas * s->sh.rep_stride;
+    s->inst_h.resize(s->E);
+    uint64_t e_tot = 0, s_tot = 0;
+    for (int i = 0; i < s->E; ++i) {
+        EnsInst &in = s->inst_h[i];
+        in.k = make_kphys(s->params[i]);
+        in.k.stot_scale = (float)s->params[i].gas_density;
+        in.v0 = s->params[i].v0;
+        const uint32_t cap = c->capacity[i];
+        const uint32_t scap = std::max<uint32_t>(1024u, cap / 8u);
+        in.e_off = (uint32_t)e_tot;
+        in.e_cap = cap;
+        in.i_off = (uint32_t)e_tot;
+        in.i_cap = cap;
+        in.s_off = (uint32_t)s_tot;
+        in.s_cap = scap;
+        e_tot += cap;
+        s_tot += scap;
+    }
+    if (e_tot > 0xffffffffull || s_tot > 0xffffffffull) {
+        delete s;
+        set_error("pic_ens_create: total capacity above 2^32");
+        return PIC_ERR_ARG;
+    }
+    s->e_total = s->i_total = (uint32_t)e_tot;
+    s->s_total = (uint32_t)s_tot;
+    auto fail = [&](int r) {
+        free_all(s);
+        delete s;
+        return r;
+    };
+    int rc;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "hipStreamCreate"));
+    for (int q = 0; q < 2; ++q)
+        for (int k = 0; k < 4; ++k) {
+            if ((rc = dalloc(&s->e[q][k], s->e_total))) return fail(rc);
+            if ((rc = dalloc(&s->ion[q][k], s->i_total))) return fail(rc);
+            if ((rc = dalloc(&s->es[q][k], s->s_total))) return fail(rc);
+        }
+    for (int k = 0; k < 4; ++k) {
+        if ((rc = dalloc(&s->e_rng[k], s->e_total))) return fail(rc);
+        if ((rc = dalloc(&s->i_rng[k], s->i_total))) return fail(rc);
+    }
+    if ((rc = dalloc(&s->inst_d, s->E))) return fail(rc);
+    if ((rc = dalloc(&s->cnt, (size_t)s->E * EC_WORDS))) return fail(rc);
+    if ((rc = dalloc(&s->clock, 1))) return fail(rc);
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = dalloc((double **)&s->e_dens[q], (size_t)s->E * s->inst_stride))) return fail(rc);
+        if ((rc = dalloc((double **)&s->i_dens[q], (size_t)s->E * s->inst_stride))) return fail(rc);
+    }
+    const size_t gridw = (size_t)s->E * s->p0.n_g;
+    if ((rc = dalloc(&s->pot, gridw))) return fail(rc);
+    if ((rc = dalloc(&s->efield, gridw))) return fail(rc);
+    if ((rc = dalloc(&s->cumul_e, gridw))) return fail(rc);
+    if ((rc = dalloc(&s->cumul_i, gridw))) return fail(rc);
+    if ((rc = dalloc(&s->counters, (size_t)s->E * 8))) return fail(rc);
+    if ((rc = dalloc(&s->cs, (size_t)PIC_N_CS * PIC_CS_RANGES))) return fail(rc);
+    if ((rc = dalloc(&s->stot_e, PIC_CS_RANGES))) return fail(rc);
+    if ((rc = dalloc(&s->stot_i, PIC_CS_RANGES))) return fail(rc);
+    for (int i = 0; i < 2; ++i)
+        if (hipEventCreate(&s->ev[i]) != hipSuccess) return fail(hip_fail(hipGetLastError(), "hipEventCreate"));
+    if (hipMemcpy(s->inst_d, s->inst_h.data(), sizeof(EnsInst) * s->E, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "upload instances"));
+    {
+        // summed cross sections (gas density 1): scaled per instance
+        std::vector<float> cs((size_t)PIC_N_CS * PIC_CS_RANGES), se(PIC_CS_RANGES), si(PIC_CS_RANGES);
+        if ((rc = pic_cross_sections(1.0, cs.data(), se.data(), si.data()))) return fail(rc);
+        if (hipMemcpy(s->cs, cs.data(), cs.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->stot_e, se.data(), se.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->stot_i, si.data(), si.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(hip_fail(hipGetLastError(), "upload tables"));
+    }
+    if ((rc = pic_seed_streams(c->seed_e, s->e_total, s->e_rng[0], s->e_rng[1], s->e_rng[2], s->e_rng[3],
+                               s->stream)))
+        return fail(rc);
+    if ((rc = pic_seed_streams(c->seed_i, s->i_total, s->i_rng[0], s->i_rng[1], s->i_rng[2], s->i_rng[3],
+                               s->stream)))
+        return fail(rc);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "seed"));
+    *out = s;
+    return PIC_OK;
+}
+
+int pic_ens_destroy(pic_ensemble_t *s) {
+    if (!s) return PIC_OK;
+    (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return PIC_OK;
+}
+
+int pic_ens_upload(pic_ensemble_t *s, int inst, int species, const float *x, const float *vx,
+                   const float *vy, const float *vz, uint32_t n) {
+    if (!s || inst < 0 || inst >= s->E || (species != PIC_ELECTRONS && species != PIC_IONS) ||
+        (n && (!x || !vx || !vy || !vz))) {
+        set_error("pic_ens_upload: bad argument");
+        return PIC_ERR_ARG;
+    }
+    const EnsInst &in = s->inst_h[inst];
+    const uint32_t cap = species == PIC_ELECTRONS ? in.e_cap : in.i_cap;
+    if (n > cap) {
+        set_error("pic_ens_upload: more particles than the instance capacity");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    float **a = species == PIC_ELECTRONS ? s->e[s->ce] : s->ion[s->ci];
+    const uint32_t off = species == PIC_ELECTRONS ? in.e_off : in.i_off;
+    const float *src[4] = {x, vx, vy, vz};
+    for (int q = 0; q < 4; ++q)
+        if (n) PIC_HIP(hipMemcpy(a[q] + off, src[q], 4ull * n, hipMemcpyHostToDevice));
+    uint32_t *c = s->cnt + (size_t)inst * EC_WORDS;
+    const uint32_t zero = 0;
+    if (species == PIC_ELECTRONS) {
+        PIC_HIP(hipMemcpy(c + EC_E0 + s->ce, &n, 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(c + EC_S0 + s->ce, &zero, 4, hipMemcpyHostToDevice));
+    } else {
+        PIC_HIP(hipMemcpy(c + EC_I0 + s->ci, &n, 4, hipMemcpyHostToDevice));
+    }
+    s->need_prologue = true;
+    invalidate_graphs(s);
+    return PIC_OK;
+}
+
+int pic_ens_download(pic_ensemble_t *s, int inst, int species, float *x, float *vx, float *vy, float *vz,
+                     uint32_t cap, uint32_t *n) {
+    if (!s || inst < 0 || inst >= s->E || (species != PIC_ELECTRONS && species != PIC_IONS)) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t c[EC_WORDS];
+    PIC_HIP(hipMemcpy(c, s->cnt + (size_t)inst * EC_WORDS, sizeof(c), hipMemcpyDeviceToHost));
+    const EnsInst &in = s->inst_h[inst];
+    float *dst[4] = {x, vx, vy, vz};
+    if (species == PIC_ELECTRONS) {
+        const uint32_t nm = std::min(c[EC_E0 + s->ce], in.e_cap), ns = std::min(c[EC_S0 + s->ce], in.s_cap);
+        if (n) *n = nm + ns;
+        const uint32_t cm = std::min(nm, cap), cs2 = std::min(ns, cap - cm);
+        for (int q = 0; q < 4; ++q) {
+            if (!dst[q]) continue;
+            if (cm) PIC_HIP(hipMemcpy(dst[q], s->e[s->ce][q] + in.e_off, 4ull * cm, hipMemcpyDeviceToHost));
+            if (cs2) PIC_HIP(hipMemcpy(dst[q] + cm, s->es[s->ce][q] + in.s_off, 4ull * cs2, hipMemcpyDeviceToHost));
+        }
+    } else {
+        const uint32_t ni = std::min(c[EC_I0 + s->ci], in.i_cap);
+        if (n) *n = ni;
+        const uint32_t cm = std::min(ni, cap);
+        for (int q = 0; q < 4; ++q)
+            if (dst[q] && cm) PIC_HIP(hipMemcpy(dst[q], s->ion[s->ci][q] + in.i_off, 4ull * cm, hipMemcpyDeviceToHost));
+    }
+    return PIC_OK;
+}
+
+int pic_ens_rng_download(pic_ensemble_t *s, int inst, int species, uint32_t *w0, uint32_t *w1,
+                         uint32_t *w2, uint32_t *w3, uint32_t n) {
+    if (!s || inst < 0 || inst >= s->E || (species != PIC_ELECTRONS && species != PIC_IONS)) return PIC_ERR_ARG;
+    const EnsInst &in = s->inst_h[inst];
+    const uint32_t off = species == PIC_ELECTRONS ? in.e_off : in.i_off;
+    const uint32_t cap = species == PIC_ELECTRONS ? in.e_cap : in.i_cap;
+    if (n > cap) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t **r = species == PIC_ELECTRONS ? s->e_rng : s->i_rng;
+    uint32_t *dst[4] = {w0, w1, w2, w3};
+    for (int q = 0; q < 4; ++q)
+        if (dst[q] && n) PIC_HIP(hipMemcpy(dst[q], r[q] + off, 4ull * n, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+int pic_ens_counts(pic_ensemble_t *s, uint32_t *n_e, uint32_t *n_i) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> c((size_t)s->E * EC_WORDS);
+    PIC_HIP(hipMemcpy(c.data(), s->cnt, 4 * c.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->E; ++i) {
+        const uint32_t *ci = &c[(size_t)i * EC_WORDS];
+        if (n_e) n_e[i] = ci[EC_E0 + s->ce] + ci[EC_S0 + s->ce];
+        if (n_i) n_i[i] = ci[EC_I0 + s->ci];
+    }
+    return PIC_OK;
+}
+
+int pic_ens_run(pic_ensemble_t *s, uint32_t n_steps, int measure) {
+    if (!s) return PIC_ERR_ARG;
+    return run(s, n_steps, measure != 0);
+}
+
+int pic_ens_synchronize(pic_ensemble_t *s) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    return PIC_OK;
+}
+
+int pic_ens_check(pic_ensemble_t *s) {
+    if (!s) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> c((size_t)s->E * EC_WORDS);
+    PIC_HIP(hipMemcpy(c.data(), s->cnt, 4 * c.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->E; ++i)
+        if (c[(size_t)i * EC_WORDS + EC_OVERFLOW]) {
+            set_error("ensemble instance " + std::to_string(i) + ": capacity exceeded (particles dropped)");
+            return PIC_ERR_CAPACITY;
+        }
+    return PIC_OK;
+}
+
+int pic_ens_time_steps(pic_ensemble_t *s, uint32_t n_steps, int measure, float *ms_total,
+                       unsigned long long *particle_steps) {
+    if (!s) return PIC_ERR_ARG;
+    if (s->need_prologue) TRY(prologue(s));
+    std::vector<uint32_t> ne0(s->E), ni0(s->E), ne1(s->E), ni1(s->E);
+    TRY(pic_ens_counts(s, ne0.data(), ni0.data()));
+    uint32_t ion_steps = 0;
+    for (uint32_t i = 0, tt = s->t; i < n_steps; ++i, tt = (tt + 1) % (uint32_t)s->p0.n_t)
+        ion_steps += (tt % (uint32_t)s->p0.n_sub) == 0;
+    PIC_HIP(hipEventRecord(s->ev[0], s->stream));
+    TRY(run(s, n_steps, measure != 0));
+    PIC_HIP(hipEventRecord(s->ev[1], s->stream));
+    PIC_HIP(hipEventSynchronize(s->ev[1]));
+    float ms = 0.0f;
+    PIC_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+    TRY(pic_ens_counts(s, ne1.data(), ni1.data()));
+    double ps = 0.0;
+    for (int i = 0; i < s->E; ++i)
+        ps += 0.5 * ((double)ne0[i] + ne1[i]) * n_steps + 0.5 * ((double)ni0[i] + ni1[i]) * ion_steps;
+    if (ms_total) *ms_total = ms;
+    if (particle_steps) *particle_steps = (unsigned long long)ps;
+    return PIC_OK;
+}
+
+int pic_ens_read_grid(pic_ensemble_t *s, int inst, float *pot, float *efield) {
+    if (!s || inst < 0 || inst >= s->E) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    const size_t n = s->p0.n_g, off = (size_t)inst * n;
+    if (pot) PIC_HIP(hipMemcpy(pot, s->pot + off, 4 * n, hipMemcpyDeviceToHost));
+    if (efield) PIC_HIP(hipMemcpy(efield, s->efield + off, 4 * n, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+int pic_ens_read_measurements(pic_ensemble_t *s, int inst, double *cumul_e, double *cumul_i,
+                              unsigned long long *counters) {
+    if (!s || inst < 0 || inst >= s->E) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    const size_t n = s->p0.n_g, off = (size_t)inst * n;
+    if (cumul_e) PIC_HIP(hipMemcpy(cumul_e, s->cumul_e + off, 8 * n, hipMemcpyDeviceToHost));
+    if (cumul_i) PIC_HIP(hipMemcpy(cumul_i, s->cumul_i + off, 8 * n, hipMemcpyDeviceToHost));
+    if (counters) PIC_HIP(hipMemcpy(counters, s->counters + (size_t)inst * 8, 64, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -295,3 +295,263 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
 }
 
 }  // namespace pic
+
+// ============================================================ ensembles
+// The fused particle step for an ensemble of independent discharges in one
+// persistent launch.  Every instance's live particles are cut into tiles; a
+// per-instance tile prefix (in LDS) maps the global tile index to
+// (instance, local tile).  Each workgroup takes a contiguous range of global
+// tiles, so it meets few instance boundaries: at each boundary it flushes
+// its LDS histogram into the finished instance's density, stages the next
+// instance's field and loads its physics (wall position, weights, gas
+// density as the cross-section scale).  Compaction, RNG slots, spawns and
+// densities are per instance, exactly as in the single-discharge kernel.
+namespace pic {
+
+constexpr int kMaxEnsemble = 256;
+
+struct EnsFusedArgs {
+    EnsFusedJob j;
+    int n_g, lds_reps, reps, rep_stride;
+};
+
+template <bool ELECTRON, int PPT, typename Acc>
+__global__ __launch_bounds__(256) void k_fused_ens(EnsFusedArgs A) {
+    const EnsFusedJob &j = A.j;
+    extern __shared__ __align__(16) float sh[];
+    __shared__ uint32_t s_tiles[kMaxEnsemble + 1], s_nmain[kMaxEnsemble], s_n[kMaxEnsemble];
+    __shared__ uint32_t s_wave[32], s_base[2];
+    const int n_g = A.n_g;
+    const int E = j.n_inst;
+    float *ef = sh;
+    float *hist = sh + n_g;
+    const int wave = threadIdx.x / kWave;
+    float *h = hist + (wave % A.lds_reps) * n_g;
+    const uint32_t tile_sz = blockDim.x * PPT;
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+        const EnsInst &in = j.inst[i];
+        const uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
+        const uint32_t nm = min(c[j.c_in], ELECTRON ? in.e_cap : in.i_cap);
+        const uint32_t ns = ELECTRON ? min(c[j.s_in], in.s_cap) : 0u;
+        s_nmain[i] = nm;
+        s_n[i] = nm + ns;
+        s_tiles[i + 1] = (nm + ns + tile_sz - 1) / tile_sz;
+    }
+    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        s_tiles[0] = 0;
+        for (int i = 0; i < E; ++i) {
+            acc += s_tiles[i + 1];
+            s_tiles[i + 1] = acc;
+        }
+        if (ELECTRON && j.time && blockIdx.x == 0) *j.time = (*j.time + 1u) % (uint32_t)j.n_t;
+    }
+    __syncthreads();
+    const uint32_t T = s_tiles[E];
+    const uint32_t t_begin = (uint32_t)((uint64_t)T * blockIdx.x / gridDim.x);
+    const uint32_t t_end = (uint32_t)((uint64_t)T * (blockIdx.x + 1) / gridDim.x);
+
+    int cur = -1;
+    KPhys k = j.inst[0].k;
+    uint32_t p_off = 0, p_cap = 0, s_off = 0, s_cap = 0, i_off = 0, i_cap = 0, n_main = 0, n = 0;
+    uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
+    bool overflow = false;
+    int buf = 0;
+    // flush the workgroup's histogram and counters into instance `who`
+    auto flush = [&](int who) {
+        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
+        if (lane_id() == 0) {
+            unsigned long long *cn = j.counters + (size_t)who * 8;
+            const int b = ELECTRON ? 0 : 2;
+            if (wp) atomicAdd(&cn[b], (unsigned long long)wp);
+            if (wg) atomicAdd(&cn[b + 1], (unsigned long long)wg);
+            if (wc) atomicAdd(&cn[ELECTRON ? 4 : 5], (unsigned long long)wc);
+        }
+        if (overflow) j.cnt[(size_t)who * EC_WORDS + EC_OVERFLOW] = 1u;
+        lost_pow = lost_gnd = ncoll = 0;
+        overflow = false;
+        __syncthreads();
+        Acc *out = (Acc *)j.dens + (size_t)who * j.inst_stride + (blockIdx.x % A.reps) * A.rep_stride;
+        for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
+            float s = 0.0f;
+            for (int r = 0; r < A.lds_reps; ++r) s += hist[r * n_g + q];
+            if (s != 0.0f) {
+                Acc v = (Acc)s;
+                if (q == 0 || q == n_g - 1) v *= (Acc)2;
+                unsafeAtomicAdd(&out[q], v);
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    };
+
+    for (uint32_t tile = t_begin; tile < t_end; ++tile, buf ^= 1) {
+        int i = cur < 0 ? 0 : cur;
+        while (tile >= s_tiles[i + 1]) ++i;
+        if (i != cur) {
+            if (cur >= 0) flush(cur);
+            __syncthreads();
+            for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[(size_t)i * n_g + q];
+            __syncthreads();
+            cur = i;
+            const EnsInst &in = j.inst[i];
+            k = in.k;
+            p_off = ELECTRON ? in.e_off : in.i_off;
+            p_cap = ELECTRON ? in.e_cap : in.i_cap;
+            s_off = in.s_off;
+            s_cap = in.s_cap;
+            i_off = in.i_off;
+            i_cap = in.i_cap;
+            n_main = s_nmain[i];
+            n = s_n[i];
+        }
+        const float factor = ELECTRON ? k.factor_e : k.factor_i;
+        const float dt = ELECTRON ? k.dt_e : k.dt_i;
+        const uint32_t local = tile - s_tiles[i];
+        float nx[PPT], nvx[PPT], vy[PPT], vz[PPT];
+        bool keep[PPT];
+        uint32_t off[PPT];
+        const uint32_t base = local * tile_sz + threadIdx.x;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint32_t v = base + u * blockDim.x;
+            keep[u] = false;
+            if (v < n) {
+                const float *X = j.sx, *VX = j.svx, *VY = j.svy, *VZ = j.svz;
+                uint32_t g = p_off + v;
+                if (ELECTRON && v >= n_main) {
+                    X = j.px;
+                    VX = j.pvx;
+                    VY = j.pvy;
+                    VZ = j.pvz;
+                    g = s_off + (v - n_main);
+                }
+                const float x = X[g];
+                const float e = field_at(ef, x, k.inv_dx, n_g);
+                nvx[u] = __builtin_fmaf(factor, e, VX[g]);
+                nx[u] = __builtin_fmaf(nvx[u], dt, x);
+                keep[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
+                if (keep[u]) {
+                    vy[u] = VY[g];
+                    vz[u] = VZ[g];
+                } else {
+                    const bool pw = nx[u] < 0.0f;
+                    lost_pow += pw;
+                    lost_gnd += !pw;
+                }
+            }
+        }
+        uint32_t wcount = 0;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint64_t m = ballot64(keep[u]);
+            off[u] = wcount + mbcnt64(m);
+            wcount += (uint32_t)__popcll(m);
+        }
+        const uint32_t wbase =
+            tile_reserve_f(wcount, s_wave, s_base, buf, j.cnt + (size_t)i * EC_WORDS + j.c_out);
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            int kind = kNone;
+            float se[3], si[3];
+            if (keep[u]) {
+                const uint32_t slot = wbase + off[u];
+                if (slot < p_cap) {
+                    const uint32_t g = p_off + slot;
+                    Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, g);
+                    float vx = nvx[u], y = vy[u], z = vz[u];
+                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
+                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s);
+                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, g, s);
+                    j.dx[g] = nx[u];
+                    j.dvx[g] = vx;
+                    j.dvy[g] = y;
+                    j.dvz[g] = z;
+                    deposit_lds<Acc>(h, nx[u], k);
+                    ncoll += kind != kNone;
+                } else {
+                    overflow = true;
+                }
+            }
+            if (ELECTRON) {
+                const uint64_t sp = ballot64(kind == kIonisation);
+                if (sp) {
+                    const int leader = __builtin_ctzll(sp);
+                    uint32_t be = 0, bi = 0;
+                    if ((int)lane_id() == leader) {
+                        const uint32_t cnt = (uint32_t)__popcll(sp);
+                        uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
+                        be = atomicAdd(&c[j.s_out], cnt);
+                        bi = atomicAdd(&c[j.c_ion], cnt);
+                    }
+                    be = __shfl(be, leader, kWave);
+                    bi = __shfl(bi, leader, kWave);
+                    if (kind == kIonisation) {
+                        const uint32_t o = mbcnt64(sp);
+                        const uint32_t es = be + o, is = bi + o;
+                        const float x = nx[u];
+                        if (es < s_cap) {
+                            j.ox[s_off + es] = x;
+                            j.ovx[s_off + es] = se[0];
+                            j.ovy[s_off + es] = se[1];
+                            j.ovz[s_off + es] = se[2];
+                            deposit_lds<Acc>(h, x, k);
+                        } else {
+                            overflow = true;
+                        }
+                        if (is < i_cap) {
+                            j.ix[i_off + is] = x;
+                            j.ivx[i_off + is] = si[0];
+                            j.ivy[i_off + is] = si[1];
+                            j.ivz[i_off + is] = si[2];
+                            deposit_global<Acc>((Acc *)j.ion_dens + (size_t)i * j.inst_stride +
+                                                    (blockIdx.x % A.reps) * A.rep_stride,
+                                                x, k);
+                        } else {
+                            overflow = true;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (cur >= 0) flush(cur);
+}
+
+template <bool E, int PPT>
+static void launch_ens_t(const EnsFusedArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
+    if (f64) k_fused_ens<E, PPT, double><<<grid, block, lds, s>>>(a);
+    else k_fused_ens<E, PPT, float><<<grid, block, lds, s>>>(a);
+}
+
+int launch_fused_ens(const EnsFusedJob &j, const Shape &sh0, int n_g, hipStream_t s) {
+    if (j.n_inst <= 0 || j.n_inst > kMaxEnsemble) {
+        set_error("ensemble: 1 to 256 instances");
+        return PIC_ERR_ARG;
+    }
+    Shape sh = resolve(sh0);
+    EnsFusedArgs a;
+    a.j = j;
+    a.n_g = n_g;
+    const int block = 256;
+    const int budget = 40 * 1024;
+    a.lds_reps = std::max(1, std::min(block / kWave, (budget - n_g * 4) / (n_g * 4)));
+    a.reps = sh.replicas;
+    a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : n_g;
+    const size_t lds = (size_t)n_g * 4 * (1 + a.lds_reps);
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    const bool f64 = j.dens_f64;
+    if (j.electrons) {
+        if (sh.ppt == 2) launch_ens_t<true, 2>(a, f64, grid, block, lds, s);
+        else launch_ens_t<true, 4>(a, f64, grid, block, lds, s);
+    } else {
+        if (sh.ppt == 2) launch_ens_t<false, 2>(a, f64, grid, block, lds, s);
+        else launch_ens_t<false, 4>(a, f64, grid, block, lds, s);
+    }
+    PIC_LAUNCHED("k_fused_ens");
+    return PIC_OK;
+}
+
+}  // namespace pic

@@ -60,10 +60,11 @@ extern "C" int pic_cross_sections(double gas_density, float *cs, float *stot_e, 
     return PIC_OK;
 }
 
-extern "C" int pic_abi_layout(uint64_t out[3]) {
+extern "C" int pic_abi_layout(uint64_t out[4]) {
     if (!out) return PIC_ERR_ARG;
     out[0] = sizeof(pic_params_t);
     out[1] = sizeof(pic_species_t);
     out[2] = sizeof(pic_sim_config_t);
+    out[3] = sizeof(pic_ensemble_config_t);
     return PIC_OK;
 }

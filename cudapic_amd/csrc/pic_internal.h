@@ -40,6 +40,12 @@ struct Shape {
 };
 Shape resolve(Shape s);
 
+// Grid geometry and constants of one Poisson solve.
+struct PoissonGeom {
+    int n_g, per, reps, rep_stride;
+    double e_charge, alpha, inv_dx, half_inv_dx, half_dx_over_eps;
+};
+
 // Poisson solve request (one workgroup of 1024 lanes).
 struct PoissonJob {
     int precision = PIC_PREC_FP64;
@@ -143,5 +149,60 @@ struct FusedJob {
     uint32_t *ifed;                        // ions only
 };
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+
+// ---------------------------------------------------------------- ensembles
+// One independent discharge of an ensemble: its physics, its driving
+// amplitude and the regions it owns in the pooled particle arrays.
+struct EnsInst {
+    KPhys k;  // k.stot_scale = the instance's gas density
+    double v0;
+    uint32_t e_off, e_cap;  // electrons (and their RNG slots)
+    uint32_t s_off, s_cap;  // newborn-electron staging
+    uint32_t i_off, i_cap;  // ions (and their RNG slots)
+};
+
+// per-instance device counters (uint32 words)
+enum EnsCnt { EC_E0 = 0, EC_S0 = 2, EC_I0 = 4, EC_OVERFLOW = 6, EC_WORDS = 8 };
+
+struct EnsPoissonJob {
+    int precision;
+    PoissonGeom g;               // reps / rep_stride of one instance's density buffer
+    const EnsInst *inst;
+    int n_inst;
+    const void *e_dens, *i_dens; // [n_inst][inst_stride]
+    int inst_stride;             // entries per instance
+    const uint32_t *time;
+    int n_t, n_sub;
+    float *pot, *efield;         // [n_inst][n_g]
+    double *cumul_e, *cumul_i;   // [n_inst][n_g] or null
+    unsigned long long *counters;  // [n_inst][8]: [6],[7] measured steps
+    void *zero_a, *zero_b;       // next accumulators [n_inst][inst_stride] (zero_b on ion steps)
+    uint32_t *cnt;               // [n_inst][EC_WORDS]
+    int zc0, zc1, zci;           // counter words cleared (zci on ion steps only)
+};
+int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s);
+
+struct EnsFusedJob {
+    bool electrons;
+    const EnsInst *inst;
+    int n_inst;
+    const float *efield;                   // [n_inst][n_g]
+    const float *sx, *svx, *svy, *svz;     // pooled input
+    const float *px, *pvx, *pvy, *pvz;     // pooled staging input (electrons)
+    float *dx, *dvx, *dvy, *dvz;           // pooled output
+    float *ox, *ovx, *ovy, *ovz;           // pooled staging output (electrons)
+    float *ix, *ivx, *ivy, *ivz;           // pooled ions (appends)
+    uint32_t *rng0, *rng1, *rng2, *rng3;   // pooled per-slot streams of this species
+    uint32_t *cnt;                         // [n_inst][EC_WORDS]
+    int c_in, c_out, s_in, s_out, c_ion;   // counter words
+    void *dens, *ion_dens;                 // [n_inst][inst_stride]
+    int inst_stride;
+    bool dens_f64;
+    const float *stot, *cs;                // stot: summed cross sections (scaled per instance)
+    unsigned long long *counters;          // [n_inst][8]
+    uint32_t *time;                        // advanced by one step (electron kernel)
+    int n_t;
+};
+int launch_fused_ens(const EnsFusedJob &j, const Shape &sh, int n_g, hipStream_t s);
 
 }  // namespace pic

@@ -78,6 +78,7 @@ KPhys make_kphys(const pic_params_t &p) {
     k.max_x = p.max_x_f;
     k.de_eepf = p.de_eepf_f;
     k.eepf_kick = p.eepf_kick_f;
+    k.stot_scale = 1.0f;
     return k;
 }
 
@@ -289,12 +290,11 @@ constexpr int kPoissonMaxPer = 4;  // n_g <= 4096
 
 struct PoissonArgs {
     const void *e_dens, *i_dens;
-    int reps, rep_stride, n_g, per;
+    PoissonGeom g;
     double pot0;
     const uint32_t *time;
     int n_t, n_sub, n_harm;
     double wave_cos[PIC_MAX_HARMONICS], wave_sin[PIC_MAX_HARMONICS];
-    double e_charge, alpha, inv_dx, half_inv_dx, half_dx_over_eps;
     float *pot, *efield;
     double *cumul_e, *cumul_i;
     unsigned long long *meas_steps;
@@ -314,48 +314,37 @@ __device__ __forceinline__ double load_dens(const D *p, int reps, int stride, in
     return s;
 }
 
+// pot0 of step t: sum_h a_h cos(2 pi (h+1) t / N_T) + b_h sin(...), the
+// phase reduced exactly in integers before sincospi (main.rs:1741).
+__device__ double waveform(uint32_t t, int n_t, int n_harm, const double *wc, const double *ws) {
+    double p0 = 0.0;
+    for (int h = 0; h < n_harm; ++h) {
+        const uint64_t m = (2ull * (uint64_t)(h + 1) * t) % (2ull * (uint64_t)n_t);
+        double sv, cv;
+        sincospi((double)m / (double)n_t, &sv, &cv);
+        p0 += wc[h] * cv + ws[h] * sv;
+    }
+    return p0;
+}
+
 // solve_poisson_scan_f32 (main.rs:408-494) generalised to three
-// precisions and n_g <= 4096 (up to four nodes per lane).  The tridiagonal
-// Dirichlet system with A = C = 1, B = -2 is solved as two inclusive scans:
+// precisions and n_g <= 4096 (up to four nodes per lane), for one discharge
+// by one workgroup of 1024 lanes.  The tridiagonal Dirichlet system with
+// A = C = 1, B = -2 is solved as two inclusive scans:
 //   G_i = sum_{j<=i} j f_j,  r_i = -G_i / ((i+1) i),  R = scan(r),
 //   pot_i = i (R_{N-2} - R_i + r_i),
 // then E by central differences with the electrode formulas of
-// C/eduPIC.cc:491-493.
+// C/eduPIC.cc:491-493.  pot_s holds n_g values of T, wtot 17.
 template <typename T, typename D>
-__global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    T *pot_s = reinterpret_cast<T *>(smem);               // n_g
-    T *wtot = pot_s + a.n_g;                              // 16
-    __shared__ double s_pot0;
-    __shared__ int s_ion;
+__device__ void poisson_solve(const PoissonGeom &pg, const D *ed, const D *id, double pot0_d, bool ion,
+                              float *pot, float *efield, double *cumul_e, double *cumul_i, T *pot_s,
+                              T *wtot) {
     const int tid = threadIdx.x;
-    const int n_g = a.n_g;
-    const int per = a.per;
-    if (tid == 0) {
-        double p0 = a.pot0;
-        int ion = 1;
-        if (a.time) {
-            const uint32_t t = *a.time;
-            ion = (t % (uint32_t)a.n_sub) == 0;
-            p0 = 0.0;
-            for (int h = 0; h < a.n_harm; ++h) {
-                // 2 pi (h+1) t / N_T reduced exactly in integers first
-                const uint64_t m = (2ull * (uint64_t)(h + 1) * t) % (2ull * (uint64_t)a.n_t);
-                double sv, cv;
-                sincospi((double)m / (double)a.n_t, &sv, &cv);
-                p0 += a.wave_cos[h] * cv + a.wave_sin[h] * sv;
-            }
-        }
-        s_pot0 = p0;
-        s_ion = ion;
-    }
-    __syncthreads();
-    const T pot0 = Num<T>::from(s_pot0);
-    const T alpha = Num<T>::from(a.alpha);
-    const T echarge = Num<T>::from(a.e_charge);
-
-    const D *ed = (const D *)a.e_dens;
-    const D *id = (const D *)a.i_dens;
+    const int n_g = pg.n_g;
+    const int per = pg.per;
+    const T pot0 = Num<T>::from(pot0_d);
+    const T alpha = Num<T>::from(pg.alpha);
+    const T echarge = Num<T>::from(pg.e_charge);
     T rho[kPoissonMaxPer], g[kPoissonMaxPer];
     const int k0 = tid * per;
     // densities (summed over the global replicas), charge density, g_i = i f_i
@@ -367,12 +356,12 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
         g[u] = Num<T>::zero();
         if (u < per && k < n_g) {
             T ne, ni;
-            if (a.reps == 1) {
+            if (pg.reps == 1) {
                 ne = Num<T>::from((double)ed[k]);
                 ni = Num<T>::from((double)id[k]);
             } else {
-                ne = Num<T>::from(load_dens(ed, a.reps, a.rep_stride, k));
-                ni = Num<T>::from(load_dens(id, a.reps, a.rep_stride, k));
+                ne = Num<T>::from(load_dens(ed, pg.reps, pg.rep_stride, k));
+                ni = Num<T>::from(load_dens(id, pg.reps, pg.rep_stride, k));
             }
             rho[u] = echarge * (ni - ne);
             if (k >= 1 && k <= n_g - 2) {
@@ -420,9 +409,9 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
         }
     }
     __syncthreads();
-    const T inv_dx = Num<T>::from(a.inv_dx);
-    const T half_inv_dx = Num<T>::from(a.half_inv_dx);
-    const T hdx_eps = Num<T>::from(a.half_dx_over_eps);
+    const T inv_dx = Num<T>::from(pg.inv_dx);
+    const T half_inv_dx = Num<T>::from(pg.half_inv_dx);
+    const T hdx_eps = Num<T>::from(pg.half_dx_over_eps);
 #pragma unroll
     for (int u = 0; u < kPoissonMaxPer; ++u) {
         const int k = k0 + u;
@@ -432,14 +421,38 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
             if (k == 0) e = (pk - pot_s[1]) * inv_dx - rho[u] * hdx_eps;
             else if (k == n_g - 1) e = (pot_s[n_g - 2] - pk) * inv_dx + rho[u] * hdx_eps;
             else e = (pot_s[k - 1] - pot_s[k + 1]) * half_inv_dx;
-            a.pot[k] = Num<T>::to_f(pk);
-            a.efield[k] = Num<T>::to_f(e);
-            if (a.cumul_e) {
-                a.cumul_e[k] += a.reps == 1 ? (double)ed[k] : load_dens(ed, a.reps, a.rep_stride, k);
-                if (s_ion) a.cumul_i[k] += a.reps == 1 ? (double)id[k] : load_dens(id, a.reps, a.rep_stride, k);
+            pot[k] = Num<T>::to_f(pk);
+            efield[k] = Num<T>::to_f(e);
+            if (cumul_e) {
+                cumul_e[k] += pg.reps == 1 ? (double)ed[k] : load_dens(ed, pg.reps, pg.rep_stride, k);
+                if (ion) cumul_i[k] += pg.reps == 1 ? (double)id[k] : load_dens(id, pg.reps, pg.rep_stride, k);
             }
         }
     }
+}
+
+template <typename T, typename D>
+__global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *pot_s = reinterpret_cast<T *>(smem);  // n_g
+    T *wtot = pot_s + a.g.n_g;               // 17
+    __shared__ double s_pot0;
+    __shared__ int s_ion;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        double p0 = a.pot0;
+        int ion = 1;
+        if (a.time) {
+            const uint32_t t = *a.time;
+            ion = (t % (uint32_t)a.n_sub) == 0;
+            p0 = waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
+        }
+        s_pot0 = p0;
+        s_ion = ion;
+    }
+    __syncthreads();
+    poisson_solve<T, D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, s_pot0, s_ion != 0, a.pot, a.efield,
+                        a.cumul_e, a.cumul_i, pot_s, wtot);
     // fused-path bookkeeping for the particle kernels that follow
     for (int w = tid; w < a.zero_a_words; w += blockDim.x) a.zero_a[w] = 0u;
     if (s_ion)
@@ -463,10 +476,10 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     memset(&a, 0, sizeof(a));
     a.e_dens = j.e_dens;
     a.i_dens = j.i_dens;
-    a.reps = std::max(1, j.replicas);
-    a.rep_stride = j.rep_stride > 0 ? j.rep_stride : j.n_g;
-    a.n_g = j.n_g;
-    a.per = (j.n_g + kPoissonThreads - 1) / kPoissonThreads;
+    a.g.reps = std::max(1, j.replicas);
+    a.g.rep_stride = j.rep_stride > 0 ? j.rep_stride : j.n_g;
+    a.g.n_g = j.n_g;
+    a.g.per = (j.n_g + kPoissonThreads - 1) / kPoissonThreads;
     a.pot0 = j.pot0;
     a.time = j.time;
     a.n_t = j.n_t;
@@ -477,11 +490,11 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
         a.wave_sin[h] = j.wave_sin[h];
         if (j.wave_cos[h] != 0.0 || j.wave_sin[h] != 0.0) a.n_harm = h + 1;
     }
-    a.e_charge = j.e_charge;
-    a.alpha = j.alpha;
-    a.inv_dx = j.inv_dx;
-    a.half_inv_dx = j.half_inv_dx;
-    a.half_dx_over_eps = j.half_dx_over_eps;
+    a.g.e_charge = j.e_charge;
+    a.g.alpha = j.alpha;
+    a.g.inv_dx = j.inv_dx;
+    a.g.half_inv_dx = j.half_inv_dx;
+    a.g.half_dx_over_eps = j.half_dx_over_eps;
     a.pot = j.pot;
     a.efield = j.efield;
     a.cumul_e = j.cumul_e;
@@ -515,6 +528,105 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
         return PIC_ERR_ARG;
     }
     PIC_LAUNCHED("k_poisson");
+    return PIC_OK;
+}
+
+// Ensemble: one workgroup per discharge, all discharges in one launch.
+struct EnsPoissonArgs {
+    PoissonGeom g;
+    const EnsInst *inst;
+    const void *e_dens, *i_dens;
+    int inst_stride;
+    const uint32_t *time;
+    int n_t, n_sub;
+    float *pot, *efield;
+    double *cumul_e, *cumul_i;
+    unsigned long long *counters;
+    uint32_t *zero_a, *zero_b;
+    int zero_words;  // 32-bit words per instance
+    uint32_t *cnt;
+    int zc0, zc1, zci;
+};
+
+template <typename T, typename D>
+__global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *pot_s = reinterpret_cast<T *>(smem);
+    T *wtot = pot_s + a.g.n_g;
+    __shared__ double s_pot0;
+    __shared__ int s_ion;
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int n_g = a.g.n_g;
+    if (tid == 0) {
+        const uint32_t t = *a.time;
+        s_ion = (t % (uint32_t)a.n_sub) == 0;
+        const double wc = a.inst[b].v0, ws = 0.0;
+        s_pot0 = waveform(t, a.n_t, 1, &wc, &ws);
+    }
+    __syncthreads();
+    const bool ion = s_ion != 0;
+    const size_t off = (size_t)b * a.inst_stride;
+    poisson_solve<T, D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, s_pot0, ion,
+                        a.pot + (size_t)b * n_g, a.efield + (size_t)b * n_g,
+                        a.cumul_e ? a.cumul_e + (size_t)b * n_g : nullptr,
+                        a.cumul_i ? a.cumul_i + (size_t)b * n_g : nullptr, pot_s, wtot);
+    const size_t zoff = (size_t)b * a.zero_words;
+    for (int w = tid; w < a.zero_words; w += blockDim.x) a.zero_a[zoff + w] = 0u;
+    if (ion)
+        for (int w = tid; w < a.zero_words; w += blockDim.x) a.zero_b[zoff + w] = 0u;
+    uint32_t *c = a.cnt + (size_t)b * EC_WORDS;
+    if (tid == 0 && a.zc0 >= 0) c[a.zc0] = 0u;
+    if (tid == 1 && a.zc1 >= 0) c[a.zc1] = 0u;
+    if (tid == 2 && ion && a.zci >= 0) c[a.zci] = 0u;
+    if (tid == 3 && a.cumul_e) {
+        a.counters[(size_t)b * 8 + 6] += 1ull;
+        if (ion) a.counters[(size_t)b * 8 + 7] += 1ull;
+    }
+}
+
+int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
+    if (j.g.n_g < 3 || j.g.n_g > kPoissonThreads * kPoissonMaxPer || j.n_inst <= 0) {
+        set_error("poisson_ens: bad arguments");
+        return PIC_ERR_ARG;
+    }
+    EnsPoissonArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = j.g;
+    a.g.per = (j.g.n_g + kPoissonThreads - 1) / kPoissonThreads;
+    a.inst = j.inst;
+    a.e_dens = j.e_dens;
+    a.i_dens = j.i_dens;
+    a.inst_stride = j.inst_stride;
+    a.time = j.time;
+    a.n_t = j.n_t;
+    a.n_sub = j.n_sub;
+    a.pot = j.pot;
+    a.efield = j.efield;
+    a.cumul_e = j.cumul_e;
+    a.cumul_i = j.cumul_i;
+    a.counters = j.counters;
+    a.zero_a = (uint32_t *)j.zero_a;
+    a.zero_b = (uint32_t *)j.zero_b;
+    a.cnt = j.cnt;
+    a.zc0 = j.zc0;
+    a.zc1 = j.zc1;
+    a.zci = j.zci;
+    switch (j.precision) {
+    case PIC_PREC_FP32:
+        a.zero_words = j.inst_stride;
+        k_poisson_ens<float, float><<<j.n_inst, kPoissonThreads, (j.g.n_g + 17) * sizeof(float), s>>>(a);
+        break;
+    case PIC_PREC_COMP:
+        a.zero_words = j.inst_stride;
+        k_poisson_ens<ff, float><<<j.n_inst, kPoissonThreads, (j.g.n_g + 17) * sizeof(ff), s>>>(a);
+        break;
+    default:
+        a.zero_words = 2 * j.inst_stride;
+        k_poisson_ens<double, double><<<j.n_inst, kPoissonThreads, (j.g.n_g + 17) * sizeof(double), s>>>(a);
+        break;
+    }
+    PIC_LAUNCHED("k_poisson_ens");
     return PIC_OK;
 }
 
@@ -1309,6 +1421,28 @@ int pic_seed_streams(const uint32_t master[4], uint32_t n, uint32_t *rng0, uint3
     k_seed_streams<<<grid, 256, 0, S(stream)>>>(mats, master[0], master[1], master[2], master[3], n,
                                                 rng0, rng1, rng2, rng3);
     PIC_LAUNCHED("k_seed_streams");
+    return PIC_OK;
+}
+
+// Streaming device copy (16-byte vector loads and stores): the calibration
+// kernel for the HBM traffic counters and a measure of achievable bandwidth.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+int pic_stream_copy(void *dst, const void *src, uint64_t bytes, pic_stream_t stream) {
+    if (!dst || !src || (bytes % 16) != 0 || ((uintptr_t)dst % 16) != 0 || ((uintptr_t)src % 16) != 0) {
+        set_error("pic_stream_copy: pointers and size must be 16-byte aligned");
+        return PIC_ERR_ARG;
+    }
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return PIC_OK;
+    const int grid = (int)std::min<uint64_t>((n16 + 255) / 256, (uint64_t)device_cus() * 16);
+    k_stream_copy<<<grid, 256, 0, S(stream)>>>((const uint4 *)src, (uint4 *)dst, n16);
+    PIC_LAUNCHED("k_stream_copy");
     return PIC_OK;
 }
 

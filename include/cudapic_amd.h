@@ -248,6 +248,46 @@ int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_to
 int pic_sim_save(pic_sim_t *s, const char *path);
 int pic_sim_load(pic_sim_t *s, const char *path);
 
+/* ------------------------------------------------------------------------
+ * Ensembles of independent discharges (one launch per stage for all)
+ * ---------------------------------------------------------------------- */
+
+/* Instances share n_g, n_t, n_sub, frequency, gap and precision (taken from
+ * params[0]); each has its own v0, pressure, gas temperature, weight and
+ * capacity.  No reference counterpart: the reference runs one discharge per
+ * process; this is the batched mode of the same step. */
+typedef struct pic_ensemble_config {
+    int32_t n_instances;          /* 1..256 */
+    const pic_params_t *params;   /* [n_instances], derived */
+    const uint32_t *capacity;     /* [n_instances] particles per species */
+    uint32_t seed_e[4], seed_i[4];
+    int32_t use_graph, grid_blocks, ppt, replicas;
+} pic_ensemble_config_t;
+
+typedef struct pic_ensemble pic_ensemble_t;
+
+int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out);
+int pic_ens_destroy(pic_ensemble_t *e);
+int pic_ens_upload(pic_ensemble_t *e, int inst, int species, const float *x, const float *vx,
+                   const float *vy, const float *vz, uint32_t n);
+int pic_ens_download(pic_ensemble_t *e, int inst, int species, float *x, float *vx, float *vy,
+                     float *vz, uint32_t cap, uint32_t *n);
+/* RNG words of the first n slots of one instance's region. */
+int pic_ens_rng_download(pic_ensemble_t *e, int inst, int species, uint32_t *w0, uint32_t *w1,
+                         uint32_t *w2, uint32_t *w3, uint32_t n);
+/* n_e[i], n_i[i] for every instance (synchronises). */
+int pic_ens_counts(pic_ensemble_t *e, uint32_t *n_e, uint32_t *n_i);
+int pic_ens_run(pic_ensemble_t *e, uint32_t n_steps, int measure);
+int pic_ens_synchronize(pic_ensemble_t *e);
+int pic_ens_check(pic_ensemble_t *e);
+/* Wall time (HIP events) of n_steps over all instances and the particle-steps done. */
+int pic_ens_time_steps(pic_ensemble_t *e, uint32_t n_steps, int measure, float *ms_total,
+                       unsigned long long *particle_steps);
+int pic_ens_read_grid(pic_ensemble_t *e, int inst, float *pot, float *efield);
+/* cumul_e/i (f64 n_g) and counters[8] (as pic_sim_read_measurements) of one instance. */
+int pic_ens_read_measurements(pic_ensemble_t *e, int inst, double *cumul_e, double *cumul_i,
+                              unsigned long long *counters);
+
 /* Device memory helpers for callers without their own allocator. */
 int pic_malloc(void **ptr, uint64_t bytes);
 int pic_free(void *ptr);
@@ -255,13 +295,18 @@ int pic_memcpy_htod(void *dst, const void *src, uint64_t bytes);
 int pic_memcpy_dtoh(void *dst, const void *src, uint64_t bytes);
 int pic_memset(void *dst, int value, uint64_t bytes);
 int pic_device_synchronize(void);
+/* Kernel copy of `bytes` (16-byte aligned) with 16-byte vector accesses: the
+ * reference point for HBM traffic counters and achievable bandwidth (no
+ * reference counterpart). */
+int pic_stream_copy(void *dst, const void *src, uint64_t bytes, pic_stream_t stream);
 /* Text of the last error on this thread. */
 const char *pic_last_error(void);
 /* Device properties used for launch shapes: compute units, wave size. */
 int pic_device_info(int *compute_units, int *wave_size, char *arch, int arch_len);
-/* sizeof(pic_params_t), sizeof(pic_species_t), sizeof(pic_sim_config_t), so
- * that a binding can verify its struct layout (no reference counterpart). */
-int pic_abi_layout(uint64_t out[3]);
+/* sizeof(pic_params_t), sizeof(pic_species_t), sizeof(pic_sim_config_t),
+ * sizeof(pic_ensemble_config_t), so that a binding can verify its struct
+ * layout (no reference counterpart). */
+int pic_abi_layout(uint64_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -77,9 +77,10 @@ def test_get_density_vs_oracle(ca, orc, f64, n):
     ca.kernels.get_density(d_x, dens, counter(ca, n), p, density_f64=f64)
     got = dens.to_host_vec().astype(np.float64)
     want = orc.deposit(x, po)
+    # each workgroup sums its particles in f32 LDS before the (f32 or f64)
+    # global flush: a few f32 roundings per node
     scale = max(want.max(), 1.0)
-    tol = (1e-12 if f64 else 2e-6) * scale + 1e-6 * scale * (n > 1000)
-    assert np.abs(got - want).max() <= tol
+    assert np.abs(got - want).max() <= 2e-6 * scale
     # charge invariant
     wf = float(np.float32(p.weight_factor))
     total = got[1:-1].sum() + 0.5 * (got[0] + got[-1])
@@ -149,6 +150,15 @@ def _push_case(ca, orc, n, species, seed, field_scale=1.0):
     p = ca.params()
     po = orc.params()
     (ex, evx, evy, evz), (ix, ivx, ivy, ivz) = synthetic_state(max(n, 1), max(n, 1), seed=seed)
+    if n >= 100:   # a tenth of the particles within a few cells of either wall
+        k = n // 10
+        wall = np.random.default_rng(seed + 2).random(k) * 4 * p.dx
+        ex = ex.copy()
+        ix = ix.copy()
+        ex[:k // 2] = wall[:k // 2]
+        ex[k // 2:k] = p.gap - wall[k // 2:]
+        ix[:k // 2] = wall[:k // 2]
+        ix[k // 2:k] = p.gap - wall[k // 2:]
     if species == "e":
         x, vx, vy, vz = ex[:n], evx[:n] * 8, evy[:n], evz[:n]   # hot electrons cross the walls
         factor, dt, hm = p.factor_e, p.dt_e_f, 0.0
@@ -240,7 +250,7 @@ def test_check_collisions_e_slot_replay(ca, orc):
     ok = dv[same] <= 1e-4 * v_in[same] + 1.0
     assert ok.all(), (dv[same][~ok][:5], v_in[same][~ok][:5])
     assert abs(int(ncoll.to_host_vec()[0]) - ncoll_o) <= (~same).sum()
-    assert ncoll_o > 5000
+    assert ncoll_o > 1000
     # spawned electrons and ions: same multisets (within the transcendental tolerance)
     assert ne_gpu - n == ne_o - n and ni_gpu == ni_o
     assert int(ovf.to_host_vec()[0]) == 0
@@ -301,7 +311,7 @@ def test_check_collisions_i_slot_replay(ca, orc):
     dv = np.sqrt((gvx - ov[0]) ** 2 + (gvy - ov[1]) ** 2 + (gvz - ov[2]) ** 2)
     assert (dv[same] <= 1e-4 * (v[same] + 270.0 * 4)).all()
     assert abs(int(ncoll.to_host_vec()[0]) - nco) <= (~same).sum()
-    assert nco > 1000
+    assert nco > 500
 
 
 # ---------------------------------------------------------------- measurement
