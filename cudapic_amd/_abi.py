@@ -46,7 +46,7 @@ class PicSpecies(C.Structure):
 class PicSimConfig(C.Structure):
     _fields_ = [
         ("params", PicParams),
-        ("capacity", C.c_uint32), ("spawn_capacity", C.c_uint32),
+        ("capacity", C.c_uint32), ("segments", C.c_uint32),
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
                                     "replicas", "measure")],
@@ -92,6 +92,7 @@ SIGNATURES = {
     "pic_sim_destroy": (I32, [P]),
     "pic_sim_upload": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_download": (I32, [P, I32, P, P, P, P, U32, C.POINTER(U32)]),
+    "pic_sim_segments": (I32, [P, I32, C.POINTER(U32), C.POINTER(U32), P]),
     "pic_sim_rng_upload": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_rng_download": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_set_tables": (I32, [P, P, P, P]),

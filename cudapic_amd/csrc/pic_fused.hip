@@ -83,69 +83,93 @@ __device__ uint32_t tile_reserve_f(uint32_t wave_count, uint32_t *s_wave, uint32
     return s_base[buf] + s_wave[buf * 16 + wave];
 }
 
+// Segmented particle arrays: a species lives in K segments of seg_size
+// slots, each dense from its start with its own live counter.  The kernel
+// cuts every input segment into tiles (an LDS prefix of per-segment tile
+// counts maps a global tile to its segment), and workgroup w writes its
+// survivors -- and, for electrons, the electrons it creates -- into output
+// segment w % K.  Each counter therefore sees 1/K of the slot reservations:
+// same-address atomics serialise in the memory-side atomic units, and one
+// counter for 4 M particles was the bottleneck of the whole step.
 template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
 __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     const FusedJob &j = A.j;
     const KPhys &k = A.k;
     extern __shared__ __align__(16) float sh[];
     __shared__ uint32_t s_wave[32], s_base[2];
+    __shared__ uint32_t s_tiles[kMaxSegments + 1], s_cnt[kMaxSegments];
     const int n_g = k.n_g;
+    const int K = j.n_seg;
     float *ef = sh;
     float *hist = sh + n_g;
     uint32_t *aux = (uint32_t *)(hist + A.lds_reps * n_g);  // EEPF (electrons) or IFED (ions)
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
+    const uint32_t tile_sz = blockDim.x * PPT;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
+    // live count of every input segment (clamped: a counter that overflowed
+    // its segment counts dropped particles) and its tile count
+    for (int q = threadIdx.x; q < K; q += blockDim.x) {
+        const uint32_t c = min(j.cnt_in[q], j.seg_size);
+        s_cnt[q] = c;
+        s_tiles[q + 1] = (c + tile_sz - 1) / tile_sz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        s_tiles[0] = 0;
+        for (int q = 0; q < K; ++q) {
+            acc += s_tiles[q + 1];
+            s_tiles[q + 1] = acc;
+        }
+    }
     __syncthreads();
     const int wave = threadIdx.x / kWave;
     float *h = hist + (wave % A.lds_reps) * n_g;
-
-    // counters that overflowed their arrays (appends are dropped and flagged)
-    // are clamped here, so no read goes past the end of an array
-    const uint32_t n_main = min(*j.n_main, j.capacity);
-    const uint32_t n_sp = ELECTRON ? min(*j.n_spawn, j.spawn_capacity) : 0u;
-    const uint32_t n = n_main + n_sp;
+    const uint32_t n_tiles = s_tiles[K];
+    const uint32_t ko = blockIdx.x % (uint32_t)K;               // output segment
+    uint32_t *const cnt_out = j.cnt_out + ko;
+    const uint32_t out_base = ko * j.seg_size;
     const float factor = ELECTRON ? k.factor_e : k.factor_i;
     const float dt = ELECTRON ? k.dt_e : k.dt_i;
-    const uint32_t tile_sz = blockDim.x * PPT;
-    const uint32_t n_tiles = (n + tile_sz - 1) / tile_sz;
     uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
     bool overflow = false;
     int buf = 0;
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x, buf ^= 1) {
+        // segment of this tile: largest q with s_tiles[q] <= tile
+        int lo = 0, hi = K - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_tiles[mid] <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t seg_n = s_cnt[lo];
+        const uint32_t in_base = (uint32_t)lo * j.seg_size;
+        const uint32_t base = (tile - s_tiles[lo]) * tile_sz + threadIdx.x;
         float nx[PPT], nvx[PPT], vy[PPT], vz[PPT];
         bool keep[PPT];
         uint32_t off[PPT];
-        const uint32_t base = tile * tile_sz + threadIdx.x;
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t v = base + u * blockDim.x;
             keep[u] = false;
-            if (v < n) {
-                const float *X = j.sx, *VX = j.svx, *VY = j.svy, *VZ = j.svz;
-                uint32_t i = v;
-                if (ELECTRON && v >= n_main) {
-                    X = j.px;
-                    VX = j.pvx;
-                    VY = j.pvy;
-                    VZ = j.pvz;
-                    i = v - n_main;
-                }
-                const float x = X[i];
+            if (v < seg_n) {
+                const uint32_t i = in_base + v;
+                const float x = j.sx[i];
                 const float e = field_at(ef, x, k.inv_dx, n_g);
-                nvx[u] = __builtin_fmaf(factor, e, VX[i]);
+                nvx[u] = __builtin_fmaf(factor, e, j.svx[i]);
                 nx[u] = __builtin_fmaf(nvx[u], dt, x);
                 keep[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
                 if (keep[u]) {
-                    vy[u] = VY[i];
-                    vz[u] = VZ[i];
+                    vy[u] = j.svy[i];
+                    vz[u] = j.svz[i];
                 } else {
                     const bool pw = nx[u] < 0.0f;
                     lost_pow += pw;
                     lost_gnd += !pw;
                     if (!ELECTRON) {
-                        const float y = VY[i], z = VZ[i];
+                        const float y = j.svy[i], z = j.svz[i];
                         const float en = k.half_mi_over_e * (nvx[u] * nvx[u] + y * y + z * z);
                         const int b = (int)en;
                         if (en >= 0.0f && b < kIfedBins) atomicAdd(&aux[(pw ? 0 : 1) * kIfedBins + b], 1u);
@@ -160,14 +184,15 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
             off[u] = wcount + mbcnt64(m);
             wcount += (uint32_t)__popcll(m);
         }
-        const uint32_t wbase = tile_reserve_f(wcount, s_wave, s_base, buf, j.n_out);
+        const uint32_t wbase = tile_reserve_f(wcount, s_wave, s_base, buf, cnt_out);
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             int kind = kNone;
             float se[3], si[3];
             if (keep[u]) {
-                const uint32_t slot = wbase + off[u];
-                if (slot < j.capacity) {
+                const uint32_t local = wbase + off[u];
+                if (local < j.seg_size) {
+                    const uint32_t slot = out_base + local;
                     Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot);
                     float vx = nvx[u], y = vy[u], z = vz[u];
                     if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
@@ -185,32 +210,37 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
                 }
             }
             if (ELECTRON) {
+                // ionisation products: the new electron joins this output
+                // segment (it is not collision-tested until the next step,
+                // main.rs:992), the new ion the same-numbered ion segment
                 const uint64_t sp = ballot64(kind == kIonisation);
                 if (sp) {
                     const int leader = __builtin_ctzll(sp);
                     uint32_t be = 0, bi = 0;
                     if ((int)lane_id() == leader) {
                         const uint32_t cnt = (uint32_t)__popcll(sp);
-                        be = atomicAdd(j.n_spawn_out, cnt);
-                        bi = atomicAdd(j.n_ions, cnt);
+                        be = atomicAdd(cnt_out, cnt);
+                        bi = atomicAdd(j.ion_cnt + ko, cnt);
                     }
                     be = __shfl(be, leader, kWave);
                     bi = __shfl(bi, leader, kWave);
                     if (kind == kIonisation) {
                         const uint32_t o = mbcnt64(sp);
-                        const uint32_t es = be + o, is = bi + o;
+                        const uint32_t el = be + o, il = bi + o;
                         const float x = nx[u];
-                        if (es < j.spawn_capacity) {
-                            j.ox[es] = x;
-                            j.ovx[es] = se[0];
-                            j.ovy[es] = se[1];
-                            j.ovz[es] = se[2];
+                        if (el < j.seg_size) {
+                            const uint32_t es = out_base + el;
+                            j.dx[es] = x;
+                            j.dvx[es] = se[0];
+                            j.dvy[es] = se[1];
+                            j.dvz[es] = se[2];
                             deposit_lds<Acc>(h, x, k);
                             if (MEASURE) eepf_lds(aux, ef, x, se[0], se[1], se[2], k);
                         } else {
                             overflow = true;
                         }
-                        if (is < j.ion_capacity) {
+                        if (il < j.ion_seg_size) {
+                            const uint32_t is = ko * j.ion_seg_size + il;
                             j.ix[is] = x;
                             j.ivx[is] = si[0];
                             j.ivy[is] = si[1];
@@ -270,6 +300,10 @@ static void launch_ppt(const FusedArgs &a, bool f64, int ppt, int grid, int bloc
 }
 
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
+    if (j.n_seg < 1 || j.n_seg > kMaxSegments || j.seg_size == 0) {
+        set_error("fused: 1 to 64 segments of non-zero size");
+        return PIC_ERR_ARG;
+    }
     Shape sh = resolve(sh0);
     FusedArgs a;
     a.j = j;

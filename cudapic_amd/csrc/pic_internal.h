@@ -64,8 +64,8 @@ struct PoissonJob {
     unsigned long long *meas_steps = nullptr;       // [2] electron / ion steps measured
     void *zero_a = nullptr; int zero_a_words = 0;   // 32-bit words to clear
     void *zero_b = nullptr; int zero_b_words = 0;   // cleared on ion steps only
-    uint32_t *zero_cnt[4] = {nullptr, nullptr, nullptr, nullptr};  // cleared every step
-    uint32_t *zero_cnt_ion = nullptr;                               // cleared on ion steps
+    uint32_t *zero_cnt = nullptr; int zero_cnt_words = 0;          // cleared every step
+    uint32_t *zero_cnt_ion = nullptr; int zero_cnt_ion_words = 0;  // cleared on ion steps
     uint32_t *advance_time = nullptr;                               // t <- (t+1) % n_t
 };
 int launch_poisson(const PoissonJob &j, hipStream_t s);
@@ -116,37 +116,33 @@ int launch_eepf(const float *x, const float *vx, const float *vy, const float *v
 // Fused particle step of the fast path (one kernel per species per step):
 // push, survivor filter, wave64 compaction, MCC collision with the
 // destination slot's RNG stream, and CIC deposit of the new positions into
-// the next step's density (plus EEPF in measurement steps).
+// the next step's density (plus EEPF in measurement steps).  A species is
+// stored as n_seg segments of seg_size slots, each dense with its own count.
+constexpr int kMaxSegments = 64;
 struct FusedJob {
     bool electrons;
     const float *efield;
-    // input: main array [0, *n_main) followed by the spawn staging [0, *n_spawn)
-    const float *sx, *svx, *svy, *svz;
-    const uint32_t *n_main;
-    const float *px, *pvx, *pvy, *pvz;   // previous step's spawns (electrons only)
-    const uint32_t *n_spawn;
-    // output
-    float *dx, *dvx, *dvy, *dvz;
-    uint32_t *n_out;
-    uint32_t capacity;
-    uint32_t *rng0, *rng1, *rng2, *rng3;  // per destination slot
-    // electron ionisation products
-    float *ox, *ovx, *ovy, *ovz;          // spawned electrons (staging)
-    uint32_t *n_spawn_out;
-    uint32_t spawn_capacity;
-    float *ix, *ivx, *ivy, *ivz;          // ion array (append)
-    uint32_t *n_ions;
-    uint32_t ion_capacity;
-    void *ion_dens;                        // next ion density (spawned ions deposit here)
+    const float *sx, *svx, *svy, *svz;    // input segments
+    const uint32_t *cnt_in;               // [n_seg]
+    float *dx, *dvx, *dvy, *dvz;          // output segments
+    uint32_t *cnt_out;                    // [n_seg], cleared before the launch
+    int n_seg;
+    uint32_t seg_size;
+    uint32_t *rng0, *rng1, *rng2, *rng3;  // per slot of the output segments
+    // electrons: ionisation products (new ions go to the current ion segments)
+    float *ix, *ivx, *ivy, *ivz;
+    uint32_t *ion_cnt;                    // [n_seg]
+    uint32_t ion_seg_size;
+    void *ion_dens;                       // next ion density (new ions deposit here)
     // densities
-    void *dens;                            // next density of this species
+    void *dens;                           // next density of this species
     bool dens_f64;
     // tables and diagnostics
     const float *stot, *cs;
     uint32_t *overflow;
-    unsigned long long *counters;          // [8] as pic_sim_read_measurements
-    uint32_t *eepf;                        // measurement steps only
-    uint32_t *ifed;                        // ions only
+    unsigned long long *counters;         // [8] as pic_sim_read_measurements
+    uint32_t *eepf;                       // measurement steps only
+    uint32_t *ifed;                       // ions only
 };
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
 

@@ -302,16 +302,40 @@ struct PoissonArgs {
     int zero_a_words;
     uint32_t *zero_b;
     int zero_b_words;
-    uint32_t *zero_cnt[4];
+    uint32_t *zero_cnt;
+    int zero_cnt_words;
     uint32_t *zero_cnt_ion;
+    int zero_cnt_ion_words;
     uint32_t *advance_time;
 };
 
+// Node k of both densities summed over the global replicas.  The replica
+// count is 1 or a multiple of 8; the loads of a group of eight replicas of
+// both species are independent, so one group costs one memory latency.
 template <typename D>
-__device__ __forceinline__ double load_dens(const D *p, int reps, int stride, int k) {
-    double s = 0.0;
-    for (int r = 0; r < reps; ++r) s += (double)p[r * stride + k];
-    return s;
+__device__ __forceinline__ void load_dens_pair(const D *ed, const D *id, int reps, int stride, int k,
+                                               double &ne, double &ni) {
+    if (reps == 1) {
+        ne = (double)ed[k];
+        ni = (double)id[k];
+        return;
+    }
+    double se = 0.0, si = 0.0;
+    for (int r0 = 0; r0 < reps; r0 += 8) {
+        D ve[8], vi[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            ve[j] = ed[(r0 + j) * stride + k];
+            vi[j] = id[(r0 + j) * stride + k];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            se += (double)ve[j];
+            si += (double)vi[j];
+        }
+    }
+    ne = se;
+    ni = si;
 }
 
 // pot0 of step t: sum_h a_h cos(2 pi (h+1) t / N_T) + b_h sin(...), the
@@ -346,8 +370,14 @@ __device__ void poisson_solve(const PoissonGeom &pg, const D *ed, const D *id, d
     const T alpha = Num<T>::from(pg.alpha);
     const T echarge = Num<T>::from(pg.e_charge);
     T rho[kPoissonMaxPer], g[kPoissonMaxPer];
+    double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
     const int k0 = tid * per;
     // densities (summed over the global replicas), charge density, g_i = i f_i
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        dne[u] = dni[u] = 0.0;
+        if (u < per && k0 + u < n_g) load_dens_pair(ed, id, pg.reps, pg.rep_stride, k0 + u, dne[u], dni[u]);
+    }
     T run = Num<T>::zero();
 #pragma unroll
     for (int u = 0; u < kPoissonMaxPer; ++u) {
@@ -355,14 +385,8 @@ __device__ void poisson_solve(const PoissonGeom &pg, const D *ed, const D *id, d
         rho[u] = Num<T>::zero();
         g[u] = Num<T>::zero();
         if (u < per && k < n_g) {
-            T ne, ni;
-            if (pg.reps == 1) {
-                ne = Num<T>::from((double)ed[k]);
-                ni = Num<T>::from((double)id[k]);
-            } else {
-                ne = Num<T>::from(load_dens(ed, pg.reps, pg.rep_stride, k));
-                ni = Num<T>::from(load_dens(id, pg.reps, pg.rep_stride, k));
-            }
+            const T ne = Num<T>::from(dne[u]);
+            const T ni = Num<T>::from(dni[u]);
             rho[u] = echarge * (ni - ne);
             if (k >= 1 && k <= n_g - 2) {
                 T f = alpha * rho[u];
@@ -424,8 +448,8 @@ __device__ void poisson_solve(const PoissonGeom &pg, const D *ed, const D *id, d
             pot[k] = Num<T>::to_f(pk);
             efield[k] = Num<T>::to_f(e);
             if (cumul_e) {
-                cumul_e[k] += pg.reps == 1 ? (double)ed[k] : load_dens(ed, pg.reps, pg.rep_stride, k);
-                if (ion) cumul_i[k] += pg.reps == 1 ? (double)id[k] : load_dens(id, pg.reps, pg.rep_stride, k);
+                cumul_e[k] += dne[u];
+                if (ion) cumul_i[k] += dni[u];
             }
         }
     }
@@ -457,9 +481,9 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     for (int w = tid; w < a.zero_a_words; w += blockDim.x) a.zero_a[w] = 0u;
     if (s_ion)
         for (int w = tid; w < a.zero_b_words; w += blockDim.x) a.zero_b[w] = 0u;
-    if (tid < 4 && a.zero_cnt[tid]) *a.zero_cnt[tid] = 0u;
-    if (tid == 4 && s_ion && a.zero_cnt_ion) *a.zero_cnt_ion = 0u;
-    if (tid == 5 && a.meas_steps && a.cumul_e) {
+    if (a.zero_cnt && tid < a.zero_cnt_words) a.zero_cnt[tid] = 0u;
+    if (s_ion && a.zero_cnt_ion && tid < a.zero_cnt_ion_words) a.zero_cnt_ion[tid] = 0u;
+    if (tid == 1023 && a.meas_steps && a.cumul_e) {
         a.meas_steps[0] += 1ull;
         if (s_ion) a.meas_steps[1] += 1ull;
     }
@@ -504,8 +528,10 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     a.zero_a_words = j.zero_a_words;
     a.zero_b = (uint32_t *)j.zero_b;
     a.zero_b_words = j.zero_b_words;
-    for (int c = 0; c < 4; ++c) a.zero_cnt[c] = j.zero_cnt[c];
+    a.zero_cnt = j.zero_cnt;
+    a.zero_cnt_words = j.zero_cnt_words;
     a.zero_cnt_ion = j.zero_cnt_ion;
+    a.zero_cnt_ion_words = j.zero_cnt_ion_words;
     a.advance_time = j.advance_time;
     switch (j.precision) {
     case PIC_PREC_FP32: {

@@ -26,16 +26,16 @@
 using namespace pic;
 
 namespace {
-// device counter block (uint32 words)
+// device counter block (uint32 words): live counts per [parity][segment]
 enum Cnt {
-    C_E0 = 0, C_E1 = 1,     // electrons in main array [parity]
-    C_S0 = 2, C_S1 = 3,     // electrons in spawn staging [parity] (fused)
-    C_I0 = 4, C_I1 = 5,     // ions [parity]
-    C_ALIVE = 6,            // reference path scratch counter
-    C_OVERFLOW = 7,
-    C_TIME = 8,             // step index within the cycle (fused path clock)
-    C_WORDS = 16
+    C_E = 0,                        // electrons [2][kMaxSegments]
+    C_I = 2 * kMaxSegments,         // ions [2][kMaxSegments]
+    C_ALIVE = 4 * kMaxSegments,     // reference path scratch counter
+    C_OVERFLOW = C_ALIVE + 1,
+    C_TIME = C_ALIVE + 2,           // step index within the cycle (fused path clock)
+    C_WORDS = C_ALIVE + 16
 };
+constexpr int kDefaultSegments = 32;
 constexpr int kMeasCounters = 8;
 constexpr uint32_t kCkptMagic = 0x43504943u;  // "CIPC"
 constexpr uint32_t kCkptVersion = 1;
@@ -49,9 +49,12 @@ struct pic_sim {
     hipStream_t stream = nullptr;
     int prec = PIC_PREC_FP32;
     bool f64 = false;
-    uint32_t cap = 0, spawn_cap = 0;
-    // particles: [parity][x, vx, vy, vz]
-    float *e[2][4] = {}, *es[2][4] = {}, *ion[2][4] = {};
+    uint32_t cap = 0;                 // MAX_PARTICLES per species
+    int nseg = 1;                     // segments per species (1 on the reference path)
+    uint32_t seg = 0;                 // slots per segment
+    uint32_t slots = 0;               // nseg * seg
+    // particles: [parity][x, vx, vy, vz], nseg segments of seg slots each
+    float *e[2][4] = {}, *ion[2][4] = {};
     uint32_t *e_rng[4] = {}, *i_rng[4] = {};
     uint32_t *cnt = nullptr;
     // grid
@@ -92,6 +95,9 @@ int dalloc(T **p, size_t n) {
         if (rc_ != PIC_OK) return rc_; \
     } while (0)
 
+uint32_t *ecnt(pic_sim *s, int parity) { return s->cnt + C_E + parity * kMaxSegments; }
+uint32_t *icnt(pic_sim *s, int parity) { return s->cnt + C_I + parity * kMaxSegments; }
+
 pic_species_t species(pic_sim *s, int which, int parity) {
     pic_species_t sp;
     float **a = which == PIC_ELECTRONS ? s->e[parity] : s->ion[parity];
@@ -104,7 +110,7 @@ pic_species_t species(pic_sim *s, int which, int parity) {
     sp.rng1 = r[1];
     sp.rng2 = r[2];
     sp.rng3 = r[3];
-    sp.capacity = s->cap;
+    sp.capacity = s->slots;
     return sp;
 }
 
@@ -147,8 +153,8 @@ double host_pot0(const pic_params_t &p, uint32_t t) {
 int step_reference(pic_sim *s, bool measure) {
     hipStream_t st = s->stream;
     const bool ion = (s->t % (uint32_t)s->p.n_sub) == 0;
-    uint32_t *ne = s->cnt + C_E0 + s->ce;
-    uint32_t *ni = s->cnt + C_I0 + s->ci;
+    uint32_t *ne = ecnt(s, s->ce);
+    uint32_t *ni = icnt(s, s->ci);
     uint32_t *alive = s->cnt + C_ALIVE;
     Shape one = s->sh;
     one.replicas = 1;
@@ -186,7 +192,7 @@ int step_reference(pic_sim *s, bool measure) {
                alive, ne, s->k.factor_e, s->k.dt_e, 0.0f, s->counters + 0, nullptr, s->cap};
     TRY(launch_move_compact(mj, s->k, s->sh, st));
     s->ce ^= 1;
-    ne = s->cnt + C_E0 + s->ce;
+    ne = ecnt(s, s->ce);
     TRY(launch_copy_u32(ne, alive, st));
     // main.rs:1770-1783: ions
     if (ion) {
@@ -197,7 +203,7 @@ int step_reference(pic_sim *s, bool measure) {
                    s->counters + 2, s->ifed, s->cap};
         TRY(launch_move_compact(mi, s->k, s->sh, st));
         s->ci ^= 1;
-        ni = s->cnt + C_I0 + s->ci;
+        ni = icnt(s, s->ci);
         TRY(launch_copy_u32(ni, alive, st, s->cap));
         TRY(launch_copy_u32(alive, ne, st));
     }
@@ -232,13 +238,12 @@ int fused_prologue(pic_sim *s) {
     TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
     TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
     TRY(launch_fill(s->i_dens[1], 0u, dens_words32(s), st));
-    uint32_t *ne = s->cnt + C_E0 + s->ce;
-    uint32_t *ns = s->cnt + C_S0 + s->ce;
-    uint32_t *ni = s->cnt + C_I0 + s->ci;
-    TRY(launch_deposit(s->e[s->ce][0], ne, s->e_dens[s->cd], s->f64, s->k, sh, st, s->cap));
-    TRY(launch_deposit(s->es[s->ce][0], ns, s->e_dens[s->cd], s->f64, s->k, sh, st, s->spawn_cap));
-    TRY(launch_deposit(s->ion[s->ci][0], ni, s->i_dens[0], s->f64, s->k, sh, st, s->cap));
-    TRY(launch_deposit(s->ion[s->ci][0], ni, s->i_dens[1], s->f64, s->k, sh, st, s->cap));
+    for (int q = 0; q < s->nseg; ++q) {
+        const size_t o = (size_t)q * s->seg;
+        TRY(launch_deposit(s->e[s->ce][0] + o, ecnt(s, s->ce) + q, s->e_dens[s->cd], s->f64, s->k, sh, st, s->seg));
+        TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[0], s->f64, s->k, sh, st, s->seg));
+        TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[1], s->f64, s->k, sh, st, s->seg));
+    }
     // the device clock follows the host's
     PIC_HIP(hipMemcpyAsync(s->cnt + C_TIME, &s->t, sizeof(uint32_t), hipMemcpyHostToDevice, st));
     PIC_HIP(hipStreamSynchronize(st));
@@ -263,9 +268,12 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
         pj.zero_b = s->i_dens[s->hi];  // old held buffer becomes the accumulator
         pj.zero_b_words = (int)dens_words32(s);
     }
-    pj.zero_cnt[0] = s->cnt + C_E0 + (1 - s->ce);
-    pj.zero_cnt[1] = s->cnt + C_S0 + (1 - s->ce);
-    if (ion) pj.zero_cnt_ion = s->cnt + C_I0 + (1 - s->ci);
+    pj.zero_cnt = ecnt(s, 1 - s->ce);
+    pj.zero_cnt_words = s->nseg;
+    if (ion) {
+        pj.zero_cnt_ion = icnt(s, 1 - s->ci);
+        pj.zero_cnt_ion_words = s->nseg;
+    }
     if (measure) {
         pj.cumul_e = s->cumul_e;
         pj.cumul_i = s->cumul_i;
@@ -275,79 +283,63 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     s->last_e = s->cd;
     s->last_i = i_read;
     const int i_acc = ion ? s->hi : 1 - s->hi;  // accumulator for the next ion step
+    FusedJob base;
+    memset(&base, 0, sizeof(base));
+    base.efield = s->efield;
+    base.n_seg = s->nseg;
+    base.seg_size = s->seg;
+    base.dens_f64 = s->f64;
+    base.cs = s->cs;
+    base.overflow = s->cnt + C_OVERFLOW;
+    base.counters = s->counters;
     if (ion) {
-        FusedJob fi;
-        memset(&fi, 0, sizeof(fi));
+        FusedJob fi = base;
         fi.electrons = false;
-        fi.efield = s->efield;
         fi.sx = s->ion[s->ci][0];
         fi.svx = s->ion[s->ci][1];
         fi.svy = s->ion[s->ci][2];
         fi.svz = s->ion[s->ci][3];
-        fi.n_main = s->cnt + C_I0 + s->ci;
+        fi.cnt_in = icnt(s, s->ci);
         fi.dx = s->ion[1 - s->ci][0];
         fi.dvx = s->ion[1 - s->ci][1];
         fi.dvy = s->ion[1 - s->ci][2];
         fi.dvz = s->ion[1 - s->ci][3];
-        fi.n_out = s->cnt + C_I0 + (1 - s->ci);
-        fi.capacity = s->cap;
+        fi.cnt_out = icnt(s, 1 - s->ci);
         fi.rng0 = s->i_rng[0];
         fi.rng1 = s->i_rng[1];
         fi.rng2 = s->i_rng[2];
         fi.rng3 = s->i_rng[3];
         fi.dens = s->i_dens[i_acc];
-        fi.dens_f64 = s->f64;
         fi.stot = s->stot_i;
-        fi.cs = s->cs;
-        fi.overflow = s->cnt + C_OVERFLOW;
-        fi.counters = s->counters;
         fi.ifed = s->ifed;
         TRY(launch_fused(fi, s->k, s->sh, st));
         s->ci ^= 1;
     }
-    FusedJob fe;
-    memset(&fe, 0, sizeof(fe));
+    FusedJob fe = base;
     fe.electrons = true;
-    fe.efield = s->efield;
     fe.sx = s->e[s->ce][0];
     fe.svx = s->e[s->ce][1];
     fe.svy = s->e[s->ce][2];
     fe.svz = s->e[s->ce][3];
-    fe.n_main = s->cnt + C_E0 + s->ce;
-    fe.px = s->es[s->ce][0];
-    fe.pvx = s->es[s->ce][1];
-    fe.pvy = s->es[s->ce][2];
-    fe.pvz = s->es[s->ce][3];
-    fe.n_spawn = s->cnt + C_S0 + s->ce;
+    fe.cnt_in = ecnt(s, s->ce);
     fe.dx = s->e[1 - s->ce][0];
     fe.dvx = s->e[1 - s->ce][1];
     fe.dvy = s->e[1 - s->ce][2];
     fe.dvz = s->e[1 - s->ce][3];
-    fe.n_out = s->cnt + C_E0 + (1 - s->ce);
-    fe.capacity = s->cap;
+    fe.cnt_out = ecnt(s, 1 - s->ce);
     fe.rng0 = s->e_rng[0];
     fe.rng1 = s->e_rng[1];
     fe.rng2 = s->e_rng[2];
     fe.rng3 = s->e_rng[3];
-    fe.ox = s->es[1 - s->ce][0];
-    fe.ovx = s->es[1 - s->ce][1];
-    fe.ovy = s->es[1 - s->ce][2];
-    fe.ovz = s->es[1 - s->ce][3];
-    fe.n_spawn_out = s->cnt + C_S0 + (1 - s->ce);
-    fe.spawn_capacity = s->spawn_cap;
     fe.ix = s->ion[s->ci][0];
     fe.ivx = s->ion[s->ci][1];
     fe.ivy = s->ion[s->ci][2];
     fe.ivz = s->ion[s->ci][3];
-    fe.n_ions = s->cnt + C_I0 + s->ci;
-    fe.ion_capacity = s->cap;
+    fe.ion_cnt = icnt(s, s->ci);
+    fe.ion_seg_size = s->seg;
     fe.ion_dens = s->i_dens[i_acc];
     fe.dens = s->e_dens[1 - s->cd];
-    fe.dens_f64 = s->f64;
     fe.stot = s->stot_e;
-    fe.cs = s->cs;
-    fe.overflow = s->cnt + C_OVERFLOW;
-    fe.counters = s->counters;
     fe.eepf = measure ? s->eepf : nullptr;
     if (time_kernel) PIC_HIP(hipEventRecord(e0, st));
     TRY(launch_fused(fe, s->k, s->sh, st));
@@ -442,7 +434,6 @@ int free_all(pic_sim *s) {
     for (int q = 0; q < 2; ++q)
         for (int c = 0; c < 4; ++c) {
             (void)hipFree(s->e[q][c]);
-            (void)hipFree(s->es[q][c]);
             (void)hipFree(s->ion[q][c]);
         }
     for (int c = 0; c < 4; ++c) {
@@ -497,14 +488,28 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     }
     s->f64 = s->prec == PIC_PREC_FP64;
     s->cap = c->capacity;
-    s->spawn_cap = c->spawn_capacity ? c->spawn_capacity : std::max<uint32_t>(4096u, c->capacity / 8u);
+    // the fused path keeps each species in segments with their own counters;
+    // the reference path in one (its stage kernels see a plain array)
+    s->nseg = c->path == PIC_PATH_FUSED ? (c->segments > 0 ? c->segments : kDefaultSegments) : 1;
+    if (s->nseg > kMaxSegments) {
+        delete s;
+        set_error("pic_sim_create: at most 64 segments");
+        return PIC_ERR_ARG;
+    }
+    if (s->nseg > 1 && (uint64_t)s->nseg * 64 > s->cap) s->nseg = std::max<int>(1, (int)(s->cap / 64));
+    s->seg = (uint32_t)(((uint64_t)s->cap + s->nseg - 1) / s->nseg);
+    if (s->nseg > 1) s->seg = (s->seg + 63) / 64 * 64;
+    s->slots = s->seg * (uint32_t)s->nseg;
     Shape sh;
     sh.block = c->block_size > 0 ? c->block_size : 256;
     // the fused kernels hold ~110 VGPRs (4 waves per SIMD): a persistent grid
     // of 4 workgroups of 256 lanes per compute unit is fully resident
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
     sh.ppt = c->ppt > 0 ? c->ppt : 4;
+    // global density replicas: 1, or a multiple of 8 (the Poisson kernel
+    // reads them in groups of eight)
     sh.replicas = c->replicas > 0 ? c->replicas : (c->path == PIC_PATH_FUSED ? 16 : 1);
+    if (sh.replicas > 1) sh.replicas = std::min(64, (sh.replicas + 7) / 8 * 8);
     // replica r of a density buffer starts at element r * rep_stride (n_g
     // rounded up to 32 elements), so the same node of different replicas
     // never shares a cache line
@@ -521,13 +526,12 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         return fail(hip_fail(hipGetLastError(), "hipStreamCreate"));
     for (int q = 0; q < 2; ++q)
         for (int k = 0; k < 4; ++k) {
-            if ((rc = dalloc(&s->e[q][k], s->cap))) return fail(rc);
-            if ((rc = dalloc(&s->ion[q][k], s->cap))) return fail(rc);
-            if ((rc = dalloc(&s->es[q][k], c->path == PIC_PATH_FUSED ? s->spawn_cap : 1))) return fail(rc);
+            if ((rc = dalloc(&s->e[q][k], s->slots))) return fail(rc);
+            if ((rc = dalloc(&s->ion[q][k], s->slots))) return fail(rc);
         }
     for (int k = 0; k < 4; ++k) {
-        if ((rc = dalloc(&s->e_rng[k], s->cap))) return fail(rc);
-        if ((rc = dalloc(&s->i_rng[k], s->cap))) return fail(rc);
+        if ((rc = dalloc(&s->e_rng[k], s->slots))) return fail(rc);
+        if ((rc = dalloc(&s->i_rng[k], s->slots))) return fail(rc);
     }
     if ((rc = dalloc(&s->cnt, C_WORDS))) return fail(rc);
     if ((rc = dalloc(&s->pot, s->p.n_g))) return fail(rc);
@@ -552,9 +556,9 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         if ((rc = pic_cross_sections(s->p.gas_density, cs.data(), se.data(), si.data()))) return fail(rc);
         if ((rc = pic_sim_set_tables(s, cs.data(), se.data(), si.data()))) return fail(rc);
     }
-    if ((rc = pic_seed_streams(c->seed_e, s->cap, s->e_rng[0], s->e_rng[1], s->e_rng[2], s->e_rng[3], s->stream)))
+    if ((rc = pic_seed_streams(c->seed_e, s->slots, s->e_rng[0], s->e_rng[1], s->e_rng[2], s->e_rng[3], s->stream)))
         return fail(rc);
-    if ((rc = pic_seed_streams(c->seed_i, s->cap, s->i_rng[0], s->i_rng[1], s->i_rng[2], s->i_rng[3], s->stream)))
+    if ((rc = pic_seed_streams(c->seed_i, s->slots, s->i_rng[0], s->i_rng[1], s->i_rng[2], s->i_rng[3], s->stream)))
         return fail(rc);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "seed"));
     *out = s;
@@ -586,51 +590,60 @@ int pic_sim_upload(pic_sim_t *s, int species_id, const float *x, const float *vx
     }
     PIC_HIP(hipStreamSynchronize(s->stream));
     float **a = species_id == PIC_ELECTRONS ? s->e[s->ce] : s->ion[s->ci];
+    uint32_t *cnt = species_id == PIC_ELECTRONS ? ecnt(s, s->ce) : icnt(s, s->ci);
     const float *src[4] = {x, vx, vy, vz};
-    for (int c = 0; c < 4; ++c)
-        if (n) PIC_HIP(hipMemcpy(a[c], src[c], sizeof(float) * n, hipMemcpyHostToDevice));
-    uint32_t cnt[2] = {n, 0u};
-    if (species_id == PIC_ELECTRONS) {
-        PIC_HIP(hipMemcpy(s->cnt + C_E0 + s->ce, &cnt[0], 4, hipMemcpyHostToDevice));
-        PIC_HIP(hipMemcpy(s->cnt + C_S0 + s->ce, &cnt[1], 4, hipMemcpyHostToDevice));
-    } else {
-        PIC_HIP(hipMemcpy(s->cnt + C_I0 + s->ci, &cnt[0], 4, hipMemcpyHostToDevice));
+    // consecutive runs of the host arrays go to consecutive segments
+    std::vector<uint32_t> c(kMaxSegments, 0u);
+    uint32_t done = 0;
+    for (int q = 0; q < s->nseg; ++q) {
+        const uint32_t m = n / s->nseg + ((uint32_t)q < n % s->nseg ? 1u : 0u);
+        c[q] = m;
+        for (int w = 0; w < 4; ++w)
+            if (m) PIC_HIP(hipMemcpy(a[w] + (size_t)q * s->seg, src[w] + done, 4ull * m, hipMemcpyHostToDevice));
+        done += m;
     }
+    PIC_HIP(hipMemcpy(cnt, c.data(), 4ull * kMaxSegments, hipMemcpyHostToDevice));
     s->need_prologue = true;
     invalidate_graphs(s);
+    return PIC_OK;
+}
+
+int pic_sim_segments(pic_sim_t *s, int species_id, uint32_t *n_seg, uint32_t *seg_size, uint32_t *counts) {
+    if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS)) return PIC_ERR_ARG;
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    if (n_seg) *n_seg = (uint32_t)s->nseg;
+    if (seg_size) *seg_size = s->seg;
+    if (counts) {
+        uint32_t *cnt = species_id == PIC_ELECTRONS ? ecnt(s, s->ce) : icnt(s, s->ci);
+        PIC_HIP(hipMemcpy(counts, cnt, 4ull * s->nseg, hipMemcpyDeviceToHost));
+        for (int q = 0; q < s->nseg; ++q) counts[q] = std::min(counts[q], s->seg);
+    }
     return PIC_OK;
 }
 
 int pic_sim_download(pic_sim_t *s, int species_id, float *x, float *vx, float *vy, float *vz,
                      uint32_t cap, uint32_t *n) {
     if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS)) return PIC_ERR_ARG;
-    PIC_HIP(hipStreamSynchronize(s->stream));
-    uint32_t c[16];
-    PIC_HIP(hipMemcpy(c, s->cnt, sizeof(c), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> c(kMaxSegments, 0u);
+    TRY(pic_sim_segments(s, species_id, nullptr, nullptr, c.data()));
+    float **a = species_id == PIC_ELECTRONS ? s->e[s->ce] : s->ion[s->ci];
     float *dst[4] = {x, vx, vy, vz};
-    if (species_id == PIC_ELECTRONS) {
-        uint32_t nm = std::min(c[C_E0 + s->ce], s->cap);
-        uint32_t ns = s->cfg.path == PIC_PATH_FUSED ? std::min(c[C_S0 + s->ce], s->spawn_cap) : 0u;
-        if (n) *n = nm + ns;
-        uint32_t cm = std::min(nm, cap), cs2 = std::min(ns, cap - cm);
-        for (int q = 0; q < 4; ++q) {
-            if (!dst[q]) continue;
-            if (cm) PIC_HIP(hipMemcpy(dst[q], s->e[s->ce][q], sizeof(float) * cm, hipMemcpyDeviceToHost));
-            if (cs2) PIC_HIP(hipMemcpy(dst[q] + cm, s->es[s->ce][q], sizeof(float) * cs2, hipMemcpyDeviceToHost));
-        }
-    } else {
-        uint32_t ni = std::min(c[C_I0 + s->ci], s->cap);
-        if (n) *n = ni;
-        uint32_t cm = std::min(ni, cap);
-        for (int q = 0; q < 4; ++q)
-            if (dst[q] && cm) PIC_HIP(hipMemcpy(dst[q], s->ion[s->ci][q], sizeof(float) * cm, hipMemcpyDeviceToHost));
+    uint32_t total = 0, copied = 0;
+    for (int q = 0; q < s->nseg; ++q) {
+        total += c[q];
+        const uint32_t m = std::min(c[q], cap - copied);
+        for (int w = 0; w < 4; ++w)
+            if (dst[w] && m)
+                PIC_HIP(hipMemcpy(dst[w] + copied, a[w] + (size_t)q * s->seg, 4ull * m, hipMemcpyDeviceToHost));
+        copied += m;
     }
+    if (n) *n = total;
     return PIC_OK;
 }
 
 int pic_sim_rng_upload(pic_sim_t *s, int species_id, const uint32_t *w0, const uint32_t *w1,
                        const uint32_t *w2, const uint32_t *w3, uint32_t n) {
-    if (!s || n > s->cap || !w0 || !w1 || !w2 || !w3) return PIC_ERR_ARG;
+    if (!s || n > s->slots || !w0 || !w1 || !w2 || !w3) return PIC_ERR_ARG;
     PIC_HIP(hipStreamSynchronize(s->stream));
     uint32_t **r = species_id == PIC_ELECTRONS ? s->e_rng : s->i_rng;
     const uint32_t *src[4] = {w0, w1, w2, w3};
@@ -640,7 +653,7 @@ int pic_sim_rng_upload(pic_sim_t *s, int species_id, const uint32_t *w0, const u
 
 int pic_sim_rng_download(pic_sim_t *s, int species_id, uint32_t *w0, uint32_t *w1, uint32_t *w2,
                          uint32_t *w3, uint32_t n) {
-    if (!s || n > s->cap) return PIC_ERR_ARG;
+    if (!s || n > s->slots) return PIC_ERR_ARG;
     PIC_HIP(hipStreamSynchronize(s->stream));
     uint32_t **r = species_id == PIC_ELECTRONS ? s->e_rng : s->i_rng;
     uint32_t *dst[4] = {w0, w1, w2, w3};
@@ -651,13 +664,14 @@ int pic_sim_rng_download(pic_sim_t *s, int species_id, uint32_t *w0, uint32_t *w
 
 int pic_sim_counts(pic_sim_t *s, uint32_t *n_e, uint32_t *n_i) {
     if (!s) return PIC_ERR_ARG;
-    PIC_HIP(hipStreamSynchronize(s->stream));
-    uint32_t c[16];
-    PIC_HIP(hipMemcpy(c, s->cnt, sizeof(c), hipMemcpyDeviceToHost));
-    uint32_t ne = c[C_E0 + s->ce];
-    if (s->cfg.path == PIC_PATH_FUSED) ne += c[C_S0 + s->ce];
-    if (n_e) *n_e = ne;
-    if (n_i) *n_i = c[C_I0 + s->ci];
+    std::vector<uint32_t> c(kMaxSegments);
+    uint32_t tot[2] = {0u, 0u};
+    for (int sp = 0; sp < 2; ++sp) {
+        TRY(pic_sim_segments(s, sp, nullptr, nullptr, c.data()));
+        for (int q = 0; q < s->nseg; ++q) tot[sp] += c[q];
+    }
+    if (n_e) *n_e = tot[0];
+    if (n_i) *n_i = tot[1];
     return PIC_OK;
 }
 
@@ -844,11 +858,11 @@ int pic_sim_save(pic_sim_t *s, const char *path) {
         fclose(f);
         return rc;
     }
-    uint32_t hdr[8] = {kCkptMagic, kCkptVersion, (uint32_t)s->p.n_g, s->t, ne, ni, s->cap, 0};
+    uint32_t hdr[8] = {kCkptMagic, kCkptVersion, (uint32_t)s->p.n_g, s->t, ne, ni, s->slots, 0};
     fwrite(hdr, sizeof(hdr), 1, f);
     fwrite(&s->steps, sizeof(s->steps), 1, f);
     std::vector<float> buf(std::max<uint32_t>(std::max(ne, ni), 1));
-    std::vector<uint32_t> w(s->cap);
+    std::vector<uint32_t> w(s->slots);
     for (int sp = 0; sp < 2; ++sp) {
         const uint32_t n = sp == 0 ? ne : ni;
         std::vector<float> a[4];
@@ -862,11 +876,11 @@ int pic_sim_save(pic_sim_t *s, const char *path) {
         for (auto &v : a) fwrite(v.data(), sizeof(float), n, f);
         for (int q = 0; q < 4; ++q) {
             uint32_t *src = (sp == 0 ? s->e_rng : s->i_rng)[q];
-            if (hipMemcpy(w.data(), src, 4ull * s->cap, hipMemcpyDeviceToHost) != hipSuccess) {
+            if (hipMemcpy(w.data(), src, 4ull * s->slots, hipMemcpyDeviceToHost) != hipSuccess) {
                 fclose(f);
                 return hip_fail(hipGetLastError(), "ckpt rng");
             }
-            fwrite(w.data(), 4, s->cap, f);
+            fwrite(w.data(), 4, s->slots, f);
         }
     }
     std::vector<double> ce(s->p.n_g), cii(s->p.n_g);
@@ -928,7 +942,7 @@ int pic_sim_load(pic_sim_t *s, const char *path) {
             v.resize(ckcap);
             if (fread(v.data(), 4, ckcap, f) != ckcap) return bad("truncated rng");
         }
-        const uint32_t nr = std::min(ckcap, s->cap);
+        const uint32_t nr = std::min(ckcap, s->slots);
         rc = pic_sim_rng_upload(s, sp, w[0].data(), w[1].data(), w[2].data(), w[3].data(), nr);
         if (rc) {
             fclose(f);

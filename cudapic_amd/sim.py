@@ -12,12 +12,12 @@ from ._abi import (PIC_ELECTRONS, PIC_IONS, PIC_N_EEPF, PIC_N_IFED, PIC_PATH_FUS
 
 class Sim:
     def __init__(self, p, capacity=120_000, path=PIC_PATH_FUSED, use_graph=True, seed_e=None,
-                 seed_i=None, spawn_capacity=0, block_size=0, grid_blocks=0, ppt=0, replicas=0):
+                 seed_i=None, segments=0, block_size=0, grid_blocks=0, ppt=0, replicas=0):
         cfg = PicSimConfig()
         lib().pic_sim_config_default(C.byref(cfg))
         cfg.params = p
         cfg.capacity = int(capacity)
-        cfg.spawn_capacity = int(spawn_capacity)
+        cfg.segments = int(segments)
         if seed_e is not None:
             cfg.seed_e[:] = [int(v) for v in seed_e]
         if seed_i is not None:
@@ -44,12 +44,28 @@ class Sim:
         call("pic_sim_download", self.h, species, *(a.ctypes.data for a in out), n.value, C.byref(n))
         return [a[:n.value] for a in out]
 
+    def segments(self, species):
+        """(n_seg, seg_size, counts[n_seg]): particle j of segment q is in slot
+        q * seg_size + j; download() returns the segments in order."""
+        ns, sz = C.c_uint32(), C.c_uint32()
+        cnt = np.zeros(64, np.uint32)
+        call("pic_sim_segments", self.h, species, C.byref(ns), C.byref(sz), cnt.ctypes.data)
+        return ns.value, sz.value, cnt[:ns.value].copy()
+
+    def slots(self, species):
+        """Slot index of every particle in download() order."""
+        ns, sz, cnt = self.segments(species)
+        return np.concatenate([q * sz + np.arange(c, dtype=np.int64) for q, c in enumerate(cnt)]) \
+            if ns else np.zeros(0, np.int64)
+
     def rng_upload(self, species, words):
         w = [np.ascontiguousarray(a, np.uint32) for a in words]
         call("pic_sim_rng_upload", self.h, species, *(a.ctypes.data for a in w), len(w[0]))
 
     def rng_download(self, species, n=None):
-        n = self.capacity if n is None else int(n)
+        if n is None:
+            ns, sz, _ = self.segments(species)
+            n = ns * sz
         out = [np.zeros(max(1, n), np.uint32) for _ in range(4)]
         call("pic_sim_rng_download", self.h, species, *(a.ctypes.data for a in out), n)
         return [a[:n] for a in out]

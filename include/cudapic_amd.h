@@ -178,7 +178,8 @@ enum pic_path {
 typedef struct pic_sim_config {
     pic_params_t params;       /* derived */
     uint32_t capacity;         /* MAX_PARTICLES per species (main.rs:79) */
-    uint32_t spawn_capacity;   /* fused path: electrons created in one step (0 = auto) */
+    uint32_t segments;         /* fused path: segments per species, each with its own
+                                  live counter (0 = 32); the reference path uses 1 */
     uint32_t seed_e[4];        /* master xoshiro states (main.rs:1667, 1671) */
     uint32_t seed_i[4];
     int32_t path;              /* enum pic_path */
@@ -203,6 +204,13 @@ int pic_sim_upload(pic_sim_t *s, int species, const float *x, const float *vx,
                    const float *vy, const float *vz, uint32_t n);
 int pic_sim_download(pic_sim_t *s, int species, float *x, float *vx, float *vy, float *vz,
                      uint32_t cap, uint32_t *n);
+/* Storage of a species: number of segments, slots per segment and the live
+ * count of each segment (counts[n_seg], may be NULL).  Particle j of segment
+ * q sits in slot q*seg_size + j; pic_sim_download returns the segments in
+ * order.  The fused path needs a few percent of headroom in `capacity`
+ * because segments fill independently. */
+int pic_sim_segments(pic_sim_t *s, int species, uint32_t *n_seg, uint32_t *seg_size,
+                     uint32_t *counts);
 /* Per-slot RNG words, planar, n slots (upload replaces, download copies). */
 int pic_sim_rng_upload(pic_sim_t *s, int species, const uint32_t *w0, const uint32_t *w1,
                        const uint32_t *w2, const uint32_t *w3, uint32_t n);

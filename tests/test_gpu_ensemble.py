@@ -50,9 +50,14 @@ def test_ensemble_step_per_instance_replay(ca, orc):
         cnt_i = _push_ref(orc, po, ef, I, "i")[0]
         re1 = ens.rng_download(i, 0, caps[i])
         ri1 = ens.rng_download(i, 1, caps[i])
-        _replay_species(orc, po, se, cs, E, ef, ge, rng0[i][0], re1, "e", cnt_e)
-        _replay_species(orc, po, si, cs, I, ef, gi, rng0[i][1], ri1, "i", cnt_i)
-        assert ni[i] - cnt_i == ne[i] - cnt_e > 0
+        # survivors fill the instance's region from its start, newborn
+        # electrons wait in the staging array (no slot)
+        slots_e = np.concatenate([np.arange(cnt_e), -np.ones(len(ge[0]) - cnt_e, np.int64)])
+        born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, slots_e, rng0[i][0], re1, "e")
+        old_ion = np.isin(gi[0], _push_ref(orc, po, ef, I, "i")[1][0])
+        _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi], np.arange(cnt_i),
+                        rng0[i][1], ri1, "i")
+        assert ni[i] - cnt_i == ne[i] - cnt_e == len(born_x) > 0
         assert cnt[6] == 1 and cnt[7] == 1 and cnt[4] > 0
     ens.check()
     ens.close()

@@ -85,65 +85,112 @@ def test_collisionless_step_stage_by_stage(ca, orc, path, prec):
         got = sim.download(sp)
         assert len(got[0]) == cnt
         assert_same_multiset(got, want)
-        # every surviving slot drew exactly one (ions: five) numbers
+        # every occupied slot drew exactly one (ions: five) numbers, every
+        # other slot is untouched
         rng1 = sim.rng_download(sp)
+        slots = sim.slots(sp)
         draws = 1 if name == "e" else 5
-        adv = [w.copy() for w in rng0]
         s4 = np.zeros(4, np.uint32)
-        for j in range(0, cnt, max(1, cnt // 2000)):
-            s4[:] = [adv[w][j] for w in range(4)]
+        for j in slots[:: max(1, cnt // 2000)]:
+            s4[:] = [rng0[w][j] for w in range(4)]
             for _ in range(draws):
                 orc.xo_next(s4)
             assert [rng1[w][j] for w in range(4)] == s4.tolist()
+        free = np.ones(len(rng0[0]), bool)
+        free[slots] = False
         for w in range(4):
-            np.testing.assert_array_equal(rng1[w][cnt:], rng0[w][cnt:])
+            np.testing.assert_array_equal(rng1[w][free], rng0[w][free])
     assert sim.counts() == (ne, ni)
     sim.check()
     sim.close()
 
 
-def _replay_species(orc, po, stot, cs, parts, efield, got, rng0, rng1, name, n_main):
-    """Match every GPU survivor to its source particle by its pushed position
-    and replay its collision with the RNG words of its slot."""
+def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, name):
+    """Match every GPU particle to its source by its pushed position and
+    replay its collision with the RNG words of its slot.
+
+    f32 positions repeat (a few hundred coincidences in 2e5 particles), so the
+    check runs over positions held by exactly one pushed particle.  There a
+    single GPU particle is a survivor (replayed); two are an ionisation: the
+    parent's slot drew its numbers, the newborn's slot (or staging entry,
+    slot -1) did not.  Returns the positions at which electrons were born
+    (one entry per newborn) and the ion velocities of the checked pairs."""
     cnt, pushed, _ = _push_ref(orc, po, efield, parts, name)
     px, pvx, pvy, pvz = pushed
-    order = np.argsort(px, kind="stable")
-    sx = px[order]
-    uniq = np.ones(cnt, bool)
-    uniq[1:] &= sx[1:] != sx[:-1]
-    uniq[:-1] &= sx[:-1] != sx[1:]
+    ux, uidx, ucnt = np.unique(px, return_index=True, return_counts=True)
     gx, gvx, gvy, gvz = got
-    assert n_main == cnt
-    pos = np.searchsorted(sx, gx[:cnt])
-    pos = np.minimum(pos, cnt - 1)
-    found = sx[pos] == gx[:cnt]
-    assert found.all()
-    checked = mism = 0
-    spawned = []
-    step = max(1, cnt // 20000)
-    for j in range(0, cnt, step):
-        k = pos[j]
-        if not uniq[k]:
-            continue
-        src = order[k]
+    n = len(gx)
+    gux, ginv, gcnt = np.unique(gx, return_inverse=True, return_counts=True)
+    k = np.searchsorted(ux, gux)
+    k = np.minimum(k, len(ux) - 1)
+    in_pushed = ux[k] == gux
+    assert in_pushed.all(), "a GPU particle has no pushed source"
+    mo = ucnt[k]                      # pushed sources per GPU position
+    born = gcnt - mo                  # newborns per GPU position
+    assert (born >= 0).all()
+    assert born.sum() == n - cnt
+
+    def moved(j):
+        sl = slots[j]
+        return sl >= 0 and any(rng1[w][sl] != rng0[w][sl] for w in range(4))
+
+    def replay(src, slot):
         v = np.array([pvx[src], pvy[src], pvz[src]], np.float32)
-        s = np.array([rng0[w][j] for w in range(4)], np.uint32)
+        st = np.array([rng0[w][slot] for w in range(4)], np.uint32)
         if name == "e":
-            kind, vo, so, sev, siv = orc.collide_e_one(stot, cs, v, s, po)
-            if kind == 3:
-                spawned.append((gx[j], sev, siv))
-        else:
-            kind, vo, so = orc.collide_i_one(stot, cs, v, s, po)
+            kind, vo, so, sev, siv = orc.collide_e_one(stot, cs, v, st, po)
+            return kind, v, vo, so, sev, siv
+        kind, vo, so = orc.collide_i_one(stot, cs, v, st, po)
+        return kind, v, vo, so, None, None
+
+    single = (gcnt == 1) & (mo == 1)
+    checked = mism = 0
+    step = max(1, n // 20000)
+    for j in range(0, n, step):
+        g = ginv[j]
+        if not single[g]:
+            continue
+        kind, v, vo, so, _, _ = replay(uidx[k[g]], slots[j])
         checked += 1
-        if so.tolist() != [rng1[w][j] for w in range(4)]:
+        if so.tolist() != [rng1[w][slots[j]] for w in range(4)]:
             mism += 1
             continue
         gv = np.array([gvx[j], gvy[j], gvz[j]], np.float64)
-        tol = 1e-4 * max(np.linalg.norm(v), 2000.0)
-        assert np.linalg.norm(gv - vo) <= tol, (name, j, gv, vo)
-    assert checked > 0.9 * (cnt // step)
+        assert np.linalg.norm(gv - vo) <= 1e-4 * max(np.linalg.norm(v), 2000.0), (name, j, gv, vo)
+    assert checked > 0.9 * (n // step) - 0.02 * n
     assert mism <= max(2, checked // 10000)
-    return spawned
+    ions = {}
+    if name == "e":
+        pair_groups = np.nonzero((gcnt == 2) & (mo == 1))[0]
+        members = {}
+        for j in np.nonzero(np.isin(ginv, pair_groups))[0]:
+            members.setdefault(ginv[j], []).append(j)
+        for g, pair in members.items():
+            par = [j for j in pair if moved(j)]
+            kid = [j for j in pair if not moved(j)]
+            assert len(par) == 1 and len(kid) == 1, (gux[g], pair)
+            kind, v, vo, so, sev, siv = replay(uidx[k[g]], slots[par[0]])
+            assert kind == 3
+            tol = 1e-4 * max(np.linalg.norm(v), 2000.0)
+            assert np.linalg.norm(np.array([gvx[par[0]], gvy[par[0]], gvz[par[0]]]) - vo) <= tol
+            assert np.linalg.norm(np.array([gvx[kid[0]], gvy[kid[0]], gvz[kid[0]]]) - sev) <= tol
+            ions[float(gux[g])] = siv
+        assert len(members) >= 0.8 * born.sum()
+    born_x = np.repeat(gux, born)
+    return born_x, ions
+
+
+def _check_new_ions(gi, pushed_ion_x, born_x, ions, check_v):
+    """Ions that are not pushed survivors are the ionisation products: one
+    per newborn electron, at the same position, with a thermal velocity."""
+    new = ~np.isin(gi[0], pushed_ion_x)
+    np.testing.assert_array_equal(np.sort(gi[0][new]), np.sort(born_x))
+    if check_v:
+        for j in np.nonzero(new)[0]:
+            x = float(gi[0][j])
+            if x in ions:
+                v = np.array([gi[1][j], gi[2][j], gi[3][j]])
+                assert np.linalg.norm(v - ions[x]) <= 1e-3 * 270 * 6
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -161,21 +208,20 @@ def test_step_with_collisions_slot_replay(ca, orc, path):
     sim.run(1, measure=True)
     _, ef, _, _ = sim.grid()
     ne, ni = sim.counts()
-    ge = sim.download(0)
-    gi = sim.download(1)
-    rng_e1 = sim.rng_download(0)
-    rng_i1 = sim.rng_download(1)
-    cnt_e = _push_ref(orc, po, ef, E, "e")[0]
-    cnt_i = _push_ref(orc, po, ef, I, "i")[0]
-    n_spawn = ne - cnt_e
-    assert n_spawn > 0
-    sp = _replay_species(orc, po, se, cs, E, ef, ge, rng_e0, rng_e1, "e", cnt_e)
-    _replay_species(orc, po, si, cs, I, ef, gi, rng_i0, rng_i1, "i", cnt_i)
-    # spawned electrons sit after the survivors, spawned ions after the ions
-    assert ni - cnt_i == n_spawn
-    spawn_x = set(ge[0][cnt_e:ne].tolist())
-    for x, sev, siv in sp:
-        assert x in spawn_x
+    ge, gi = sim.download(0), sim.download(1)
+    rng_e1, rng_i1 = sim.rng_download(0), sim.rng_download(1)
+    born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, sim.slots(0), rng_e0, rng_e1, "e")
+    assert len(born_x) > 0
+    cnt_i, pushed_i, _ = _push_ref(orc, po, ef, I, "i")
+    assert ni == cnt_i + len(born_x)
+    # the ion replay covers the pushed ions (new ions have no pushed source)
+    slots_i = sim.slots(1)
+    old_ion = np.isin(gi[0], pushed_i[0])
+    _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi], slots_i[old_ion],
+                    rng_i0, rng_i1, "i")
+    # new ions: at their parents' positions and thermal (on the reference path
+    # they may collide within the same ion step, so only positions are compared)
+    _check_new_ions(gi, pushed_i[0], born_x, ions, check_v=(path == 1))
     m = sim.measurements()
     c = m["counters"]
     assert c["e_meas_steps"] == 1 and c["i_meas_steps"] == 1
@@ -293,7 +339,7 @@ def test_checkpoint_round_trip(ca, orc, tmp_path):
 def test_capacity_overflow_is_reported(ca, orc):
     p = ca.params(weight=2.0e3)
     E, I = synthetic_state(20_000, 20_000, seed=71, te_ev=30.0)
-    sim = make_sim(ca, p, 20_010, 1, spawn_capacity=4)
+    sim = make_sim(ca, p, 20_010, 1)
     sim.upload(0, *E)
     sim.upload(1, *I)
     sim.run(20)
