@@ -55,8 +55,10 @@ def parse():
     ap.add_argument("--precision", choices=["fp32", "comp", "fp64"], default="fp64")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--grid", type=int, default=0)
-    ap.add_argument("--ppt", type=int, default=0)
+    ap.add_argument("--ppt", type=int, default=0, help="particles per thread, compacting steps")
+    ap.add_argument("--ppt-inplace", type=int, default=0, help="particles per thread, in-place steps")
     ap.add_argument("--replicas", type=int, default=0)
+    ap.add_argument("--segments", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="particles per species of the CPU sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -171,7 +173,8 @@ def main():
           f"{args.n_e} e + {args.n_i} i", file=sys.stderr, flush=True)
     E, I = synthetic.state(args.n_e, args.n_i, seed=1)
     sim = ca.Sim(p, capacity=args.capacity, path=path, use_graph=not args.no_graph,
-                 grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas)
+                 grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
+                 ppt_inplace=args.ppt_inplace)
     sim.upload(ca.PIC_ELECTRONS, *E)
     sim.upload(ca.PIC_IONS, *I)
     del E, I

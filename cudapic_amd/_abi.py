@@ -49,7 +49,7 @@ class PicSimConfig(C.Structure):
         ("capacity", C.c_uint32), ("segments", C.c_uint32),
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
-                                    "replicas", "measure")],
+                                    "replicas", "measure", "ppt_inplace")],
     ]
 
 
@@ -93,6 +93,7 @@ SIGNATURES = {
     "pic_sim_upload": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_download": (I32, [P, I32, P, P, P, P, U32, C.POINTER(U32)]),
     "pic_sim_segments": (I32, [P, I32, C.POINTER(U32), C.POINTER(U32), P]),
+    "pic_sim_download_slots": (I32, [P, I32, P, U32, C.POINTER(U32)]),
     "pic_sim_rng_upload": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_rng_download": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_set_tables": (I32, [P, P, P, P]),

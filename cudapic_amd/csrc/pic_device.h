@@ -170,30 +170,37 @@ __device__ __forceinline__ void rotate(float g, float chi, float eta, const Fram
 
 enum CollKind { kNone = 0, kElastic = 1, kExcitation = 2, kIonisation = 3 };
 
-// Collision test and scattering of one electron (check_collisions_e +
-// collision_e, main.rs:996-1165).  v is updated in place; on ionisation the
-// ejected electron and the new ion velocities are returned.  RNG draw order:
-// test, process, then (elastic/excitation) chi, eta or (ionisation) energy
-// share, eta, and four draws for the ion's thermal velocity.
-__device__ __forceinline__ int collide_electron(const float *__restrict__ stot_e,
-                                                const float *__restrict__ cs, const KPhys &k,
-                                                float &vx, float &vy, float &vz, Xo &s,
-                                                float *se, float *si) {
+// The collision of one electron (check_collisions_e + collision_e,
+// main.rs:996-1165) in three parts, so that a kernel can gather the table
+// entries of several particles at once: e_index (energy bin and speed),
+// e_test (collision probability; one draw) and e_scatter (process choice and
+// scattering with the three cross sections of the bin).  v is updated in
+// place; on ionisation the ejected electron and the new ion velocities are
+// returned.  RNG draw order: test, process, then (elastic/excitation) chi,
+// eta or (ionisation) energy share, eta, and four draws for the ion's
+// thermal velocity.
+__device__ __forceinline__ int e_index(const KPhys &k, float vx, float vy, float vz, float &velocity) {
     const float v2 = vx * vx + vy * vy + vz * vz;
     const float energy = k.half_me_over_e * v2;
-    const int idx = energy_index(energy, k.de_cs);
-    const float velocity = __builtin_sqrtf(v2);
-    const float nu = stot_e[idx] * k.stot_scale * velocity;
+    velocity = __builtin_sqrtf(v2);
+    return energy_index(energy, k.de_cs);
+}
+
+__device__ __forceinline__ bool e_test(const KPhys &k, float stot, float velocity, Xo &s) {
+    const float nu = stot * k.stot_scale * velocity;
     const float rv = draw(s);
     const float pc = 1.0f - expf(-nu * k.dt_e);
-    if (!(rv < pc)) return kNone;
+    return rv < pc;
+}
 
+__device__ __forceinline__ int e_scatter(const KPhys &k, float c0, float c1, float c2, float &vx,
+                                         float &vy, float &vz, Xo &s, float *se, float *si) {
     float gx = vx, gy = vy, gz = vz;
     float g = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
     const float wx = k.f1 * vx, wy = k.f1 * vy, wz = k.f1 * vz;
-    const float t0 = cs[idx];
-    const float t1 = t0 + cs[kCsRanges + idx];
-    const float t2 = t1 + cs[2 * kCsRanges + idx];
+    const float t0 = c0;
+    const float t1 = t0 + c1;
+    const float t2 = t1 + c2;
     const Frame f = frame_of(gx, gy, gz);
     float chi, eta;
     int kind;
@@ -237,25 +244,50 @@ __device__ __forceinline__ int collide_electron(const float *__restrict__ stot_e
     return kind;
 }
 
-// Collision test and scattering of one ion against a Maxwellian atom
-// (check_collisions_i + collision_i, main.rs:1185-1292).
-__device__ __forceinline__ int collide_ion(const float *__restrict__ stot_i,
-                                           const float *__restrict__ cs, const KPhys &k,
-                                           float &vx, float &vy, float &vz, Xo &s) {
-    float ax, ay, az;
+// Total cross section of bin idx: from the first stc_n bins staged in LDS
+// when the bin is there, else from the table in global memory.
+__device__ __forceinline__ float stot_at(const float *__restrict__ stot, const float *stc, int stc_n, int idx) {
+    return idx < stc_n ? stc[idx] : stot[idx];
+}
+
+__device__ __forceinline__ int collide_electron(const float *__restrict__ stot_e,
+                                                const float *__restrict__ cs, const KPhys &k,
+                                                float &vx, float &vy, float &vz, Xo &s,
+                                                float *se, float *si, const float *stc = nullptr,
+                                                int stc_n = 0) {
+    float velocity;
+    const int idx = e_index(k, vx, vy, vz, velocity);
+    if (!e_test(k, stot_at(stot_e, stc, stc_n, idx), velocity, s)) return kNone;
+    return e_scatter(k, cs[idx], cs[kCsRanges + idx], cs[2 * kCsRanges + idx], vx, vy, vz, s, se, si);
+}
+
+// The collision of one ion against a Maxwellian atom (check_collisions_i +
+// collision_i, main.rs:1185-1292) in the same three parts: i_index draws the
+// atom's velocity a (three normals) and bins the relative energy, i_test
+// draws the collision test, i_scatter picks isotropic or backward
+// scattering with the two cross sections of the bin.
+__device__ __forceinline__ int i_index(const KPhys &k, float vx, float vy, float vz, Xo &s, float &ax,
+                                       float &ay, float &az, float &g) {
     three_normal(s, k.normal_range, ax, ay, az);
     const float gx = vx - ax, gy = vy - ay, gz = vz - az;
     const float g2 = gx * gx + gy * gy + gz * gz;
     const float energy = k.half_mu_over_e * g2;
-    const int idx = energy_index(energy, k.de_cs);
-    const float g = __builtin_sqrtf(g2);
-    const float nu = stot_i[idx] * k.stot_scale * g;
+    g = __builtin_sqrtf(g2);
+    return energy_index(energy, k.de_cs);
+}
+
+__device__ __forceinline__ bool i_test(const KPhys &k, float stot, float g, Xo &s) {
+    const float nu = stot * k.stot_scale * g;
     const float rv = draw(s);
     const float pc = 1.0f - expf(-nu * k.dt_i);
-    if (!(rv < pc)) return kNone;
+    return rv < pc;
+}
 
-    const float t0 = cs[3 * kCsRanges + idx];
-    const float t1 = t0 + cs[4 * kCsRanges + idx];
+__device__ __forceinline__ int i_scatter(const KPhys &k, float c0, float c1, float &vx, float &vy,
+                                         float &vz, float ax, float ay, float az, float g, Xo &s) {
+    const float gx = vx - ax, gy = vy - ay, gz = vz - az;
+    const float t0 = c0;
+    const float t1 = t0 + c1;
     const float wx = 0.5f * (vx + ax), wy = 0.5f * (vy + ay), wz = 0.5f * (vz + az);
     const Frame f = frame_of(gx, gy, gz);
     float chi;
@@ -275,6 +307,16 @@ __device__ __forceinline__ int collide_ion(const float *__restrict__ stot_i,
     vy = wy + 0.5f * oy;
     vz = wz + 0.5f * oz;
     return kind;
+}
+
+__device__ __forceinline__ int collide_ion(const float *__restrict__ stot_i,
+                                           const float *__restrict__ cs, const KPhys &k,
+                                           float &vx, float &vy, float &vz, Xo &s,
+                                           const float *stc = nullptr, int stc_n = 0) {
+    float ax, ay, az, g;
+    const int idx = i_index(k, vx, vy, vz, s, ax, ay, az, g);
+    if (!i_test(k, stot_at(stot_i, stc, stc_n, idx), g, s)) return kNone;
+    return i_scatter(k, cs[3 * kCsRanges + idx], cs[4 * kCsRanges + idx], vx, vy, vz, ax, ay, az, g, s);
 }
 
 }  // namespace pic

@@ -329,7 +329,9 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     sh.block = 256;
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
     sh.ppt = c->ppt > 0 ? c->ppt : 4;
-    sh.replicas = c->replicas > 0 ? c->replicas : 4;
+    // global density replicas: 1 or a multiple of 8 (read in groups of eight)
+    sh.replicas = c->replicas > 0 ? c->replicas : 8;
+    if (sh.replicas > 1) sh.replicas = std::min(64, (sh.replicas + 7) / 8 * 8);
     sh.rep_stride = ((s->p0.n_g + 31) / 32) * 32;
     s->sh = resolve(sh);
     s->inst_stride = s->sh.replicas * s->sh.rep_stride;

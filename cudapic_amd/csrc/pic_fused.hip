@@ -31,6 +31,7 @@ struct FusedArgs {
     KPhys k;
     int lds_reps;
     int reps, rep_stride;
+    int stc_n;  // leading bins of the total cross-section table staged in LDS
 };
 
 template <typename Acc>
@@ -108,6 +109,8 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
+    float *stc = (float *)(aux + aux_n);
+    for (int q = threadIdx.x; q < A.stc_n; q += blockDim.x) stc[q] = j.stot[q];
     // live count of every input segment (clamped: a counter that overflowed
     // its segment counts dropped particles) and its tile count
     for (int q = threadIdx.x; q < K; q += blockDim.x) {
@@ -154,9 +157,9 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
         for (int u = 0; u < PPT; ++u) {
             const uint32_t v = base + u * blockDim.x;
             keep[u] = false;
-            if (v < seg_n) {
-                const uint32_t i = in_base + v;
-                const float x = j.sx[i];
+            const uint32_t i = in_base + v;
+            const float x = v < seg_n ? j.sx[i] : -1.0f;
+            if (x >= 0.0f) {  // x < 0 marks a hole left by an in-place step
                 const float e = field_at(ef, x, k.inv_dx, n_g);
                 nvx[u] = __builtin_fmaf(factor, e, j.svx[i]);
                 nx[u] = __builtin_fmaf(nvx[u], dt, x);
@@ -195,8 +198,8 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
                     const uint32_t slot = out_base + local;
                     Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot);
                     float vx = nvx[u], y = vy[u], z = vz[u];
-                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
-                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s);
+                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si, stc, A.stc_n);
+                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s, stc, A.stc_n);
                     rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
                     j.dx[slot] = nx[u];
                     j.dvx[slot] = vx;
@@ -283,19 +286,376 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
             if (aux[q]) atomicAdd(&aux_out[q], aux[q]);
 }
 
+// The in-place step (every step that does not compact): each particle is
+// updated in the slot it occupies, so all of its loads -- x, v and the four
+// RNG words of the slot -- are independent and issued together, and there
+// is no slot reservation.  A particle that leaves the gap becomes a hole
+// (x = -1); electrons born in the step are appended after the segment's
+// watermark (input count cnt_in, copied into cnt_out before the launch),
+// so they are neither pushed nor collision-tested until the next step
+// (main.rs:992).  Per particle the traffic is read x, v, RNG (32 B), write
+// x, vx, RNG (24 B) and vy, vz only after a collision.
+//
+// Collisions are rare (well under 1% of particle-steps) and their
+// scattering code is long, so a wave that meets one would run it for a
+// single lane.  Instead the collision test only queues the colliding
+// particle (slot, velocity, RNG state after the test draw, energy bin) in a
+// per-wave LDS ring, and the wave scatters 64 queued particles at a time
+// with every lane busy; the RNG stream of each particle continues from the
+// queued state, so the draws are the same as in sequential order.
+constexpr int kQ = 128;       // ring entries per wave (pending < 64 + 64 new)
+constexpr int kQWordsE = 10;  // slot, x, vx, vy, vz, rng[4], bin
+constexpr int kQWordsI = 13;  // slot, vx, vy, vz, rng[4], bin, ax, ay, az, g
+
+template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
+__global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
+    const FusedJob &j = A.j;
+    const KPhys &k = A.k;
+    extern __shared__ __align__(16) float sh[];
+    __shared__ uint32_t s_tiles[kMaxSegments + 1], s_cnt[kMaxSegments];
+    constexpr int QW = ELECTRON ? kQWordsE : kQWordsI;
+    const int n_g = k.n_g;
+    const int K = j.n_seg;
+    float *ef = sh;
+    float *hist = sh + n_g;
+    uint32_t *aux = (uint32_t *)(hist + A.lds_reps * n_g);
+    const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
+    uint32_t *queue = aux + aux_n;
+    float *stc = (float *)(queue + (blockDim.x / kWave) * QW * kQ);
+    const uint32_t tile_sz = blockDim.x * PPT;
+    for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
+    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
+    for (int q = threadIdx.x; q < A.stc_n; q += blockDim.x) stc[q] = j.stot[q];
+    for (int q = threadIdx.x; q < K; q += blockDim.x) {
+        const uint32_t c = min(j.cnt_in[q], j.seg_size);
+        s_cnt[q] = c;
+        s_tiles[q + 1] = (c + tile_sz - 1) / tile_sz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        s_tiles[0] = 0;
+        for (int q = 0; q < K; ++q) {
+            acc += s_tiles[q + 1];
+            s_tiles[q + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / kWave;
+    float *h = hist + (wave % A.lds_reps) * n_g;
+    uint32_t *q = queue + wave * QW * kQ;
+    uint32_t q_head = 0, q_tail = 0;  // ring positions (wave-uniform)
+    const uint32_t n_tiles = s_tiles[K];
+    const uint32_t ko = blockIdx.x % (uint32_t)K;  // segment that receives this workgroup's newborns
+    const float factor = ELECTRON ? k.factor_e : k.factor_i;
+    const float dt = ELECTRON ? k.dt_e : k.dt_i;
+    float *const X = (float *)j.sx, *const VX = (float *)j.svx, *const VY = (float *)j.svy,
+                 *const VZ = (float *)j.svz;
+    uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
+    bool overflow = false;
+
+    // Scatter the n (<= 64) oldest queued collisions, one per lane.
+    auto scatter_queued = [&](uint32_t n) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t l = lane_id();
+        const uint32_t e = (q_head + l) & (kQ - 1);
+        int kind = kNone;
+        float se[3], si[3], px = 0.0f;
+        if (l < n) {
+            const uint32_t slot = q[e];
+            if (ELECTRON) {
+                px = __uint_as_float(q[kQ + e]);
+                float vx = __uint_as_float(q[2 * kQ + e]), vy = __uint_as_float(q[3 * kQ + e]),
+                      vz = __uint_as_float(q[4 * kQ + e]);
+                Xo s{q[5 * kQ + e], q[6 * kQ + e], q[7 * kQ + e], q[8 * kQ + e]};
+                const int idx = (int)q[9 * kQ + e];
+                kind = e_scatter(k, j.cs[idx], j.cs[kCsRanges + idx], j.cs[2 * kCsRanges + idx], vx, vy, vz, s,
+                                 se, si);
+                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+                VX[slot] = vx;
+                VY[slot] = vy;
+                VZ[slot] = vz;
+                if (MEASURE) eepf_lds(aux, ef, px, vx, vy, vz, k);
+            } else {
+                float vx = __uint_as_float(q[kQ + e]), vy = __uint_as_float(q[2 * kQ + e]),
+                      vz = __uint_as_float(q[3 * kQ + e]);
+                Xo s{q[4 * kQ + e], q[5 * kQ + e], q[6 * kQ + e], q[7 * kQ + e]};
+                const int idx = (int)q[8 * kQ + e];
+                const float ax = __uint_as_float(q[9 * kQ + e]), ay = __uint_as_float(q[10 * kQ + e]),
+                            az = __uint_as_float(q[11 * kQ + e]), g = __uint_as_float(q[12 * kQ + e]);
+                kind = i_scatter(k, j.cs[3 * kCsRanges + idx], j.cs[4 * kCsRanges + idx], vx, vy, vz, ax, ay, az,
+                                 g, s);
+                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+                VX[slot] = vx;
+                VY[slot] = vy;
+                VZ[slot] = vz;
+            }
+            ++ncoll;
+        }
+        q_head += n;
+        if (ELECTRON) {
+            // ionisation products: the new electron joins this workgroup's
+            // output segment, the new ion the same-numbered ion segment
+            const uint64_t sp = ballot64(kind == kIonisation);
+            if (sp) {
+                const int leader = __builtin_ctzll(sp);
+                uint32_t be = 0, bi = 0;
+                if ((int)lane_id() == leader) {
+                    const uint32_t cnt = (uint32_t)__popcll(sp);
+                    be = atomicAdd(j.cnt_out + ko, cnt);
+                    bi = atomicAdd(j.ion_cnt + ko, cnt);
+                }
+                be = __shfl(be, leader, kWave);
+                bi = __shfl(bi, leader, kWave);
+                if (kind == kIonisation) {
+                    const uint32_t o = mbcnt64(sp);
+                    const uint32_t el = be + o, il = bi + o;
+                    if (el < j.seg_size) {
+                        const uint32_t es = ko * j.seg_size + el;
+                        X[es] = px;
+                        VX[es] = se[0];
+                        VY[es] = se[1];
+                        VZ[es] = se[2];
+                        deposit_lds<Acc>(h, px, k);
+                        if (MEASURE) eepf_lds(aux, ef, px, se[0], se[1], se[2], k);
+                    } else {
+                        overflow = true;
+                    }
+                    if (il < j.ion_seg_size) {
+                        const uint32_t is = ko * j.ion_seg_size + il;
+                        j.ix[is] = px;
+                        j.ivx[is] = si[0];
+                        j.ivy[is] = si[1];
+                        j.ivz[is] = si[2];
+                        deposit_global<Acc>((Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride, px, k);
+                    } else {
+                        overflow = true;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    // (1) particle and RNG loads of all PPT particles of a tile; tile t + G
+    // (G = grid size) is loaded while tile t is processed, so each wave keeps
+    // a tile of loads in flight behind its compute and gathers.
+    // The loads are unconditional (lanes past the segment's count read the
+    // segment's first slot) and the count test is applied where the values
+    // are used, so that no wait for them is needed at the point of issue.
+    auto load_tile = [&](uint32_t tile, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
+                         float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT]) {
+        uint32_t safe = 0;
+        avail = 0;
+        s0 = 0;
+        if (tile < n_tiles) {
+            int lo = 0, hi = K - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_tiles[mid] <= tile) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint32_t local0 = (tile - s_tiles[lo]) * tile_sz + threadIdx.x;
+            safe = (uint32_t)lo * j.seg_size;
+            s0 = safe + local0;
+            avail = s_cnt[lo] > local0 ? s_cnt[lo] - local0 : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint32_t i = u * blockDim.x < avail ? s0 + u * blockDim.x : safe;
+            lx[u] = X[i];
+            lvx[u] = VX[i];
+            lvy[u] = VY[i];
+            lvz[u] = VZ[i];
+            lst[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, i);
+        }
+    };
+    float x[PPT], vx[PPT], vy[PPT], vz[PPT];
+    Xo st[PPT];
+    uint32_t slot0 = 0, avail = 0;
+    load_tile(blockIdx.x, slot0, avail, x, vx, vy, vz, st);
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            if (!(u * blockDim.x < avail)) x[u] = -1.0f;  // past the segment's count
+        // (2) push; in-gap particles bin their energy and gather stot
+        float nx[PPT], spd[PPT], sig[PPT];
+        float ax[PPT], ay[PPT], az[PPT];  // ions: velocity of the collision partner
+        int idx[PPT];
+        bool in[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            in[u] = false;
+            nx[u] = 0.0f;
+            idx[u] = 0;
+            spd[u] = 0.0f;
+            ax[u] = ay[u] = az[u] = 0.0f;
+            if (x[u] >= 0.0f) {
+                const float e = field_at(ef, x[u], k.inv_dx, n_g);
+                vx[u] = __builtin_fmaf(factor, e, vx[u]);
+                nx[u] = __builtin_fmaf(vx[u], dt, x[u]);
+                in[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
+            }
+            if (in[u]) {
+                if (ELECTRON) idx[u] = e_index(k, vx[u], vy[u], vz[u], spd[u]);
+                else idx[u] = i_index(k, vx[u], vy[u], vz[u], st[u], ax[u], ay[u], az[u], spd[u]);
+            }
+        }
+        // bins past the LDS copy come from global memory; the gather is
+        // issued for every lane (the others read bin 0, one shared line) so
+        // that it stays in flight while the next tile's loads are issued
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) sig[u] = j.stot[(in[u] && idx[u] >= A.stc_n) ? idx[u] : 0];
+        float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
+        Xo pst[PPT];
+        uint32_t pslot0, pavail;
+        load_tile(tile + gridDim.x, pslot0, pavail, px, pvx, pvy, pvz, pst);
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            if (idx[u] < A.stc_n) sig[u] = stc[idx[u]];
+        // (3) collision tests, write-back of the particles that do not
+        // collide, deposits; colliding particles are queued; lost ones
+        // leave a hole
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint32_t i = slot0 + u * blockDim.x;
+            bool hit = false;
+            if (in[u]) {
+                hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
+                X[i] = nx[u];
+                deposit_lds<Acc>(h, nx[u], k);
+                if (!hit) {
+                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                    VX[i] = vx[u];
+                    if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], vx[u], vy[u], vz[u], k);
+                }
+            } else if (x[u] >= 0.0f) {
+                X[i] = -1.0f;  // hole
+                const bool pw = nx[u] < 0.0f;
+                lost_pow += pw;
+                lost_gnd += !pw;
+                if (!ELECTRON) {
+                    const float en = k.half_mi_over_e * (vx[u] * vx[u] + vy[u] * vy[u] + vz[u] * vz[u]);
+                    const int b = (int)en;
+                    if (en >= 0.0f && b < kIfedBins) atomicAdd(&aux[(pw ? 0 : 1) * kIfedBins + b], 1u);
+                }
+            }
+            const uint64_t m = ballot64(hit);
+            if (m) {
+                if (hit) {
+                    const uint32_t e = (q_tail + mbcnt64(m)) & (kQ - 1);
+                    q[e] = i;
+                    if (ELECTRON) {
+                        q[kQ + e] = __float_as_uint(nx[u]);
+                        q[2 * kQ + e] = __float_as_uint(vx[u]);
+                        q[3 * kQ + e] = __float_as_uint(vy[u]);
+                        q[4 * kQ + e] = __float_as_uint(vz[u]);
+                        q[5 * kQ + e] = st[u].s0;
+                        q[6 * kQ + e] = st[u].s1;
+                        q[7 * kQ + e] = st[u].s2;
+                        q[8 * kQ + e] = st[u].s3;
+                        q[9 * kQ + e] = (uint32_t)idx[u];
+                    } else {
+                        q[kQ + e] = __float_as_uint(vx[u]);
+                        q[2 * kQ + e] = __float_as_uint(vy[u]);
+                        q[3 * kQ + e] = __float_as_uint(vz[u]);
+                        q[4 * kQ + e] = st[u].s0;
+                        q[5 * kQ + e] = st[u].s1;
+                        q[6 * kQ + e] = st[u].s2;
+                        q[7 * kQ + e] = st[u].s3;
+                        q[8 * kQ + e] = (uint32_t)idx[u];
+                        q[9 * kQ + e] = __float_as_uint(ax[u]);
+                        q[10 * kQ + e] = __float_as_uint(ay[u]);
+                        q[11 * kQ + e] = __float_as_uint(az[u]);
+                        q[12 * kQ + e] = __float_as_uint(spd[u]);
+                    }
+                }
+                q_tail += (uint32_t)__popcll(m);
+                if (q_tail - q_head >= (uint32_t)kWave) scatter_queued(kWave);
+            }
+        }
+        slot0 = pslot0;
+        avail = pavail;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            x[u] = px[u];
+            vx[u] = pvx[u];
+            vy[u] = pvy[u];
+            vz[u] = pvz[u];
+            st[u] = pst[u];
+        }
+    }
+    if (q_tail != q_head) scatter_queued(q_tail - q_head);
+    {
+        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
+        if (lane_id() == 0 && j.counters) {
+            const int b = ELECTRON ? 0 : 2;
+            if (wp) atomicAdd(&j.counters[b], (unsigned long long)wp);
+            if (wg) atomicAdd(&j.counters[b + 1], (unsigned long long)wg);
+            if (wc) atomicAdd(&j.counters[ELECTRON ? 4 : 5], (unsigned long long)wc);
+        }
+        if (overflow) *j.overflow = 1u;
+    }
+    if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
+    Acc *out = (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride;
+    for (int q2 = threadIdx.x; q2 < n_g; q2 += blockDim.x) {
+        float sum = 0.0f;
+        for (int r = 0; r < A.lds_reps; ++r) sum += hist[r * n_g + q2];
+        if (sum != 0.0f) {
+            Acc v = (Acc)sum;
+            if (q2 == 0 || q2 == n_g - 1) v *= (Acc)2;
+            unsafeAtomicAdd(&out[q2], v);
+        }
+    }
+    uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;
+    if (aux_out)
+        for (int q2 = threadIdx.x; q2 < aux_n; q2 += blockDim.x)
+            if (aux[q2]) atomicAdd(&aux_out[q2], aux[q2]);
+}
+
+// Live particles (x >= 0) of a segmented species, for the host's counts.
+__global__ __launch_bounds__(256) void k_count_live(const float *__restrict__ x, const uint32_t *__restrict__ cnt,
+                                                   int n_seg, uint32_t seg_size, uint32_t *__restrict__ out) {
+    uint32_t c = 0;
+    for (int q = 0; q < n_seg; ++q) {
+        const uint32_t n = min(cnt[q], seg_size);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            c += x[(size_t)q * seg_size + i] >= 0.0f;
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+int launch_count_live(const float *x, const uint32_t *cnt, int n_seg, uint32_t seg_size, uint32_t *out,
+                      hipStream_t s) {
+    k_count_live<<<device_cus() * 4, 256, 0, s>>>(x, cnt, n_seg, seg_size, out);
+    PIC_LAUNCHED("k_count_live");
+    return PIC_OK;
+}
+
 template <bool E, bool M, int PPT>
-static void launch_t(const FusedArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
-    if (f64) k_fused<E, M, PPT, double><<<grid, block, lds, s>>>(a);
-    else k_fused<E, M, PPT, float><<<grid, block, lds, s>>>(a);
+static void launch_t(const FusedArgs &a, bool f64, bool inplace, int grid, int block, size_t lds, hipStream_t s) {
+    if (inplace) {
+        if (f64) k_inplace<E, M, PPT, double><<<grid, block, lds, s>>>(a);
+        else k_inplace<E, M, PPT, float><<<grid, block, lds, s>>>(a);
+    } else {
+        if (f64) k_fused<E, M, PPT, double><<<grid, block, lds, s>>>(a);
+        else k_fused<E, M, PPT, float><<<grid, block, lds, s>>>(a);
+    }
 }
 
 template <bool E, bool M>
-static void launch_ppt(const FusedArgs &a, bool f64, int ppt, int grid, int block, size_t lds,
+static void launch_ppt(const FusedArgs &a, bool f64, bool inplace, int ppt, int grid, int block, size_t lds,
                        hipStream_t s) {
     switch (ppt) {
-    case 1: launch_t<E, M, 1>(a, f64, grid, block, lds, s); break;
-    case 2: launch_t<E, M, 2>(a, f64, grid, block, lds, s); break;
-    default: launch_t<E, M, 4>(a, f64, grid, block, lds, s); break;
+    case 1: launch_t<E, M, 1>(a, f64, inplace, grid, block, lds, s); break;
+    case 2: launch_t<E, M, 2>(a, f64, inplace, grid, block, lds, s); break;
+    default: launch_t<E, M, 4>(a, f64, inplace, grid, block, lds, s); break;
     }
 }
 
@@ -311,20 +671,31 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
     const int block = 256;
     const bool measure = j.electrons && j.eepf != nullptr;
     const int aux_words = j.electrons ? (measure ? kEepfBins : 0) : 2 * kIfedBins;
-    const int fixed = (k.n_g + aux_words) * 4;
-    const int budget = 40 * 1024;
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    // LDS per workgroup: what the resident workgroups of a compute unit can
+    // share (2 KB kept for the static arrays), at most what one may allocate.
+    // It holds the field, the EEPF / IFED bins, the collision queues of the
+    // in-place kernel, one histogram replica per wave (fewer when space is
+    // short) and then as many leading bins of the total cross-section table
+    // as fit (the collision test reads one bin per particle).
+    const int per_cu = std::max(1, (grid + device_cus() - 1) / device_cus());
+    const int budget = std::min(device_lds_per_block(), device_lds_per_cu() / per_cu - 2048);
+    const int queue_words = j.inplace ? (block / kWave) * (j.electrons ? kQWordsE : kQWordsI) * kQ : 0;
+    const int fixed = (k.n_g + aux_words + queue_words) * 4;
     a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (k.n_g * 4)));
     a.reps = sh.replicas;
     a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
-    const size_t lds = (size_t)fixed + (size_t)a.lds_reps * k.n_g * 4;
-    const int grid = std::max(1, sh.grid * sh.block / block);
+    const int used = fixed + a.lds_reps * k.n_g * 4;
+    a.stc_n = std::max(0, std::min(kCsRanges, (budget - used) / 4));
+    const size_t lds = (size_t)used + (size_t)a.stc_n * 4;
+    const int ppt = j.inplace ? sh.ppt_inplace : sh.ppt;
     if (j.electrons) {
-        if (measure) launch_ppt<true, true>(a, j.dens_f64, sh.ppt, grid, block, lds, s);
-        else launch_ppt<true, false>(a, j.dens_f64, sh.ppt, grid, block, lds, s);
+        if (measure) launch_ppt<true, true>(a, j.dens_f64, j.inplace, ppt, grid, block, lds, s);
+        else launch_ppt<true, false>(a, j.dens_f64, j.inplace, ppt, grid, block, lds, s);
     } else {
-        launch_ppt<false, false>(a, j.dens_f64, sh.ppt, grid, block, lds, s);
+        launch_ppt<false, false>(a, j.dens_f64, j.inplace, ppt, grid, block, lds, s);
     }
-    PIC_LAUNCHED("k_fused");
+    PIC_LAUNCHED(j.inplace ? "k_inplace" : "k_fused");
     return PIC_OK;
 }
 

@@ -26,7 +26,9 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::pic::hip_fail(e_, what);    \
     } while (0)
 
-int device_cus();  // compute units of the current device (cached)
+int device_cus();            // compute units of the current device (cached)
+int device_lds_per_cu();     // LDS bytes per compute unit
+int device_lds_per_block();  // LDS bytes one workgroup may allocate
 
 KPhys make_kphys(const pic_params_t &p);
 
@@ -34,7 +36,8 @@ KPhys make_kphys(const pic_params_t &p);
 struct Shape {
     int block = 256;   // threads per workgroup (multiple of 64, <= 1024)
     int grid = 0;      // persistent workgroups (0 = 8 per compute unit)
-    int ppt = 4;       // particles per thread per tile
+    int ppt = 4;       // particles per thread per tile (compacting kernels)
+    int ppt_inplace = 2;  // particles per thread per tile (in-place kernel)
     int replicas = 1;  // global density replicas the flush spreads over
     int rep_stride = 0;
 };
@@ -66,6 +69,7 @@ struct PoissonJob {
     void *zero_b = nullptr; int zero_b_words = 0;   // cleared on ion steps only
     uint32_t *zero_cnt = nullptr; int zero_cnt_words = 0;          // cleared every step
     uint32_t *zero_cnt_ion = nullptr; int zero_cnt_ion_words = 0;  // cleared on ion steps
+    const uint32_t *copy_cnt_src = nullptr; uint32_t *copy_cnt_dst = nullptr; int copy_cnt_words = 0;
     uint32_t *advance_time = nullptr;                               // t <- (t+1) % n_t
 };
 int launch_poisson(const PoissonJob &j, hipStream_t s);
@@ -121,6 +125,7 @@ int launch_eepf(const float *x, const float *vx, const float *vy, const float *v
 constexpr int kMaxSegments = 64;
 struct FusedJob {
     bool electrons;
+    bool inplace;                         // update in the occupied slots (sx.. are written)
     const float *efield;
     const float *sx, *svx, *svy, *svz;    // input segments
     const uint32_t *cnt_in;               // [n_seg]
@@ -145,6 +150,9 @@ struct FusedJob {
     uint32_t *ifed;                       // ions only
 };
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+// *out += number of x >= 0 in the first cnt[q] slots of each segment q
+int launch_count_live(const float *x, const uint32_t *cnt, int n_seg, uint32_t seg_size, uint32_t *out,
+                      hipStream_t s);
 
 // ---------------------------------------------------------------- ensembles
 // One independent discharge of an ensemble: its physics, its driving

@@ -28,27 +28,40 @@ int hip_fail(hipError_t e, const char *what) {
     return PIC_ERR_HIP;
 }
 
-int device_cus() {
+namespace {
+// One integer attribute of the current device, cached per device; `dflt`
+// if the query fails.
+int device_attr(hipDeviceAttribute_t attr, int slot, int dflt) {
     static std::mutex mu;
     static std::vector<int> cache;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipGetDevice(&dev) != hipSuccess) return dflt;
     std::lock_guard<std::mutex> lk(mu);
-    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
-    if (cache[dev] == 0) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        cache[dev] = cus;
+    const size_t at = (size_t)dev * 4 + slot;
+    if (cache.size() <= at) cache.resize(at + 1, 0);
+    if (cache[at] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) v = dflt;
+        cache[at] = v;
     }
-    return cache[dev];
+    return cache[at];
 }
+}  // namespace
+
+int device_cus() { return device_attr(hipDeviceAttributeMultiprocessorCount, 0, 256); }
+
+int device_lds_per_cu() {
+    return device_attr(hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 1, 64 * 1024);
+}
+
+int device_lds_per_block() { return device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, 2, 64 * 1024); }
 
 Shape resolve(Shape s) {
     if (s.block <= 0) s.block = 256;
     s.block = std::min(1024, std::max(64, (s.block / 64) * 64));
     if (s.grid <= 0) s.grid = device_cus() * std::max(1, 2048 / s.block);
     if (s.ppt <= 0) s.ppt = 4;
+    if (s.ppt_inplace <= 0) s.ppt_inplace = 2;
     if (s.replicas <= 0) s.replicas = 1;
     return s;
 }
@@ -145,7 +158,7 @@ __global__ __launch_bounds__(256) void k_deposit(const float *__restrict__ x,
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            if (i + u * stride < n) {
+            if (i + u * stride < n && xv[u] >= 0.0f) {  // x < 0: a hole
                 const float pos = xv[u] * inv_dx;
                 const int q = cell_of(pos, n_g);
                 const float rem = pos - (float)q;
@@ -306,6 +319,9 @@ struct PoissonArgs {
     int zero_cnt_words;
     uint32_t *zero_cnt_ion;
     int zero_cnt_ion_words;
+    const uint32_t *copy_cnt_src;
+    uint32_t *copy_cnt_dst;
+    int copy_cnt_words;
     uint32_t *advance_time;
 };
 
@@ -483,6 +499,7 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
         for (int w = tid; w < a.zero_b_words; w += blockDim.x) a.zero_b[w] = 0u;
     if (a.zero_cnt && tid < a.zero_cnt_words) a.zero_cnt[tid] = 0u;
     if (s_ion && a.zero_cnt_ion && tid < a.zero_cnt_ion_words) a.zero_cnt_ion[tid] = 0u;
+    if (a.copy_cnt_dst && tid < a.copy_cnt_words) a.copy_cnt_dst[tid] = a.copy_cnt_src[tid];
     if (tid == 1023 && a.meas_steps && a.cumul_e) {
         a.meas_steps[0] += 1ull;
         if (s_ion) a.meas_steps[1] += 1ull;
@@ -494,6 +511,10 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
 int launch_poisson(const PoissonJob &j, hipStream_t s) {
     if (j.n_g < 3 || j.n_g > kPoissonThreads * kPoissonMaxPer) {
         set_error("poisson: n_g must be in [3, 4096]");
+        return PIC_ERR_ARG;
+    }
+    if (j.replicas != 1 && (j.replicas % 8) != 0) {
+        set_error("poisson: density replicas must be 1 or a multiple of 8");
         return PIC_ERR_ARG;
     }
     PoissonArgs a;
@@ -532,6 +553,9 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     a.zero_cnt_words = j.zero_cnt_words;
     a.zero_cnt_ion = j.zero_cnt_ion;
     a.zero_cnt_ion_words = j.zero_cnt_ion_words;
+    a.copy_cnt_src = j.copy_cnt_src;
+    a.copy_cnt_dst = j.copy_cnt_dst;
+    a.copy_cnt_words = j.copy_cnt_words;
     a.advance_time = j.advance_time;
     switch (j.precision) {
     case PIC_PREC_FP32: {
@@ -612,7 +636,8 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
 }
 
 int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
-    if (j.g.n_g < 3 || j.g.n_g > kPoissonThreads * kPoissonMaxPer || j.n_inst <= 0) {
+    if (j.g.n_g < 3 || j.g.n_g > kPoissonThreads * kPoissonMaxPer || j.n_inst <= 0 ||
+        (j.g.reps != 1 && (j.g.reps % 8) != 0)) {
         set_error("poisson_ens: bad arguments");
         return PIC_ERR_ARG;
     }

@@ -10,7 +10,7 @@
 //  * PIC_PATH_FUSED runs Poisson (which also clears the next step's
 //    accumulators and counters and advances the device clock) and one fused
 //    particle kernel per species.  Its launch sequence repeats every
-//    2*N_SUB steps with identical arguments, so it is captured once into a
+//    4*N_SUB steps with identical arguments, so it is captured once into a
 //    hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
@@ -68,7 +68,8 @@ struct pic_sim {
     // tables
     float *cs = nullptr, *stot_e = nullptr, *stot_i = nullptr;
     // bookkeeping
-    int ce = 0, ci = 0;      // current particle parity
+    int ce = 0, ci = 0;      // current particle-array parity (electrons, ions)
+    int pe = 0;              // electron counter parity (flips every fused step)
     int cd = 0, hi = 0;      // fused: current e density, held ion density
     int last_e = 0, last_i = 0;  // densities used by the last Poisson solve
     uint32_t t = 0;          // next step's index in the cycle (host mirror)
@@ -76,7 +77,7 @@ struct pic_sim {
     bool need_prologue = true;
     unsigned long long ref_meas[2] = {0, 0};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_roles[2][4] = {};
+    int graph_roles[2][5] = {};
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -153,7 +154,11 @@ double host_pot0(const pic_params_t &p, uint32_t t) {
 int step_reference(pic_sim *s, bool measure) {
     hipStream_t st = s->stream;
     const bool ion = (s->t % (uint32_t)s->p.n_sub) == 0;
-    uint32_t *ne = ecnt(s, s->ce);
+    // the ion density is refreshed on ion steps; after an upload, a load or
+    // a clock change it is deposited on the first step whatever its index
+    const bool dep_i = ion || s->need_prologue;
+    s->need_prologue = false;
+    uint32_t *ne = ecnt(s, s->pe);
     uint32_t *ni = icnt(s, s->ci);
     uint32_t *alive = s->cnt + C_ALIVE;
     Shape one = s->sh;
@@ -161,7 +166,7 @@ int step_reference(pic_sim *s, bool measure) {
     // main.rs:1729-1739
     TRY(launch_fill(s->e_dens[0], 0u, (uint64_t)s->p.n_g * (s->f64 ? 2 : 1), st));
     TRY(launch_deposit(s->e[s->ce][0], ne, s->e_dens[0], s->f64, s->k, one, st, s->cap));
-    if (ion) {
+    if (dep_i) {
         TRY(launch_fill(s->i_dens[0], 0u, (uint64_t)s->p.n_g * (s->f64 ? 2 : 1), st));
         TRY(launch_deposit(s->ion[s->ci][0], ni, s->i_dens[0], s->f64, s->k, one, st, s->cap));
     }
@@ -192,7 +197,8 @@ int step_reference(pic_sim *s, bool measure) {
                alive, ne, s->k.factor_e, s->k.dt_e, 0.0f, s->counters + 0, nullptr, s->cap};
     TRY(launch_move_compact(mj, s->k, s->sh, st));
     s->ce ^= 1;
-    ne = ecnt(s, s->ce);
+    s->pe ^= 1;
+    ne = ecnt(s, s->pe);
     TRY(launch_copy_u32(ne, alive, st));
     // main.rs:1770-1783: ions
     if (ion) {
@@ -240,7 +246,7 @@ int fused_prologue(pic_sim *s) {
     TRY(launch_fill(s->i_dens[1], 0u, dens_words32(s), st));
     for (int q = 0; q < s->nseg; ++q) {
         const size_t o = (size_t)q * s->seg;
-        TRY(launch_deposit(s->e[s->ce][0] + o, ecnt(s, s->ce) + q, s->e_dens[s->cd], s->f64, s->k, sh, st, s->seg));
+        TRY(launch_deposit(s->e[s->ce][0] + o, ecnt(s, s->pe) + q, s->e_dens[s->cd], s->f64, s->k, sh, st, s->seg));
         TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[0], s->f64, s->k, sh, st, s->seg));
         TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[1], s->f64, s->k, sh, st, s->seg));
     }
@@ -251,9 +257,17 @@ int fused_prologue(pic_sim *s) {
     return PIC_OK;
 }
 
+// One fused step.  Every 2*N_SUB steps the particle kernels compact (src ->
+// dst arrays, holes squeezed out, survivors take the RNG stream of their new
+// slot); the other steps update the particles in place.  The electron live
+// counters are double-buffered every step (the in-place kernel appends
+// newborns after a frozen copy of the watermark).
 int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEvent_t e1) {
     hipStream_t st = s->stream;
-    const bool ion = (s->t % (uint32_t)s->p.n_sub) == 0;
+    const uint32_t t = s->t;
+    const bool ion = (t % (uint32_t)s->p.n_sub) == 0;
+    const bool comp_e = (t % (2u * (uint32_t)s->p.n_sub)) == 0;
+    const bool comp_i = ion && comp_e;
     // Poisson of this step: e density accumulated by the previous step,
     // ion density = the held one, or on ion steps the accumulated one
     PoissonJob pj = base_poisson(s);
@@ -268,9 +282,15 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
         pj.zero_b = s->i_dens[s->hi];  // old held buffer becomes the accumulator
         pj.zero_b_words = (int)dens_words32(s);
     }
-    pj.zero_cnt = ecnt(s, 1 - s->ce);
-    pj.zero_cnt_words = s->nseg;
-    if (ion) {
+    if (comp_e) {
+        pj.zero_cnt = ecnt(s, 1 - s->pe);
+        pj.zero_cnt_words = s->nseg;
+    } else {
+        pj.copy_cnt_src = ecnt(s, s->pe);
+        pj.copy_cnt_dst = ecnt(s, 1 - s->pe);
+        pj.copy_cnt_words = s->nseg;
+    }
+    if (comp_i) {
         pj.zero_cnt_ion = icnt(s, 1 - s->ci);
         pj.zero_cnt_ion_words = s->nseg;
     }
@@ -295,6 +315,7 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     if (ion) {
         FusedJob fi = base;
         fi.electrons = false;
+        fi.inplace = !comp_i;
         fi.sx = s->ion[s->ci][0];
         fi.svx = s->ion[s->ci][1];
         fi.svy = s->ion[s->ci][2];
@@ -313,20 +334,21 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
         fi.stot = s->stot_i;
         fi.ifed = s->ifed;
         TRY(launch_fused(fi, s->k, s->sh, st));
-        s->ci ^= 1;
+        if (comp_i) s->ci ^= 1;
     }
     FusedJob fe = base;
     fe.electrons = true;
+    fe.inplace = !comp_e;
     fe.sx = s->e[s->ce][0];
     fe.svx = s->e[s->ce][1];
     fe.svy = s->e[s->ce][2];
     fe.svz = s->e[s->ce][3];
-    fe.cnt_in = ecnt(s, s->ce);
+    fe.cnt_in = ecnt(s, s->pe);
     fe.dx = s->e[1 - s->ce][0];
     fe.dvx = s->e[1 - s->ce][1];
     fe.dvy = s->e[1 - s->ce][2];
     fe.dvz = s->e[1 - s->ce][3];
-    fe.cnt_out = ecnt(s, 1 - s->ce);
+    fe.cnt_out = ecnt(s, 1 - s->pe);
     fe.rng0 = s->e_rng[0];
     fe.rng1 = s->e_rng[1];
     fe.rng2 = s->e_rng[2];
@@ -344,7 +366,8 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     if (time_kernel) PIC_HIP(hipEventRecord(e0, st));
     TRY(launch_fused(fe, s->k, s->sh, st));
     if (time_kernel) PIC_HIP(hipEventRecord(e1, st));
-    s->ce ^= 1;
+    if (comp_e) s->ce ^= 1;
+    s->pe ^= 1;
     s->cd ^= 1;
     if (ion) s->hi ^= 1;
     return PIC_OK;
@@ -367,12 +390,12 @@ int one_step(pic_sim *s, bool measure) {
     return PIC_OK;
 }
 
-// Capture 2*N_SUB fused steps (every role flag returns to its start value
+// Capture 4*N_SUB fused steps (every role flag returns to its start value
 // after that many steps) into a graph for this measure flag.
 int capture_graph(pic_sim *s, bool measure) {
     const int m = measure ? 1 : 0;
-    const int period = 2 * s->p.n_sub;
-    int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+    const int period = 4 * s->p.n_sub;
+    int roles[5] = {s->ce, s->ci, s->cd, s->hi, s->pe};
     uint32_t t0 = s->t;
     unsigned long long steps0 = s->steps;
     hipGraph_t g = nullptr;
@@ -387,6 +410,7 @@ int capture_graph(pic_sim *s, bool measure) {
     s->ci = roles[1];
     s->cd = roles[2];
     s->hi = roles[3];
+    s->pe = roles[4];
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -403,13 +427,13 @@ int capture_graph(pic_sim *s, bool measure) {
 
 int run_steps(pic_sim *s, uint32_t n, bool measure) {
     if (s->cfg.path == PIC_PATH_FUSED && s->need_prologue) TRY(fused_prologue(s));
-    const uint32_t period = 2u * (uint32_t)s->p.n_sub;
+    const uint32_t period = 4u * (uint32_t)s->p.n_sub;
     const bool graphs = s->cfg.use_graph && s->cfg.path == PIC_PATH_FUSED &&
                         (uint32_t)s->p.n_t % period == 0;
     while (n > 0) {
         if (graphs && s->t % period == 0 && n >= period) {
             const int m = measure ? 1 : 0;
-            int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+            int roles[5] = {s->ce, s->ci, s->cd, s->hi, s->pe};
             if (s->graph[m] && memcmp(roles, s->graph_roles[m], sizeof(roles)) != 0) {
                 (void)hipGraphExecDestroy(s->graph[m]);
                 s->graph[m] = nullptr;
@@ -506,6 +530,7 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     // of 4 workgroups of 256 lanes per compute unit is fully resident
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
     sh.ppt = c->ppt > 0 ? c->ppt : 4;
+    sh.ppt_inplace = c->ppt_inplace > 0 ? c->ppt_inplace : 2;
     // global density replicas: 1, or a multiple of 8 (the Poisson kernel
     // reads them in groups of eight)
     sh.replicas = c->replicas > 0 ? c->replicas : (c->path == PIC_PATH_FUSED ? 16 : 1);
@@ -590,7 +615,7 @@ int pic_sim_upload(pic_sim_t *s, int species_id, const float *x, const float *vx
     }
     PIC_HIP(hipStreamSynchronize(s->stream));
     float **a = species_id == PIC_ELECTRONS ? s->e[s->ce] : s->ion[s->ci];
-    uint32_t *cnt = species_id == PIC_ELECTRONS ? ecnt(s, s->ce) : icnt(s, s->ci);
+    uint32_t *cnt = species_id == PIC_ELECTRONS ? ecnt(s, s->pe) : icnt(s, s->ci);
     const float *src[4] = {x, vx, vy, vz};
     // consecutive runs of the host arrays go to consecutive segments
     std::vector<uint32_t> c(kMaxSegments, 0u);
@@ -614,31 +639,54 @@ int pic_sim_segments(pic_sim_t *s, int species_id, uint32_t *n_seg, uint32_t *se
     if (n_seg) *n_seg = (uint32_t)s->nseg;
     if (seg_size) *seg_size = s->seg;
     if (counts) {
-        uint32_t *cnt = species_id == PIC_ELECTRONS ? ecnt(s, s->ce) : icnt(s, s->ci);
+        uint32_t *cnt = species_id == PIC_ELECTRONS ? ecnt(s, s->pe) : icnt(s, s->ci);
         PIC_HIP(hipMemcpy(counts, cnt, 4ull * s->nseg, hipMemcpyDeviceToHost));
         for (int q = 0; q < s->nseg; ++q) counts[q] = std::min(counts[q], s->seg);
     }
     return PIC_OK;
 }
 
-int pic_sim_download(pic_sim_t *s, int species_id, float *x, float *vx, float *vy, float *vz,
-                     uint32_t cap, uint32_t *n) {
-    if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS)) return PIC_ERR_ARG;
+// Live particles of a species in segment order, holes (x < 0) skipped, with
+// the slot of each when `slots` is given.
+static int gather_live(pic_sim *s, int species_id, float *x, float *vx, float *vy, float *vz,
+                       uint32_t *slots, uint32_t cap, uint32_t *n) {
     std::vector<uint32_t> c(kMaxSegments, 0u);
     TRY(pic_sim_segments(s, species_id, nullptr, nullptr, c.data()));
     float **a = species_id == PIC_ELECTRONS ? s->e[s->ce] : s->ion[s->ci];
     float *dst[4] = {x, vx, vy, vz};
-    uint32_t total = 0, copied = 0;
+    std::vector<float> buf[4];
+    uint32_t total = 0;
     for (int q = 0; q < s->nseg; ++q) {
-        total += c[q];
-        const uint32_t m = std::min(c[q], cap - copied);
-        for (int w = 0; w < 4; ++w)
-            if (dst[w] && m)
-                PIC_HIP(hipMemcpy(dst[w] + copied, a[w] + (size_t)q * s->seg, 4ull * m, hipMemcpyDeviceToHost));
-        copied += m;
+        const uint32_t m = c[q];
+        if (!m) continue;
+        for (int w = 0; w < 4; ++w) {
+            buf[w].resize(m);
+            if (w == 0 || dst[w])
+                PIC_HIP(hipMemcpy(buf[w].data(), a[w] + (size_t)q * s->seg, 4ull * m, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < m; ++i) {
+            if (!(buf[0][i] >= 0.0f)) continue;
+            if (total < cap) {
+                for (int w = 0; w < 4; ++w)
+                    if (dst[w]) dst[w][total] = buf[w][i];
+                if (slots) slots[total] = q * s->seg + i;
+            }
+            ++total;
+        }
     }
     if (n) *n = total;
     return PIC_OK;
+}
+
+int pic_sim_download(pic_sim_t *s, int species_id, float *x, float *vx, float *vy, float *vz,
+                     uint32_t cap, uint32_t *n) {
+    if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS)) return PIC_ERR_ARG;
+    return gather_live(s, species_id, x, vx, vy, vz, nullptr, cap, n);
+}
+
+int pic_sim_download_slots(pic_sim_t *s, int species_id, uint32_t *slots, uint32_t cap, uint32_t *n) {
+    if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS)) return PIC_ERR_ARG;
+    return gather_live(s, species_id, nullptr, nullptr, nullptr, nullptr, slots, cap, n);
 }
 
 int pic_sim_rng_upload(pic_sim_t *s, int species_id, const uint32_t *w0, const uint32_t *w1,
@@ -664,11 +712,24 @@ int pic_sim_rng_download(pic_sim_t *s, int species_id, uint32_t *w0, uint32_t *w
 
 int pic_sim_counts(pic_sim_t *s, uint32_t *n_e, uint32_t *n_i) {
     if (!s) return PIC_ERR_ARG;
-    std::vector<uint32_t> c(kMaxSegments);
     uint32_t tot[2] = {0u, 0u};
-    for (int sp = 0; sp < 2; ++sp) {
-        TRY(pic_sim_segments(s, sp, nullptr, nullptr, c.data()));
-        for (int q = 0; q < s->nseg; ++q) tot[sp] += c[q];
+    if (s->cfg.path == PIC_PATH_FUSED) {
+        // in-place steps leave holes: count the live slots on the device
+        uint32_t *d = s->cnt + C_ALIVE;
+        for (int sp = 0; sp < 2; ++sp) {
+            TRY(launch_copy_u32(d, nullptr, s->stream));
+            float *x = sp == 0 ? s->e[s->ce][0] : s->ion[s->ci][0];
+            uint32_t *c = sp == 0 ? ecnt(s, s->pe) : icnt(s, s->ci);
+            TRY(launch_count_live(x, c, s->nseg, s->seg, d, s->stream));
+            PIC_HIP(hipMemcpyAsync(&tot[sp], d, 4, hipMemcpyDeviceToHost, s->stream));
+            PIC_HIP(hipStreamSynchronize(s->stream));
+        }
+    } else {
+        std::vector<uint32_t> c(kMaxSegments);
+        for (int sp = 0; sp < 2; ++sp) {
+            TRY(pic_sim_segments(s, sp, nullptr, nullptr, c.data()));
+            for (int q = 0; q < s->nseg; ++q) tot[sp] += c[q];
+        }
     }
     if (n_e) *n_e = tot[0];
     if (n_i) *n_i = tot[1];

@@ -12,7 +12,8 @@ from ._abi import (PIC_ELECTRONS, PIC_IONS, PIC_N_EEPF, PIC_N_IFED, PIC_PATH_FUS
 
 class Sim:
     def __init__(self, p, capacity=120_000, path=PIC_PATH_FUSED, use_graph=True, seed_e=None,
-                 seed_i=None, segments=0, block_size=0, grid_blocks=0, ppt=0, replicas=0):
+                 seed_i=None, segments=0, block_size=0, grid_blocks=0, ppt=0, replicas=0,
+                 ppt_inplace=0):
         cfg = PicSimConfig()
         lib().pic_sim_config_default(C.byref(cfg))
         cfg.params = p
@@ -25,6 +26,7 @@ class Sim:
         cfg.path = int(path)
         cfg.use_graph = int(bool(use_graph))
         cfg.block_size, cfg.grid_blocks, cfg.ppt, cfg.replicas = block_size, grid_blocks, ppt, replicas
+        cfg.ppt_inplace = ppt_inplace
         self.p = p
         self.cfg = cfg
         self.capacity = int(capacity)
@@ -54,9 +56,11 @@ class Sim:
 
     def slots(self, species):
         """Slot index of every particle in download() order."""
-        ns, sz, cnt = self.segments(species)
-        return np.concatenate([q * sz + np.arange(c, dtype=np.int64) for q, c in enumerate(cnt)]) \
-            if ns else np.zeros(0, np.int64)
+        n = C.c_uint32()
+        call("pic_sim_download_slots", self.h, species, None, 0, C.byref(n))
+        out = np.zeros(max(1, n.value), np.uint32)
+        call("pic_sim_download_slots", self.h, species, out.ctypes.data, n.value, C.byref(n))
+        return out[:n.value].astype(np.int64)
 
     def rng_upload(self, species, words):
         w = [np.ascontiguousarray(a, np.uint32) for a in words]

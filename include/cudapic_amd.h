@@ -183,12 +183,15 @@ typedef struct pic_sim_config {
     uint32_t seed_e[4];        /* master xoshiro states (main.rs:1667, 1671) */
     uint32_t seed_i[4];
     int32_t path;              /* enum pic_path */
-    int32_t use_graph;         /* capture 2*N_SUB steps into a hipGraph */
+    int32_t use_graph;         /* fused path: capture 4*N_SUB steps into a hipGraph */
     int32_t block_size;        /* particle kernels: 0 = tuned default */
     int32_t grid_blocks;       /* persistent grid: 0 = tuned default */
-    int32_t ppt;               /* particles per thread per tile: 0 = default */
+    int32_t ppt;               /* particles per thread per tile, compacting
+                                  steps: 0 = default (4) */
     int32_t replicas;          /* global density replicas: 0 = default */
     int32_t measure;           /* allocate measurement buffers */
+    int32_t ppt_inplace;       /* particles per thread per tile, in-place
+                                  steps: 0 = default (2) */
 } pic_sim_config_t;
 
 typedef struct pic_sim pic_sim_t;
@@ -204,13 +207,18 @@ int pic_sim_upload(pic_sim_t *s, int species, const float *x, const float *vx,
                    const float *vy, const float *vz, uint32_t n);
 int pic_sim_download(pic_sim_t *s, int species, float *x, float *vx, float *vy, float *vz,
                      uint32_t cap, uint32_t *n);
-/* Storage of a species: number of segments, slots per segment and the live
- * count of each segment (counts[n_seg], may be NULL).  Particle j of segment
- * q sits in slot q*seg_size + j; pic_sim_download returns the segments in
- * order.  The fused path needs a few percent of headroom in `capacity`
- * because segments fill independently. */
+/* Storage of a species: number of segments, slots per segment and the
+ * number of slots in use in each segment (counts[n_seg], may be NULL).
+ * Slots in use q*seg_size + [0, counts[q]) hold the particles; on the fused
+ * path a slot whose particle left the gap in an in-place step is a hole
+ * (x < 0) until the next compacting step (every 2*N_SUB steps).
+ * pic_sim_download returns the live particles segment by segment; the
+ * fused path needs a few percent of headroom in `capacity` because segments
+ * fill independently. */
 int pic_sim_segments(pic_sim_t *s, int species, uint32_t *n_seg, uint32_t *seg_size,
                      uint32_t *counts);
+/* Slot index of every particle pic_sim_download returns, in the same order. */
+int pic_sim_download_slots(pic_sim_t *s, int species, uint32_t *slots, uint32_t cap, uint32_t *n);
 /* Per-slot RNG words, planar, n slots (upload replaces, download copies). */
 int pic_sim_rng_upload(pic_sim_t *s, int species, const uint32_t *w0, const uint32_t *w1,
                        const uint32_t *w2, const uint32_t *w3, uint32_t n);

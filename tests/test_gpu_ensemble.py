@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from helpers import SEED_E, SEED_I, synthetic_state
-from test_gpu_sim import _push_ref, _replay_species
+from test_gpu_sim import _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
 
@@ -54,9 +54,11 @@ def test_ensemble_step_per_instance_replay(ca, orc):
         # electrons wait in the staging array (no slot)
         slots_e = np.concatenate([np.arange(cnt_e), -np.ones(len(ge[0]) - cnt_e, np.int64)])
         born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, slots_e, rng0[i][0], re1, "e")
-        old_ion = np.isin(gi[0], _push_ref(orc, po, ef, I, "i")[1][0])
-        _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi], np.arange(cnt_i),
-                        rng0[i][1], ri1, "i")
+        _, old_ion, n_amb, _ = _split_ions(gi, _push_ref(orc, po, ef, I, "i")[1][0])
+        # survivors fill the first cnt_i slots in order
+        assert not old_ion[cnt_i:].any()
+        _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi],
+                        np.arange(len(gi[0]))[old_ion], rng0[i][1], ri1, "i", missing=n_amb)
         assert ni[i] - cnt_i == ne[i] - cnt_e == len(born_x) > 0
         assert cnt[6] == 1 and cnt[7] == 1 and cnt[4] > 0
     ens.check()

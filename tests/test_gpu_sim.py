@@ -47,14 +47,40 @@ def _push_ref(orc, po, efield, parts, species):
                                 mass_over_2e=np.float32(0.5 * 6.63352090e-26 / 1.60217662e-19))
 
 
-@pytest.mark.parametrize("path", PATHS)
+# start steps: 0 is an ion step on both paths and a compacting step on the
+# fused path; on the fused path 1 updates electrons in place (no ion step)
+# and 20 (= N_SUB) updates both species in place
+STARTS = [(0, 0), (1, 0), (1, 1), (1, 20)]
+
+
+def _pot0(t0):
+    return float(np.float32(250.0 * np.cos(2.0 * np.pi * t0 / 4000.0)))
+
+
+def _check_slot_draws(orc, rng0, rng1, slots, cnt, draws):
+    """Every occupied slot advanced its stream `draws` times, every other slot
+    is untouched."""
+    s4 = np.zeros(4, np.uint32)
+    for j in slots[:: max(1, cnt // 2000)]:
+        s4[:] = [rng0[w][j] for w in range(4)]
+        for _ in range(draws):
+            orc.xo_next(s4)
+        assert [rng1[w][j] for w in range(4)] == s4.tolist()
+    free = np.ones(len(rng0[0]), bool)
+    free[slots] = False
+    for w in range(4):
+        np.testing.assert_array_equal(rng1[w][free], rng0[w][free])
+
+
+@pytest.mark.parametrize("path,t0", STARTS)
 @pytest.mark.parametrize("prec", [0, 2])
-def test_collisionless_step_stage_by_stage(ca, orc, path, prec):
+def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec):
     p = ca.params(weight=2.0e3, precision=prec)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 300_000, 320_000, 400_000
-    E, I = synthetic_state(n_e, n_i, seed=31)
+    E, I = synthetic_state(n_e, n_i, seed=31 + t0)
     sim = make_sim(ca, p, cap, path)
+    sim.set_time(t0)
     cs, se0, si0 = zero_tables(orc, po)
     sim.set_tables(cs, se0, si0)
     sim.upload(0, *E)
@@ -62,13 +88,14 @@ def test_collisionless_step_stage_by_stage(ca, orc, path, prec):
     rng_e0 = sim.rng_download(0)
     rng_i0 = sim.rng_download(1)
     sim.run(1)
+    assert sim.time()[0] == t0 + 1
     pot, ef, ed, idn = sim.grid()
     # densities of the initial state
     for got, parts in ((ed, E), (idn, I)):
         want = orc.deposit(parts[0], po)
         assert np.abs(got - want).max() <= 2e-6 * want.max()
-    # field from the GPU densities (t = 0: pot0 = V0)
-    pw, ew = orc.poisson_scan_f64(ed, idn, float(np.float32(250.0)), po)
+    # field from the GPU densities (pot0 = V0 cos(2 pi t0 / N_T))
+    pw, ew = orc.poisson_scan_f64(ed, idn, _pot0(t0), po)
     # FP64: 1e-5 of the max norm; FP32: the n_g*eps*|pot| error of the f32
     # scan, which the central difference divides by 2 dx
     eps = np.finfo(np.float32).eps
@@ -80,32 +107,26 @@ def test_collisionless_step_stage_by_stage(ca, orc, path, prec):
         assert np.abs(ef - ew).max() <= 2 * p.n_g * eps * np.abs(pw).max() / p.dx
     # particles: bit-exact multisets of the push in the GPU field
     ne, ni = sim.counts()
+    ion_step = t0 % 20 == 0
     for sp, parts, rng0, name in ((0, E, rng_e0, "e"), (1, I, rng_i0, "i")):
-        cnt, want, _ = _push_ref(orc, po, ef, parts, name)
         got = sim.download(sp)
+        rng1 = sim.rng_download(sp)
+        if name == "i" and not ion_step:
+            assert_same_multiset(got, parts)
+            for w in range(4):
+                np.testing.assert_array_equal(rng1[w], rng0[w])
+            continue
+        cnt, want, _ = _push_ref(orc, po, ef, parts, name)
         assert len(got[0]) == cnt
         assert_same_multiset(got, want)
-        # every occupied slot drew exactly one (ions: five) numbers, every
-        # other slot is untouched
-        rng1 = sim.rng_download(sp)
-        slots = sim.slots(sp)
-        draws = 1 if name == "e" else 5
-        s4 = np.zeros(4, np.uint32)
-        for j in slots[:: max(1, cnt // 2000)]:
-            s4[:] = [rng0[w][j] for w in range(4)]
-            for _ in range(draws):
-                orc.xo_next(s4)
-            assert [rng1[w][j] for w in range(4)] == s4.tolist()
-        free = np.ones(len(rng0[0]), bool)
-        free[slots] = False
-        for w in range(4):
-            np.testing.assert_array_equal(rng1[w][free], rng0[w][free])
+        # every occupied slot drew exactly one (ions: five) numbers
+        _check_slot_draws(orc, rng0, rng1, sim.slots(sp), cnt, 1 if name == "e" else 5)
     assert sim.counts() == (ne, ni)
     sim.check()
     sim.close()
 
 
-def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, name):
+def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, name, missing=0):
     """Match every GPU particle to its source by its pushed position and
     replay its collision with the RNG words of its slot.
 
@@ -114,7 +135,8 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
     single GPU particle is a survivor (replayed); two are an ionisation: the
     parent's slot drew its numbers, the newborn's slot (or staging entry,
     slot -1) did not.  Returns the positions at which electrons were born
-    (one entry per newborn) and the ion velocities of the checked pairs."""
+    (one entry per newborn) and the ion velocities of the checked pairs.
+    `missing`: pushed particles whose GPU rows were left out of `got`."""
     cnt, pushed, _ = _push_ref(orc, po, efield, parts, name)
     px, pvx, pvy, pvz = pushed
     ux, uidx, ucnt = np.unique(px, return_index=True, return_counts=True)
@@ -128,7 +150,7 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
     mo = ucnt[k]                      # pushed sources per GPU position
     born = gcnt - mo                  # newborns per GPU position
     assert (born >= 0).all()
-    assert born.sum() == n - cnt
+    assert born.sum() == n - cnt + missing
 
     def moved(j):
         sl = slots[j]
@@ -180,11 +202,31 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
     return born_x, ions
 
 
+def _split_ions(gi, old_x):
+    """Separate the GPU ions into the old ones (positions old_x) and the
+    ionisation products.  f32 positions can coincide, so this works on
+    multisets: returns the positions of the new ions (the multiset
+    difference), a mask of the GPU ions that are certainly old (their
+    position is held by old ions only), the number of old ions left out of
+    that mask and the ambiguous positions."""
+    ux, uc = np.unique(old_x, return_counts=True)
+    gx, gc = np.unique(gi[0], return_counts=True)
+    k = np.clip(np.searchsorted(ux, gx), 0, max(0, len(ux) - 1))
+    have = np.where(ux[k] == gx, uc[k], 0) if len(ux) else np.zeros_like(gc)
+    surplus = gc - have
+    assert (surplus >= 0).all(), "an old ion is missing"
+    amb_sel = (surplus > 0) & (have > 0)
+    amb = gx[amb_sel]
+    old_mask = np.isin(gi[0], old_x) & ~np.isin(gi[0], amb)
+    return np.repeat(gx, surplus), old_mask, int(have[amb_sel].sum()), amb
+
+
 def _check_new_ions(gi, pushed_ion_x, born_x, ions, check_v):
     """Ions that are not pushed survivors are the ionisation products: one
     per newborn electron, at the same position, with a thermal velocity."""
+    new_x = _split_ions(gi, pushed_ion_x)[0]
+    np.testing.assert_array_equal(np.sort(new_x), np.sort(born_x))
     new = ~np.isin(gi[0], pushed_ion_x)
-    np.testing.assert_array_equal(np.sort(gi[0][new]), np.sort(born_x))
     if check_v:
         for j in np.nonzero(new)[0]:
             x = float(gi[0][j])
@@ -193,13 +235,14 @@ def _check_new_ions(gi, pushed_ion_x, born_x, ions, check_v):
                 assert np.linalg.norm(v - ions[x]) <= 1e-3 * 270 * 6
 
 
-@pytest.mark.parametrize("path", PATHS)
-def test_step_with_collisions_slot_replay(ca, orc, path):
+@pytest.mark.parametrize("path,t0", STARTS)
+def test_step_with_collisions_slot_replay(ca, orc, path, t0):
     p = ca.params(weight=2.0e3, precision=2)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 200_000, 210_000, 300_000
-    E, I = synthetic_state(n_e, n_i, seed=41, te_ev=6.0)
+    E, I = synthetic_state(n_e, n_i, seed=41 + t0, te_ev=6.0)
     sim = make_sim(ca, p, cap, path)
+    sim.set_time(t0)
     cs, se, si = orc.cross_sections(po.gas_density)
     sim.upload(0, *E)
     sim.upload(1, *I)
@@ -212,20 +255,31 @@ def test_step_with_collisions_slot_replay(ca, orc, path):
     rng_e1, rng_i1 = sim.rng_download(0), sim.rng_download(1)
     born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, sim.slots(0), rng_e0, rng_e1, "e")
     assert len(born_x) > 0
-    cnt_i, pushed_i, _ = _push_ref(orc, po, ef, I, "i")
+    ion_step = t0 % 20 == 0
+    if ion_step:
+        cnt_i, pushed_i, _ = _push_ref(orc, po, ef, I, "i")
+    else:
+        cnt_i, pushed_i = len(I[0]), I
     assert ni == cnt_i + len(born_x)
-    # the ion replay covers the pushed ions (new ions have no pushed source)
     slots_i = sim.slots(1)
-    old_ion = np.isin(gi[0], pushed_i[0])
-    _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi], slots_i[old_ion],
-                    rng_i0, rng_i1, "i")
+    _, old_ion, n_amb, amb = _split_ions(gi, pushed_i[0])
+    if ion_step:
+        # the ion replay covers the pushed ions (new ions have no pushed source)
+        _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi], slots_i[old_ion],
+                        rng_i0, rng_i1, "i", missing=n_amb)
+    else:
+        keep = ~np.isin(I[0], amb)
+        assert_same_multiset([a[old_ion] for a in gi], [a[keep] for a in I])
+        for w in range(4):
+            np.testing.assert_array_equal(rng_i1[w], rng_i0[w])
     # new ions: at their parents' positions and thermal (on the reference path
     # they may collide within the same ion step, so only positions are compared)
-    _check_new_ions(gi, pushed_i[0], born_x, ions, check_v=(path == 1))
+    _check_new_ions(gi, pushed_i[0], born_x, ions, check_v=(path == 1 or not ion_step))
     m = sim.measurements()
     c = m["counters"]
-    assert c["e_meas_steps"] == 1 and c["i_meas_steps"] == 1
-    assert c["e_collisions"] > 1000 and c["i_collisions"] > 100
+    assert c["e_meas_steps"] == 1 and c["i_meas_steps"] == int(ion_step)
+    assert c["e_collisions"] > 1000
+    assert (c["i_collisions"] > 100) == ion_step
     # EEPF of the post-collision electrons in the step's field (main.rs:1802-1807)
     want = orc.accumulate_eepf(*ge, ef, po)
     np.testing.assert_array_equal(m["eepf"], want)
@@ -267,7 +321,7 @@ def test_init_cycle_statistics(ca, orc, path, graph):
 
 
 def test_graph_replays_match_eager(ca, orc):
-    """Graph replays (2*N_SUB steps each) against the same steps run eagerly,
+    """Graph replays (4*N_SUB steps each) against the same steps run eagerly,
     collisionless and in FP64 so that both runs see the same fields up to the
     summation order of the density atomics; eager work and synchronisations
     between the replays."""
@@ -284,7 +338,7 @@ def test_graph_replays_match_eager(ca, orc):
         sims.append(s)
     for rep in range(3):
         for s in sims:
-            s.run(40)
+            s.run(160)
             s.synchronize()
         ce = [s.counts() for s in sims]
         assert abs(ce[0][0] - ce[1][0]) <= 2 and abs(ce[0][1] - ce[1][1]) <= 2, ce
@@ -298,9 +352,9 @@ def test_graph_replays_match_eager(ca, orc):
         # eager work between replays
         for s in sims:
             s.run(3)
-            s.run(37)
+            s.run(77)
     for s in sims:
-        assert s.time()[0] == 240 % 4000
+        assert s.time()[0] == 720
         s.close()
 
 
