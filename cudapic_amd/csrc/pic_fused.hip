@@ -297,15 +297,72 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
 // x, vx, RNG (24 B) and vy, vz only after a collision.
 //
 // Collisions are rare (well under 1% of particle-steps) and their
-// scattering code is long, so a wave that meets one would run it for a
-// single lane.  Instead the collision test only queues the colliding
-// particle (slot, velocity, RNG state after the test draw, energy bin) in a
-// per-wave LDS ring, and the wave scatters 64 queued particles at a time
-// with every lane busy; the RNG stream of each particle continues from the
-// queued state, so the draws are the same as in sequential order.
-constexpr int kQ = 128;       // ring entries per wave (pending < 64 + 64 new)
-constexpr int kQWordsE = 10;  // slot, x, vx, vy, vz, rng[4], bin
-constexpr int kQWordsI = 13;  // slot, vx, vy, vz, rng[4], bin, ax, ay, az, g
+// scattering code is long: a wave that ran it for its one or two colliding
+// lanes in every tile would spend most of the step there, and four waves per
+// SIMD doing that at once saturate the vector ALUs.  The collision test
+// therefore only lists the colliding particle (slot, position, velocity,
+// RNG state after the test draw, energy bin; ions also the partner velocity
+// and relative speed) in a workgroup list in LDS, and after the last tile
+// the workgroup's waves scatter the list 64 entries at a time, one entry per
+// lane.  The RNG stream of a particle continues from the listed state, so
+// its draws are those of the sequential order.  A list that fills up (far
+// beyond the expected tens of entries per workgroup) scatters the excess in
+// place.
+constexpr uint32_t kCollPerWg = 256;  // list entries per workgroup
+enum CollField {
+    CF_SLOT = 0, CF_X, CF_VX, CF_VY, CF_VZ, CF_R0, CF_R1, CF_R2, CF_R3, CF_BIN, CF_AX, CF_AY, CF_AZ,
+    CF_G = CF_X  // ions: relative speed in the (unused) position field; CF_AX.. unused by electrons
+};
+constexpr int kCollFieldsE = CF_BIN + 1;
+constexpr int kCollFieldsI = CF_AZ + 1;
+
+// Apply an electron collision's products: the new electron goes to output
+// segment `ko` (into the LDS density and EEPF), the new ion to the same ion
+// segment and the global ion density.  Wave-aggregated slot reservation;
+// `kind` per lane.
+template <bool MEASURE, typename Acc>
+__device__ __forceinline__ void ionisation_products(const FusedJob &j, const KPhys &k, int kind, uint32_t ko,
+                                                    float px, const float *se, const float *si, float *h,
+                                                    Acc *i_dens_g, uint32_t *eepf_h, const float *ef,
+                                                    bool &overflow) {
+    const uint64_t sp = ballot64(kind == kIonisation);
+    if (!sp) return;
+    const int leader = __builtin_ctzll(sp);
+    uint32_t be = 0, bi = 0;
+    if ((int)lane_id() == leader) {
+        const uint32_t cnt = (uint32_t)__popcll(sp);
+        be = atomicAdd(j.cnt_out + ko, cnt);
+        bi = atomicAdd(j.ion_cnt + ko, cnt);
+    }
+    be = __shfl(be, leader, kWave);
+    bi = __shfl(bi, leader, kWave);
+    if (kind != kIonisation) return;
+    const uint32_t o = mbcnt64(sp);
+    const uint32_t el = be + o, il = bi + o;
+    float *const X = (float *)j.sx, *const VX = (float *)j.svx, *const VY = (float *)j.svy,
+                 *const VZ = (float *)j.svz;
+    if (el < j.seg_size) {
+        const uint32_t es = ko * j.seg_size + el;
+        X[es] = px;
+        VX[es] = se[0];
+        VY[es] = se[1];
+        VZ[es] = se[2];
+        deposit_lds<Acc>(h, px, k);
+        if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
+    } else {
+        overflow = true;
+    }
+    if (il < j.ion_seg_size) {
+        const uint32_t is = ko * j.ion_seg_size + il;
+        j.ix[is] = px;
+        j.ivx[is] = si[0];
+        j.ivy[is] = si[1];
+        j.ivz[is] = si[2];
+        deposit_global<Acc>(i_dens_g, px, k);
+    } else {
+        overflow = true;
+    }
+}
 
 template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
 __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
@@ -313,15 +370,16 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     const KPhys &k = A.k;
     extern __shared__ __align__(16) float sh[];
     __shared__ uint32_t s_tiles[kMaxSegments + 1], s_cnt[kMaxSegments];
-    constexpr int QW = ELECTRON ? kQWordsE : kQWordsI;
+    __shared__ uint32_t s_ncoll;
+    constexpr int NF = ELECTRON ? kCollFieldsE : kCollFieldsI;
     const int n_g = k.n_g;
     const int K = j.n_seg;
     float *ef = sh;
     float *hist = sh + n_g;
     uint32_t *aux = (uint32_t *)(hist + A.lds_reps * n_g);
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
-    uint32_t *queue = aux + aux_n;
-    float *stc = (float *)(queue + (blockDim.x / kWave) * QW * kQ);
+    uint32_t *list = aux + aux_n;  // [NF][kCollPerWg]
+    float *stc = (float *)(list + NF * kCollPerWg);
     const uint32_t tile_sz = blockDim.x * PPT;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
@@ -332,6 +390,7 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
         s_cnt[q] = c;
         s_tiles[q + 1] = (c + tile_sz - 1) / tile_sz;
     }
+    if (threadIdx.x == 0) s_ncoll = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
@@ -344,110 +403,22 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     __syncthreads();
     const int wave = threadIdx.x / kWave;
     float *h = hist + (wave % A.lds_reps) * n_g;
-    uint32_t *q = queue + wave * QW * kQ;
-    uint32_t q_head = 0, q_tail = 0;  // ring positions (wave-uniform)
     const uint32_t n_tiles = s_tiles[K];
     const uint32_t ko = blockIdx.x % (uint32_t)K;  // segment that receives this workgroup's newborns
     const float factor = ELECTRON ? k.factor_e : k.factor_i;
     const float dt = ELECTRON ? k.dt_e : k.dt_i;
     float *const X = (float *)j.sx, *const VX = (float *)j.svx, *const VY = (float *)j.svy,
                  *const VZ = (float *)j.svz;
+    Acc *const i_dens_g = (Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride;
     uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
     bool overflow = false;
 
-    // Scatter the n (<= 64) oldest queued collisions, one per lane.
-    auto scatter_queued = [&](uint32_t n) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t l = lane_id();
-        const uint32_t e = (q_head + l) & (kQ - 1);
-        int kind = kNone;
-        float se[3], si[3], px = 0.0f;
-        if (l < n) {
-            const uint32_t slot = q[e];
-            if (ELECTRON) {
-                px = __uint_as_float(q[kQ + e]);
-                float vx = __uint_as_float(q[2 * kQ + e]), vy = __uint_as_float(q[3 * kQ + e]),
-                      vz = __uint_as_float(q[4 * kQ + e]);
-                Xo s{q[5 * kQ + e], q[6 * kQ + e], q[7 * kQ + e], q[8 * kQ + e]};
-                const int idx = (int)q[9 * kQ + e];
-                kind = e_scatter(k, j.cs[idx], j.cs[kCsRanges + idx], j.cs[2 * kCsRanges + idx], vx, vy, vz, s,
-                                 se, si);
-                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
-                VX[slot] = vx;
-                VY[slot] = vy;
-                VZ[slot] = vz;
-                if (MEASURE) eepf_lds(aux, ef, px, vx, vy, vz, k);
-            } else {
-                float vx = __uint_as_float(q[kQ + e]), vy = __uint_as_float(q[2 * kQ + e]),
-                      vz = __uint_as_float(q[3 * kQ + e]);
-                Xo s{q[4 * kQ + e], q[5 * kQ + e], q[6 * kQ + e], q[7 * kQ + e]};
-                const int idx = (int)q[8 * kQ + e];
-                const float ax = __uint_as_float(q[9 * kQ + e]), ay = __uint_as_float(q[10 * kQ + e]),
-                            az = __uint_as_float(q[11 * kQ + e]), g = __uint_as_float(q[12 * kQ + e]);
-                kind = i_scatter(k, j.cs[3 * kCsRanges + idx], j.cs[4 * kCsRanges + idx], vx, vy, vz, ax, ay, az,
-                                 g, s);
-                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
-                VX[slot] = vx;
-                VY[slot] = vy;
-                VZ[slot] = vz;
-            }
-            ++ncoll;
-        }
-        q_head += n;
-        if (ELECTRON) {
-            // ionisation products: the new electron joins this workgroup's
-            // output segment, the new ion the same-numbered ion segment
-            const uint64_t sp = ballot64(kind == kIonisation);
-            if (sp) {
-                const int leader = __builtin_ctzll(sp);
-                uint32_t be = 0, bi = 0;
-                if ((int)lane_id() == leader) {
-                    const uint32_t cnt = (uint32_t)__popcll(sp);
-                    be = atomicAdd(j.cnt_out + ko, cnt);
-                    bi = atomicAdd(j.ion_cnt + ko, cnt);
-                }
-                be = __shfl(be, leader, kWave);
-                bi = __shfl(bi, leader, kWave);
-                if (kind == kIonisation) {
-                    const uint32_t o = mbcnt64(sp);
-                    const uint32_t el = be + o, il = bi + o;
-                    if (el < j.seg_size) {
-                        const uint32_t es = ko * j.seg_size + el;
-                        X[es] = px;
-                        VX[es] = se[0];
-                        VY[es] = se[1];
-                        VZ[es] = se[2];
-                        deposit_lds<Acc>(h, px, k);
-                        if (MEASURE) eepf_lds(aux, ef, px, se[0], se[1], se[2], k);
-                    } else {
-                        overflow = true;
-                    }
-                    if (il < j.ion_seg_size) {
-                        const uint32_t is = ko * j.ion_seg_size + il;
-                        j.ix[is] = px;
-                        j.ivx[is] = si[0];
-                        j.ivy[is] = si[1];
-                        j.ivz[is] = si[2];
-                        deposit_global<Acc>((Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride, px, k);
-                    } else {
-                        overflow = true;
-                    }
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-
     // (1) particle and RNG loads of all PPT particles of a tile; tile t + G
     // (G = grid size) is loaded while tile t is processed, so each wave keeps
-    // a tile of loads in flight behind its compute and gathers.
-    // The loads are unconditional (lanes past the segment's count read the
-    // segment's first slot) and the count test is applied where the values
-    // are used, so that no wait for them is needed at the point of issue.
+    // a tile of loads in flight behind its compute and gathers.  The loads
+    // are unconditional (lanes past the segment's count read the segment's
+    // first slot) and the count test is applied where the values are used,
+    // so that no wait for them is needed at the point of issue.
     auto load_tile = [&](uint32_t tile, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
                          float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT]) {
         uint32_t safe = 0;
@@ -519,7 +490,7 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
         for (int u = 0; u < PPT; ++u)
             if (idx[u] < A.stc_n) sig[u] = stc[idx[u]];
         // (3) collision tests, write-back of the particles that do not
-        // collide, deposits; colliding particles are queued; lost ones
+        // collide, deposits; colliding particles are listed; lost ones
         // leave a hole
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
@@ -547,36 +518,58 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
             }
             const uint64_t m = ballot64(hit);
             if (m) {
-                if (hit) {
-                    const uint32_t e = (q_tail + mbcnt64(m)) & (kQ - 1);
-                    q[e] = i;
+                const int leader = __builtin_ctzll(m);
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&s_ncoll, (uint32_t)__popcll(m));
+                base = __shfl(base, leader, kWave);
+                const uint32_t pos = base + mbcnt64(m);
+                const bool listed = hit && pos < kCollPerWg;
+                if (listed) {
+                    uint32_t *e = list + pos;
+                    e[CF_SLOT * kCollPerWg] = i;
+                    e[CF_VX * kCollPerWg] = __float_as_uint(vx[u]);
+                    e[CF_VY * kCollPerWg] = __float_as_uint(vy[u]);
+                    e[CF_VZ * kCollPerWg] = __float_as_uint(vz[u]);
+                    e[CF_R0 * kCollPerWg] = st[u].s0;
+                    e[CF_R1 * kCollPerWg] = st[u].s1;
+                    e[CF_R2 * kCollPerWg] = st[u].s2;
+                    e[CF_R3 * kCollPerWg] = st[u].s3;
+                    e[CF_BIN * kCollPerWg] = (uint32_t)idx[u];
                     if (ELECTRON) {
-                        q[kQ + e] = __float_as_uint(nx[u]);
-                        q[2 * kQ + e] = __float_as_uint(vx[u]);
-                        q[3 * kQ + e] = __float_as_uint(vy[u]);
-                        q[4 * kQ + e] = __float_as_uint(vz[u]);
-                        q[5 * kQ + e] = st[u].s0;
-                        q[6 * kQ + e] = st[u].s1;
-                        q[7 * kQ + e] = st[u].s2;
-                        q[8 * kQ + e] = st[u].s3;
-                        q[9 * kQ + e] = (uint32_t)idx[u];
+                        e[CF_X * kCollPerWg] = __float_as_uint(nx[u]);
                     } else {
-                        q[kQ + e] = __float_as_uint(vx[u]);
-                        q[2 * kQ + e] = __float_as_uint(vy[u]);
-                        q[3 * kQ + e] = __float_as_uint(vz[u]);
-                        q[4 * kQ + e] = st[u].s0;
-                        q[5 * kQ + e] = st[u].s1;
-                        q[6 * kQ + e] = st[u].s2;
-                        q[7 * kQ + e] = st[u].s3;
-                        q[8 * kQ + e] = (uint32_t)idx[u];
-                        q[9 * kQ + e] = __float_as_uint(ax[u]);
-                        q[10 * kQ + e] = __float_as_uint(ay[u]);
-                        q[11 * kQ + e] = __float_as_uint(az[u]);
-                        q[12 * kQ + e] = __float_as_uint(spd[u]);
+                        e[CF_AX * kCollPerWg] = __float_as_uint(ax[u]);
+                        e[CF_AY * kCollPerWg] = __float_as_uint(ay[u]);
+                        e[CF_AZ * kCollPerWg] = __float_as_uint(az[u]);
+                        e[CF_G * kCollPerWg] = __float_as_uint(spd[u]);
                     }
                 }
-                q_tail += (uint32_t)__popcll(m);
-                if (q_tail - q_head >= (uint32_t)kWave) scatter_queued(kWave);
+                // a full list: scatter here
+                const bool local = hit && !listed;
+                if (ballot64(local)) {
+                    int kind = kNone;
+                    float se[3], si[3];
+                    if (local) {
+                        float w = vx[u], y = vy[u], z = vz[u];
+                        Xo s = st[u];
+                        const int b = idx[u];
+                        if (ELECTRON)
+                            kind = e_scatter(k, j.cs[b], j.cs[kCsRanges + b], j.cs[2 * kCsRanges + b], w, y, z, s,
+                                             se, si);
+                        else
+                            kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
+                                             ay[u], az[u], spd[u], s);
+                        rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
+                        VX[i] = w;
+                        VY[i] = y;
+                        VZ[i] = z;
+                        ++ncoll;
+                        if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], w, y, z, k);
+                    }
+                    if (ELECTRON)
+                        ionisation_products<MEASURE, Acc>(j, k, kind, ko, nx[u], se, si, h, i_dens_g, aux, ef,
+                                                          overflow);
+                }
             }
         }
         slot0 = pslot0;
@@ -590,7 +583,37 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
             st[u] = pst[u];
         }
     }
-    if (q_tail != q_head) scatter_queued(q_tail - q_head);
+    // (4) scatter the listed collisions, 64 per wave pass
+    __syncthreads();
+    const uint32_t n_list = min(s_ncoll, kCollPerWg);
+    for (uint32_t b0 = (uint32_t)wave * kWave; b0 < n_list; b0 += blockDim.x) {
+        const uint32_t e = b0 + lane_id();
+        int kind = kNone;
+        float se[3], si[3], px = 0.0f;
+        if (e < n_list) {
+            const uint32_t *q = list + e;
+            const uint32_t slot = q[CF_SLOT * kCollPerWg];
+            float w = __uint_as_float(q[CF_VX * kCollPerWg]), y = __uint_as_float(q[CF_VY * kCollPerWg]),
+                  z = __uint_as_float(q[CF_VZ * kCollPerWg]);
+            Xo s{q[CF_R0 * kCollPerWg], q[CF_R1 * kCollPerWg], q[CF_R2 * kCollPerWg], q[CF_R3 * kCollPerWg]};
+            const int b = (int)q[CF_BIN * kCollPerWg];
+            if (ELECTRON) {
+                px = __uint_as_float(q[CF_X * kCollPerWg]);
+                kind = e_scatter(k, j.cs[b], j.cs[kCsRanges + b], j.cs[2 * kCsRanges + b], w, y, z, s, se, si);
+                if (MEASURE) eepf_lds(aux, ef, px, w, y, z, k);
+            } else {
+                kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z,
+                                 __uint_as_float(q[CF_AX * kCollPerWg]), __uint_as_float(q[CF_AY * kCollPerWg]),
+                                 __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]), s);
+            }
+            rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+            VX[slot] = w;
+            VY[slot] = y;
+            VZ[slot] = z;
+            ++ncoll;
+        }
+        if (ELECTRON) ionisation_products<MEASURE, Acc>(j, k, kind, ko, px, se, si, h, i_dens_g, aux, ef, overflow);
+    }
     {
         const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
         if (lane_id() == 0 && j.counters) {
@@ -674,14 +697,14 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
     const int grid = std::max(1, sh.grid * sh.block / block);
     // LDS per workgroup: what the resident workgroups of a compute unit can
     // share (2 KB kept for the static arrays), at most what one may allocate.
-    // It holds the field, the EEPF / IFED bins, the collision queues of the
+    // It holds the field, the EEPF / IFED bins, the collision list of the
     // in-place kernel, one histogram replica per wave (fewer when space is
     // short) and then as many leading bins of the total cross-section table
     // as fit (the collision test reads one bin per particle).
     const int per_cu = std::max(1, (grid + device_cus() - 1) / device_cus());
     const int budget = std::min(device_lds_per_block(), device_lds_per_cu() / per_cu - 2048);
-    const int queue_words = j.inplace ? (block / kWave) * (j.electrons ? kQWordsE : kQWordsI) * kQ : 0;
-    const int fixed = (k.n_g + aux_words + queue_words) * 4;
+    const int list_words = j.inplace ? (j.electrons ? kCollFieldsE : kCollFieldsI) * (int)kCollPerWg : 0;
+    const int fixed = (k.n_g + aux_words + list_words) * 4;
     a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (k.n_g * 4)));
     a.reps = sh.replicas;
     a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
