@@ -202,31 +202,32 @@ __device__ __forceinline__ int e_scatter(const KPhys &k, float c0, float c1, flo
     const float t1 = t0 + c1;
     const float t2 = t1 + c2;
     const Frame f = frame_of(gx, gy, gz);
-    float chi, eta;
-    int kind;
     const float rnd = draw(s);
-    if (rnd < t0 / t2) {
-        chi = acosf(1.0f - 2.0f * draw(s));
-        eta = kTwoPiF * draw(s);
-        kind = kElastic;
-    } else if (rnd < t1 / t2) {
-        float en = 0.5f * k.me * g * g;
-        en = fabsf(en - k.exc_th * k.e_charge);
+    const int kind = rnd < t0 / t2 ? kElastic : (rnd < t1 / t2 ? kExcitation : kIonisation);
+    // every process draws two numbers next: the scattering angle (elastic,
+    // excitation) or the energy share (ionisation), then the azimuth; the
+    // processes share one acosf and one rotation of the scattered electron
+    const float d1 = draw(s);
+    const float d2 = draw(s);
+    float chi_arg = 1.0f - 2.0f * d1;
+    const float en0 = 0.5f * k.me * g * g;
+    float g_new = 0.0f, chi_new_arg = 0.0f;
+    if (kind == kExcitation) {
+        const float en = fabsf(en0 - k.exc_th * k.e_charge);
         g = __builtin_sqrtf(2.0f * en / k.me);
-        chi = acosf(1.0f - 2.0f * draw(s));
-        eta = kTwoPiF * draw(s);
-        kind = kExcitation;
-    } else {
-        float en = 0.5f * k.me * g * g;
-        en = fabsf(en - k.ion_th * k.e_charge);
-        const float r = draw(s);
-        const float e_new = 10.0f * tanf(r * atanf(en / k.e_charge / 20.0f)) * k.e_charge;
+    } else if (kind == kIonisation) {
+        const float en = fabsf(en0 - k.ion_th * k.e_charge);
+        const float e_new = 10.0f * tanf(d1 * atanf(en / k.e_charge / 20.0f)) * k.e_charge;
         const float e_orig = fabsf(en - e_new);
         g = __builtin_sqrtf(2.0f * e_orig / k.me);
-        const float g_new = __builtin_sqrtf(2.0f * e_new / k.me);
-        chi = acosf(__builtin_sqrtf(e_orig / en));
-        const float chi_new = acosf(__builtin_sqrtf(e_new / en));
-        eta = kTwoPiF * draw(s);
+        g_new = __builtin_sqrtf(2.0f * e_new / k.me);
+        chi_arg = __builtin_sqrtf(e_orig / en);
+        chi_new_arg = __builtin_sqrtf(e_new / en);
+    }
+    const float chi = acosf(chi_arg);
+    const float eta = kTwoPiF * d2;
+    if (kind == kIonisation) {
+        const float chi_new = acosf(chi_new_arg);
         const float eta_new = eta + kPiF;
         float nx, ny, nz;
         rotate(g_new, chi_new, eta_new, f, nx, ny, nz);
@@ -234,7 +235,6 @@ __device__ __forceinline__ int e_scatter(const KPhys &k, float c0, float c1, flo
         se[1] = wy + k.f2 * ny;
         se[2] = wz + k.f2 * nz;
         three_normal(s, k.normal_range, si[0], si[1], si[2]);
-        kind = kIonisation;
     }
     float ox, oy, oz;
     rotate(g, chi, eta, f, ox, oy, oz);

@@ -70,7 +70,7 @@ struct PoissonJob {
     uint32_t *zero_cnt = nullptr; int zero_cnt_words = 0;          // cleared every step
     uint32_t *zero_cnt_ion = nullptr; int zero_cnt_ion_words = 0;  // cleared on ion steps
     const uint32_t *copy_cnt_src = nullptr; uint32_t *copy_cnt_dst = nullptr; int copy_cnt_words = 0;
-    uint32_t *advance_time = nullptr;                               // t <- (t+1) % n_t
+    uint32_t *advance_time = nullptr;  // == time: *time <- (*time + 1) % n_t after the solve
 };
 int launch_poisson(const PoissonJob &j, hipStream_t s);
 

@@ -325,9 +325,11 @@ struct PoissonArgs {
     uint32_t *advance_time;
 };
 
-// Node k of both densities summed over the global replicas.  The replica
-// count is 1 or a multiple of 8; the loads of a group of eight replicas of
-// both species are independent, so one group costs one memory latency.
+// Node k of both densities summed over the global replicas (in replica
+// order).  The replica count is 1 or a multiple of 8; the loads of up to 16
+// replicas of both species are independent, so they cost one memory
+// latency (a group of 8 also loads replica reps-1 eight more times, from
+// cache, to keep one code path).
 template <typename D>
 __device__ __forceinline__ void load_dens_pair(const D *ed, const D *id, int reps, int stride, int k,
                                                double &ne, double &ni) {
@@ -337,21 +339,35 @@ __device__ __forceinline__ void load_dens_pair(const D *ed, const D *id, int rep
         return;
     }
     double se = 0.0, si = 0.0;
-    for (int r0 = 0; r0 < reps; r0 += 8) {
-        D ve[8], vi[8];
+    for (int r0 = 0; r0 < reps; r0 += 16) {
+        D ve[16], vi[16];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            ve[j] = ed[(r0 + j) * stride + k];
-            vi[j] = id[(r0 + j) * stride + k];
+        for (int j = 0; j < 16; ++j) {
+            const int r = min(r0 + j, reps - 1);
+            ve[j] = ed[r * stride + k];
+            vi[j] = id[r * stride + k];
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            se += (double)ve[j];
-            si += (double)vi[j];
-        }
+        for (int j = 0; j < 16; ++j)
+            if (r0 + j < reps) {
+                se += (double)ve[j];
+                si += (double)vi[j];
+            }
     }
     ne = se;
     ni = si;
+}
+
+// Densities of the (up to 4) nodes of this lane, summed over the replicas.
+template <typename D>
+__device__ __forceinline__ void load_node_densities(const PoissonGeom &pg, const D *ed, const D *id, double *dne,
+                                                    double *dni) {
+    const int k0 = threadIdx.x * pg.per;
+#pragma unroll
+    for (int u = 0; u < kPoissonMaxPer; ++u) {
+        dne[u] = dni[u] = 0.0;
+        if (u < pg.per && k0 + u < pg.n_g) load_dens_pair(ed, id, pg.reps, pg.rep_stride, k0 + u, dne[u], dni[u]);
+    }
 }
 
 // pot0 of step t: sum_h a_h cos(2 pi (h+1) t / N_T) + b_h sin(...), the
@@ -375,8 +391,9 @@ __device__ double waveform(uint32_t t, int n_t, int n_harm, const double *wc, co
 //   pot_i = i (R_{N-2} - R_i + r_i),
 // then E by central differences with the electrode formulas of
 // C/eduPIC.cc:491-493.  pot_s holds n_g values of T, wtot 17.
-template <typename T, typename D>
-__device__ void poisson_solve(const PoissonGeom &pg, const D *ed, const D *id, double pot0_d, bool ion,
+// The densities of the lane's nodes come in dne / dni (load_node_densities).
+template <typename T>
+__device__ void poisson_solve(const PoissonGeom &pg, const double *dne, const double *dni, double pot0_d, bool ion,
                               float *pot, float *efield, double *cumul_e, double *cumul_i, T *pot_s,
                               T *wtot) {
     const int tid = threadIdx.x;
@@ -386,14 +403,8 @@ __device__ void poisson_solve(const PoissonGeom &pg, const D *ed, const D *id, d
     const T alpha = Num<T>::from(pg.alpha);
     const T echarge = Num<T>::from(pg.e_charge);
     T rho[kPoissonMaxPer], g[kPoissonMaxPer];
-    double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
     const int k0 = tid * per;
-    // densities (summed over the global replicas), charge density, g_i = i f_i
-#pragma unroll
-    for (int u = 0; u < kPoissonMaxPer; ++u) {
-        dne[u] = dni[u] = 0.0;
-        if (u < per && k0 + u < n_g) load_dens_pair(ed, id, pg.reps, pg.rep_stride, k0 + u, dne[u], dni[u]);
-    }
+    // charge density, g_i = i f_i
     T run = Num<T>::zero();
 #pragma unroll
     for (int u = 0; u < kPoissonMaxPer; ++u) {
@@ -479,11 +490,16 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     __shared__ double s_pot0;
     __shared__ int s_ion;
     const int tid = threadIdx.x;
+    // the density loads are issued before the clock is read, so both
+    // latencies overlap
+    double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
+    load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, dne, dni);
+    uint32_t t = 0;
     if (tid == 0) {
         double p0 = a.pot0;
         int ion = 1;
         if (a.time) {
-            const uint32_t t = *a.time;
+            t = *a.time;
             ion = (t % (uint32_t)a.n_sub) == 0;
             p0 = waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
         }
@@ -491,8 +507,7 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
         s_ion = ion;
     }
     __syncthreads();
-    poisson_solve<T, D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, s_pot0, s_ion != 0, a.pot, a.efield,
-                        a.cumul_e, a.cumul_i, pot_s, wtot);
+    poisson_solve<T>(a.g, dne, dni, s_pot0, s_ion != 0, a.pot, a.efield, a.cumul_e, a.cumul_i, pot_s, wtot);
     // fused-path bookkeeping for the particle kernels that follow
     for (int w = tid; w < a.zero_a_words; w += blockDim.x) a.zero_a[w] = 0u;
     if (s_ion)
@@ -504,8 +519,8 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
         a.meas_steps[0] += 1ull;
         if (s_ion) a.meas_steps[1] += 1ull;
     }
-    __syncthreads();
-    if (tid == 0 && a.advance_time) *a.advance_time = (*a.advance_time + 1u) % (uint32_t)a.n_t;
+    // the clock was read by this lane (the same word when advance_time is set)
+    if (tid == 0 && a.advance_time) *a.advance_time = (t + 1u) % (uint32_t)a.n_t;
 }
 
 int launch_poisson(const PoissonJob &j, hipStream_t s) {
@@ -515,6 +530,10 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     }
     if (j.replicas != 1 && (j.replicas % 8) != 0) {
         set_error("poisson: density replicas must be 1 or a multiple of 8");
+        return PIC_ERR_ARG;
+    }
+    if (j.advance_time && j.advance_time != j.time) {
+        set_error("poisson: the clock advanced must be the clock read");
         return PIC_ERR_ARG;
     }
     PoissonArgs a;
@@ -608,6 +627,9 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int n_g = a.g.n_g;
+    const size_t off = (size_t)b * a.inst_stride;
+    double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
+    load_node_densities<D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, dne, dni);
     if (tid == 0) {
         const uint32_t t = *a.time;
         s_ion = (t % (uint32_t)a.n_sub) == 0;
@@ -616,8 +638,7 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
     }
     __syncthreads();
     const bool ion = s_ion != 0;
-    const size_t off = (size_t)b * a.inst_stride;
-    poisson_solve<T, D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, s_pot0, ion,
+    poisson_solve<T>(a.g, dne, dni, s_pot0, ion,
                         a.pot + (size_t)b * n_g, a.efield + (size_t)b * n_g,
                         a.cumul_e ? a.cumul_e + (size_t)b * n_g : nullptr,
                         a.cumul_i ? a.cumul_i + (size_t)b * n_g : nullptr, pot_s, wtot);
