@@ -34,6 +34,49 @@ struct FusedArgs {
     int stc_n;  // leading bins of the total cross-section table staged in LDS
 };
 
+// Workgroup density histograms of the single-discharge kernels are fixed
+// point: LDS integer atomics run at full rate, float ones at a small
+// fraction of it.  A particle adds exactly `scale` units split by its CIC
+// weights (weight wf = scale units), so the sum is exact; each weight is
+// rounded to 1/scale.  The scale is the largest power of two (at most 2^20)
+// for which no replica can overflow: a bound on the deposits of one
+// replica (two per particle -- the particle and an ionisation product --
+// for each tile of its waves, plus the collision list) times the scale
+// stays below 2^32.
+__device__ __forceinline__ void deposit_fix(uint32_t *h, float x, const KPhys &k, float scale) {
+    const float pos = x * k.inv_dx;
+    const int q = cell_of(pos, k.n_g);
+    const float rem = pos - (float)q;
+    const uint32_t u2 = (uint32_t)(rem * scale + 0.5f);
+    atomicAdd(&h[q], (uint32_t)scale - u2);
+    atomicAdd(&h[q + 1], u2);
+}
+
+__device__ __forceinline__ float fix_scale(uint32_t n_tiles, int ppt, int lds_reps) {
+    const uint32_t my_tiles = n_tiles > blockIdx.x ? (n_tiles - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+    const uint32_t waves_per_rep = (blockDim.x / kWave + lds_reps - 1) / lds_reps;
+    const uint64_t bound = (uint64_t)my_tiles * waves_per_rep * 2u * ppt * kWave + 256u + 1u;
+    int s = 20;
+    while (s > 0 && (bound << s) >= (1ull << 32)) --s;
+    return (float)(1u << s);
+}
+
+// Flush of fixed-point workgroup histograms into a global density replica.
+template <typename Acc>
+__device__ __forceinline__ void flush_fix(const uint32_t *hist, int lds_reps, int n_g, float scale, float wf,
+                                          Acc *out) {
+    const Acc unit = (Acc)wf / (Acc)scale;
+    for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
+        uint64_t sum = 0;
+        for (int r = 0; r < lds_reps; ++r) sum += hist[r * n_g + q];
+        if (sum) {
+            Acc v = (Acc)sum * unit;
+            if (q == 0 || q == n_g - 1) v *= (Acc)2;
+            unsafeAtomicAdd(&out[q], v);
+        }
+    }
+}
+
 template <typename Acc>
 __device__ __forceinline__ void deposit_lds(float *h, float x, const KPhys &k) {
     const float pos = x * k.inv_dx;
@@ -102,12 +145,12 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     const int n_g = k.n_g;
     const int K = j.n_seg;
     float *ef = sh;
-    float *hist = sh + n_g;
-    uint32_t *aux = (uint32_t *)(hist + A.lds_reps * n_g);  // EEPF (electrons) or IFED (ions)
+    uint32_t *hist = (uint32_t *)(sh + n_g);
+    uint32_t *aux = hist + A.lds_reps * n_g;  // EEPF (electrons) or IFED (ions)
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     const uint32_t tile_sz = blockDim.x * PPT;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
-    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
     float *stc = (float *)(aux + aux_n);
     for (int q = threadIdx.x; q < A.stc_n; q += blockDim.x) stc[q] = j.stot[q];
@@ -129,8 +172,9 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave;
-    float *h = hist + (wave % A.lds_reps) * n_g;
+    uint32_t *h = hist + (wave % A.lds_reps) * n_g;
     const uint32_t n_tiles = s_tiles[K];
+    const float scale = fix_scale(n_tiles, PPT, A.lds_reps);
     const uint32_t ko = blockIdx.x % (uint32_t)K;               // output segment
     uint32_t *const cnt_out = j.cnt_out + ko;
     const uint32_t out_base = ko * j.seg_size;
@@ -205,7 +249,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
                     j.dvx[slot] = vx;
                     j.dvy[slot] = y;
                     j.dvz[slot] = z;
-                    deposit_lds<Acc>(h, nx[u], k);
+                    deposit_fix(h, nx[u], k, scale);
                     if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], vx, y, z, k);
                     ncoll += kind != kNone;
                 } else {
@@ -237,7 +281,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
                             j.dvx[es] = se[0];
                             j.dvy[es] = se[1];
                             j.dvz[es] = se[2];
-                            deposit_lds<Acc>(h, x, k);
+                            deposit_fix(h, x, k, scale);
                             if (MEASURE) eepf_lds(aux, ef, x, se[0], se[1], se[2], k);
                         } else {
                             overflow = true;
@@ -270,16 +314,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     }
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
     // one flush of the workgroup histogram into the next step's density
-    Acc *out = (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride;
-    for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
-        float s = 0.0f;
-        for (int r = 0; r < A.lds_reps; ++r) s += hist[r * n_g + q];
-        if (s != 0.0f) {
-            Acc v = (Acc)s;
-            if (q == 0 || q == n_g - 1) v *= (Acc)2;
-            unsafeAtomicAdd(&out[q], v);
-        }
-    }
+    flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
     uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;
     if (aux_out)
         for (int q = threadIdx.x; q < aux_n; q += blockDim.x)
@@ -322,9 +357,9 @@ constexpr int kCollFieldsI = CF_AZ + 1;
 // `kind` per lane.
 template <bool MEASURE, typename Acc>
 __device__ __forceinline__ void ionisation_products(const FusedJob &j, const KPhys &k, int kind, uint32_t ko,
-                                                    float px, const float *se, const float *si, float *h,
-                                                    Acc *i_dens_g, uint32_t *eepf_h, const float *ef,
-                                                    bool &overflow) {
+                                                    float px, const float *se, const float *si, uint32_t *h,
+                                                    float scale, Acc *i_dens_g, uint32_t *eepf_h,
+                                                    const float *ef, bool &overflow) {
     const uint64_t sp = ballot64(kind == kIonisation);
     if (!sp) return;
     const int leader = __builtin_ctzll(sp);
@@ -347,7 +382,7 @@ __device__ __forceinline__ void ionisation_products(const FusedJob &j, const KPh
         VX[es] = se[0];
         VY[es] = se[1];
         VZ[es] = se[2];
-        deposit_lds<Acc>(h, px, k);
+        deposit_fix(h, px, k, scale);
         if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
     } else {
         overflow = true;
@@ -375,14 +410,14 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     const int n_g = k.n_g;
     const int K = j.n_seg;
     float *ef = sh;
-    float *hist = sh + n_g;
-    uint32_t *aux = (uint32_t *)(hist + A.lds_reps * n_g);
+    uint32_t *hist = (uint32_t *)(sh + n_g);
+    uint32_t *aux = hist + A.lds_reps * n_g;
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     uint32_t *list = aux + aux_n;  // [NF][kCollPerWg]
     float *stc = (float *)(list + NF * kCollPerWg);
     const uint32_t tile_sz = blockDim.x * PPT;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
-    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
     for (int q = threadIdx.x; q < A.stc_n; q += blockDim.x) stc[q] = j.stot[q];
     for (int q = threadIdx.x; q < K; q += blockDim.x) {
@@ -402,8 +437,9 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave;
-    float *h = hist + (wave % A.lds_reps) * n_g;
+    uint32_t *h = hist + (wave % A.lds_reps) * n_g;
     const uint32_t n_tiles = s_tiles[K];
+    const float scale = fix_scale(n_tiles, PPT, A.lds_reps);
     const uint32_t ko = blockIdx.x % (uint32_t)K;  // segment that receives this workgroup's newborns
     const float factor = ELECTRON ? k.factor_e : k.factor_i;
     const float dt = ELECTRON ? k.dt_e : k.dt_i;
@@ -499,7 +535,7 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
             if (in[u]) {
                 hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
                 X[i] = nx[u];
-                deposit_lds<Acc>(h, nx[u], k);
+                deposit_fix(h, nx[u], k, scale);
                 if (!hit) {
                     rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
                     VX[i] = vx[u];
@@ -567,8 +603,8 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
                         if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], w, y, z, k);
                     }
                     if (ELECTRON)
-                        ionisation_products<MEASURE, Acc>(j, k, kind, ko, nx[u], se, si, h, i_dens_g, aux, ef,
-                                                          overflow);
+                        ionisation_products<MEASURE, Acc>(j, k, kind, ko, nx[u], se, si, h, scale, i_dens_g, aux,
+                                                          ef, overflow);
                 }
             }
         }
@@ -612,7 +648,8 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
             VZ[slot] = z;
             ++ncoll;
         }
-        if (ELECTRON) ionisation_products<MEASURE, Acc>(j, k, kind, ko, px, se, si, h, i_dens_g, aux, ef, overflow);
+        if (ELECTRON)
+            ionisation_products<MEASURE, Acc>(j, k, kind, ko, px, se, si, h, scale, i_dens_g, aux, ef, overflow);
     }
     {
         const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
@@ -625,16 +662,7 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
         if (overflow) *j.overflow = 1u;
     }
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
-    Acc *out = (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride;
-    for (int q2 = threadIdx.x; q2 < n_g; q2 += blockDim.x) {
-        float sum = 0.0f;
-        for (int r = 0; r < A.lds_reps; ++r) sum += hist[r * n_g + q2];
-        if (sum != 0.0f) {
-            Acc v = (Acc)sum;
-            if (q2 == 0 || q2 == n_g - 1) v *= (Acc)2;
-            unsafeAtomicAdd(&out[q2], v);
-        }
-    }
+    flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
     uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;
     if (aux_out)
         for (int q2 = threadIdx.x; q2 < aux_n; q2 += blockDim.x)
