@@ -77,6 +77,32 @@ __device__ __forceinline__ void flush_fix(const uint32_t *hist, int lds_reps, in
     }
 }
 
+// Wall losses and collisions of the workgroup: summed in LDS, then one
+// atomic per counter into the workgroup's copy of the counters.  Contains a
+// barrier: every thread of the workgroup calls it.
+template <bool ELECTRON>
+__device__ __forceinline__ void add_diagnostics(const FusedJob &j, uint32_t lost_pow, uint32_t lost_gnd,
+                                                uint32_t ncoll, bool overflow) {
+    __shared__ uint32_t s_diag[3];
+    if (threadIdx.x < 3) s_diag[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
+    if (lane_id() == 0) {
+        if (wp) atomicAdd(&s_diag[0], wp);
+        if (wg) atomicAdd(&s_diag[1], wg);
+        if (wc) atomicAdd(&s_diag[2], wc);
+    }
+    if (overflow) *j.overflow = 1u;
+    __syncthreads();
+    if (threadIdx.x == 0 && j.counters) {
+        unsigned long long *c = j.counters + (size_t)(blockIdx.x % kCounterLanes) * kCounterWords;
+        const int b = ELECTRON ? 0 : 2;
+        if (s_diag[0]) atomicAdd(&c[b], (unsigned long long)s_diag[0]);
+        if (s_diag[1]) atomicAdd(&c[b + 1], (unsigned long long)s_diag[1]);
+        if (s_diag[2]) atomicAdd(&c[ELECTRON ? 4 : 5], (unsigned long long)s_diag[2]);
+    }
+}
+
 template <typename Acc>
 __device__ __forceinline__ void deposit_lds(float *h, float x, const KPhys &k) {
     const float pos = x * k.inv_dx;
@@ -301,17 +327,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
             }
         }
     }
-    // per-workgroup diagnostics: one atomic per counter
-    {
-        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
-        if (lane_id() == 0 && j.counters) {
-            const int b = ELECTRON ? 0 : 2;
-            if (wp) atomicAdd(&j.counters[b], (unsigned long long)wp);
-            if (wg) atomicAdd(&j.counters[b + 1], (unsigned long long)wg);
-            if (wc) atomicAdd(&j.counters[ELECTRON ? 4 : 5], (unsigned long long)wc);
-        }
-        if (overflow) *j.overflow = 1u;
-    }
+    add_diagnostics<ELECTRON>(j, lost_pow, lost_gnd, ncoll, overflow);
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
     // one flush of the workgroup histogram into the next step's density
     flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
@@ -487,9 +503,12 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     uint32_t slot0 = 0, avail = 0;
     load_tile(blockIdx.x, slot0, avail, x, vx, vy, vz, st);
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        bool valid[PPT];
 #pragma unroll
-        for (int u = 0; u < PPT; ++u)
-            if (!(u * blockDim.x < avail)) x[u] = -1.0f;  // past the segment's count
+        for (int u = 0; u < PPT; ++u) {
+            valid[u] = u * blockDim.x < avail;
+            if (!valid[u]) x[u] = -1.0f;  // past the segment's count
+        }
         // (2) push; in-gap particles bin their energy and gather stot
         float nx[PPT], spd[PPT], sig[PPT];
         float ax[PPT], ay[PPT], az[PPT];  // ions: velocity of the collision partner
@@ -525,24 +544,28 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
             if (idx[u] < A.stc_n) sig[u] = stc[idx[u]];
-        // (3) collision tests, write-back of the particles that do not
-        // collide, deposits; colliding particles are listed; lost ones
-        // leave a hole
+        // (3) collision tests, write-back, deposits; colliding particles
+        // are listed; lost ones leave a hole.  Every slot of the tile is
+        // written back (x, vx, RNG words; a colliding particle's values are
+        // overwritten by its scattering after the last tile, holes and lost
+        // particles write their unchanged RNG words): stores that skip
+        // lanes leave partly written lines, which cost a read-modify-write
+        // in memory.
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t i = slot0 + u * blockDim.x;
             bool hit = false;
             if (in[u]) {
                 hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
-                X[i] = nx[u];
                 deposit_fix(h, nx[u], k, scale);
-                if (!hit) {
-                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
-                    VX[i] = vx[u];
-                    if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], vx[u], vy[u], vz[u], k);
-                }
-            } else if (x[u] >= 0.0f) {
-                X[i] = -1.0f;  // hole
+                if (MEASURE && ELECTRON && !hit) eepf_lds(aux, ef, nx[u], vx[u], vy[u], vz[u], k);
+            }
+            if (valid[u]) {
+                X[i] = in[u] ? nx[u] : -1.0f;
+                VX[i] = vx[u];
+                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+            }
+            if (!in[u] && x[u] >= 0.0f) {  // lost in this step
                 const bool pw = nx[u] < 0.0f;
                 lost_pow += pw;
                 lost_gnd += !pw;
@@ -651,16 +674,7 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
         if (ELECTRON)
             ionisation_products<MEASURE, Acc>(j, k, kind, ko, px, se, si, h, scale, i_dens_g, aux, ef, overflow);
     }
-    {
-        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
-        if (lane_id() == 0 && j.counters) {
-            const int b = ELECTRON ? 0 : 2;
-            if (wp) atomicAdd(&j.counters[b], (unsigned long long)wp);
-            if (wg) atomicAdd(&j.counters[b + 1], (unsigned long long)wg);
-            if (wc) atomicAdd(&j.counters[ELECTRON ? 4 : 5], (unsigned long long)wc);
-        }
-        if (overflow) *j.overflow = 1u;
-    }
+    add_diagnostics<ELECTRON>(j, lost_pow, lost_gnd, ncoll, overflow);
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
     flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
     uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;

@@ -123,6 +123,12 @@ int launch_eepf(const float *x, const float *vx, const float *vy, const float *v
 // the next step's density (plus EEPF in measurement steps).  A species is
 // stored as n_seg segments of seg_size slots, each dense with its own count.
 constexpr int kMaxSegments = 64;
+// Measurement counters (pic_sim_read_measurements order) are kept in
+// kCounterLanes copies of kCounterWords each; workgroup b of a particle
+// kernel adds into copy b % kCounterLanes, so that thousands of workgroups
+// ending together do not serialise on one address.  Readers sum the copies.
+constexpr int kCounterWords = 8;
+constexpr int kCounterLanes = 64;
 struct FusedJob {
     bool electrons;
     bool inplace;                         // update in the occupied slots (sx.. are written)
@@ -145,7 +151,7 @@ struct FusedJob {
     // tables and diagnostics
     const float *stot, *cs;
     uint32_t *overflow;
-    unsigned long long *counters;         // [8] as pic_sim_read_measurements
+    unsigned long long *counters;         // [kCounterLanes][kCounterWords]
     uint32_t *eepf;                       // measurement steps only
     uint32_t *ifed;                       // ions only
 };

@@ -36,7 +36,7 @@ enum Cnt {
     C_WORDS = C_ALIVE + 16
 };
 constexpr int kDefaultSegments = 32;
-constexpr int kMeasCounters = 8;
+constexpr int kMeasCounters = kCounterWords;
 constexpr uint32_t kCkptMagic = 0x43504943u;  // "CIPC"
 constexpr uint32_t kCkptVersion = 1;
 }  // namespace
@@ -569,7 +569,7 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     if ((rc = dalloc(&s->cumul_i, s->p.n_g))) return fail(rc);
     if ((rc = dalloc(&s->eepf, PIC_N_EEPF))) return fail(rc);
     if ((rc = dalloc(&s->ifed, 2 * PIC_N_IFED))) return fail(rc);
-    if ((rc = dalloc(&s->counters, kMeasCounters))) return fail(rc);
+    if ((rc = dalloc(&s->counters, (size_t)kCounterLanes * kMeasCounters))) return fail(rc);
     if ((rc = dalloc(&s->cs, (size_t)PIC_N_CS * PIC_CS_RANGES))) return fail(rc);
     if ((rc = dalloc(&s->stot_e, PIC_CS_RANGES))) return fail(rc);
     if ((rc = dalloc(&s->stot_i, PIC_CS_RANGES))) return fail(rc);
@@ -815,8 +815,13 @@ int pic_sim_read_measurements(pic_sim_t *s, double *cumul_e, double *cumul_i, ui
     if (eepf) PIC_HIP(hipMemcpy(eepf, s->eepf, sizeof(uint32_t) * PIC_N_EEPF, hipMemcpyDeviceToHost));
     if (ifed) PIC_HIP(hipMemcpy(ifed, s->ifed, sizeof(uint32_t) * 2 * PIC_N_IFED, hipMemcpyDeviceToHost));
     if (counters) {
-        PIC_HIP(hipMemcpy(counters, s->counters, sizeof(unsigned long long) * kMeasCounters,
+        std::vector<unsigned long long> lanes((size_t)kCounterLanes * kMeasCounters);
+        PIC_HIP(hipMemcpy(lanes.data(), s->counters, sizeof(unsigned long long) * lanes.size(),
                           hipMemcpyDeviceToHost));
+        for (int c = 0; c < kMeasCounters; ++c) {
+            counters[c] = 0;
+            for (int l = 0; l < kCounterLanes; ++l) counters[c] += lanes[(size_t)l * kMeasCounters + c];
+        }
         counters[6] += s->ref_meas[0];
         counters[7] += s->ref_meas[1];
     }
@@ -830,7 +835,7 @@ int pic_sim_reset_measurements(pic_sim_t *s) {
     PIC_HIP(hipMemset(s->cumul_i, 0, sizeof(double) * s->p.n_g));
     PIC_HIP(hipMemset(s->eepf, 0, sizeof(uint32_t) * PIC_N_EEPF));
     PIC_HIP(hipMemset(s->ifed, 0, sizeof(uint32_t) * 2 * PIC_N_IFED));
-    PIC_HIP(hipMemset(s->counters, 0, sizeof(unsigned long long) * kMeasCounters));
+    PIC_HIP(hipMemset(s->counters, 0, sizeof(unsigned long long) * kCounterLanes * kMeasCounters));
     s->ref_meas[0] = s->ref_meas[1] = 0;
     return PIC_OK;
 }
@@ -1022,7 +1027,8 @@ int pic_sim_load(pic_sim_t *s, const char *path) {
     PIC_HIP(hipMemcpy(s->cumul_i, cii.data(), 8ull * s->p.n_g, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(s->eepf, ee.data(), 4ull * PIC_N_EEPF, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(s->ifed, ifd.data(), 4ull * 2 * PIC_N_IFED, hipMemcpyHostToDevice));
-    PIC_HIP(hipMemcpy(s->counters, cnt, 8ull * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemset(s->counters, 0, sizeof(unsigned long long) * kCounterLanes * kMeasCounters));
+    PIC_HIP(hipMemcpy(s->counters, cnt, 8ull * kMeasCounters, hipMemcpyHostToDevice));
     s->ref_meas[0] = s->ref_meas[1] = 0;
     s->t = hdr[3] % (uint32_t)s->p.n_t;
     s->steps = steps;

@@ -32,6 +32,7 @@ def test_ensemble_step_per_instance_replay(ca, orc):
     rng0 = [(ens.rng_download(i, 0, caps[i]), ens.rng_download(i, 1, caps[i])) for i in range(3)]
     ens.run(1, measure=True)
     ne, ni = ens.counts()
+    born_total = 0
     for i, (v, pr, w, n) in enumerate(specs):
         po = orc.params(v0=v, pressure=pr, weight=w)
         E, I = states[i]
@@ -59,8 +60,12 @@ def test_ensemble_step_per_instance_replay(ca, orc):
         assert not old_ion[cnt_i:].any()
         _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi],
                         np.arange(len(gi[0]))[old_ion], rng0[i][1], ri1, "i", missing=n_amb)
-        assert ni[i] - cnt_i == ne[i] - cnt_e == len(born_x) > 0
+        # (slots are assigned in atomic order, so the RNG stream a particle
+        # meets and the number of ionisations vary from run to run)
+        assert ni[i] - cnt_i == ne[i] - cnt_e == len(born_x)
+        born_total += len(born_x)
         assert cnt[6] == 1 and cnt[7] == 1 and cnt[4] > 0
+    assert born_total > 0
     ens.check()
     ens.close()
 
