@@ -244,20 +244,13 @@ __device__ __forceinline__ int e_scatter(const KPhys &k, float c0, float c1, flo
     return kind;
 }
 
-// Total cross section of bin idx: from the first stc_n bins staged in LDS
-// when the bin is there, else from the table in global memory.
-__device__ __forceinline__ float stot_at(const float *__restrict__ stot, const float *stc, int stc_n, int idx) {
-    return idx < stc_n ? stc[idx] : stot[idx];
-}
-
 __device__ __forceinline__ int collide_electron(const float *__restrict__ stot_e,
                                                 const float *__restrict__ cs, const KPhys &k,
                                                 float &vx, float &vy, float &vz, Xo &s,
-                                                float *se, float *si, const float *stc = nullptr,
-                                                int stc_n = 0) {
+                                                float *se, float *si) {
     float velocity;
     const int idx = e_index(k, vx, vy, vz, velocity);
-    if (!e_test(k, stot_at(stot_e, stc, stc_n, idx), velocity, s)) return kNone;
+    if (!e_test(k, stot_e[idx], velocity, s)) return kNone;
     return e_scatter(k, cs[idx], cs[kCsRanges + idx], cs[2 * kCsRanges + idx], vx, vy, vz, s, se, si);
 }
 
@@ -311,11 +304,10 @@ __device__ __forceinline__ int i_scatter(const KPhys &k, float c0, float c1, flo
 
 __device__ __forceinline__ int collide_ion(const float *__restrict__ stot_i,
                                            const float *__restrict__ cs, const KPhys &k,
-                                           float &vx, float &vy, float &vz, Xo &s,
-                                           const float *stc = nullptr, int stc_n = 0) {
+                                           float &vx, float &vy, float &vz, Xo &s) {
     float ax, ay, az, g;
     const int idx = i_index(k, vx, vy, vz, s, ax, ay, az, g);
-    if (!i_test(k, stot_at(stot_i, stc, stc_n, idx), g, s)) return kNone;
+    if (!i_test(k, stot_i[idx], g, s)) return kNone;
     return i_scatter(k, cs[3 * kCsRanges + idx], cs[4 * kCsRanges + idx], vx, vy, vz, ax, ay, az, g, s);
 }
 

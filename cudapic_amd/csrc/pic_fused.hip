@@ -31,7 +31,6 @@ struct FusedArgs {
     KPhys k;
     int lds_reps;
     int reps, rep_stride;
-    int stc_n;  // leading bins of the total cross-section table staged in LDS
 };
 
 // Workgroup density histograms of the single-discharge kernels are fixed
@@ -75,6 +74,15 @@ __device__ __forceinline__ void flush_fix(const uint32_t *hist, int lds_reps, in
             unsafeAtomicAdd(&out[q], v);
         }
     }
+}
+
+// This workgroup's share of a buffer the particle kernels clear while they
+// run.
+__device__ __forceinline__ void clear_slice(uint32_t *buf, uint32_t words) {
+    if (!buf) return;
+    const uint32_t per = (words + gridDim.x - 1) / gridDim.x;
+    const uint32_t w0 = blockIdx.x * per, w1 = min(words, w0 + per);
+    for (uint32_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) buf[w] = 0u;
 }
 
 // Wall losses and collisions of the workgroup: summed in LDS, then one
@@ -178,8 +186,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
-    float *stc = (float *)(aux + aux_n);
-    for (int q = threadIdx.x; q < A.stc_n; q += blockDim.x) stc[q] = j.stot[q];
+    clear_slice(j.zero, j.zero_words);
     // live count of every input segment (clamped: a counter that overflowed
     // its segment counts dropped particles) and its tile count
     for (int q = threadIdx.x; q < K; q += blockDim.x) {
@@ -268,8 +275,8 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
                     const uint32_t slot = out_base + local;
                     Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot);
                     float vx = nvx[u], y = vy[u], z = vz[u];
-                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si, stc, A.stc_n);
-                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s, stc, A.stc_n);
+                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
+                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s);
                     rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
                     j.dx[slot] = nx[u];
                     j.dvx[slot] = vx;
@@ -430,12 +437,11 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     uint32_t *aux = hist + A.lds_reps * n_g;
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     uint32_t *list = aux + aux_n;  // [NF][kCollPerWg]
-    float *stc = (float *)(list + NF * kCollPerWg);
     const uint32_t tile_sz = blockDim.x * PPT;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
-    for (int q = threadIdx.x; q < A.stc_n; q += blockDim.x) stc[q] = j.stot[q];
+    clear_slice(j.zero, j.zero_words);
     for (int q = threadIdx.x; q < K; q += blockDim.x) {
         const uint32_t c = min(j.cnt_in[q], j.seg_size);
         s_cnt[q] = c;
@@ -455,7 +461,13 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     const int wave = threadIdx.x / kWave;
     uint32_t *h = hist + (wave % A.lds_reps) * n_g;
     const uint32_t n_tiles = s_tiles[K];
-    const float scale = fix_scale(n_tiles, PPT, A.lds_reps);
+    // fixed-point scale 2^20; a replica takes at most cap tiles of deposits
+    // (two per particle) plus the collision list's before it could overflow,
+    // so the histograms are flushed and cleared every cap tiles
+    const float scale = 1048576.0f;
+    const uint32_t waves_per_rep = (blockDim.x / kWave + A.lds_reps - 1) / A.lds_reps;
+    const uint32_t cap = (4096u - kCollPerWg - 1u) / (waves_per_rep * 2u * PPT * kWave);
+    uint32_t since_flush = 0;
     const uint32_t ko = blockIdx.x % (uint32_t)K;  // segment that receives this workgroup's newborns
     const float factor = ELECTRON ? k.factor_e : k.factor_i;
     const float dt = ELECTRON ? k.dt_e : k.dt_i;
@@ -501,8 +513,11 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     float x[PPT], vx[PPT], vy[PPT], vz[PPT];
     Xo st[PPT];
     uint32_t slot0 = 0, avail = 0;
-    load_tile(blockIdx.x, slot0, avail, x, vx, vy, vz, st);
-    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    // workgroup b takes tiles b, b + G, b + 2G, ... (G = grid size)
+    uint32_t tile = blockIdx.x;
+    load_tile(tile, slot0, avail, x, vx, vy, vz, st);
+    while (tile < n_tiles) {
+        const uint32_t tile_next = tile + gridDim.x;
         bool valid[PPT];
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
@@ -532,18 +547,15 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
                 else idx[u] = i_index(k, vx[u], vy[u], vz[u], st[u], ax[u], ay[u], az[u], spd[u]);
             }
         }
-        // bins past the LDS copy come from global memory; the gather is
-        // issued for every lane (the others read bin 0, one shared line) so
-        // that it stays in flight while the next tile's loads are issued
+        // the gather is issued for every lane (the others read bin 0, one
+        // shared line) so that it stays in flight while the next tile's
+        // loads are issued
 #pragma unroll
-        for (int u = 0; u < PPT; ++u) sig[u] = j.stot[(in[u] && idx[u] >= A.stc_n) ? idx[u] : 0];
+        for (int u = 0; u < PPT; ++u) sig[u] = j.stot[in[u] ? idx[u] : 0];
         float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
         Xo pst[PPT];
         uint32_t pslot0, pavail;
-        load_tile(tile + gridDim.x, pslot0, pavail, px, pvx, pvy, pvz, pst);
-#pragma unroll
-        for (int u = 0; u < PPT; ++u)
-            if (idx[u] < A.stc_n) sig[u] = stc[idx[u]];
+        load_tile(tile_next, pslot0, pavail, px, pvx, pvy, pvz, pst);
         // (3) collision tests, write-back, deposits; colliding particles
         // are listed; lost ones leave a hole.  Every slot of the tile is
         // written back (x, vx, RNG words; a colliding particle's values are
@@ -640,6 +652,15 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
             vy[u] = pvy[u];
             vz[u] = pvz[u];
             st[u] = pst[u];
+        }
+        tile = tile_next;
+        if (++since_flush == cap && tile < n_tiles) {  // uniform over the workgroup
+            since_flush = 0;
+            __syncthreads();
+            flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
+            __syncthreads();
+            for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
+            __syncthreads();
         }
     }
     // (4) scatter the listed collisions, 64 per wave pass
@@ -740,9 +761,8 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
     // LDS per workgroup: what the resident workgroups of a compute unit can
     // share (2 KB kept for the static arrays), at most what one may allocate.
     // It holds the field, the EEPF / IFED bins, the collision list of the
-    // in-place kernel, one histogram replica per wave (fewer when space is
-    // short) and then as many leading bins of the total cross-section table
-    // as fit (the collision test reads one bin per particle).
+    // in-place kernel and one histogram replica per wave (fewer when space is
+    // short).
     const int per_cu = std::max(1, (grid + device_cus() - 1) / device_cus());
     const int budget = std::min(device_lds_per_block(), device_lds_per_cu() / per_cu - 2048);
     const int list_words = j.inplace ? (j.electrons ? kCollFieldsE : kCollFieldsI) * (int)kCollPerWg : 0;
@@ -750,9 +770,7 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
     a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (k.n_g * 4)));
     a.reps = sh.replicas;
     a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
-    const int used = fixed + a.lds_reps * k.n_g * 4;
-    a.stc_n = std::max(0, std::min(kCsRanges, (budget - used) / 4));
-    const size_t lds = (size_t)used + (size_t)a.stc_n * 4;
+    const size_t lds = (size_t)fixed + (size_t)a.lds_reps * k.n_g * 4;
     const int ppt = j.inplace ? sh.ppt_inplace : sh.ppt;
     if (j.electrons) {
         if (measure) launch_ppt<true, true>(a, j.dens_f64, j.inplace, ppt, grid, block, lds, s);

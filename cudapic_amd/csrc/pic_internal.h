@@ -58,6 +58,12 @@ struct PoissonJob {
     int replicas = 1, rep_stride = 0;
     double pot0 = 0.0;                // used when time == nullptr
     const uint32_t *time = nullptr;   // device step index within the cycle
+    // with time: pot0 of the step read from pot0_in when given (else computed
+    // from the waveform), pot0 of the next step written to pot0_next
+    const double *pot0_in = nullptr;
+    double *pot0_next = nullptr;
+    // the summed node densities the solve used (n_g doubles each), optional
+    double *dens_e_out = nullptr, *dens_i_out = nullptr;
     int n_t = 4000, n_sub = 20;
     double wave_cos[PIC_MAX_HARMONICS] = {}, wave_sin[PIC_MAX_HARMONICS] = {};
     double e_charge = 0, alpha = 0, inv_dx = 0, half_inv_dx = 0, half_dx_over_eps = 0;
@@ -73,6 +79,8 @@ struct PoissonJob {
     uint32_t *advance_time = nullptr;  // == time: *time <- (*time + 1) % n_t after the solve
 };
 int launch_poisson(const PoissonJob &j, hipStream_t s);
+// *out = pot0 of the step *time (the waveform the Poisson kernel evaluates)
+int launch_pot0(const PoissonJob &j, double *out, hipStream_t s);
 
 // n_cap bounds the live count read from n_active (an overflowed counter can
 // exceed the array it counts)
@@ -154,6 +162,10 @@ struct FusedJob {
     unsigned long long *counters;         // [kCounterLanes][kCounterWords]
     uint32_t *eepf;                       // measurement steps only
     uint32_t *ifed;                       // ions only
+    // 32-bit words the workgroups clear between them while they run (the
+    // density buffer the step's Poisson solve has read; optional)
+    uint32_t *zero;
+    uint32_t zero_words;
 };
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
 // *out += number of x >= 0 in the first cnt[q] slots of each segment q

@@ -301,11 +301,22 @@ __device__ __forceinline__ T block_scan_incl(T v, T *wtot) {
 constexpr int kPoissonThreads = 1024;
 constexpr int kPoissonMaxPer = 4;  // n_g <= 4096
 
+// Nodes per lane and lanes of a Poisson workgroup: as few waves as cover the
+// grid (their barriers and the inter-wave scans are the solve's latency).
+inline int poisson_per(int n_g) { return (n_g + kPoissonThreads - 1) / kPoissonThreads; }
+inline int poisson_threads(int n_g) {
+    const int lanes = (n_g + poisson_per(n_g) - 1) / poisson_per(n_g);
+    return std::min(kPoissonThreads, (lanes + kWave - 1) / kWave * kWave);
+}
+
 struct PoissonArgs {
     const void *e_dens, *i_dens;
     PoissonGeom g;
     double pot0;
     const uint32_t *time;
+    const double *pot0_in;
+    double *pot0_next;
+    double *dens_e_out, *dens_i_out;
     int n_t, n_sub, n_harm;
     double wave_cos[PIC_MAX_HARMONICS], wave_sin[PIC_MAX_HARMONICS];
     float *pot, *efield;
@@ -490,37 +501,60 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     __shared__ double s_pot0;
     __shared__ int s_ion;
     const int tid = threadIdx.x;
-    // the density loads are issued before the clock is read, so both
-    // latencies overlap
+    // the clock, the step's pot0, the counters to copy and the densities are
+    // loaded together
+    uint32_t t = 0;
+    double p0_in = 0.0;
+    if (tid == 0 && a.time) {
+        t = *a.time;
+        if (a.pot0_in) p0_in = *a.pot0_in;
+    }
+    uint32_t cnt_copy = 0;
+    if (a.copy_cnt_dst && tid < a.copy_cnt_words) cnt_copy = a.copy_cnt_src[tid];
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
     load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, dne, dni);
-    uint32_t t = 0;
     if (tid == 0) {
         double p0 = a.pot0;
         int ion = 1;
         if (a.time) {
-            t = *a.time;
             ion = (t % (uint32_t)a.n_sub) == 0;
-            p0 = waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
+            p0 = a.pot0_in ? p0_in : waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
         }
         s_pot0 = p0;
         s_ion = ion;
     }
     __syncthreads();
     poisson_solve<T>(a.g, dne, dni, s_pot0, s_ion != 0, a.pot, a.efield, a.cumul_e, a.cumul_i, pot_s, wtot);
+    if (a.dens_e_out) {
+        const int k0 = tid * a.g.per;
+#pragma unroll
+        for (int u = 0; u < kPoissonMaxPer; ++u)
+            if (u < a.g.per && k0 + u < a.g.n_g) {
+                a.dens_e_out[k0 + u] = dne[u];
+                a.dens_i_out[k0 + u] = dni[u];
+            }
+    }
     // fused-path bookkeeping for the particle kernels that follow
     for (int w = tid; w < a.zero_a_words; w += blockDim.x) a.zero_a[w] = 0u;
     if (s_ion)
         for (int w = tid; w < a.zero_b_words; w += blockDim.x) a.zero_b[w] = 0u;
     if (a.zero_cnt && tid < a.zero_cnt_words) a.zero_cnt[tid] = 0u;
     if (s_ion && a.zero_cnt_ion && tid < a.zero_cnt_ion_words) a.zero_cnt_ion[tid] = 0u;
-    if (a.copy_cnt_dst && tid < a.copy_cnt_words) a.copy_cnt_dst[tid] = a.copy_cnt_src[tid];
-    if (tid == 1023 && a.meas_steps && a.cumul_e) {
+    if (a.copy_cnt_dst && tid < a.copy_cnt_words) a.copy_cnt_dst[tid] = cnt_copy;
+    if (tid == (int)blockDim.x - 1 && a.meas_steps && a.cumul_e) {
         a.meas_steps[0] += 1ull;
         if (s_ion) a.meas_steps[1] += 1ull;
     }
     // the clock was read by this lane (the same word when advance_time is set)
-    if (tid == 0 && a.advance_time) *a.advance_time = (t + 1u) % (uint32_t)a.n_t;
+    if (tid == 0 && a.advance_time) {
+        const uint32_t tn = (t + 1u) % (uint32_t)a.n_t;
+        *a.advance_time = tn;
+        if (a.pot0_next) *a.pot0_next = waveform(tn, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
+    }
+}
+
+__global__ void k_pot0(const uint32_t *time, int n_t, int n_harm, PoissonArgs a, double *out) {
+    *out = waveform(*time, n_t, n_harm, a.wave_cos, a.wave_sin);
 }
 
 int launch_poisson(const PoissonJob &j, hipStream_t s) {
@@ -536,6 +570,14 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
         set_error("poisson: the clock advanced must be the clock read");
         return PIC_ERR_ARG;
     }
+    if ((j.pot0_in || j.pot0_next) && !j.time) {
+        set_error("poisson: a precomputed pot0 needs the device clock");
+        return PIC_ERR_ARG;
+    }
+    if ((j.dens_e_out == nullptr) != (j.dens_i_out == nullptr)) {
+        set_error("poisson: both density outputs or neither");
+        return PIC_ERR_ARG;
+    }
     PoissonArgs a;
     memset(&a, 0, sizeof(a));
     a.e_dens = j.e_dens;
@@ -543,9 +585,13 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     a.g.reps = std::max(1, j.replicas);
     a.g.rep_stride = j.rep_stride > 0 ? j.rep_stride : j.n_g;
     a.g.n_g = j.n_g;
-    a.g.per = (j.n_g + kPoissonThreads - 1) / kPoissonThreads;
+    a.g.per = poisson_per(j.n_g);
     a.pot0 = j.pot0;
     a.time = j.time;
+    a.pot0_in = j.pot0_in;
+    a.pot0_next = j.pot0_next;
+    a.dens_e_out = j.dens_e_out;
+    a.dens_i_out = j.dens_i_out;
     a.n_t = j.n_t;
     a.n_sub = j.n_sub;
     a.n_harm = 0;
@@ -579,17 +625,17 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     switch (j.precision) {
     case PIC_PREC_FP32: {
         size_t lds = (j.n_g + 17) * sizeof(float);
-        k_poisson<float, float><<<1, kPoissonThreads, lds, s>>>(a);
+        k_poisson<float, float><<<1, poisson_threads(j.n_g), lds, s>>>(a);
         break;
     }
     case PIC_PREC_COMP: {
         size_t lds = (j.n_g + 17) * sizeof(ff);
-        k_poisson<ff, float><<<1, kPoissonThreads, lds, s>>>(a);
+        k_poisson<ff, float><<<1, poisson_threads(j.n_g), lds, s>>>(a);
         break;
     }
     case PIC_PREC_FP64: {
         size_t lds = (j.n_g + 17) * sizeof(double);
-        k_poisson<double, double><<<1, kPoissonThreads, lds, s>>>(a);
+        k_poisson<double, double><<<1, poisson_threads(j.n_g), lds, s>>>(a);
         break;
     }
     default:
@@ -597,6 +643,24 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
         return PIC_ERR_ARG;
     }
     PIC_LAUNCHED("k_poisson");
+    return PIC_OK;
+}
+
+int launch_pot0(const PoissonJob &j, double *out, hipStream_t s) {
+    if (!j.time || !out || j.n_t <= 0) {
+        set_error("pot0: needs the device clock and an output");
+        return PIC_ERR_ARG;
+    }
+    PoissonArgs a;
+    memset(&a, 0, sizeof(a));
+    int n_harm = 0;
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) {
+        a.wave_cos[h] = j.wave_cos[h];
+        a.wave_sin[h] = j.wave_sin[h];
+        if (j.wave_cos[h] != 0.0 || j.wave_sin[h] != 0.0) n_harm = h + 1;
+    }
+    k_pot0<<<1, 1, 0, s>>>(j.time, j.n_t, n_harm, a, out);
+    PIC_LAUNCHED("k_pot0");
     return PIC_OK;
 }
 
@@ -628,10 +692,11 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
     const int tid = threadIdx.x;
     const int n_g = a.g.n_g;
     const size_t off = (size_t)b * a.inst_stride;
+    uint32_t t = 0;
+    if (tid == 0) t = *a.time;
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
     load_node_densities<D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, dne, dni);
     if (tid == 0) {
-        const uint32_t t = *a.time;
         s_ion = (t % (uint32_t)a.n_sub) == 0;
         const double wc = a.inst[b].v0, ws = 0.0;
         s_pot0 = waveform(t, a.n_t, 1, &wc, &ws);
@@ -665,7 +730,7 @@ int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
     EnsPoissonArgs a;
     memset(&a, 0, sizeof(a));
     a.g = j.g;
-    a.g.per = (j.g.n_g + kPoissonThreads - 1) / kPoissonThreads;
+    a.g.per = poisson_per(j.g.n_g);
     a.inst = j.inst;
     a.e_dens = j.e_dens;
     a.i_dens = j.i_dens;
@@ -687,15 +752,15 @@ int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
     switch (j.precision) {
     case PIC_PREC_FP32:
         a.zero_words = j.inst_stride;
-        k_poisson_ens<float, float><<<j.n_inst, kPoissonThreads, (j.g.n_g + 17) * sizeof(float), s>>>(a);
+        k_poisson_ens<float, float><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + 17) * sizeof(float), s>>>(a);
         break;
     case PIC_PREC_COMP:
         a.zero_words = j.inst_stride;
-        k_poisson_ens<ff, float><<<j.n_inst, kPoissonThreads, (j.g.n_g + 17) * sizeof(ff), s>>>(a);
+        k_poisson_ens<ff, float><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + 17) * sizeof(ff), s>>>(a);
         break;
     default:
         a.zero_words = 2 * j.inst_stride;
-        k_poisson_ens<double, double><<<j.n_inst, kPoissonThreads, (j.g.n_g + 17) * sizeof(double), s>>>(a);
+        k_poisson_ens<double, double><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + 17) * sizeof(double), s>>>(a);
         break;
     }
     PIC_LAUNCHED("k_poisson_ens");

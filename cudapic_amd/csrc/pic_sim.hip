@@ -33,6 +33,7 @@ enum Cnt {
     C_ALIVE = 4 * kMaxSegments,     // reference path scratch counter
     C_OVERFLOW = C_ALIVE + 1,
     C_TIME = C_ALIVE + 2,           // step index within the cycle (fused path clock)
+    C_P0 = C_ALIVE + 4,             // pot0 of that step (double, 8-byte aligned)
     C_WORDS = C_ALIVE + 16
 };
 constexpr int kDefaultSegments = 32;
@@ -59,6 +60,7 @@ struct pic_sim {
     uint32_t *cnt = nullptr;
     // grid
     float *pot = nullptr, *efield = nullptr;
+    double *grid_e = nullptr, *grid_i = nullptr;  // node densities of the last solve
     void *e_dens[2] = {}, *i_dens[2] = {};  // replicas * stride entries of 8 bytes
     int dens_words = 0;                      // entries per buffer (replicas * stride)
     // measurement
@@ -71,7 +73,6 @@ struct pic_sim {
     int ce = 0, ci = 0;      // current particle-array parity (electrons, ions)
     int pe = 0;              // electron counter parity (flips every fused step)
     int cd = 0, hi = 0;      // fused: current e density, held ion density
-    int last_e = 0, last_i = 0;  // densities used by the last Poisson solve
     uint32_t t = 0;          // next step's index in the cycle (host mirror)
     unsigned long long steps = 0;
     bool need_prologue = true;
@@ -134,6 +135,8 @@ PoissonJob base_poisson(pic_sim *s) {
     }
     j.pot = s->pot;
     j.efield = s->efield;
+    j.dens_e_out = s->grid_e;
+    j.dens_i_out = s->grid_i;
     j.replicas = s->sh.replicas;
     j.rep_stride = s->sh.rep_stride;
     return j;
@@ -179,8 +182,6 @@ int step_reference(pic_sim *s, bool measure) {
     const double p0 = host_pot0(s->p, s->t);
     pj.pot0 = s->prec == PIC_PREC_FP64 ? p0 : (double)(float)p0;
     TRY(launch_poisson(pj, st));
-    s->last_e = 0;
-    s->last_i = 0;
     // main.rs:1746-1756
     if (measure) {
         TRY(launch_accum_density(s->e_dens[0], s->f64, s->cumul_e, s->p.n_g, st));
@@ -242,6 +243,7 @@ int fused_prologue(pic_sim *s) {
     hipStream_t st = s->stream;
     Shape sh = s->sh;
     TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
+    TRY(launch_fill(s->e_dens[1 - s->cd], 0u, dens_words32(s), st));  // the first step deposits here
     TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
     TRY(launch_fill(s->i_dens[1], 0u, dens_words32(s), st));
     for (int q = 0; q < s->nseg; ++q) {
@@ -250,8 +252,14 @@ int fused_prologue(pic_sim *s) {
         TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[0], s->f64, s->k, sh, st, s->seg));
         TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[1], s->f64, s->k, sh, st, s->seg));
     }
-    // the device clock follows the host's
+    // the device clock follows the host's; the pot0 of its step is computed
+    // by the device waveform (each Poisson solve then computes the next one)
     PIC_HIP(hipMemcpyAsync(s->cnt + C_TIME, &s->t, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    {
+        PoissonJob pj = base_poisson(s);
+        pj.time = s->cnt + C_TIME;
+        TRY(launch_pot0(pj, (double *)(s->cnt + C_P0), st));
+    }
     PIC_HIP(hipStreamSynchronize(st));
     s->need_prologue = false;
     return PIC_OK;
@@ -276,8 +284,8 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     pj.i_dens = s->i_dens[i_read];
     pj.time = s->cnt + C_TIME;
     pj.advance_time = s->cnt + C_TIME;
-    pj.zero_a = s->e_dens[1 - s->cd];
-    pj.zero_a_words = (int)dens_words32(s);
+    pj.pot0_in = (const double *)(s->cnt + C_P0);
+    pj.pot0_next = (double *)(s->cnt + C_P0);
     if (ion) {
         pj.zero_b = s->i_dens[s->hi];  // old held buffer becomes the accumulator
         pj.zero_b_words = (int)dens_words32(s);
@@ -300,8 +308,6 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
         pj.meas_steps = s->counters + 6;
     }
     TRY(launch_poisson(pj, st));
-    s->last_e = s->cd;
-    s->last_i = i_read;
     const int i_acc = ion ? s->hi : 1 - s->hi;  // accumulator for the next ion step
     FusedJob base;
     memset(&base, 0, sizeof(base));
@@ -363,6 +369,9 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     fe.dens = s->e_dens[1 - s->cd];
     fe.stot = s->stot_e;
     fe.eepf = measure ? s->eepf : nullptr;
+    // the density this step's solve read is the next step's accumulator
+    fe.zero = (uint32_t *)s->e_dens[s->cd];
+    fe.zero_words = (uint32_t)dens_words32(s);
     if (time_kernel) PIC_HIP(hipEventRecord(e0, st));
     TRY(launch_fused(fe, s->k, s->sh, st));
     if (time_kernel) PIC_HIP(hipEventRecord(e1, st));
@@ -464,7 +473,7 @@ int free_all(pic_sim *s) {
         (void)hipFree(s->e_rng[c]);
         (void)hipFree(s->i_rng[c]);
     }
-    void *bufs[] = {s->cnt, s->pot, s->efield, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
+    void *bufs[] = {s->cnt, s->pot, s->efield, s->grid_e, s->grid_i, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->cumul_e, s->cumul_i, s->eepf, s->ifed, s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
     for (int i = 0; i < 2; ++i)
@@ -533,7 +542,7 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     sh.ppt_inplace = c->ppt_inplace > 0 ? c->ppt_inplace : 2;
     // global density replicas: 1, or a multiple of 8 (the Poisson kernel
     // reads them in groups of eight)
-    sh.replicas = c->replicas > 0 ? c->replicas : (c->path == PIC_PATH_FUSED ? 16 : 1);
+    sh.replicas = c->replicas > 0 ? c->replicas : (c->path == PIC_PATH_FUSED ? 8 : 1);
     if (sh.replicas > 1) sh.replicas = std::min(64, (sh.replicas + 7) / 8 * 8);
     // replica r of a density buffer starts at element r * rep_stride (n_g
     // rounded up to 32 elements), so the same node of different replicas
@@ -561,6 +570,8 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     if ((rc = dalloc(&s->cnt, C_WORDS))) return fail(rc);
     if ((rc = dalloc(&s->pot, s->p.n_g))) return fail(rc);
     if ((rc = dalloc(&s->efield, s->p.n_g))) return fail(rc);
+    if ((rc = dalloc(&s->grid_e, s->p.n_g))) return fail(rc);
+    if ((rc = dalloc(&s->grid_i, s->p.n_g))) return fail(rc);
     for (int q = 0; q < 2; ++q) {
         if ((rc = dalloc((double **)&s->e_dens[q], s->dens_words))) return fail(rc);
         if ((rc = dalloc((double **)&s->i_dens[q], s->dens_words))) return fail(rc);
@@ -781,27 +792,9 @@ int pic_sim_read_grid(pic_sim_t *s, float *pot, float *efield, double *e_dens, d
     const int n_g = s->p.n_g;
     if (pot) PIC_HIP(hipMemcpy(pot, s->pot, sizeof(float) * n_g, hipMemcpyDeviceToHost));
     if (efield) PIC_HIP(hipMemcpy(efield, s->efield, sizeof(float) * n_g, hipMemcpyDeviceToHost));
-    void *src[2] = {s->cfg.path == PIC_PATH_FUSED ? s->e_dens[s->last_e] : s->e_dens[0],
-                    s->cfg.path == PIC_PATH_FUSED ? s->i_dens[s->last_i] : s->i_dens[0]};
-    double *dst[2] = {e_dens, i_dens};
-    const int reps = s->cfg.path == PIC_PATH_FUSED ? s->sh.replicas : 1;
-    const int stride = s->cfg.path == PIC_PATH_FUSED ? s->sh.rep_stride : n_g;
-    for (int q = 0; q < 2; ++q) {
-        if (!dst[q]) continue;
-        std::vector<double> h((size_t)reps * stride);
-        if (s->f64) {
-            PIC_HIP(hipMemcpy(h.data(), src[q], sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-        } else {
-            std::vector<float> f(h.size());
-            PIC_HIP(hipMemcpy(f.data(), src[q], sizeof(float) * f.size(), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < f.size(); ++i) h[i] = f[i];
-        }
-        for (int k = 0; k < n_g; ++k) {
-            double acc = 0.0;
-            for (int r = 0; r < reps; ++r) acc += h[(size_t)r * stride + k];
-            dst[q][k] = acc;
-        }
-    }
+    // the summed node densities the last Poisson solve wrote out
+    if (e_dens) PIC_HIP(hipMemcpy(e_dens, s->grid_e, sizeof(double) * n_g, hipMemcpyDeviceToHost));
+    if (i_dens) PIC_HIP(hipMemcpy(i_dens, s->grid_i, sizeof(double) * n_g, hipMemcpyDeviceToHost));
     return PIC_OK;
 }
 
