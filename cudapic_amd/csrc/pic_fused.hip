@@ -4,20 +4,22 @@
 // (Rust-CUDA/src/main.rs:566-678, 971-1292, 356-406, 1301-1339) in a single
 // pass over the particles.
 //
-// Per particle-step the HBM traffic is the compulsory 64 B: read x,vx,vy,vz
-// (16 B), write them (16 B), read and write the four xoshiro words of the
-// destination slot (32 B).  The field, the histograms and the cross-section
-// gathers stay in LDS / L2.
+// A species is stored in segments (each dense from its start, with its own
+// live count).  Most steps update the particles in place (k_inplace): a
+// particle stays in its slot and collides with that slot's RNG stream, a
+// particle that leaves the gap becomes a hole (x = -1).  Every 2*N_SUB steps
+// the compacting kernel (k_fused) squeezes the holes out into the other
+// array of the pair; there, as in the reference (main.rs:1785-1791), a
+// survivor collides with the RNG stream of the slot it lands in.  Electrons
+// born in a step are appended after their segment's watermark and are not
+// collision-tested in that step (the loop bound is the watermark read at
+// launch, main.rs:992); the next step's density includes every survivor and
+// every newborn particle.
 //
-// Semantics kept from the reference: survivors are compacted first, then the
-// collision of a survivor uses the RNG stream of the slot it lands in
-// (main.rs:1785-1791 runs check_collisions_e over the compacted array with
-// per-slot streams); electrons born in a step are not collision-tested in
-// that step (the collision bound is the pre-spawn count, main.rs:992); the
-// next step's density includes every survivor and every newborn particle.
-// The fused kernel stages newborn electrons in a side array that the next
-// step reads after the main array, since the main array's survivor count is
-// only final when the kernel ends.
+// HBM traffic per in-place particle-step: read x, v and the four RNG words
+// (32 B), write x, vx and the RNG words (24 B), vy and vz only after a
+// collision.  The field, the histograms and the cross-section gathers stay
+// in LDS / L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
