@@ -36,7 +36,13 @@ enum Cnt {
     C_P0 = C_ALIVE + 4,             // pot0 of that step (double, 8-byte aligned)
     C_WORDS = C_ALIVE + 16
 };
+// Default segment count: one per 128 Ki slots, at most 32.  Segments spread
+// the slot reservations of millions of particles over separate counters; a
+// compaction sends each tile's survivors to one segment, so a segment must
+// hold its share of tiles plus one tile, which small capacities cannot
+// afford with many segments.
 constexpr int kDefaultSegments = 32;
+constexpr uint32_t kSlotsPerDefaultSegment = 131072;
 constexpr int kMeasCounters = kCounterWords;
 constexpr uint32_t kCkptMagic = 0x43504943u;  // "CIPC"
 constexpr uint32_t kCkptVersion = 1;
@@ -523,7 +529,11 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     s->cap = c->capacity;
     // the fused path keeps each species in segments with their own counters;
     // the reference path in one (its stage kernels see a plain array)
-    s->nseg = c->path == PIC_PATH_FUSED ? (c->segments > 0 ? c->segments : kDefaultSegments) : 1;
+    s->nseg = c->path == PIC_PATH_FUSED
+                  ? (c->segments > 0 ? (int)c->segments
+                                     : (int)std::max<uint32_t>(1u, std::min<uint32_t>(kDefaultSegments,
+                                                                                c->capacity / kSlotsPerDefaultSegment)))
+                  : 1;
     if (s->nseg > kMaxSegments) {
         delete s;
         set_error("pic_sim_create: at most 64 segments");

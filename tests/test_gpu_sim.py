@@ -51,6 +51,9 @@ def _push_ref(orc, po, efield, parts, species):
 # fused path; on the fused path 1 updates electrons in place (no ion step)
 # and 20 (= N_SUB) updates both species in place
 STARTS = [(0, 0), (1, 0), (1, 1), (1, 20)]
+# fused-path segment counts: the default for the test capacities (a few) and
+# the 32 of a multi-million-particle run
+SEGMENTS = [0, 32]
 
 
 def _pot0(t0):
@@ -74,12 +77,15 @@ def _check_slot_draws(orc, rng0, rng1, slots, cnt, draws):
 
 @pytest.mark.parametrize("path,t0", STARTS)
 @pytest.mark.parametrize("prec", [0, 2])
-def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec):
+@pytest.mark.parametrize("segments", SEGMENTS)
+def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec, segments):
+    if path == 0 and segments:
+        pytest.skip("the reference path keeps one segment")
     p = ca.params(weight=2.0e3, precision=prec)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 300_000, 320_000, 400_000
     E, I = synthetic_state(n_e, n_i, seed=31 + t0)
-    sim = make_sim(ca, p, cap, path)
+    sim = make_sim(ca, p, cap, path, segments=segments)
     sim.set_time(t0)
     cs, se0, si0 = zero_tables(orc, po)
     sim.set_tables(cs, se0, si0)
@@ -236,12 +242,15 @@ def _check_new_ions(gi, pushed_ion_x, born_x, ions, check_v):
 
 
 @pytest.mark.parametrize("path,t0", STARTS)
-def test_step_with_collisions_slot_replay(ca, orc, path, t0):
+@pytest.mark.parametrize("segments", SEGMENTS)
+def test_step_with_collisions_slot_replay(ca, orc, path, t0, segments):
+    if path == 0 and segments:
+        pytest.skip("the reference path keeps one segment")
     p = ca.params(weight=2.0e3, precision=2)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 200_000, 210_000, 300_000
     E, I = synthetic_state(n_e, n_i, seed=41 + t0, te_ev=6.0)
-    sim = make_sim(ca, p, cap, path)
+    sim = make_sim(ca, p, cap, path, segments=segments)
     sim.set_time(t0)
     cs, se, si = orc.cross_sections(po.gas_density)
     sim.upload(0, *E)
