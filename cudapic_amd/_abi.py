@@ -114,6 +114,8 @@ SIGNATURES = {
     "pic_ens_destroy": (I32, [P]),
     "pic_ens_upload": (I32, [P, I32, I32, P, P, P, P, U32]),
     "pic_ens_download": (I32, [P, I32, I32, P, P, P, P, U32, C.POINTER(U32)]),
+    "pic_ens_download_slots": (I32, [P, I32, I32, P, U32, C.POINTER(U32)]),
+    "pic_ens_set_time": (I32, [P, U32]),
     "pic_ens_counts": (I32, [P, P, P]),
     "pic_ens_rng_download": (I32, [P, I32, I32, P, P, P, P, U32]),
     "pic_ens_run": (I32, [P, U32, I32]),

@@ -1,7 +1,7 @@
 // pic_ensemble.hip -- ensembles of independent discharges on one GPU: every
 // stage of every instance in one launch (one Poisson workgroup per
 // instance, one persistent particle kernel per species over all instances),
-// the 2*N_SUB-step launch pattern captured into a hipGraph.
+// the 4*N_SUB-step launch pattern captured into a hipGraph.
 //
 // Instances share the grid (N_G, gap), the time step (frequency, N_T,
 // N_SUB) and the precision; each has its own driving amplitude, gas
@@ -27,11 +27,12 @@ struct pic_ensemble {
     std::vector<pic_params_t> params;
     std::vector<EnsInst> inst_h;
     EnsInst *inst_d = nullptr;
-    uint32_t e_total = 0, s_total = 0, i_total = 0;
-    float *e[2][4] = {}, *es[2][4] = {}, *ion[2][4] = {};
+    uint32_t e_total = 0, i_total = 0;
+    float *e[2][4] = {}, *ion[2][4] = {};
     uint32_t *e_rng[4] = {}, *i_rng[4] = {};
     uint32_t *cnt = nullptr;   // [E][EC_WORDS]
     uint32_t *clock = nullptr; // device step index within the cycle
+    uint32_t *live = nullptr;  // [2][E] live counts for the host
     void *e_dens[2] = {}, *i_dens[2] = {};
     int inst_stride = 0;       // density entries per instance
     float *pot = nullptr, *efield = nullptr;
@@ -42,13 +43,16 @@ struct pic_ensemble {
     bool f64 = true;
     int prec = PIC_PREC_FP64;
     hipStream_t stream = nullptr;
-    int ce = 0, ci = 0, cd = 0, hi = 0;
+    // roles: ce / ci the electron / ion array of the pair in use, cd / hi the
+    // density buffers, pe the electron watermark parity, ac set after a
+    // compaction step (the watermark is then the compaction's output count)
+    int ce = 0, ci = 0, cd = 0, hi = 0, pe = 0, ac = 0;
     uint32_t t = 0;
     unsigned long long steps = 0;
     bool need_prologue = true;
     bool use_graph = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_roles[2][4] = {};
+    int graph_roles[2][6] = {};
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -68,6 +72,10 @@ int dalloc(T **p, size_t n) {
 }
 
 uint64_t dens_words32(pic_ensemble *s) { return (uint64_t)s->E * s->inst_stride * (s->f64 ? 2 : 1); }
+
+// counter word holding the electron watermark the next step reads
+int e_word(const pic_ensemble *s) { return s->ac ? EC_C : EC_E0 + s->pe; }
+int i_word(const pic_ensemble *s) { return EC_I0 + s->ci; }
 
 EnsPoissonJob poisson_job(pic_ensemble *s) {
     const double E = 1.60217662e-19, EPS0 = 8.85418781e-12;
@@ -92,13 +100,14 @@ EnsPoissonJob poisson_job(pic_ensemble *s) {
     j.efield = s->efield;
     j.counters = s->counters;
     j.cnt = s->cnt;
-    j.zc0 = j.zc1 = j.zci = -1;
+    j.copy_src = j.copy_dst = j.zc0 = j.zc1 = -1;
     return j;
 }
 
 int prologue(pic_ensemble *s) {
     hipStream_t st = s->stream;
     TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
+    TRY(launch_fill(s->e_dens[1 - s->cd], 0u, dens_words32(s), st));
     TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
     TRY(launch_fill(s->i_dens[1], 0u, dens_words32(s), st));
     Shape sh = s->sh;
@@ -107,11 +116,10 @@ int prologue(pic_ensemble *s) {
         const EnsInst &in = s->inst_h[i];
         uint32_t *c = s->cnt + (size_t)i * EC_WORDS;
         char *ed = (char *)s->e_dens[s->cd] + (size_t)i * s->inst_stride * es;
-        TRY(launch_deposit(s->e[s->ce][0] + in.e_off, c + EC_E0 + s->ce, ed, s->f64, in.k, sh, st, in.e_cap));
-        TRY(launch_deposit(s->es[s->ce][0] + in.s_off, c + EC_S0 + s->ce, ed, s->f64, in.k, sh, st, in.s_cap));
+        TRY(launch_deposit(s->e[s->ce][0] + in.e_off, c + e_word(s), ed, s->f64, in.k, sh, st, in.e_cap));
         for (int b = 0; b < 2; ++b) {
             char *id = (char *)s->i_dens[b] + (size_t)i * s->inst_stride * es;
-            TRY(launch_deposit(s->ion[s->ci][0] + in.i_off, c + EC_I0 + s->ci, id, s->f64, in.k, sh, st, in.i_cap));
+            TRY(launch_deposit(s->ion[s->ci][0] + in.i_off, c + i_word(s), id, s->f64, in.k, sh, st, in.i_cap));
         }
     }
     PIC_HIP(hipMemcpyAsync(s->clock, &s->t, 4, hipMemcpyHostToDevice, st));
@@ -120,39 +128,56 @@ int prologue(pic_ensemble *s) {
     return PIC_OK;
 }
 
-void fill_species(EnsFusedJob &f, float *const *src, float *const *dst, pic_ensemble *s) {
-    f.sx = src[0];
-    f.svx = src[1];
-    f.svy = src[2];
-    f.svz = src[3];
-    f.dx = dst[0];
-    f.dvx = dst[1];
-    f.dvy = dst[2];
-    f.dvz = dst[3];
-    (void)s;
+EnsCompactJob compact_job(pic_ensemble *s, bool electrons, float *const *src, float *const *dst, int c_in,
+                          int c_out) {
+    EnsCompactJob j;
+    memset(&j, 0, sizeof(j));
+    j.electrons = electrons;
+    j.inst = s->inst_d;
+    j.n_inst = s->E;
+    for (int q = 0; q < 4; ++q) {
+        j.src[q] = src[q];
+        j.dst[q] = dst[q];
+    }
+    j.cnt = s->cnt;
+    j.c_in = c_in;
+    j.c_out = c_out;
+    return j;
 }
 
+// One ensemble step.  Particles are updated in place (lost ones leave
+// holes, newborn electrons and ions are appended after the watermarks);
+// every 2*N_SUB steps a compaction pass after the step packs each species
+// into the other array of its pair.  Electron watermarks alternate between
+// two counter words every step: the step reads one, the Poisson kernel
+// copies it into the other, newborns are appended there.
 int step(pic_ensemble *s, bool measure) {
     hipStream_t st = s->stream;
-    const bool ion = (s->t % (uint32_t)s->p0.n_sub) == 0;
+    const uint32_t t = s->t;
+    const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
+    const bool comp_e = (t % (2u * (uint32_t)s->p0.n_sub)) == 0;
+    const bool comp_i = ion && comp_e;
+    const int in_e = e_word(s), out_e = EC_E0 + (1 - s->pe);
     EnsPoissonJob pj = poisson_job(s);
     pj.e_dens = s->e_dens[s->cd];
     pj.i_dens = s->i_dens[ion ? 1 - s->hi : s->hi];
     pj.zero_a = s->e_dens[1 - s->cd];
     pj.zero_b = s->i_dens[s->hi];
-    pj.zc0 = EC_E0 + (1 - s->ce);
-    pj.zc1 = EC_S0 + (1 - s->ce);
-    pj.zci = EC_I0 + (1 - s->ci);
+    pj.copy_src = in_e;
+    pj.copy_dst = out_e;
+    if (comp_e) pj.zc0 = EC_C;
+    if (comp_i) pj.zc1 = EC_I0 + (1 - s->ci);
     if (measure) {
         pj.cumul_e = s->cumul_e;
         pj.cumul_i = s->cumul_i;
     }
     TRY(launch_poisson_ens(pj, st));
     const int i_acc = ion ? s->hi : 1 - s->hi;
-    EnsFusedJob base;
+    EnsStepJob base;
     memset(&base, 0, sizeof(base));
     base.inst = s->inst_d;
     base.n_inst = s->E;
+    base.k = s->inst_h[0].k;
     base.efield = s->efield;
     base.cnt = s->cnt;
     base.inst_stride = s->inst_stride;
@@ -161,50 +186,54 @@ int step(pic_ensemble *s, bool measure) {
     base.counters = s->counters;
     base.n_t = s->p0.n_t;
     if (ion) {
-        EnsFusedJob fi = base;
+        EnsStepJob fi = base;
         fi.electrons = false;
-        fill_species(fi, s->ion[s->ci], s->ion[1 - s->ci], s);
+        fi.x = s->ion[s->ci][0];
+        fi.vx = s->ion[s->ci][1];
+        fi.vy = s->ion[s->ci][2];
+        fi.vz = s->ion[s->ci][3];
         fi.rng0 = s->i_rng[0];
         fi.rng1 = s->i_rng[1];
         fi.rng2 = s->i_rng[2];
         fi.rng3 = s->i_rng[3];
-        fi.c_in = EC_I0 + s->ci;
-        fi.c_out = EC_I0 + (1 - s->ci);
+        fi.c_in = i_word(s);
         fi.dens = s->i_dens[i_acc];
         fi.stot = s->stot_i;
-        TRY(launch_fused_ens(fi, s->sh, s->p0.n_g, st));
-        s->ci ^= 1;
+        TRY(launch_inplace_ens(fi, s->sh, st));
+        if (comp_i) {
+            TRY(launch_compact_ens(
+                compact_job(s, false, s->ion[s->ci], s->ion[1 - s->ci], EC_I0 + s->ci, EC_I0 + (1 - s->ci)), st));
+            s->ci ^= 1;
+        }
     }
-    EnsFusedJob fe = base;
+    EnsStepJob fe = base;
     fe.electrons = true;
-    fill_species(fe, s->e[s->ce], s->e[1 - s->ce], s);
-    fe.px = s->es[s->ce][0];
-    fe.pvx = s->es[s->ce][1];
-    fe.pvy = s->es[s->ce][2];
-    fe.pvz = s->es[s->ce][3];
-    fe.ox = s->es[1 - s->ce][0];
-    fe.ovx = s->es[1 - s->ce][1];
-    fe.ovy = s->es[1 - s->ce][2];
-    fe.ovz = s->es[1 - s->ce][3];
-    fe.ix = s->ion[s->ci][0];
-    fe.ivx = s->ion[s->ci][1];
-    fe.ivy = s->ion[s->ci][2];
-    fe.ivz = s->ion[s->ci][3];
+    fe.x = s->e[s->ce][0];
+    fe.vx = s->e[s->ce][1];
+    fe.vy = s->e[s->ce][2];
+    fe.vz = s->e[s->ce][3];
     fe.rng0 = s->e_rng[0];
     fe.rng1 = s->e_rng[1];
     fe.rng2 = s->e_rng[2];
     fe.rng3 = s->e_rng[3];
-    fe.c_in = EC_E0 + s->ce;
-    fe.c_out = EC_E0 + (1 - s->ce);
-    fe.s_in = EC_S0 + s->ce;
-    fe.s_out = EC_S0 + (1 - s->ce);
-    fe.c_ion = EC_I0 + s->ci;
+    fe.ix = s->ion[s->ci][0];
+    fe.ivx = s->ion[s->ci][1];
+    fe.ivy = s->ion[s->ci][2];
+    fe.ivz = s->ion[s->ci][3];
+    fe.c_in = in_e;
+    fe.c_out = out_e;
+    fe.c_ion = i_word(s);
     fe.dens = s->e_dens[1 - s->cd];
     fe.ion_dens = s->i_dens[i_acc];
     fe.stot = s->stot_e;
     fe.time = s->clock;
-    TRY(launch_fused_ens(fe, s->sh, s->p0.n_g, st));
-    s->ce ^= 1;
+    TRY(launch_inplace_ens(fe, s->sh, st));
+    if (comp_e) {
+        TRY(launch_compact_ens(compact_job(s, true, s->e[s->ce], s->e[1 - s->ce], out_e, EC_C), st));
+        s->ce ^= 1;
+    }
+    s->ac = comp_e ? 1 : 0;
+    s->pe ^= 1;
     s->cd ^= 1;
     if (ion) s->hi ^= 1;
     s->t = (s->t + 1) % (uint32_t)s->p0.n_t;
@@ -220,10 +249,31 @@ void invalidate_graphs(pic_ensemble *s) {
         }
 }
 
+void get_roles(const pic_ensemble *s, int *r) {
+    r[0] = s->ce;
+    r[1] = s->ci;
+    r[2] = s->cd;
+    r[3] = s->hi;
+    r[4] = s->pe;
+    r[5] = s->ac;
+}
+
+void set_roles(pic_ensemble *s, const int *r) {
+    s->ce = r[0];
+    s->ci = r[1];
+    s->cd = r[2];
+    s->hi = r[3];
+    s->pe = r[4];
+    s->ac = r[5];
+}
+
+// Capture 4*N_SUB steps (every role returns to its start value after that
+// many) into a graph for this measure flag.
 int capture(pic_ensemble *s, bool measure) {
     const int m = measure ? 1 : 0;
-    const int period = 2 * s->p0.n_sub;
-    int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+    const int period = 4 * s->p0.n_sub;
+    int roles[6];
+    get_roles(s, roles);
     const uint32_t t0 = s->t;
     const unsigned long long st0 = s->steps;
     hipGraph_t g = nullptr;
@@ -231,12 +281,10 @@ int capture(pic_ensemble *s, bool measure) {
     int rc = PIC_OK;
     for (int i = 0; i < period && rc == PIC_OK; ++i) rc = step(s, measure);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
+    // capture does not execute: restore the host mirror
     s->t = t0;
     s->steps = st0;
-    s->ce = roles[0];
-    s->ci = roles[1];
-    s->cd = roles[2];
-    s->hi = roles[3];
+    set_roles(s, roles);
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -253,12 +301,13 @@ int capture(pic_ensemble *s, bool measure) {
 
 int run(pic_ensemble *s, uint32_t n, bool measure) {
     if (s->need_prologue) TRY(prologue(s));
-    const uint32_t period = 2u * (uint32_t)s->p0.n_sub;
+    const uint32_t period = 4u * (uint32_t)s->p0.n_sub;
     const bool graphs = s->use_graph && (uint32_t)s->p0.n_t % period == 0;
     while (n > 0) {
         if (graphs && s->t % period == 0 && n >= period) {
             const int m = measure ? 1 : 0;
-            int roles[4] = {s->ce, s->ci, s->cd, s->hi};
+            int roles[6];
+            get_roles(s, roles);
             if (s->graph[m] && memcmp(roles, s->graph_roles[m], sizeof(roles)) != 0) {
                 (void)hipGraphExecDestroy(s->graph[m]);
                 s->graph[m] = nullptr;
@@ -278,19 +327,51 @@ int run(pic_ensemble *s, uint32_t n, bool measure) {
     return PIC_OK;
 }
 
+// Live particles of one instance in slot order (and their slots relative
+// to the instance's region); *n = the live count.
+int gather_live(pic_ensemble *s, int inst, int species, float *x, float *vx, float *vy, float *vz,
+                uint32_t *slots, uint32_t cap, uint32_t *n) {
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    const bool el = species == PIC_ELECTRONS;
+    uint32_t c = 0;
+    PIC_HIP(hipMemcpy(&c, s->cnt + (size_t)inst * EC_WORDS + (el ? e_word(s) : i_word(s)), 4,
+                      hipMemcpyDeviceToHost));
+    const EnsInst &in = s->inst_h[inst];
+    const uint32_t m = std::min(c, el ? in.e_cap : in.i_cap), off = el ? in.e_off : in.i_off;
+    float **a = el ? s->e[s->ce] : s->ion[s->ci];
+    float *dst[4] = {x, vx, vy, vz};
+    std::vector<float> buf[4];
+    for (int w = 0; w < 4; ++w)
+        if (m && (w == 0 || dst[w])) {
+            buf[w].resize(m);
+            PIC_HIP(hipMemcpy(buf[w].data(), a[w] + off, 4ull * m, hipMemcpyDeviceToHost));
+        }
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < m; ++i) {
+        if (!(buf[0][i] >= 0.0f)) continue;  // a hole
+        if (total < cap) {
+            for (int w = 0; w < 4; ++w)
+                if (dst[w]) dst[w][total] = buf[w][i];
+            if (slots) slots[total] = i;
+        }
+        ++total;
+    }
+    if (n) *n = total;
+    return PIC_OK;
+}
+
 void free_all(pic_ensemble *s) {
     invalidate_graphs(s);
     for (int q = 0; q < 2; ++q)
         for (int c = 0; c < 4; ++c) {
             (void)hipFree(s->e[q][c]);
-            (void)hipFree(s->es[q][c]);
             (void)hipFree(s->ion[q][c]);
         }
     for (int c = 0; c < 4; ++c) {
         (void)hipFree(s->e_rng[c]);
         (void)hipFree(s->i_rng[c]);
     }
-    void *bufs[] = {s->inst_d, s->cnt, s->clock, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
+    void *bufs[] = {s->inst_d, s->cnt, s->clock, s->live, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->pot, s->efield, s->cumul_e, s->cumul_i, s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
     for (int i = 0; i < 2; ++i)
@@ -328,7 +409,7 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     Shape sh;
     sh.block = 256;
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
-    sh.ppt = c->ppt > 0 ? c->ppt : 4;
+    sh.ppt_inplace = c->ppt > 0 ? c->ppt : 2;
     // global density replicas: 1 or a multiple of 8 (read in groups of eight)
     sh.replicas = c->replicas > 0 ? c->replicas : 8;
     if (sh.replicas > 1) sh.replicas = std::min(64, (sh.replicas + 7) / 8 * 8);
@@ -336,30 +417,35 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     s->sh = resolve(sh);
     s->inst_stride = s->sh.replicas * s->sh.rep_stride;
     s->inst_h.resize(s->E);
-    uint64_t e_tot = 0, s_tot = 0;
+    uint64_t tot = 0;
+    const KPhys k0 = make_kphys(s->params[0]);
     for (int i = 0; i < s->E; ++i) {
         EnsInst &in = s->inst_h[i];
         in.k = make_kphys(s->params[i]);
         in.k.stot_scale = (float)s->params[i].gas_density;
         in.v0 = s->params[i].v0;
+        // the particle kernels take everything else from instance 0
+        KPhys kc = in.k;
+        kc.wf = k0.wf;
+        kc.normal_range = k0.normal_range;
+        kc.stot_scale = k0.stot_scale;
+        if (memcmp(&kc, &k0, sizeof(KPhys)) != 0) {
+            delete s;
+            set_error("pic_ens_create: instances may differ in v0, pressure, gas temperature and weight only");
+            return PIC_ERR_ARG;
+        }
+        // regions start on 256-slot boundaries (whole cache lines per tile)
         const uint32_t cap = c->capacity[i];
-        const uint32_t scap = std::max<uint32_t>(1024u, cap / 8u);
-        in.e_off = (uint32_t)e_tot;
-        in.e_cap = cap;
-        in.i_off = (uint32_t)e_tot;
-        in.i_cap = cap;
-        in.s_off = (uint32_t)s_tot;
-        in.s_cap = scap;
-        e_tot += cap;
-        s_tot += scap;
+        in.e_off = in.i_off = (uint32_t)tot;
+        in.e_cap = in.i_cap = cap;
+        tot += ((uint64_t)cap + 255u) / 256u * 256u;
     }
-    if (e_tot > 0xffffffffull || s_tot > 0xffffffffull) {
+    if (tot > 0xffffffffull) {
         delete s;
         set_error("pic_ens_create: total capacity above 2^32");
         return PIC_ERR_ARG;
     }
-    s->e_total = s->i_total = (uint32_t)e_tot;
-    s->s_total = (uint32_t)s_tot;
+    s->e_total = s->i_total = (uint32_t)tot;
     auto fail = [&](int r) {
         free_all(s);
         delete s;
@@ -372,7 +458,6 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
         for (int k = 0; k < 4; ++k) {
             if ((rc = dalloc(&s->e[q][k], s->e_total))) return fail(rc);
             if ((rc = dalloc(&s->ion[q][k], s->i_total))) return fail(rc);
-            if ((rc = dalloc(&s->es[q][k], s->s_total))) return fail(rc);
         }
     for (int k = 0; k < 4; ++k) {
         if ((rc = dalloc(&s->e_rng[k], s->e_total))) return fail(rc);
@@ -381,6 +466,7 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     if ((rc = dalloc(&s->inst_d, s->E))) return fail(rc);
     if ((rc = dalloc(&s->cnt, (size_t)s->E * EC_WORDS))) return fail(rc);
     if ((rc = dalloc(&s->clock, 1))) return fail(rc);
+    if ((rc = dalloc(&s->live, 2 * (size_t)s->E))) return fail(rc);
     for (int q = 0; q < 2; ++q) {
         if ((rc = dalloc((double **)&s->e_dens[q], (size_t)s->E * s->inst_stride))) return fail(rc);
         if ((rc = dalloc((double **)&s->i_dens[q], (size_t)s->E * s->inst_stride))) return fail(rc);
@@ -446,13 +532,7 @@ int pic_ens_upload(pic_ensemble_t *s, int inst, int species, const float *x, con
     for (int q = 0; q < 4; ++q)
         if (n) PIC_HIP(hipMemcpy(a[q] + off, src[q], 4ull * n, hipMemcpyHostToDevice));
     uint32_t *c = s->cnt + (size_t)inst * EC_WORDS;
-    const uint32_t zero = 0;
-    if (species == PIC_ELECTRONS) {
-        PIC_HIP(hipMemcpy(c + EC_E0 + s->ce, &n, 4, hipMemcpyHostToDevice));
-        PIC_HIP(hipMemcpy(c + EC_S0 + s->ce, &zero, 4, hipMemcpyHostToDevice));
-    } else {
-        PIC_HIP(hipMemcpy(c + EC_I0 + s->ci, &n, 4, hipMemcpyHostToDevice));
-    }
+    PIC_HIP(hipMemcpy(c + (species == PIC_ELECTRONS ? e_word(s) : i_word(s)), &n, 4, hipMemcpyHostToDevice));
     s->need_prologue = true;
     invalidate_graphs(s);
     return PIC_OK;
@@ -461,27 +541,20 @@ int pic_ens_upload(pic_ensemble_t *s, int inst, int species, const float *x, con
 int pic_ens_download(pic_ensemble_t *s, int inst, int species, float *x, float *vx, float *vy, float *vz,
                      uint32_t cap, uint32_t *n) {
     if (!s || inst < 0 || inst >= s->E || (species != PIC_ELECTRONS && species != PIC_IONS)) return PIC_ERR_ARG;
+    return gather_live(s, inst, species, x, vx, vy, vz, nullptr, cap, n);
+}
+
+int pic_ens_download_slots(pic_ensemble_t *s, int inst, int species, uint32_t *slots, uint32_t cap, uint32_t *n) {
+    if (!s || inst < 0 || inst >= s->E || (species != PIC_ELECTRONS && species != PIC_IONS)) return PIC_ERR_ARG;
+    return gather_live(s, inst, species, nullptr, nullptr, nullptr, nullptr, slots, cap, n);
+}
+
+int pic_ens_set_time(pic_ensemble_t *s, uint32_t t) {
+    if (!s || t >= (uint32_t)s->p0.n_t) return PIC_ERR_ARG;
     PIC_HIP(hipStreamSynchronize(s->stream));
-    uint32_t c[EC_WORDS];
-    PIC_HIP(hipMemcpy(c, s->cnt + (size_t)inst * EC_WORDS, sizeof(c), hipMemcpyDeviceToHost));
-    const EnsInst &in = s->inst_h[inst];
-    float *dst[4] = {x, vx, vy, vz};
-    if (species == PIC_ELECTRONS) {
-        const uint32_t nm = std::min(c[EC_E0 + s->ce], in.e_cap), ns = std::min(c[EC_S0 + s->ce], in.s_cap);
-        if (n) *n = nm + ns;
-        const uint32_t cm = std::min(nm, cap), cs2 = std::min(ns, cap - cm);
-        for (int q = 0; q < 4; ++q) {
-            if (!dst[q]) continue;
-            if (cm) PIC_HIP(hipMemcpy(dst[q], s->e[s->ce][q] + in.e_off, 4ull * cm, hipMemcpyDeviceToHost));
-            if (cs2) PIC_HIP(hipMemcpy(dst[q] + cm, s->es[s->ce][q] + in.s_off, 4ull * cs2, hipMemcpyDeviceToHost));
-        }
-    } else {
-        const uint32_t ni = std::min(c[EC_I0 + s->ci], in.i_cap);
-        if (n) *n = ni;
-        const uint32_t cm = std::min(ni, cap);
-        for (int q = 0; q < 4; ++q)
-            if (dst[q] && cm) PIC_HIP(hipMemcpy(dst[q], s->ion[s->ci][q] + in.i_off, 4ull * cm, hipMemcpyDeviceToHost));
-    }
+    s->t = t;
+    s->need_prologue = true;
+    invalidate_graphs(s);
     return PIC_OK;
 }
 
@@ -502,13 +575,17 @@ int pic_ens_rng_download(pic_ensemble_t *s, int inst, int species, uint32_t *w0,
 
 int pic_ens_counts(pic_ensemble_t *s, uint32_t *n_e, uint32_t *n_i) {
     if (!s) return PIC_ERR_ARG;
+    // in-place steps leave holes: count the live slots on the device
+    TRY(launch_fill(s->live, 0u, 2ull * s->E, s->stream));
+    TRY(launch_count_live_ens(s->e[s->ce][0], s->inst_d, s->E, s->cnt, e_word(s), true, s->live, s->stream));
+    TRY(launch_count_live_ens(s->ion[s->ci][0], s->inst_d, s->E, s->cnt, i_word(s), false, s->live + s->E,
+                              s->stream));
+    std::vector<uint32_t> c(2 * (size_t)s->E);
+    PIC_HIP(hipMemcpyAsync(c.data(), s->live, 4 * c.size(), hipMemcpyDeviceToHost, s->stream));
     PIC_HIP(hipStreamSynchronize(s->stream));
-    std::vector<uint32_t> c((size_t)s->E * EC_WORDS);
-    PIC_HIP(hipMemcpy(c.data(), s->cnt, 4 * c.size(), hipMemcpyDeviceToHost));
     for (int i = 0; i < s->E; ++i) {
-        const uint32_t *ci = &c[(size_t)i * EC_WORDS];
-        if (n_e) n_e[i] = ci[EC_E0 + s->ce] + ci[EC_S0 + s->ce];
-        if (n_i) n_i[i] = ci[EC_I0 + s->ci];
+        if (n_e) n_e[i] = c[i];
+        if (n_i) n_i[i] = c[s->E + i];
     }
     return PIC_OK;
 }

@@ -88,11 +88,14 @@ __device__ __forceinline__ void clear_slice(uint32_t *buf, uint32_t words) {
 }
 
 // Wall losses and collisions of the workgroup: summed in LDS, then one
-// atomic per counter into the workgroup's copy of the counters.  Contains a
-// barrier: every thread of the workgroup calls it.
+// atomic per counter into the workgroup's copy of the counters.  Contains
+// barriers: every thread of the workgroup calls it, and a second call needs
+// a barrier between (thread 0 reads the LDS sums after the last one).
+// `c`: the counter words to add into (null: none), `overflow_flag` set when
+// any thread saw an overflow.
 template <bool ELECTRON>
-__device__ __forceinline__ void add_diagnostics(const FusedJob &j, uint32_t lost_pow, uint32_t lost_gnd,
-                                                uint32_t ncoll, bool overflow) {
+__device__ __forceinline__ void add_diagnostics(unsigned long long *c, uint32_t *overflow_flag, uint32_t lost_pow,
+                                                uint32_t lost_gnd, uint32_t ncoll, bool overflow) {
     __shared__ uint32_t s_diag[3];
     if (threadIdx.x < 3) s_diag[threadIdx.x] = 0u;
     __syncthreads();
@@ -102,10 +105,9 @@ __device__ __forceinline__ void add_diagnostics(const FusedJob &j, uint32_t lost
         if (wg) atomicAdd(&s_diag[1], wg);
         if (wc) atomicAdd(&s_diag[2], wc);
     }
-    if (overflow) *j.overflow = 1u;
+    if (overflow) *overflow_flag = 1u;
     __syncthreads();
-    if (threadIdx.x == 0 && j.counters) {
-        unsigned long long *c = j.counters + (size_t)(blockIdx.x % kCounterLanes) * kCounterWords;
+    if (threadIdx.x == 0 && c) {
         const int b = ELECTRON ? 0 : 2;
         if (s_diag[0]) atomicAdd(&c[b], (unsigned long long)s_diag[0]);
         if (s_diag[1]) atomicAdd(&c[b + 1], (unsigned long long)s_diag[1]);
@@ -163,6 +165,64 @@ __device__ uint32_t tile_reserve_f(uint32_t wave_count, uint32_t *s_wave, uint32
     return s_base[buf] + s_wave[buf * 16 + wave];
 }
 
+// Where an electron step puts its ionisation products: the new electron
+// after the watermark *e_cnt of the electron region that starts at slot
+// e_base (e_cap slots), the new ion after *i_cnt of the ion region at
+// i_base (i_cap slots).
+struct Appends {
+    float *ex, *evx, *evy, *evz;
+    uint32_t *e_cnt;
+    uint32_t e_base, e_cap;
+    float *ix, *ivx, *ivy, *ivz;
+    uint32_t *i_cnt;
+    uint32_t i_base, i_cap;
+};
+
+// Apply an electron collision's products: the new electron (into the LDS
+// density and EEPF) and the new ion (into the global ion density).
+// Wave-aggregated slot reservation; `kind` per lane.
+template <bool MEASURE, typename Acc>
+__device__ __forceinline__ void ionisation_products(const Appends &a, const KPhys &k, int kind, float px,
+                                                    const float *se, const float *si, uint32_t *h, float scale,
+                                                    Acc *i_dens_g, uint32_t *eepf_h, const float *ef,
+                                                    bool &overflow) {
+    const uint64_t sp = ballot64(kind == kIonisation);
+    if (!sp) return;
+    const int leader = __builtin_ctzll(sp);
+    uint32_t be = 0, bi = 0;
+    if ((int)lane_id() == leader) {
+        const uint32_t cnt = (uint32_t)__popcll(sp);
+        be = atomicAdd(a.e_cnt, cnt);
+        bi = atomicAdd(a.i_cnt, cnt);
+    }
+    be = __shfl(be, leader, kWave);
+    bi = __shfl(bi, leader, kWave);
+    if (kind != kIonisation) return;
+    const uint32_t o = mbcnt64(sp);
+    const uint32_t el = be + o, il = bi + o;
+    if (el < a.e_cap) {
+        const uint32_t es = a.e_base + el;
+        a.ex[es] = px;
+        a.evx[es] = se[0];
+        a.evy[es] = se[1];
+        a.evz[es] = se[2];
+        deposit_fix(h, px, k, scale);
+        if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
+    } else {
+        overflow = true;
+    }
+    if (il < a.i_cap) {
+        const uint32_t is = a.i_base + il;
+        a.ix[is] = px;
+        a.ivx[is] = si[0];
+        a.ivy[is] = si[1];
+        a.ivz[is] = si[2];
+        deposit_global<Acc>(i_dens_g, px, k);
+    } else {
+        overflow = true;
+    }
+}
+
 // Segmented particle arrays: a species lives in K segments of seg_size
 // slots, each dense from its start with its own live counter.  The kernel
 // cuts every input segment into tiles (an LDS prefix of per-segment tile
@@ -215,6 +275,9 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
     const uint32_t out_base = ko * j.seg_size;
     const float factor = ELECTRON ? k.factor_e : k.factor_i;
     const float dt = ELECTRON ? k.dt_e : k.dt_i;
+    Acc *const i_dens_g = (Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride;
+    const Appends app{j.dx,  j.dvx, j.dvy, j.dvz, cnt_out,        out_base,            j.seg_size,
+                      j.ix,  j.ivx, j.ivy, j.ivz, j.ion_cnt + ko, ko * j.ion_seg_size, j.ion_seg_size};
     uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
     bool overflow = false;
     int buf = 0;
@@ -291,52 +354,17 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
                     overflow = true;
                 }
             }
-            if (ELECTRON) {
-                // ionisation products: the new electron joins this output
-                // segment (it is not collision-tested until the next step,
-                // main.rs:992), the new ion the same-numbered ion segment
-                const uint64_t sp = ballot64(kind == kIonisation);
-                if (sp) {
-                    const int leader = __builtin_ctzll(sp);
-                    uint32_t be = 0, bi = 0;
-                    if ((int)lane_id() == leader) {
-                        const uint32_t cnt = (uint32_t)__popcll(sp);
-                        be = atomicAdd(cnt_out, cnt);
-                        bi = atomicAdd(j.ion_cnt + ko, cnt);
-                    }
-                    be = __shfl(be, leader, kWave);
-                    bi = __shfl(bi, leader, kWave);
-                    if (kind == kIonisation) {
-                        const uint32_t o = mbcnt64(sp);
-                        const uint32_t el = be + o, il = bi + o;
-                        const float x = nx[u];
-                        if (el < j.seg_size) {
-                            const uint32_t es = out_base + el;
-                            j.dx[es] = x;
-                            j.dvx[es] = se[0];
-                            j.dvy[es] = se[1];
-                            j.dvz[es] = se[2];
-                            deposit_fix(h, x, k, scale);
-                            if (MEASURE) eepf_lds(aux, ef, x, se[0], se[1], se[2], k);
-                        } else {
-                            overflow = true;
-                        }
-                        if (il < j.ion_seg_size) {
-                            const uint32_t is = ko * j.ion_seg_size + il;
-                            j.ix[is] = x;
-                            j.ivx[is] = si[0];
-                            j.ivy[is] = si[1];
-                            j.ivz[is] = si[2];
-                            deposit_global<Acc>((Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride, x, k);
-                        } else {
-                            overflow = true;
-                        }
-                    }
-                }
-            }
+            // ionisation products: the new electron joins this output
+            // segment (it is not collision-tested until the next step,
+            // main.rs:992), the new ion the same-numbered ion segment
+            if (ELECTRON)
+                ionisation_products<MEASURE, Acc>(app, k, kind, nx[u], se, si, h, scale, i_dens_g, aux, ef,
+                                                  overflow);
         }
     }
-    add_diagnostics<ELECTRON>(j, lost_pow, lost_gnd, ncoll, overflow);
+    add_diagnostics<ELECTRON>(
+        j.counters ? j.counters + (size_t)(blockIdx.x % kCounterLanes) * kCounterWords : nullptr, j.overflow,
+        lost_pow, lost_gnd, ncoll, overflow);
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
     // one flush of the workgroup histogram into the next step's density
     flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
@@ -375,54 +403,6 @@ enum CollField {
 };
 constexpr int kCollFieldsE = CF_BIN + 1;
 constexpr int kCollFieldsI = CF_AZ + 1;
-
-// Apply an electron collision's products: the new electron goes to output
-// segment `ko` (into the LDS density and EEPF), the new ion to the same ion
-// segment and the global ion density.  Wave-aggregated slot reservation;
-// `kind` per lane.
-template <bool MEASURE, typename Acc>
-__device__ __forceinline__ void ionisation_products(const FusedJob &j, const KPhys &k, int kind, uint32_t ko,
-                                                    float px, const float *se, const float *si, uint32_t *h,
-                                                    float scale, Acc *i_dens_g, uint32_t *eepf_h,
-                                                    const float *ef, bool &overflow) {
-    const uint64_t sp = ballot64(kind == kIonisation);
-    if (!sp) return;
-    const int leader = __builtin_ctzll(sp);
-    uint32_t be = 0, bi = 0;
-    if ((int)lane_id() == leader) {
-        const uint32_t cnt = (uint32_t)__popcll(sp);
-        be = atomicAdd(j.cnt_out + ko, cnt);
-        bi = atomicAdd(j.ion_cnt + ko, cnt);
-    }
-    be = __shfl(be, leader, kWave);
-    bi = __shfl(bi, leader, kWave);
-    if (kind != kIonisation) return;
-    const uint32_t o = mbcnt64(sp);
-    const uint32_t el = be + o, il = bi + o;
-    float *const X = (float *)j.sx, *const VX = (float *)j.svx, *const VY = (float *)j.svy,
-                 *const VZ = (float *)j.svz;
-    if (el < j.seg_size) {
-        const uint32_t es = ko * j.seg_size + el;
-        X[es] = px;
-        VX[es] = se[0];
-        VY[es] = se[1];
-        VZ[es] = se[2];
-        deposit_fix(h, px, k, scale);
-        if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
-    } else {
-        overflow = true;
-    }
-    if (il < j.ion_seg_size) {
-        const uint32_t is = ko * j.ion_seg_size + il;
-        j.ix[is] = px;
-        j.ivx[is] = si[0];
-        j.ivy[is] = si[1];
-        j.ivz[is] = si[2];
-        deposit_global<Acc>(i_dens_g, px, k);
-    } else {
-        overflow = true;
-    }
-}
 
 template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
 __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
@@ -476,6 +456,8 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     float *const X = (float *)j.sx, *const VX = (float *)j.svx, *const VY = (float *)j.svy,
                  *const VZ = (float *)j.svz;
     Acc *const i_dens_g = (Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride;
+    const Appends app{X,        VX,       VY,       VZ,       j.cnt_out + ko, ko * j.seg_size,     j.seg_size,
+                      j.ix,     j.ivx,    j.ivy,    j.ivz,    j.ion_cnt + ko, ko * j.ion_seg_size, j.ion_seg_size};
     uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
     bool overflow = false;
 
@@ -640,7 +622,7 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
                         if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], w, y, z, k);
                     }
                     if (ELECTRON)
-                        ionisation_products<MEASURE, Acc>(j, k, kind, ko, nx[u], se, si, h, scale, i_dens_g, aux,
+                        ionisation_products<MEASURE, Acc>(app, k, kind, nx[u], se, si, h, scale, i_dens_g, aux,
                                                           ef, overflow);
                 }
             }
@@ -695,9 +677,11 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
             ++ncoll;
         }
         if (ELECTRON)
-            ionisation_products<MEASURE, Acc>(j, k, kind, ko, px, se, si, h, scale, i_dens_g, aux, ef, overflow);
+            ionisation_products<MEASURE, Acc>(app, k, kind, px, se, si, h, scale, i_dens_g, aux, ef, overflow);
     }
-    add_diagnostics<ELECTRON>(j, lost_pow, lost_gnd, ncoll, overflow);
+    add_diagnostics<ELECTRON>(
+        j.counters ? j.counters + (size_t)(blockIdx.x % kCounterLanes) * kCounterWords : nullptr, j.overflow,
+        lost_pow, lost_gnd, ncoll, overflow);
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
     flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
     uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;
@@ -787,47 +771,47 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
 }  // namespace pic
 
 // ============================================================ ensembles
-// The fused particle step for an ensemble of independent discharges in one
-// persistent launch.  Every instance's live particles are cut into tiles; a
-// per-instance tile prefix (in LDS) maps the global tile index to
-// (instance, local tile).  Each workgroup takes a contiguous range of global
-// tiles, so it meets few instance boundaries: at each boundary it flushes
-// its LDS histogram into the finished instance's density, stages the next
-// instance's field and loads its physics (wall position, weights, gas
-// density as the cross-section scale).  Compaction, RNG slots, spawns and
-// densities are per instance, exactly as in the single-discharge kernel.
+// The in-place particle step for an ensemble of independent discharges in
+// one persistent launch, and the compaction pass that squeezes out holes
+// every 2*N_SUB steps.  Every instance's slots are cut into tiles; a
+// per-instance tile prefix (in LDS) maps a global tile to its instance.
+// Each workgroup takes a contiguous range of global tiles, so it meets few
+// instance boundaries.  At a boundary it finishes the instance it leaves --
+// scatters its listed collisions, adds its diagnostics, flushes its density
+// histogram -- and stages the next instance's field and physics.  The tile
+// body is that of the single-discharge in-place kernel: prefetch of the
+// next tile, collision test, full-line write-back, colliding particles
+// listed in LDS and scattered when the workgroup leaves the instance.
 namespace pic {
 
 constexpr int kMaxEnsemble = 256;
 
-struct EnsFusedArgs {
-    EnsFusedJob j;
-    int n_g, lds_reps, reps, rep_stride;
+struct EnsArgs {
+    EnsStepJob j;
+    int lds_reps, reps, rep_stride;
 };
 
 template <bool ELECTRON, int PPT, typename Acc>
-__global__ __launch_bounds__(256) void k_fused_ens(EnsFusedArgs A) {
-    const EnsFusedJob &j = A.j;
+__global__ __launch_bounds__(256) void k_inplace_ens(EnsArgs A) {
+    const EnsStepJob &j = A.j;
     extern __shared__ __align__(16) float sh[];
-    __shared__ uint32_t s_tiles[kMaxEnsemble + 1], s_nmain[kMaxEnsemble], s_n[kMaxEnsemble];
-    __shared__ uint32_t s_wave[32], s_base[2];
-    const int n_g = A.n_g;
+    __shared__ uint32_t s_tiles[kMaxEnsemble + 1], s_cnt[kMaxEnsemble], s_off[kMaxEnsemble];
+    __shared__ uint32_t s_ncoll;
+    const int n_g = j.k.n_g;
     const int E = j.n_inst;
     float *ef = sh;
-    float *hist = sh + n_g;
-    const int wave = threadIdx.x / kWave;
-    float *h = hist + (wave % A.lds_reps) * n_g;
+    uint32_t *hist = (uint32_t *)(sh + n_g);
+    uint32_t *list = hist + A.lds_reps * n_g;  // [NF][kCollPerWg]
     const uint32_t tile_sz = blockDim.x * PPT;
     for (int i = threadIdx.x; i < E; i += blockDim.x) {
         const EnsInst &in = j.inst[i];
-        const uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
-        const uint32_t nm = min(c[j.c_in], ELECTRON ? in.e_cap : in.i_cap);
-        const uint32_t ns = ELECTRON ? min(c[j.s_in], in.s_cap) : 0u;
-        s_nmain[i] = nm;
-        s_n[i] = nm + ns;
-        s_tiles[i + 1] = (nm + ns + tile_sz - 1) / tile_sz;
+        const uint32_t c = min(j.cnt[(size_t)i * EC_WORDS + j.c_in], ELECTRON ? in.e_cap : in.i_cap);
+        s_cnt[i] = c;
+        s_off[i] = ELECTRON ? in.e_off : in.i_off;
+        s_tiles[i + 1] = (c + tile_sz - 1) / tile_sz;
     }
-    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
+    if (threadIdx.x == 0) s_ncoll = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
@@ -842,205 +826,407 @@ __global__ __launch_bounds__(256) void k_fused_ens(EnsFusedArgs A) {
     const uint32_t T = s_tiles[E];
     const uint32_t t_begin = (uint32_t)((uint64_t)T * blockIdx.x / gridDim.x);
     const uint32_t t_end = (uint32_t)((uint64_t)T * (blockIdx.x + 1) / gridDim.x);
+    if (t_begin >= t_end) return;  // uniform over the workgroup
 
-    int cur = -1;
-    KPhys k = j.inst[0].k;
-    uint32_t p_off = 0, p_cap = 0, s_off = 0, s_cap = 0, i_off = 0, i_cap = 0, n_main = 0, n = 0;
-    uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
-    bool overflow = false;
-    int buf = 0;
-    // flush the workgroup's histogram and counters into instance `who`
-    auto flush = [&](int who) {
-        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd), wc = wave_sum(ncoll);
-        if (lane_id() == 0) {
-            unsigned long long *cn = j.counters + (size_t)who * 8;
-            const int b = ELECTRON ? 0 : 2;
-            if (wp) atomicAdd(&cn[b], (unsigned long long)wp);
-            if (wg) atomicAdd(&cn[b + 1], (unsigned long long)wg);
-            if (wc) atomicAdd(&cn[ELECTRON ? 4 : 5], (unsigned long long)wc);
-        }
-        if (overflow) j.cnt[(size_t)who * EC_WORDS + EC_OVERFLOW] = 1u;
-        lost_pow = lost_gnd = ncoll = 0;
-        overflow = false;
-        __syncthreads();
-        Acc *out = (Acc *)j.dens + (size_t)who * j.inst_stride + (blockIdx.x % A.reps) * A.rep_stride;
-        for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
-            float s = 0.0f;
-            for (int r = 0; r < A.lds_reps; ++r) s += hist[r * n_g + q];
-            if (s != 0.0f) {
-                Acc v = (Acc)s;
-                if (q == 0 || q == n_g - 1) v *= (Acc)2;
-                unsafeAtomicAdd(&out[q], v);
-            }
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0.0f;
+    const int wave = threadIdx.x / kWave;
+    uint32_t *h = hist + (wave % A.lds_reps) * n_g;
+    const float scale = 1048576.0f;  // fixed point 2^20, flushed every cap tiles
+    const uint32_t waves_per_rep = (blockDim.x / kWave + A.lds_reps - 1) / A.lds_reps;
+    const uint32_t cap = (4096u - kCollPerWg - 1u) / (waves_per_rep * 2u * PPT * kWave);
+    const float factor = ELECTRON ? j.k.factor_e : j.k.factor_i;
+    const float dt = ELECTRON ? j.k.dt_e : j.k.dt_i;
+    float *const X = j.x, *const VX = j.vx, *const VY = j.vy, *const VZ = j.vz;
+    const int rep = blockIdx.x % A.reps;
+
+    // instance of a tile: the largest i with s_tiles[i] <= tile (instances
+    // without tiles are skipped); tiles only grow, so the search walks on
+    auto inst_of = [&](uint32_t tile, int from) {
+        int i = from;
+        while (i + 1 < E && s_tiles[i + 1] <= tile) ++i;
+        return i;
     };
-
-    for (uint32_t tile = t_begin; tile < t_end; ++tile, buf ^= 1) {
-        int i = cur < 0 ? 0 : cur;
-        while (tile >= s_tiles[i + 1]) ++i;
-        if (i != cur) {
-            if (cur >= 0) flush(cur);
-            __syncthreads();
-            for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[(size_t)i * n_g + q];
-            __syncthreads();
-            cur = i;
-            const EnsInst &in = j.inst[i];
-            k = in.k;
-            p_off = ELECTRON ? in.e_off : in.i_off;
-            p_cap = ELECTRON ? in.e_cap : in.i_cap;
-            s_off = in.s_off;
-            s_cap = in.s_cap;
-            i_off = in.i_off;
-            i_cap = in.i_cap;
-            n_main = s_nmain[i];
-            n = s_n[i];
+    // loads of one tile (unconditional: lanes past the count read the
+    // instance's first slot; the count test is applied at use)
+    auto load_tile = [&](uint32_t tile, int i, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
+                         float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT]) {
+        uint32_t safe = 0;
+        avail = 0;
+        s0 = 0;
+        if (tile < t_end) {
+            safe = s_off[i];
+            const uint32_t local0 = (tile - s_tiles[i]) * tile_sz + threadIdx.x;
+            s0 = safe + local0;
+            avail = s_cnt[i] > local0 ? s_cnt[i] - local0 : 0u;
         }
-        const float factor = ELECTRON ? k.factor_e : k.factor_i;
-        const float dt = ELECTRON ? k.dt_e : k.dt_i;
-        const uint32_t local = tile - s_tiles[i];
-        float nx[PPT], nvx[PPT], vy[PPT], vz[PPT];
-        bool keep[PPT];
-        uint32_t off[PPT];
-        const uint32_t base = local * tile_sz + threadIdx.x;
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
-            const uint32_t v = base + u * blockDim.x;
-            keep[u] = false;
-            if (v < n) {
-                const float *X = j.sx, *VX = j.svx, *VY = j.svy, *VZ = j.svz;
-                uint32_t g = p_off + v;
-                if (ELECTRON && v >= n_main) {
-                    X = j.px;
-                    VX = j.pvx;
-                    VY = j.pvy;
-                    VZ = j.pvz;
-                    g = s_off + (v - n_main);
-                }
-                const float x = X[g];
-                const float e = field_at(ef, x, k.inv_dx, n_g);
-                nvx[u] = __builtin_fmaf(factor, e, VX[g]);
-                nx[u] = __builtin_fmaf(nvx[u], dt, x);
-                keep[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
-                if (keep[u]) {
-                    vy[u] = VY[g];
-                    vz[u] = VZ[g];
+            const uint32_t g = u * blockDim.x < avail ? s0 + u * blockDim.x : safe;
+            lx[u] = X[g];
+            lvx[u] = VX[g];
+            lvy[u] = VY[g];
+            lvz[u] = VZ[g];
+            lst[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, g);
+        }
+    };
+
+    // the instance being processed and its physics
+    int cur = -1;
+    KPhys k = j.k;
+    Appends app{};
+    Acc *i_dens_g = nullptr;
+    uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0, since_flush = 0;
+    bool overflow = false;
+
+    // leave instance `cur`: scatter its listed collisions, add its
+    // diagnostics, flush and clear the histograms
+    auto finish = [&]() {
+        __syncthreads();
+        const uint32_t n_list = min(s_ncoll, kCollPerWg);
+        for (uint32_t b0 = (uint32_t)wave * kWave; b0 < n_list; b0 += blockDim.x) {
+            const uint32_t e = b0 + lane_id();
+            int kind = kNone;
+            float se[3], si[3], px = 0.0f;
+            if (e < n_list) {
+                const uint32_t *q = list + e;
+                const uint32_t slot = q[CF_SLOT * kCollPerWg];
+                float w = __uint_as_float(q[CF_VX * kCollPerWg]), y = __uint_as_float(q[CF_VY * kCollPerWg]),
+                      z = __uint_as_float(q[CF_VZ * kCollPerWg]);
+                Xo s{q[CF_R0 * kCollPerWg], q[CF_R1 * kCollPerWg], q[CF_R2 * kCollPerWg], q[CF_R3 * kCollPerWg]};
+                const int b = (int)q[CF_BIN * kCollPerWg];
+                if (ELECTRON) {
+                    px = __uint_as_float(q[CF_X * kCollPerWg]);
+                    kind = e_scatter(k, j.cs[b], j.cs[kCsRanges + b], j.cs[2 * kCsRanges + b], w, y, z, s, se, si);
                 } else {
-                    const bool pw = nx[u] < 0.0f;
-                    lost_pow += pw;
-                    lost_gnd += !pw;
+                    kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z,
+                                     __uint_as_float(q[CF_AX * kCollPerWg]), __uint_as_float(q[CF_AY * kCollPerWg]),
+                                     __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]),
+                                     s);
+                }
+                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+                VX[slot] = w;
+                VY[slot] = y;
+                VZ[slot] = z;
+                ++ncoll;
+            }
+            if (ELECTRON)
+                ionisation_products<false, Acc>(app, k, kind, px, se, si, h, scale, i_dens_g, nullptr, ef,
+                                                overflow);
+        }
+        add_diagnostics<ELECTRON>(j.counters ? j.counters + (size_t)cur * 8 : nullptr,
+                                  j.cnt + (size_t)cur * EC_WORDS + EC_OVERFLOW, lost_pow, lost_gnd, ncoll, overflow);
+        lost_pow = lost_gnd = ncoll = 0;
+        overflow = false;
+        flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf,
+                       (Acc *)j.dens + (size_t)cur * j.inst_stride + rep * A.rep_stride);
+        __syncthreads();
+        for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
+        if (threadIdx.x == 0) s_ncoll = 0;
+        since_flush = 0;
+    };
+    // enter instance i: its field and physics, its append regions
+    auto enter = [&](int i) {
+        for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[(size_t)i * n_g + q];
+        const EnsInst &in = j.inst[i];
+        k.wf = in.k.wf;
+        k.stot_scale = in.k.stot_scale;
+        k.normal_range = in.k.normal_range;
+        if (ELECTRON) {
+            uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
+            app = Appends{X,    VX,    VY,    VZ,    c + j.c_out, in.e_off, in.e_cap,
+                          j.ix, j.ivx, j.ivy, j.ivz, c + j.c_ion, in.i_off, in.i_cap};
+            i_dens_g = (Acc *)j.ion_dens + (size_t)i * j.inst_stride + rep * A.rep_stride;
+        }
+        cur = i;
+        __syncthreads();
+    };
+
+    float x[PPT], vx[PPT], vy[PPT], vz[PPT];
+    Xo st[PPT];
+    uint32_t slot0 = 0, avail = 0;
+    uint32_t tile = t_begin;
+    int ti = 0;
+    {
+        int lo = 0, hi = E - 1;  // binary search for the first tile's instance
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_tiles[mid] <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        ti = inst_of(tile, lo);
+    }
+    ti = __builtin_amdgcn_readfirstlane(ti);
+    load_tile(tile, ti, slot0, avail, x, vx, vy, vz, st);
+    while (tile < t_end) {
+        if (ti != cur) {
+            if (cur >= 0) finish();
+            enter(ti);
+        }
+        const uint32_t tile_next = tile + 1;
+        const int ti_next = __builtin_amdgcn_readfirstlane(tile_next < t_end ? inst_of(tile_next, ti) : ti);
+        bool valid[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            valid[u] = u * blockDim.x < avail;
+            if (!valid[u]) x[u] = -1.0f;
+        }
+        float nx[PPT], spd[PPT], sig[PPT];
+        float ax[PPT], ay[PPT], az[PPT];
+        int idx[PPT];
+        bool in[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            in[u] = false;
+            nx[u] = 0.0f;
+            idx[u] = 0;
+            spd[u] = 0.0f;
+            ax[u] = ay[u] = az[u] = 0.0f;
+            if (x[u] >= 0.0f) {
+                const float e = field_at(ef, x[u], k.inv_dx, n_g);
+                vx[u] = __builtin_fmaf(factor, e, vx[u]);
+                nx[u] = __builtin_fmaf(vx[u], dt, x[u]);
+                in[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
+            }
+            if (in[u]) {
+                if (ELECTRON) idx[u] = e_index(k, vx[u], vy[u], vz[u], spd[u]);
+                else idx[u] = i_index(k, vx[u], vy[u], vz[u], st[u], ax[u], ay[u], az[u], spd[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) sig[u] = j.stot[in[u] ? idx[u] : 0];
+        float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
+        Xo pst[PPT];
+        uint32_t pslot0, pavail;
+        load_tile(tile_next, ti_next, pslot0, pavail, px, pvx, pvy, pvz, pst);
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const uint32_t i = slot0 + u * blockDim.x;
+            bool hit = false;
+            if (in[u]) {
+                hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
+                deposit_fix(h, nx[u], k, scale);
+            }
+            if (valid[u]) {
+                X[i] = in[u] ? nx[u] : -1.0f;
+                VX[i] = vx[u];
+                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+            }
+            if (!in[u] && x[u] >= 0.0f) {
+                const bool pw = nx[u] < 0.0f;
+                lost_pow += pw;
+                lost_gnd += !pw;
+            }
+            const uint64_t m = ballot64(hit);
+            if (m) {
+                const int leader = __builtin_ctzll(m);
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&s_ncoll, (uint32_t)__popcll(m));
+                base = __shfl(base, leader, kWave);
+                const uint32_t pos = base + mbcnt64(m);
+                const bool listed = hit && pos < kCollPerWg;
+                if (listed) {
+                    uint32_t *e = list + pos;
+                    e[CF_SLOT * kCollPerWg] = i;
+                    e[CF_VX * kCollPerWg] = __float_as_uint(vx[u]);
+                    e[CF_VY * kCollPerWg] = __float_as_uint(vy[u]);
+                    e[CF_VZ * kCollPerWg] = __float_as_uint(vz[u]);
+                    e[CF_R0 * kCollPerWg] = st[u].s0;
+                    e[CF_R1 * kCollPerWg] = st[u].s1;
+                    e[CF_R2 * kCollPerWg] = st[u].s2;
+                    e[CF_R3 * kCollPerWg] = st[u].s3;
+                    e[CF_BIN * kCollPerWg] = (uint32_t)idx[u];
+                    if (ELECTRON) {
+                        e[CF_X * kCollPerWg] = __float_as_uint(nx[u]);
+                    } else {
+                        e[CF_AX * kCollPerWg] = __float_as_uint(ax[u]);
+                        e[CF_AY * kCollPerWg] = __float_as_uint(ay[u]);
+                        e[CF_AZ * kCollPerWg] = __float_as_uint(az[u]);
+                        e[CF_G * kCollPerWg] = __float_as_uint(spd[u]);
+                    }
+                }
+                const bool local = hit && !listed;  // a full list: scatter here
+                if (ballot64(local)) {
+                    int kind = kNone;
+                    float se[3], si[3];
+                    if (local) {
+                        float w = vx[u], y = vy[u], z = vz[u];
+                        Xo s = st[u];
+                        const int b = idx[u];
+                        if (ELECTRON)
+                            kind = e_scatter(k, j.cs[b], j.cs[kCsRanges + b], j.cs[2 * kCsRanges + b], w, y, z, s,
+                                             se, si);
+                        else
+                            kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
+                                             ay[u], az[u], spd[u], s);
+                        rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
+                        VX[i] = w;
+                        VY[i] = y;
+                        VZ[i] = z;
+                        ++ncoll;
+                    }
+                    if (ELECTRON)
+                        ionisation_products<false, Acc>(app, k, kind, nx[u], se, si, h, scale, i_dens_g, nullptr,
+                                                        ef, overflow);
                 }
             }
         }
+        slot0 = pslot0;
+        avail = pavail;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            x[u] = px[u];
+            vx[u] = pvx[u];
+            vy[u] = pvy[u];
+            vz[u] = pvz[u];
+            st[u] = pst[u];
+        }
+        tile = tile_next;
+        ti = ti_next;
+        if (++since_flush == cap && tile < t_end && ti == cur) {  // uniform over the workgroup
+            since_flush = 0;
+            __syncthreads();
+            flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf,
+                           (Acc *)j.dens + (size_t)cur * j.inst_stride + rep * A.rep_stride);
+            __syncthreads();
+            for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
+            __syncthreads();
+        }
+    }
+    finish();
+}
+
+// Compaction pass of an ensemble species (see EnsCompactJob).
+template <int PPT>
+__global__ __launch_bounds__(256) void k_compact_ens(EnsCompactJob j) {
+    __shared__ uint32_t s_tiles[kMaxEnsemble + 1], s_cnt[kMaxEnsemble];
+    __shared__ uint32_t s_wave[32], s_base[2];
+    const int E = j.n_inst;
+    const uint32_t tile_sz = blockDim.x * PPT;
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+        const EnsInst &in = j.inst[i];
+        const uint32_t c = min(j.cnt[(size_t)i * EC_WORDS + j.c_in], j.electrons ? in.e_cap : in.i_cap);
+        s_cnt[i] = c;
+        s_tiles[i + 1] = (c + tile_sz - 1) / tile_sz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        s_tiles[0] = 0;
+        for (int i = 0; i < E; ++i) {
+            acc += s_tiles[i + 1];
+            s_tiles[i + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const uint32_t T = s_tiles[E];
+    const uint32_t t_begin = (uint32_t)((uint64_t)T * blockIdx.x / gridDim.x);
+    const uint32_t t_end = (uint32_t)((uint64_t)T * (blockIdx.x + 1) / gridDim.x);
+    int i = 0;
+    int buf = 0;
+    for (uint32_t tile = t_begin; tile < t_end; ++tile, buf ^= 1) {
+        while (i + 1 < E && s_tiles[i + 1] <= tile) ++i;
+        const EnsInst &in = j.inst[i];
+        const uint32_t off = j.electrons ? in.e_off : in.i_off;
+        const uint32_t base = (tile - s_tiles[i]) * tile_sz + threadIdx.x;
+        float v[4][PPT];
+        bool keep[PPT];
+        uint32_t o[PPT];
         uint32_t wcount = 0;
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
+            const uint32_t l = base + u * blockDim.x;
+            keep[u] = false;
+            if (l < s_cnt[i]) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[c][u] = j.src[c][off + l];
+                keep[u] = v[0][u] >= 0.0f;  // x < 0: a hole
+            }
             const uint64_t m = ballot64(keep[u]);
-            off[u] = wcount + mbcnt64(m);
+            o[u] = wcount + mbcnt64(m);
             wcount += (uint32_t)__popcll(m);
         }
-        const uint32_t wbase =
-            tile_reserve_f(wcount, s_wave, s_base, buf, j.cnt + (size_t)i * EC_WORDS + j.c_out);
+        const uint32_t wbase = tile_reserve_f(wcount, s_wave, s_base, buf, j.cnt + (size_t)i * EC_WORDS + j.c_out);
 #pragma unroll
-        for (int u = 0; u < PPT; ++u) {
-            int kind = kNone;
-            float se[3], si[3];
+        for (int u = 0; u < PPT; ++u)
             if (keep[u]) {
-                const uint32_t slot = wbase + off[u];
-                if (slot < p_cap) {
-                    const uint32_t g = p_off + slot;
-                    Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, g);
-                    float vx = nvx[u], y = vy[u], z = vz[u];
-                    if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
-                    else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s);
-                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, g, s);
-                    j.dx[g] = nx[u];
-                    j.dvx[g] = vx;
-                    j.dvy[g] = y;
-                    j.dvz[g] = z;
-                    deposit_lds<Acc>(h, nx[u], k);
-                    ncoll += kind != kNone;
-                } else {
-                    overflow = true;
-                }
+                const uint32_t d = off + wbase + o[u];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) j.dst[c][d] = v[c][u];
             }
-            if (ELECTRON) {
-                const uint64_t sp = ballot64(kind == kIonisation);
-                if (sp) {
-                    const int leader = __builtin_ctzll(sp);
-                    uint32_t be = 0, bi = 0;
-                    if ((int)lane_id() == leader) {
-                        const uint32_t cnt = (uint32_t)__popcll(sp);
-                        uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
-                        be = atomicAdd(&c[j.s_out], cnt);
-                        bi = atomicAdd(&c[j.c_ion], cnt);
-                    }
-                    be = __shfl(be, leader, kWave);
-                    bi = __shfl(bi, leader, kWave);
-                    if (kind == kIonisation) {
-                        const uint32_t o = mbcnt64(sp);
-                        const uint32_t es = be + o, is = bi + o;
-                        const float x = nx[u];
-                        if (es < s_cap) {
-                            j.ox[s_off + es] = x;
-                            j.ovx[s_off + es] = se[0];
-                            j.ovy[s_off + es] = se[1];
-                            j.ovz[s_off + es] = se[2];
-                            deposit_lds<Acc>(h, x, k);
-                        } else {
-                            overflow = true;
-                        }
-                        if (is < i_cap) {
-                            j.ix[i_off + is] = x;
-                            j.ivx[i_off + is] = si[0];
-                            j.ivy[i_off + is] = si[1];
-                            j.ivz[i_off + is] = si[2];
-                            deposit_global<Acc>((Acc *)j.ion_dens + (size_t)i * j.inst_stride +
-                                                    (blockIdx.x % A.reps) * A.rep_stride,
-                                                x, k);
-                        } else {
-                            overflow = true;
-                        }
-                    }
-                }
-            }
-        }
     }
-    if (cur >= 0) flush(cur);
+}
+
+// Live particles (x >= 0) of every instance: out[i] += count of instance i
+// (grid: blocks x instances).
+__global__ __launch_bounds__(256) void k_count_live_ens(const float *__restrict__ x, const EnsInst *inst,
+                                                       const uint32_t *__restrict__ cnt, int word, bool electrons,
+                                                       uint32_t *__restrict__ out) {
+    const int i = blockIdx.y;
+    const EnsInst &in = inst[i];
+    const uint32_t off = electrons ? in.e_off : in.i_off;
+    const uint32_t n = min(cnt[(size_t)i * EC_WORDS + word], electrons ? in.e_cap : in.i_cap);
+    uint32_t c = 0;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x)
+        c += x[off + l] >= 0.0f;
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(out + i, c);
+}
+
+int launch_count_live_ens(const float *x, const EnsInst *inst, int n_inst, const uint32_t *cnt, int word,
+                          bool electrons, uint32_t *out, hipStream_t s) {
+    k_count_live_ens<<<dim3(16, n_inst), 256, 0, s>>>(x, inst, cnt, word, electrons, out);
+    PIC_LAUNCHED("k_count_live_ens");
+    return PIC_OK;
 }
 
 template <bool E, int PPT>
-static void launch_ens_t(const EnsFusedArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
-    if (f64) k_fused_ens<E, PPT, double><<<grid, block, lds, s>>>(a);
-    else k_fused_ens<E, PPT, float><<<grid, block, lds, s>>>(a);
+static void launch_ens_t(const EnsArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
+    if (f64) k_inplace_ens<E, PPT, double><<<grid, block, lds, s>>>(a);
+    else k_inplace_ens<E, PPT, float><<<grid, block, lds, s>>>(a);
 }
 
-int launch_fused_ens(const EnsFusedJob &j, const Shape &sh0, int n_g, hipStream_t s) {
+int launch_inplace_ens(const EnsStepJob &j, const Shape &sh0, hipStream_t s) {
     if (j.n_inst <= 0 || j.n_inst > kMaxEnsemble) {
         set_error("ensemble: 1 to 256 instances");
         return PIC_ERR_ARG;
     }
     Shape sh = resolve(sh0);
-    EnsFusedArgs a;
+    EnsArgs a;
     a.j = j;
-    a.n_g = n_g;
+    const int n_g = j.k.n_g;
     const int block = 256;
-    const int budget = 40 * 1024;
-    a.lds_reps = std::max(1, std::min(block / kWave, (budget - n_g * 4) / (n_g * 4)));
+    const int grid = std::max(1, sh.grid * sh.block / block);
+    // LDS as in the single-discharge kernel: the field, the collision list
+    // and up to one histogram replica per wave in what the resident
+    // workgroups of a compute unit can share
+    const int per_cu = std::max(1, (grid + device_cus() - 1) / device_cus());
+    const int budget = std::min(device_lds_per_block(), device_lds_per_cu() / per_cu - 2048);
+    const int fixed = (n_g + (j.electrons ? kCollFieldsE : kCollFieldsI) * (int)kCollPerWg) * 4;
+    a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (n_g * 4)));
     a.reps = sh.replicas;
     a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : n_g;
-    const size_t lds = (size_t)n_g * 4 * (1 + a.lds_reps);
-    const int grid = std::max(1, sh.grid * sh.block / block);
+    const size_t lds = (size_t)fixed + (size_t)a.lds_reps * n_g * 4;
     const bool f64 = j.dens_f64;
+    const int ppt = sh.ppt_inplace;
     if (j.electrons) {
-        if (sh.ppt == 2) launch_ens_t<true, 2>(a, f64, grid, block, lds, s);
-        else launch_ens_t<true, 4>(a, f64, grid, block, lds, s);
+        if (ppt == 1) launch_ens_t<true, 1>(a, f64, grid, block, lds, s);
+        else if (ppt == 4) launch_ens_t<true, 4>(a, f64, grid, block, lds, s);
+        else launch_ens_t<true, 2>(a, f64, grid, block, lds, s);
     } else {
-        if (sh.ppt == 2) launch_ens_t<false, 2>(a, f64, grid, block, lds, s);
-        else launch_ens_t<false, 4>(a, f64, grid, block, lds, s);
+        if (ppt == 1) launch_ens_t<false, 1>(a, f64, grid, block, lds, s);
+        else if (ppt == 4) launch_ens_t<false, 4>(a, f64, grid, block, lds, s);
+        else launch_ens_t<false, 2>(a, f64, grid, block, lds, s);
     }
-    PIC_LAUNCHED("k_fused_ens");
+    PIC_LAUNCHED("k_inplace_ens");
+    return PIC_OK;
+}
+
+int launch_compact_ens(const EnsCompactJob &j, hipStream_t s) {
+    if (j.n_inst <= 0 || j.n_inst > kMaxEnsemble) {
+        set_error("ensemble: 1 to 256 instances");
+        return PIC_ERR_ARG;
+    }
+    k_compact_ens<4><<<device_cus() * 4, 256, 0, s>>>(j);
+    PIC_LAUNCHED("k_compact_ens");
     return PIC_OK;
 }
 

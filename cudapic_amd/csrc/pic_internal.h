@@ -175,16 +175,22 @@ int launch_count_live(const float *x, const uint32_t *cnt, int n_seg, uint32_t s
 // ---------------------------------------------------------------- ensembles
 // One independent discharge of an ensemble: its physics, its driving
 // amplitude and the regions it owns in the pooled particle arrays.
+// Instances share the grid, the time step and the masses, so they differ in
+// KPhys only by the weight factor wf, the gas density (stot_scale, the
+// summed cross-section table is shared) and the thermal velocity spread of
+// the gas atoms (normal_range).
 struct EnsInst {
-    KPhys k;  // k.stot_scale = the instance's gas density
+    KPhys k;
     double v0;
     uint32_t e_off, e_cap;  // electrons (and their RNG slots)
-    uint32_t s_off, s_cap;  // newborn-electron staging
     uint32_t i_off, i_cap;  // ions (and their RNG slots)
 };
 
-// per-instance device counters (uint32 words)
-enum EnsCnt { EC_E0 = 0, EC_S0 = 2, EC_I0 = 4, EC_OVERFLOW = 6, EC_WORDS = 8 };
+// per-instance device counters (uint32 words): the electron watermark of
+// the in-place steps (two parities: the step reads one and appends newborns
+// after a copy of it in the other), the ion watermark of the two ion
+// buffers, the compaction pass's output count and the overflow flag
+enum EnsCnt { EC_E0 = 0, EC_I0 = 2, EC_C = 4, EC_OVERFLOW = 5, EC_WORDS = 8 };
 
 struct EnsPoissonJob {
     int precision;
@@ -200,31 +206,52 @@ struct EnsPoissonJob {
     unsigned long long *counters;  // [n_inst][8]: [6],[7] measured steps
     void *zero_a, *zero_b;       // next accumulators [n_inst][inst_stride] (zero_b on ion steps)
     uint32_t *cnt;               // [n_inst][EC_WORDS]
-    int zc0, zc1, zci;           // counter words cleared (zci on ion steps only)
+    // counter words: c[copy_dst] = c[copy_src] (when copy_dst >= 0), then
+    // c[zc0] = 0 and c[zc1] = 0 (each when >= 0)
+    int copy_src, copy_dst, zc0, zc1;
 };
 int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s);
 
-struct EnsFusedJob {
+// In-place particle step of every instance in one launch (the ensemble
+// counterpart of the in-place FusedJob): pooled arrays updated in their
+// slots, lost particles leave holes, newborn electrons and ions appended
+// after each instance's watermark.
+struct EnsStepJob {
     bool electrons;
     const EnsInst *inst;
     int n_inst;
+    KPhys k;                               // shared physics (instance 0)
     const float *efield;                   // [n_inst][n_g]
-    const float *sx, *svx, *svy, *svz;     // pooled input
-    const float *px, *pvx, *pvy, *pvz;     // pooled staging input (electrons)
-    float *dx, *dvx, *dvy, *dvz;           // pooled output
-    float *ox, *ovx, *ovy, *ovz;           // pooled staging output (electrons)
-    float *ix, *ivx, *ivy, *ivz;           // pooled ions (appends)
+    float *x, *vx, *vy, *vz;               // pooled particles of this species
     uint32_t *rng0, *rng1, *rng2, *rng3;   // pooled per-slot streams of this species
+    float *ix, *ivx, *ivy, *ivz;           // pooled ions (electron step: appends)
     uint32_t *cnt;                         // [n_inst][EC_WORDS]
-    int c_in, c_out, s_in, s_out, c_ion;   // counter words
+    int c_in, c_out, c_ion;                // counter words (c_out, c_ion: electron step)
     void *dens, *ion_dens;                 // [n_inst][inst_stride]
     int inst_stride;
     bool dens_f64;
     const float *stot, *cs;                // stot: summed cross sections (scaled per instance)
     unsigned long long *counters;          // [n_inst][8]
-    uint32_t *time;                        // advanced by one step (electron kernel)
+    uint32_t *time;                        // advanced by one step (electron step)
     int n_t;
 };
-int launch_fused_ens(const EnsFusedJob &j, const Shape &sh, int n_g, hipStream_t s);
+int launch_inplace_ens(const EnsStepJob &j, const Shape &sh, hipStream_t s);
+
+// Compaction pass: the live particles (x >= 0) of each instance's first
+// c[c_in] slots of src are packed into the same region of dst, counted in
+// c[c_out] (cleared before the launch).  RNG streams stay with the slots.
+struct EnsCompactJob {
+    bool electrons;
+    const EnsInst *inst;
+    int n_inst;
+    const float *src[4];
+    float *dst[4];
+    uint32_t *cnt;
+    int c_in, c_out;
+};
+int launch_compact_ens(const EnsCompactJob &j, hipStream_t s);
+// out[i] += live particles (x >= 0) in the first c[word] slots of instance i
+int launch_count_live_ens(const float *x, const EnsInst *inst, int n_inst, const uint32_t *cnt, int word,
+                          bool electrons, uint32_t *out, hipStream_t s);
 
 }  // namespace pic

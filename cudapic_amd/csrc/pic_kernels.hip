@@ -678,7 +678,7 @@ struct EnsPoissonArgs {
     uint32_t *zero_a, *zero_b;
     int zero_words;  // 32-bit words per instance
     uint32_t *cnt;
-    int zc0, zc1, zci;
+    int copy_src, copy_dst, zc0, zc1;
 };
 
 template <typename T, typename D>
@@ -712,9 +712,11 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
     if (ion)
         for (int w = tid; w < a.zero_words; w += blockDim.x) a.zero_b[zoff + w] = 0u;
     uint32_t *c = a.cnt + (size_t)b * EC_WORDS;
-    if (tid == 0 && a.zc0 >= 0) c[a.zc0] = 0u;
-    if (tid == 1 && a.zc1 >= 0) c[a.zc1] = 0u;
-    if (tid == 2 && ion && a.zci >= 0) c[a.zci] = 0u;
+    if (tid == 0) {
+        if (a.copy_dst >= 0) c[a.copy_dst] = c[a.copy_src];
+        if (a.zc0 >= 0) c[a.zc0] = 0u;
+        if (a.zc1 >= 0) c[a.zc1] = 0u;
+    }
     if (tid == 3 && a.cumul_e) {
         a.counters[(size_t)b * 8 + 6] += 1ull;
         if (ion) a.counters[(size_t)b * 8 + 7] += 1ull;
@@ -746,9 +748,10 @@ int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
     a.zero_a = (uint32_t *)j.zero_a;
     a.zero_b = (uint32_t *)j.zero_b;
     a.cnt = j.cnt;
+    a.copy_src = j.copy_src;
+    a.copy_dst = j.copy_dst;
     a.zc0 = j.zc0;
     a.zc1 = j.zc1;
-    a.zci = j.zci;
     switch (j.precision) {
     case PIC_PREC_FP32:
         a.zero_words = j.inst_stride;

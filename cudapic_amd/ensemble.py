@@ -1,5 +1,5 @@
 """Ensembles of independent discharges (pic_ens_* of the C ABI): every
-stage of every instance in one launch, the 2*N_SUB-step pattern replayed
+stage of every instance in one launch, the 4*N_SUB-step pattern replayed
 from a hipGraph."""
 from __future__ import annotations
 
@@ -40,6 +40,17 @@ class Ensemble:
         out = [np.zeros(max(1, n.value), np.float32) for _ in range(4)]
         call("pic_ens_download", self.h, inst, species, *(a.ctypes.data for a in out), n.value, C.byref(n))
         return [a[:n.value] for a in out]
+
+    def slots(self, inst, species):
+        """Slot (within the instance's region) of each particle download() returns."""
+        n = C.c_uint32()
+        call("pic_ens_download_slots", self.h, inst, species, None, 0, C.byref(n))
+        out = np.zeros(max(1, n.value), np.uint32)
+        call("pic_ens_download_slots", self.h, inst, species, out.ctypes.data, n.value, C.byref(n))
+        return out[:n.value].astype(np.int64)
+
+    def set_time(self, t):
+        call("pic_ens_set_time", self.h, int(t))
 
     def rng_download(self, inst, species, n):
         out = [np.zeros(max(1, n), np.uint32) for _ in range(4)]

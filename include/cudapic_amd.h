@@ -272,12 +272,19 @@ int pic_sim_load(pic_sim_t *s, const char *path);
 /* Instances share n_g, n_t, n_sub, frequency, gap and precision (taken from
  * params[0]); each has its own v0, pressure, gas temperature, weight and
  * capacity.  No reference counterpart: the reference runs one discharge per
- * process; this is the batched mode of the same step. */
+ * process; this is the batched mode of the same step.  Particles are
+ * updated in the slots they occupy, as on the fused single-discharge path
+ * (lost particles leave holes, a compaction pass every 2*N_SUB steps), so
+ * a particle's RNG stream is its slot's until the next compaction. */
 typedef struct pic_ensemble_config {
     int32_t n_instances;          /* 1..256 */
     const pic_params_t *params;   /* [n_instances], derived */
-    const uint32_t *capacity;     /* [n_instances] particles per species */
+    const uint32_t *capacity;     /* [n_instances] slots per species */
     uint32_t seed_e[4], seed_i[4];
+    /* use_graph: replay 4*N_SUB steps from a hipGraph; grid_blocks (0 = 4
+     * per compute unit), ppt (particles per thread per tile, 1, 2 or 4; 0 =
+     * 2) and replicas (global density replicas, 0 = 8) shape the particle
+     * kernels */
     int32_t use_graph, grid_blocks, ppt, replicas;
 } pic_ensemble_config_t;
 
@@ -287,8 +294,15 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out);
 int pic_ens_destroy(pic_ensemble_t *e);
 int pic_ens_upload(pic_ensemble_t *e, int inst, int species, const float *x, const float *vx,
                    const float *vy, const float *vz, uint32_t n);
+/* The live particles of one instance in slot order (*n = live count). */
 int pic_ens_download(pic_ensemble_t *e, int inst, int species, float *x, float *vx, float *vy,
                      float *vz, uint32_t cap, uint32_t *n);
+/* Slot (relative to the instance's region) of each particle
+ * pic_ens_download returns, in the same order. */
+int pic_ens_download_slots(pic_ensemble_t *e, int inst, int species, uint32_t *slots, uint32_t cap,
+                           uint32_t *n);
+/* Step index within the RF cycle of the next step (as pic_sim_set_time). */
+int pic_ens_set_time(pic_ensemble_t *e, uint32_t t_in_cycle);
 /* RNG words of the first n slots of one instance's region. */
 int pic_ens_rng_download(pic_ensemble_t *e, int inst, int species, uint32_t *w0, uint32_t *w1,
                          uint32_t *w2, uint32_t *w3, uint32_t n);

@@ -5,8 +5,8 @@ whole runs reproduce the reference's init-cycle statistics per instance."""
 import numpy as np
 import pytest
 
-from helpers import SEED_E, SEED_I, synthetic_state
-from test_gpu_sim import _push_ref, _replay_species, _split_ions
+from helpers import SEED_E, SEED_I, assert_same_multiset, sort_rows, synthetic_state
+from test_gpu_sim import _check_new_ions, _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
 
@@ -18,20 +18,34 @@ def ca():
     return cudapic_amd
 
 
-def test_ensemble_step_per_instance_replay(ca, orc):
+def _source_slots(orc, po, ef, parts, name):
+    """Slot (upload index) of every pushed survivor: the push leaves vy
+    untouched and keeps the order, so an index carried in vy comes back."""
+    x, vx, _, vz = parts
+    idx = np.arange(len(x), dtype=np.float32)
+    return _push_ref(orc, po, ef, [x, vx, idx, vz], name)[1][2].astype(np.int64)
+
+
+# start steps: 1 updates electrons in place, 20 (= N_SUB) both species in
+# place, 0 both species in place followed by the compaction pass
+@pytest.mark.parametrize("t0", [0, 1, 20])
+def test_ensemble_step_per_instance_replay(ca, orc, t0):
     specs = [(250.0, 10.0, 2.0e3, 60_000), (120.0, 4.0, 5.0e3, 30_000), (380.0, 25.0, 1.0e3, 90_000)]
     ps = [ca.params(v0=v, pressure=pr, weight=w, precision=2) for v, pr, w, _ in specs]
     caps = [int(n * 1.3) for *_, n in specs]
     ens = ca.Ensemble(ps, caps, use_graph=False)
+    ens.set_time(t0)
     states = []
     for i, (v, pr, w, n) in enumerate(specs):
-        E, I = synthetic_state(n, int(n * 1.05), seed=80 + i, te_ev=6.0)
+        E, I = synthetic_state(n, int(n * 1.05), seed=80 + i + t0, te_ev=6.0)
         ens.upload(i, 0, *E)
         ens.upload(i, 1, *I)
         states.append((E, I))
     rng0 = [(ens.rng_download(i, 0, caps[i]), ens.rng_download(i, 1, caps[i])) for i in range(3)]
     ens.run(1, measure=True)
     ne, ni = ens.counts()
+    ion_step = t0 % 20 == 0
+    compacted = t0 % 40 == 0
     born_total = 0
     for i, (v, pr, w, n) in enumerate(specs):
         po = orc.params(v0=v, pressure=pr, weight=w)
@@ -39,35 +53,92 @@ def test_ensemble_step_per_instance_replay(ca, orc):
         pot, ef = ens.grid(i)
         ce, ci, cnt = ens.measurements(i)
         # densities of the initial state (cumul after one measured step)
-        for got, parts in ((ce, E), (ci, I)):
-            want = orc.deposit(parts[0], po)
-            assert np.abs(got - want).max() <= 2e-6 * want.max()
-        pw, ew = orc.poisson_scan_f64(ce, ci, v, po)
+        want = orc.deposit(E[0], po)
+        assert np.abs(ce - want).max() <= 2e-6 * want.max()
+        if ion_step:
+            want = orc.deposit(I[0], po)
+            assert np.abs(ci - want).max() <= 2e-6 * want.max()
+        pw, ew = orc.poisson_scan_f64(ce, orc.deposit(I[0], po), v * np.cos(2 * np.pi * t0 / 4000), po)
         assert np.abs(pot - pw).max() <= 1e-5 * np.abs(pw).max()
         assert np.abs(ef - ew).max() <= 1e-5 * np.abs(ew).max()
         cs, se, si = orc.cross_sections(po.gas_density)
         ge, gi = ens.download(i, 0), ens.download(i, 1)
         cnt_e = _push_ref(orc, po, ef, E, "e")[0]
-        cnt_i = _push_ref(orc, po, ef, I, "i")[0]
         re1 = ens.rng_download(i, 0, caps[i])
         ri1 = ens.rng_download(i, 1, caps[i])
-        # survivors fill the instance's region from its start, newborn
-        # electrons wait in the staging array (no slot)
-        slots_e = np.concatenate([np.arange(cnt_e), -np.ones(len(ge[0]) - cnt_e, np.int64)])
-        born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, slots_e, rng0[i][0], re1, "e")
-        _, old_ion, n_amb, _ = _split_ions(gi, _push_ref(orc, po, ef, I, "i")[1][0])
-        # survivors fill the first cnt_i slots in order
-        assert not old_ion[cnt_i:].any()
-        _replay_species(orc, po, si, cs, I, ef, [a[old_ion] for a in gi],
-                        np.arange(len(gi[0]))[old_ion], rng0[i][1], ri1, "i", missing=n_amb)
-        # (slots are assigned in atomic order, so the RNG stream a particle
-        # meets and the number of ionisations vary from run to run)
-        assert ni[i] - cnt_i == ne[i] - cnt_e == len(born_x)
+        if compacted:
+            born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, None, rng0[i][0], re1, "e",
+                                           src_slots=_source_slots(orc, po, ef, E, "e"))
+        else:
+            born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, ens.slots(i, 0), rng0[i][0], re1, "e")
+        assert ne[i] == len(ge[0]) == cnt_e + len(born_x)
+        if ion_step:
+            cnt_i, pushed_i, _ = _push_ref(orc, po, ef, I, "i")
+        else:
+            cnt_i, pushed_i = len(I[0]), I
+        assert ni[i] == len(gi[0]) == cnt_i + len(born_x)
+        _, old_ion, n_amb, amb = _split_ions(gi, pushed_i[0])
+        if ion_step:
+            sel = [a[old_ion] for a in gi]
+            if compacted:
+                _replay_species(orc, po, si, cs, I, ef, sel, None, rng0[i][1], ri1, "i", missing=n_amb,
+                                src_slots=_source_slots(orc, po, ef, I, "i"))
+            else:
+                _replay_species(orc, po, si, cs, I, ef, sel, ens.slots(i, 1)[old_ion], rng0[i][1], ri1, "i",
+                                missing=n_amb)
+        else:
+            keep = ~np.isin(I[0], amb)
+            assert_same_multiset([a[old_ion] for a in gi], [a[keep] for a in I])
+            for q in range(4):
+                np.testing.assert_array_equal(ri1[q], rng0[i][1][q])
+        _check_new_ions(gi, pushed_i[0], born_x, ions, check_v=True)
         born_total += len(born_x)
-        assert cnt[6] == 1 and cnt[7] == 1 and cnt[4] > 0
+        assert cnt[6] == 1 and cnt[7] == int(ion_step) and cnt[4] > 0
+        assert (cnt[5] > 0) == ion_step
     assert born_total > 0
     ens.check()
     ens.close()
+
+
+def test_ensemble_graph_replays_match_eager(ca):
+    """Graph replays (4*N_SUB steps each) against the same steps run eagerly,
+    with eager work between the replays.  At 1e-9 Pa the discharges are
+    collisionless (collision probability ~1e-13 per particle-step), so both
+    runs see the same fields up to the summation order of the density
+    atomics.  The weights put both at the reference case's plasma density,
+    so that the plasma shields the RF voltage and keeps its electrons."""
+    specs = [(250.0, 1.5e5, 40_000), (150.0, 3.0e5, 20_000)]
+    ps = [ca.params(v0=v, pressure=1e-9, weight=w, precision=2) for v, w, _ in specs]
+    caps = [int(n * 1.5) for *_, n in specs]
+    ens = []
+    for graph in (False, True):
+        e = ca.Ensemble(ps, caps, use_graph=graph)
+        for i, (*_, n) in enumerate(specs):
+            E, I = synthetic_state(n, int(n * 1.05), seed=7 + i)
+            e.upload(i, 0, *E)
+            e.upload(i, 1, *I)
+        ens.append(e)
+    for rep in range(3):
+        for e in ens:
+            e.run(160)
+            e.synchronize()
+        ca_, cb = ens[0].counts(), ens[1].counts()
+        print("rep", rep, "counts eager", ca_, "graph", cb, flush=True)
+        for i in range(len(specs)):
+            assert ca_[0][i] > 0.5 * specs[i][2]
+            assert abs(int(ca_[0][i]) - int(cb[0][i])) <= 2 and abs(int(ca_[1][i]) - int(cb[1][i])) <= 2
+            a = sort_rows(*ens[0].download(i, 0))
+            b = sort_rows(*ens[1].download(i, 0))
+            m = min(len(a), len(b))
+            assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
+            pa, pb = ens[0].grid(i)[0], ens[1].grid(i)[0]
+            assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
+        for e in ens:
+            e.run(3)
+            e.run(77)
+    for e in ens:
+        e.check()
+        e.close()
 
 
 def test_ensemble_init_cycle_statistics(ca):

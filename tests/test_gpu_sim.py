@@ -132,7 +132,8 @@ def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec, segments):
     sim.close()
 
 
-def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, name, missing=0):
+def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, name, missing=0,
+                    src_slots=None):
     """Match every GPU particle to its source by its pushed position and
     replay its collision with the RNG words of its slot.
 
@@ -142,7 +143,10 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
     parent's slot drew its numbers, the newborn's slot (or staging entry,
     slot -1) did not.  Returns the positions at which electrons were born
     (one entry per newborn) and the ion velocities of the checked pairs.
-    `missing`: pushed particles whose GPU rows were left out of `got`."""
+    `missing`: pushed particles whose GPU rows were left out of `got`.
+    `src_slots`: for a step that compacted after colliding, the slot whose
+    stream pushed source p used (its slot before the compaction); the parent
+    of a pair is then the one with the replayed velocity."""
     cnt, pushed, _ = _push_ref(orc, po, efield, parts, name)
     px, pvx, pvy, pvz = pushed
     ux, uidx, ucnt = np.unique(px, return_index=True, return_counts=True)
@@ -171,6 +175,9 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
         kind, vo, so = orc.collide_i_one(stot, cs, v, st, po)
         return kind, v, vo, so, None, None
 
+    def rng_slot(j, g):
+        return src_slots[uidx[k[g]]] if src_slots is not None else slots[j]
+
     single = (gcnt == 1) & (mo == 1)
     checked = mism = 0
     step = max(1, n // 20000)
@@ -178,9 +185,10 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
         g = ginv[j]
         if not single[g]:
             continue
-        kind, v, vo, so, _, _ = replay(uidx[k[g]], slots[j])
+        rs = rng_slot(j, g)
+        kind, v, vo, so, _, _ = replay(uidx[k[g]], rs)
         checked += 1
-        if so.tolist() != [rng1[w][slots[j]] for w in range(4)]:
+        if so.tolist() != [rng1[w][rs] for w in range(4)]:
             mism += 1
             continue
         gv = np.array([gvx[j], gvy[j], gvz[j]], np.float64)
@@ -194,10 +202,19 @@ def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, na
         for j in np.nonzero(np.isin(ginv, pair_groups))[0]:
             members.setdefault(ginv[j], []).append(j)
         for g, pair in members.items():
-            par = [j for j in pair if moved(j)]
-            kid = [j for j in pair if not moved(j)]
-            assert len(par) == 1 and len(kid) == 1, (gux[g], pair)
-            kind, v, vo, so, sev, siv = replay(uidx[k[g]], slots[par[0]])
+            if src_slots is None:
+                par = [j for j in pair if moved(j)]
+                kid = [j for j in pair if not moved(j)]
+                assert len(par) == 1 and len(kid) == 1, (gux[g], pair)
+                kind, v, vo, so, sev, siv = replay(uidx[k[g]], slots[par[0]])
+            else:
+                rs = src_slots[uidx[k[g]]]
+                kind, v, vo, so, sev, siv = replay(uidx[k[g]], rs)
+                assert so.tolist() == [rng1[w][rs] for w in range(4)]
+                gv = [np.array([gvx[j], gvy[j], gvz[j]], np.float64) for j in pair]
+                swap = (np.linalg.norm(gv[1] - vo) + np.linalg.norm(gv[0] - sev)
+                        < np.linalg.norm(gv[0] - vo) + np.linalg.norm(gv[1] - sev))
+                par, kid = ([pair[1]], [pair[0]]) if swap else ([pair[0]], [pair[1]])
             assert kind == 3
             tol = 1e-4 * max(np.linalg.norm(v), 2000.0)
             assert np.linalg.norm(np.array([gvx[par[0]], gvy[par[0]], gvz[par[0]]]) - vo) <= tol
