@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--ppt-inplace", type=int, default=0, help="particles per thread, in-place steps")
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--segments", type=int, default=0)
+    ap.add_argument("--lds-replicas", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick grid, particles per thread and replication with cudapic_amd.autotune "
+                         "(cached per device and workload; the search runs before the warm-up)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="particles per species of the CPU sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -172,9 +176,15 @@ def main():
     print(f"bench: device {arch}, {cu} CUs, wave{ws}; building state "
           f"{args.n_e} e + {args.n_i} i", file=sys.stderr, flush=True)
     E, I = synthetic.state(args.n_e, args.n_i, seed=1)
-    sim = ca.Sim(p, capacity=args.capacity, path=path, use_graph=not args.no_graph,
-                 grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
-                 ppt_inplace=args.ppt_inplace)
+    shape = dict(grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
+                 ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas)
+    tuned = None
+    if args.autotune and path == ca.PIC_PATH_FUSED:
+        from cudapic_amd import autotune
+        tuned = autotune.tune_sim(p, args.capacity, E, I)
+        shape = tuned["config"]
+        print(f"bench: autotuned {shape}", file=sys.stderr, flush=True)
+    sim = ca.Sim(p, capacity=args.capacity, path=path, use_graph=not args.no_graph, **shape)
     sim.upload(ca.PIC_ELECTRONS, *E)
     sim.upload(ca.PIC_IONS, *I)
     del E, I
@@ -228,7 +238,8 @@ def main():
                                "N_G=400, N_T=4000, N_SUB=20 (plain method at scale)",
                    "n_electrons_end": n_e, "n_ions_end": n_i, "path": args.path,
                    "graph": not args.no_graph, "poisson_and_density_flush": args.precision,
-                   "step_hbm_frac_80B": round(step_frac, 4)},
+                   "step_hbm_frac_80B": round(step_frac, 4),
+                   **({"autotuned": tuned["config"]} if tuned else {})},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

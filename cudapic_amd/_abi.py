@@ -49,7 +49,7 @@ class PicSimConfig(C.Structure):
         ("capacity", C.c_uint32), ("segments", C.c_uint32),
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
-                                    "replicas", "measure", "ppt_inplace")],
+                                    "replicas", "measure", "ppt_inplace", "lds_replicas")],
     ]
 
 
@@ -59,7 +59,7 @@ class PicEnsembleConfig(C.Structure):
         ("params", C.POINTER(PicParams)),
         ("capacity", C.POINTER(C.c_uint32)),
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
-        *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas")],
+        *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas", "lds_replicas")],
     ]
 
 

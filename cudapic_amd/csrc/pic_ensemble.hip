@@ -410,6 +410,7 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     sh.block = 256;
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
     sh.ppt_inplace = c->ppt > 0 ? c->ppt : 2;
+    sh.lds_reps = std::max(0, c->lds_replicas);
     // global density replicas: 1 or a multiple of 8 (read in groups of eight)
     sh.replicas = c->replicas > 0 ? c->replicas : 8;
     if (sh.replicas > 1) sh.replicas = std::min(64, (sh.replicas + 7) / 8 * 8);

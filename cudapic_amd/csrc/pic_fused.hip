@@ -754,6 +754,7 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
     const int list_words = j.inplace ? (j.electrons ? kCollFieldsE : kCollFieldsI) * (int)kCollPerWg : 0;
     const int fixed = (k.n_g + aux_words + list_words) * 4;
     a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (k.n_g * 4)));
+    if (sh.lds_reps > 0) a.lds_reps = std::min(a.lds_reps, sh.lds_reps);
     a.reps = sh.replicas;
     a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
     const size_t lds = (size_t)fixed + (size_t)a.lds_reps * k.n_g * 4;
@@ -789,6 +790,9 @@ constexpr int kMaxEnsemble = 256;
 struct EnsArgs {
     EnsStepJob j;
     int lds_reps, reps, rep_stride;
+    // tile order: 0 = each workgroup a contiguous range of tiles; c > 0 =
+    // chunks of c tiles dealt round-robin to the workgroups
+    int chunk;
 };
 
 template <bool ELECTRON, int PPT, typename Acc>
@@ -824,9 +828,14 @@ __global__ __launch_bounds__(256) void k_inplace_ens(EnsArgs A) {
     }
     __syncthreads();
     const uint32_t T = s_tiles[E];
-    const uint32_t t_begin = (uint32_t)((uint64_t)T * blockIdx.x / gridDim.x);
-    const uint32_t t_end = (uint32_t)((uint64_t)T * (blockIdx.x + 1) / gridDim.x);
+    const uint32_t C = (uint32_t)A.chunk;
+    const uint32_t t_begin = C ? blockIdx.x * C : (uint32_t)((uint64_t)T * blockIdx.x / gridDim.x);
+    const uint32_t t_end = C ? T : (uint32_t)((uint64_t)T * (blockIdx.x + 1) / gridDim.x);
     if (t_begin >= t_end) return;  // uniform over the workgroup
+    // the workgroup's next tile (tiles only grow)
+    auto next_tile = [&](uint32_t t) {
+        return (C == 0 || (t + 1) % C != 0) ? t + 1 : t + 1 + (gridDim.x - 1) * C;
+    };
 
     const int wave = threadIdx.x / kWave;
     uint32_t *h = hist + (wave % A.lds_reps) * n_g;
@@ -961,7 +970,7 @@ __global__ __launch_bounds__(256) void k_inplace_ens(EnsArgs A) {
             if (cur >= 0) finish();
             enter(ti);
         }
-        const uint32_t tile_next = tile + 1;
+        const uint32_t tile_next = next_tile(tile);
         const int ti_next = __builtin_amdgcn_readfirstlane(tile_next < t_end ? inst_of(tile_next, ti) : ti);
         bool valid[PPT];
 #pragma unroll
@@ -1202,8 +1211,10 @@ int launch_inplace_ens(const EnsStepJob &j, const Shape &sh0, hipStream_t s) {
     const int budget = std::min(device_lds_per_block(), device_lds_per_cu() / per_cu - 2048);
     const int fixed = (n_g + (j.electrons ? kCollFieldsE : kCollFieldsI) * (int)kCollPerWg) * 4;
     a.lds_reps = std::max(1, std::min(block / kWave, (budget - fixed) / (n_g * 4)));
+    if (sh.lds_reps > 0) a.lds_reps = std::min(a.lds_reps, sh.lds_reps);
     a.reps = sh.replicas;
     a.rep_stride = sh.rep_stride > 0 ? sh.rep_stride : n_g;
+    a.chunk = 0;
     const size_t lds = (size_t)fixed + (size_t)a.lds_reps * n_g * 4;
     const bool f64 = j.dens_f64;
     const int ppt = sh.ppt_inplace;

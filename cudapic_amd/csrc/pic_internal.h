@@ -40,6 +40,7 @@ struct Shape {
     int ppt_inplace = 2;  // particles per thread per tile (in-place kernel)
     int replicas = 1;  // global density replicas the flush spreads over
     int rep_stride = 0;
+    int lds_reps = 0;  // LDS histograms per workgroup at most (0 = one per wave as LDS allows)
 };
 Shape resolve(Shape s);
 

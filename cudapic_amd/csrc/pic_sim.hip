@@ -550,6 +550,7 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
     sh.ppt = c->ppt > 0 ? c->ppt : 4;
     sh.ppt_inplace = c->ppt_inplace > 0 ? c->ppt_inplace : 2;
+    sh.lds_reps = std::max(0, c->lds_replicas);
     // global density replicas: 1, or a multiple of 8 (the Poisson kernel
     // reads them in groups of eight)
     sh.replicas = c->replicas > 0 ? c->replicas : (c->path == PIC_PATH_FUSED ? 8 : 1);

@@ -14,8 +14,9 @@ W = 7e3 cannot shield a few hundred volts and runs away into an ionisation
 avalanche within a few hundred steps.  The same instances, from
 the same states, are run once as one ensemble (one Poisson launch and one
 particle launch per species per step for all of them, hipGraph replay) and
-once one after the other as single-discharge simulations (each on the fused
-path with its own hipGraph).  Both runs advance every instance by the same
+once one after the other as single-discharge simulations: on the fused fast
+path with its own hipGraph, and on the reference's kernel sequence (push,
+compaction, collisions and deposit as separate launches, main.rs:1725-1808).  Both runs advance every instance by the same
 number of steps, so their statistical errors are the same; the figure of
 merit is total particle-steps per second.  Prints one JSON line.
 """
@@ -44,6 +45,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=80)
     ap.add_argument("--weight", type=float, default=7.0e3, help="weight of a 1.2M-particle instance")
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--sequential-path", choices=["fused", "reference", "both"], default="both",
+                    help="single-discharge path of the sequential runs: the fused fast path, the "
+                         "reference's kernel sequence, or both")
+    ap.add_argument("--grid", type=int, default=0, help="particle-kernel workgroups (0 = 4 per compute unit)")
+    ap.add_argument("--ppt", type=int, default=0, help="particles per thread per tile (0 = 2)")
+    ap.add_argument("--replicas", type=int, default=0, help="global density replicas (0 = 8)")
     args = ap.parse_args()
     import cudapic_amd as ca
     from cudapic_amd import synthetic
@@ -56,7 +63,7 @@ def main():
     print(f"ens_bench: {args.instances} instances, {int(cnt.sum())} electrons in total "
           f"({int(cnt.min())}..{int(cnt.max())})", file=sys.stderr, flush=True)
 
-    ens = ca.Ensemble(ps, caps)
+    ens = ca.Ensemble(ps, caps, grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas)
     for i, (E, I) in enumerate(states):
         ens.upload(i, 0, *E)
         ens.upload(i, 1, *I)
@@ -70,11 +77,10 @@ def main():
     ens.close()
     print(f"ens_bench: ensemble {args.steps} steps in {ens_ms:.1f} ms", file=sys.stderr, flush=True)
 
-    seq = None
-    if not args.no_sequential:
+    def sequential(path):
         tot_ps, tot_ms = 0, 0.0
         for i, (E, I) in enumerate(states):
-            with ca.Sim(ps[i], capacity=caps[i]) as sim:
+            with ca.Sim(ps[i], capacity=caps[i], path=path) as sim:
                 sim.upload(0, *E)
                 sim.upload(1, *I)
                 sim.run(args.warmup)
@@ -83,9 +89,16 @@ def main():
                 sim.check()
             tot_ps += r["particle_steps"]
             tot_ms += r["ms_total"]
-        seq = {"value": tot_ps / (tot_ms * 1e-3), "ms_total": round(tot_ms, 2)}
-        print(f"ens_bench: sequential {args.steps} steps x {args.instances} in {tot_ms:.1f} ms",
-              file=sys.stderr, flush=True)
+        print(f"ens_bench: sequential ({'fused' if path == ca.PIC_PATH_FUSED else 'reference'} path) "
+              f"{args.steps} steps x {args.instances} in {tot_ms:.1f} ms", file=sys.stderr, flush=True)
+        return {"value": round(tot_ps / (tot_ms * 1e-3), 1), "ms_total": round(tot_ms, 2)}
+
+    seq, seq_ref = None, None
+    if not args.no_sequential:
+        if args.sequential_path in ("fused", "both"):
+            seq = sequential(ca.PIC_PATH_FUSED)
+        if args.sequential_path in ("reference", "both"):
+            seq_ref = sequential(ca.PIC_PATH_REFERENCE)
     out = {
         "metric": "ensemble particle-steps/s (batched vs sequential single-discharge runs)",
         "unit": "particle-steps/s",
@@ -95,6 +108,8 @@ def main():
         "ensemble": {"value": round(ens_rate, 1), "ms_total": round(ens_ms, 2)},
         "sequential": seq,
         "speedup": round(ens_rate / seq["value"], 3) if seq else None,
+        "sequential_reference_path": seq_ref,
+        "speedup_vs_reference_path": round(ens_rate / seq_ref["value"], 3) if seq_ref else None,
         "config": {"v0": "U[100,400] V", "pressure": "logU[3,30] Pa",
                    "weight": f"{args.weight} x 1.2e6 / count", "n_g": 400,
                    "counts": "logU[5e4,2e6] per species per instance", "precision": "fp64 Poisson"},

@@ -193,6 +193,9 @@ typedef struct pic_sim_config {
     int32_t measure;           /* allocate measurement buffers */
     int32_t ppt_inplace;       /* particles per thread per tile, in-place
                                   steps: 0 = default (2) */
+    int32_t lds_replicas;      /* fused path: at most this many LDS density
+                                  histograms per workgroup (shared by its
+                                  waves): 0 = one per wave as LDS allows */
 } pic_sim_config_t;
 
 typedef struct pic_sim pic_sim_t;
@@ -284,8 +287,8 @@ typedef struct pic_ensemble_config {
     /* use_graph: replay 4*N_SUB steps from a hipGraph; grid_blocks (0 = 4
      * per compute unit), ppt (particles per thread per tile, 1, 2 or 4; 0 =
      * 2) and replicas (global density replicas, 0 = 8) shape the particle
-     * kernels */
-    int32_t use_graph, grid_blocks, ppt, replicas;
+     * kernels; lds_replicas as in pic_sim_config_t */
+    int32_t use_graph, grid_blocks, ppt, replicas, lds_replicas;
 } pic_ensemble_config_t;
 
 typedef struct pic_ensemble pic_ensemble_t;
