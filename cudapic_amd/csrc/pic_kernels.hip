@@ -141,13 +141,24 @@ __global__ __launch_bounds__(256) void k_deposit(const float *__restrict__ x,
                                                  uint32_t n_cap, Acc *__restrict__ dens, int reps,
                                                  int rep_stride, float inv_dx, float wf, int n_g,
                                                  int lds_reps) {
-    extern __shared__ float sh[];
+    // Fixed-point LDS histograms (integer LDS atomics run at full rate,
+    // float ones at a fraction of it): a particle adds exactly `scale` units
+    // split by its CIC weights.  The scale is the largest power of two (at
+    // most 2^20) for which a replica cannot overflow: its waves deposit at
+    // most iters * 4 * 64 particles each.
+    extern __shared__ uint32_t hs[];
     const int wave = threadIdx.x / kWave;
-    for (int k = threadIdx.x; k < lds_reps * n_g; k += blockDim.x) sh[k] = 0.0f;
+    for (int k = threadIdx.x; k < lds_reps * n_g; k += blockDim.x) hs[k] = 0u;
     __syncthreads();
-    float *h = sh + (wave % lds_reps) * n_g;
+    uint32_t *h = hs + (wave % lds_reps) * n_g;
     const uint32_t n = min(*n_active, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint64_t iters = ((uint64_t)n + 4ull * stride - 1) / (4ull * stride);
+    const uint64_t waves_per_rep = (blockDim.x / kWave + lds_reps - 1) / lds_reps;
+    const uint64_t bound = iters * 4u * kWave * waves_per_rep + 1u;
+    int sh_bits = 20;
+    while (sh_bits > 0 && (bound << sh_bits) >= (1ull << 32)) --sh_bits;
+    const float scale = (float)(1u << sh_bits);
     bool any = false;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 4 * stride) {
         float xv[4];
@@ -162,19 +173,21 @@ __global__ __launch_bounds__(256) void k_deposit(const float *__restrict__ x,
                 const float pos = xv[u] * inv_dx;
                 const int q = cell_of(pos, n_g);
                 const float rem = pos - (float)q;
-                atomicAdd(&h[q], (1.0f - rem) * wf);
-                atomicAdd(&h[q + 1], rem * wf);
+                const uint32_t u2 = (uint32_t)(rem * scale + 0.5f);
+                atomicAdd(&h[q], (uint32_t)scale - u2);
+                atomicAdd(&h[q + 1], u2);
             }
         }
         any = true;
     }
     if (!__syncthreads_or(any)) return;
     Acc *out = dens + (blockIdx.x % reps) * rep_stride;
+    const Acc unit = (Acc)wf / (Acc)scale;
     for (int k = threadIdx.x; k < n_g; k += blockDim.x) {
-        float s = 0.0f;
-        for (int r = 0; r < lds_reps; ++r) s += sh[r * n_g + k];
-        if (s != 0.0f) {
-            Acc v = (Acc)s;
+        uint64_t s = 0;
+        for (int r = 0; r < lds_reps; ++r) s += hs[r * n_g + k];
+        if (s) {
+            Acc v = (Acc)s * unit;
             if (k == 0 || k == n_g - 1) v *= (Acc)2;
             unsafeAtomicAdd(&out[k], v);
         }
@@ -186,9 +199,12 @@ int launch_deposit(const float *x, const uint32_t *n_active, void *dens, bool f6
     Shape sh = resolve(sh0);
     const int block = 256;
     const int lr = lds_replicas(block, k.n_g, 0);
-    const size_t lds = (size_t)lr * k.n_g * sizeof(float);
+    const size_t lds = (size_t)lr * k.n_g * sizeof(uint32_t);
     const int stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
-    const int grid = std::max(1, sh.grid * sh.block / block / 2);
+    // one workgroup per compute unit: each flushes n_g atomics into the
+    // (possibly single) global density, so fewer workgroups means less
+    // same-address contention at the flush
+    const int grid = std::max(1, std::min(device_cus(), sh.grid * sh.block / block / 2));
     if (f64)
         k_deposit<double><<<grid, block, lds, s>>>(x, n_active, n_cap, (double *)dens, sh.replicas, stride,
                                                    k.inv_dx, k.wf, k.n_g, lr);
@@ -872,6 +888,21 @@ __device__ __forceinline__ uint32_t tile_reserve(uint32_t wave_count, uint32_t *
 // tile makes one global atomic.  The efield array is staged in LDS.
 // Lost particles are counted per electrode (one atomic per workgroup) and,
 // for ions, binned by energy (IFED, C/eduPIC.cc:625-667).
+// Sum v over the workgroup and add the total to *dst with one atomic: the
+// counters of these kernels are single words, and thousands of waves adding
+// to one address serialise in the memory-side atomic unit.  Every thread of
+// the workgroup calls it; only thread 0 reads the LDS sum, so consecutive
+// calls need no barrier between them.
+__device__ void wg_atomic_add(unsigned long long *dst, uint32_t v) {
+    __shared__ uint32_t s_sum;
+    if (threadIdx.x == 0) s_sum = 0u;
+    __syncthreads();
+    const uint32_t w = wave_sum(v);
+    if (lane_id() == 0 && w) atomicAdd(&s_sum, w);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(dst, (unsigned long long)s_sum);
+}
+
 template <int PPT>
 __global__ __launch_bounds__(256) void k_move_compact(MoveArgs a) {
     extern __shared__ float ef[];  // n_g
@@ -938,11 +969,8 @@ __global__ __launch_bounds__(256) void k_move_compact(MoveArgs a) {
         }
     }
     if (a.abs_counts) {
-        const uint32_t wp = wave_sum(lost_pow), wg = wave_sum(lost_gnd);
-        if (lane_id() == 0) {
-            if (wp) atomicAdd(&a.abs_counts[0], (unsigned long long)wp);
-            if (wg) atomicAdd(&a.abs_counts[1], (unsigned long long)wg);
-        }
+        wg_atomic_add(&a.abs_counts[0], lost_pow);
+        wg_atomic_add(&a.abs_counts[1], lost_gnd);
     }
     if (do_ifed) {
         __syncthreads();
@@ -1060,10 +1088,7 @@ __global__ __launch_bounds__(256) void k_collide_e(CollEArgs a) {
             }
         }
     }
-    if (a.n_coll) {
-        const uint32_t c = wave_sum(ncoll);
-        if (lane == 0 && c) atomicAdd(a.n_coll, (unsigned long long)c);
-    }
+    if (a.n_coll) wg_atomic_add(a.n_coll, ncoll);
 }
 
 int launch_collide_e(const CollideEJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
@@ -1125,10 +1150,7 @@ __global__ __launch_bounds__(256) void k_collide_i(CollIArgs a) {
             ++ncoll;
         }
     }
-    if (a.n_coll) {
-        const uint32_t c = wave_sum(ncoll);
-        if (lane_id() == 0 && c) atomicAdd(a.n_coll, (unsigned long long)c);
-    }
+    if (a.n_coll) wg_atomic_add(a.n_coll, ncoll);
 }
 
 int launch_collide_i(const CollideIJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {

@@ -548,7 +548,9 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     // the fused kernels hold ~110 VGPRs (4 waves per SIMD): a persistent grid
     // of 4 workgroups of 256 lanes per compute unit is fully resident
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
-    sh.ppt = c->ppt > 0 ? c->ppt : 4;
+    // compaction reserves output slots with one atomic per tile on a single
+    // counter on the reference path: 8 particles per lane halves those
+    sh.ppt = c->ppt > 0 ? c->ppt : (c->path == PIC_PATH_FUSED ? 4 : 8);
     sh.ppt_inplace = c->ppt_inplace > 0 ? c->ppt_inplace : 2;
     sh.lds_reps = std::max(0, c->lds_replicas);
     // global density replicas: 1, or a multiple of 8 (the Poisson kernel

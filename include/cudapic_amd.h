@@ -188,7 +188,7 @@ typedef struct pic_sim_config {
     int32_t block_size;        /* particle kernels: 0 = tuned default */
     int32_t grid_blocks;       /* persistent grid: 0 = tuned default */
     int32_t ppt;               /* particles per thread per tile, compacting
-                                  steps: 0 = default (4) */
+                                  steps: 0 = default (4 fused, 8 reference) */
     int32_t replicas;          /* global density replicas: 0 = default */
     int32_t measure;           /* allocate measurement buffers */
     int32_t ppt_inplace;       /* particles per thread per tile, in-place
