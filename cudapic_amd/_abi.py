@@ -63,6 +63,14 @@ class PicEnsembleConfig(C.Structure):
     ]
 
 
+class PicAdjConfig(C.Structure):
+    _fields_ = [
+        ("params", PicParams),
+        *[(n, C.c_uint32) for n in ("capacity_e", "capacity_i", "steps", "checkpoint", "t0", "tape_capacity")],
+        ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
+    ]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 U64 = C.c_uint64
@@ -124,6 +132,16 @@ SIGNATURES = {
     "pic_ens_time_steps": (I32, [P, U32, I32, C.POINTER(F32), C.POINTER(C.c_ulonglong)]),
     "pic_ens_read_grid": (I32, [P, I32, P, P]),
     "pic_ens_read_measurements": (I32, [P, I32, P, P, P]),
+    "pic_adj_create": (I32, [C.POINTER(PicAdjConfig), C.POINTER(P)]),
+    "pic_adj_destroy": (I32, [P]),
+    "pic_adj_upload": (I32, [P, I32, P, P, P, P, U32]),
+    "pic_adj_set_target": (I32, [P, P]),
+    "pic_adj_record": (I32, [P, C.POINTER(C.c_double)]),
+    "pic_adj_forward": (I32, [P, P, P, C.POINTER(C.c_double), P]),
+    "pic_adj_backward": (I32, [P, P, P, P]),
+    "pic_adj_tape_sizes": (I32, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
+                                 C.POINTER(U32)]),
+    "pic_adj_download_tape": (I32, [P] + [P] * 16),
     "pic_malloc": (I32, [C.POINTER(P), U64]),
     "pic_free": (I32, [P]),
     "pic_memcpy_htod": (I32, [P, P, U64]),

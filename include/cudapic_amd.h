@@ -322,6 +322,63 @@ int pic_ens_read_grid(pic_ensemble_t *e, int inst, float *pot, float *efield);
 int pic_ens_read_measurements(pic_ensemble_t *e, int inst, double *cumul_e, double *cumul_i,
                               unsigned long long *counters);
 
+/* ------------------------------------------------------------------------
+ * Adjoint: gradients of the window-averaged ion density misfit
+ * ---------------------------------------------------------------------- */
+
+/* No reference counterpart: the reference has no differentiation.  A record
+ * pass runs the RF step (the reference's stage order) over a window of
+ * `steps` steps and tapes its discrete events (collisions with their
+ * post-collision v_x, absorptions, ionisations with the newborns' slots and
+ * v_x); later passes replay the tape, so x and v_x are differentiable
+ * functions of the waveform pot0(t) = sum_h wave_cos[h] cos(2 pi (h+1) t/N_T)
+ * + wave_sin[h] sin(...).  Objective J = sum_k (nbar_k - target_k)^2 /
+ * sum_k target_k^2, nbar the ion density averaged over the window's ion
+ * steps.  The backward pass restores a checkpoint every `checkpoint` steps
+ * and recomputes the segment before its adjoint steps (push, deposit, field
+ * solve).  Densities are accumulated in 64-bit fixed point, so every pass is
+ * deterministic. */
+typedef struct pic_adj_config {
+    pic_params_t params;        /* derived; the waveform is wave_cos / wave_sin */
+    uint32_t capacity_e, capacity_i;
+    uint32_t steps;             /* window length (a multiple of n_sub) */
+    uint32_t checkpoint;        /* steps between checkpoints (multiple of n_sub; 0 = 200) */
+    uint32_t t0;                /* step of the RF cycle the window starts at (multiple of n_sub) */
+    uint32_t tape_capacity;     /* events per kind (0 = steps * capacity_e / 64) */
+    uint32_t seed_e[4], seed_i[4];
+} pic_adj_config_t;
+
+typedef struct pic_adjoint pic_adjoint_t;
+
+int pic_adj_create(const pic_adj_config_t *c, pic_adjoint_t **out);
+int pic_adj_destroy(pic_adjoint_t *a);
+/* Starting state of the window (before pic_adj_record). */
+int pic_adj_upload(pic_adjoint_t *a, int species, const float *x, const float *vx, const float *vy,
+                   const float *vz, uint32_t n);
+/* Target ion density profile (n_g values, m^-3). */
+int pic_adj_set_target(pic_adjoint_t *a, const double *n_i_target);
+/* The record pass (once): Monte Carlo collisions, event tape, *J. */
+int pic_adj_record(pic_adjoint_t *a, double *J);
+/* A frozen-event forward pass with another waveform (NULL keeps the
+ * current coefficients); *J and nbar[n_g] (either may be NULL). */
+int pic_adj_forward(pic_adjoint_t *a, const double *wave_cos, const double *wave_sin, double *J,
+                    double *nbar);
+/* dJ/dwave_cos[16], dJ/dwave_sin[16] and dJ/dpot0(t) for every window step
+ * (any may be NULL) at the waveform of the last forward or record pass. */
+int pic_adj_backward(pic_adjoint_t *a, double *grad_cos, double *grad_sin, double *dJ_dV);
+/* Event counts of the tape and the slots in use at the window's end. */
+int pic_adj_tape_sizes(pic_adjoint_t *a, uint32_t *n_coll_e, uint32_t *n_coll_i, uint32_t *n_ionisations,
+                       uint32_t *slots_e, uint32_t *slots_i);
+/* The tape: per-step offsets [steps+1] into the collision lists (slot, v_x)
+ * of each species and into the ionisation list (parent, new electron slot,
+ * new ion slot, their v_x), the slot watermarks at the start of each step
+ * [steps+1] and every slot's death step (step + 1 of its absorption, or
+ * 0xffffffff).  Any pointer may be NULL. */
+int pic_adj_download_tape(pic_adjoint_t *a, uint32_t *off_e, uint32_t *slot_e, float *v_e, uint32_t *off_i,
+                          uint32_t *slot_i, float *v_i, uint32_t *off_ion, uint32_t *parent, uint32_t *new_e,
+                          uint32_t *new_i, float *v_new_e, float *v_new_i, uint32_t *slots_e, uint32_t *slots_i,
+                          uint32_t *death_e, uint32_t *death_i);
+
 /* Device memory helpers for callers without their own allocator. */
 int pic_malloc(void **ptr, uint64_t bytes);
 int pic_free(void *ptr);
