@@ -106,6 +106,52 @@ __device__ __forceinline__ void flush_fix64(const unsigned long long *h, int n_g
         if (h[q]) atomicAdd(&out[q], h[q]);
 }
 
+// Inclusive prefix sum over the 256 lanes of the workgroup (tmp: 512
+// doubles of LDS).  Every lane calls it.
+__device__ double block_scan256(double v, double *tmp) {
+    double *a = tmp, *b = tmp + 256;
+    const int tid = threadIdx.x;
+    a[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        b[tid] = a[tid] + (tid >= off ? a[tid - off] : 0.0);
+        __syncthreads();
+        double *c = a;
+        a = b;
+        b = c;
+    }
+    return a[tid];
+}
+
+// Solves tridiag(1, -2, 1) y = f on the interior nodes 1..n_g-2 (f and y in
+// LDS, y may alias f) by the prefix-sum form of the Thomas recurrences
+// (the engine's Poisson scan): G_i = sum_{j<=i} j f_j, r_i = -G_i / (i (i+1)),
+// y_i = i sum_{j>=i} r_j.  Each lane takes a contiguous run of nodes.
+// tmp: 512 doubles.  Every lane of a 256-lane workgroup calls it.
+__device__ void scan_solve(const double *f, double *y, int n_g, double *tmp) {
+    const int m = n_g - 2, per = (m + 255) / 256;
+    const int i0 = 1 + threadIdx.x * per, i1 = min(i0 + per, n_g - 1);
+    double loc = 0.0;
+    for (int i = i0; i < i1; ++i) loc += (double)i * f[i];
+    const double excl = block_scan256(loc, tmp) - loc;
+    double r[16];
+    double G = excl, rs = 0.0;
+    for (int i = i0, u = 0; i < i1; ++i, ++u) {
+        G += (double)i * f[i];
+        r[u] = -G / ((double)i * (double)(i + 1));
+        rs += r[u];
+    }
+    __syncthreads();  // f may be overwritten below (y aliases f)
+    const double incl = block_scan256(rs, tmp);
+    const double total = tmp[255];  // the scan's last round leaves the total in lane 255's slot
+    double suffix = total - (incl - rs);  // sum_{j >= i0} r_j
+    for (int i = i0, u = 0; i < i1; ++i, ++u) {
+        y[i] = (double)i * suffix;
+        suffix -= r[u];
+    }
+    __syncthreads();
+}
+
 struct PushArgs {
     bool record, electrons, store_traj;
     uint32_t t;                  // window step
@@ -283,18 +329,26 @@ struct FieldArgs {
 // (Thomas, FP64, one thread) and the field; the record pass also stamps the
 // slot watermarks and tape offsets of the step.
 __global__ __launch_bounds__(256) void k_adj_field(FieldArgs a) {
-    extern __shared__ double fs[];  // pot, aux
-    double *pot = fs, *w = fs + a.n_g;
+    extern __shared__ double fs[];  // pot, rho, scan scratch
     const int n_g = a.n_g;
+    double *pot = fs, *rho = fs + n_g, *tmp = fs + 2 * n_g;
+    const double alpha = -a.dx * a.dx / kEpsilon0;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
         const double f = (q == 0 || q == n_g - 1) ? 2.0 : 1.0;
-        a.n_e[q] = (double)a.acc_e[q] * a.wf_unit * f;
+        const double ne = (double)a.acc_e[q] * a.wf_unit * f;
+        a.n_e[q] = ne;
         a.acc_e[q] = 0ull;
+        double ni;
         if (a.ion_step) {
-            a.n_i[q] = (double)a.acc_i[q] * a.wf_unit * f;
+            ni = (double)a.acc_i[q] * a.wf_unit * f;
+            a.n_i[q] = ni;
             a.acc_i[q] = 0ull;
-            if (a.accumulate) a.nbar[q] += a.n_i[q];
+            if (a.accumulate) a.nbar[q] += ni;
+        } else {
+            ni = a.n_i[q];
         }
+        rho[q] = kECharge * (ni - ne);
+        pot[q] = alpha * rho[q] - (q == 1 ? a.v : 0.0);  // right-hand side f_q (interior nodes)
     }
     if (a.record && threadIdx.x == 0) {
         a.n_slots_e[a.t] = a.cnt[AC_SLOTS_E];
@@ -304,30 +358,17 @@ __global__ __launch_bounds__(256) void k_adj_field(FieldArgs a) {
         a.tape.off_ion[a.t] = a.cnt[AC_ION];
     }
     __syncthreads();
+    scan_solve(pot, pot, n_g, tmp);
     if (threadIdx.x == 0) {
-        const double alpha = -a.dx * a.dx / kEpsilon0;
-        // Thomas for A = C = 1, B = -2 on nodes 1..n_g-2 (eduPIC's solver)
         pot[0] = a.v;
         pot[n_g - 1] = 0.0;
-        double wp = 0.0, gp = 0.0;
-        for (int i = 1; i <= n_g - 2; ++i) {
-            double f = alpha * kECharge * (a.n_i[i] - a.n_e[i]);
-            if (i == 1) f -= pot[0];
-            const double den = (i == 1) ? -2.0 : (-2.0 - wp);
-            w[i] = 1.0 / den;
-            gp = (i == 1) ? f / den : (f - gp) / den;
-            wp = w[i];
-            pot[i] = gp;  // g_i for now
-        }
-        for (int i = n_g - 3; i >= 1; --i) pot[i] = pot[i] - w[i] * pot[i + 1];
     }
     __syncthreads();
     const double s = 0.5 / a.dx;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
         double e;
-        if (q == 0) e = (pot[0] - pot[1]) / a.dx - kECharge * (a.n_i[0] - a.n_e[0]) * a.dx / (2.0 * kEpsilon0);
-        else if (q == n_g - 1)
-            e = (pot[n_g - 2] - pot[n_g - 1]) / a.dx + kECharge * (a.n_i[q] - a.n_e[q]) * a.dx / (2.0 * kEpsilon0);
+        if (q == 0) e = (pot[0] - pot[1]) / a.dx - rho[0] * a.dx / (2.0 * kEpsilon0);
+        else if (q == n_g - 1) e = (pot[n_g - 2] - pot[n_g - 1]) / a.dx + rho[q] * a.dx / (2.0 * kEpsilon0);
         else e = (pot[q - 1] - pot[q + 1]) * s;
         a.efield[q] = (float)e;
         if (a.ehist) a.ehist[q] = (float)e;
@@ -411,45 +452,35 @@ struct BackFieldArgs {
 __global__ __launch_bounds__(256) void k_adj_field_back(BackFieldArgs a) {
     extern __shared__ double bs[];
     const int n_g = a.n_g;
-    double *lphi = bs, *lrho = bs + n_g, *w = bs + 2 * n_g;
+    double *lphi = bs, *lrho = bs + n_g, *le = bs + 2 * n_g, *tmp = bs + 3 * n_g;
+    for (int q = threadIdx.x; q < n_g; q += blockDim.x) le[q] = a.lE[q];
+    __syncthreads();
+    // lambda_phi = D^T lambda_E (interior central differences, one-sided
+    // boundary formulas) and the boundary charges' part of lambda_rho
+    const double s = 0.5 / a.dx;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
-        lphi[q] = 0.0;
-        lrho[q] = 0.0;
+        double v = 0.0;
+        if (q + 1 <= n_g - 2) v += le[q + 1] * s;
+        if (q - 1 >= 1) v -= le[q - 1] * s;
+        if (q == 0) v += le[0] / a.dx;
+        if (q == 1) v -= le[0] / a.dx;
+        if (q == n_g - 2) v += le[n_g - 1] / a.dx;
+        if (q == n_g - 1) v -= le[n_g - 1] / a.dx;
+        lphi[q] = v;
+        lrho[q] = q == 0 ? -le[0] * a.dx / (2.0 * kEpsilon0)
+                         : (q == n_g - 1 ? le[n_g - 1] * a.dx / (2.0 * kEpsilon0) : lphi[q]);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double *lE = a.lE;
-        const double s = 0.5 / a.dx;
-        for (int i = 1; i <= n_g - 2; ++i) {
-            lphi[i - 1] += lE[i] * s;
-            lphi[i + 1] -= lE[i] * s;
-        }
-        lphi[0] += lE[0] / a.dx;
-        lphi[1] -= lE[0] / a.dx;
-        lrho[0] = -lE[0] * a.dx / (2.0 * kEpsilon0);
-        lphi[n_g - 2] += lE[n_g - 1] / a.dx;
-        lphi[n_g - 1] -= lE[n_g - 1] / a.dx;
-        lrho[n_g - 1] = lE[n_g - 1] * a.dx / (2.0 * kEpsilon0);
-        // u = A^-1 lphi_interior (A symmetric): the forward's Thomas with f = lphi
-        double wp = 0.0, gp = 0.0;
-        for (int i = 1; i <= n_g - 2; ++i) {
-            const double den = (i == 1) ? -2.0 : (-2.0 - wp);
-            w[i] = 1.0 / den;
-            gp = (i == 1) ? lphi[i] / den : (lphi[i] - gp) / den;
-            wp = w[i];
-            lrho[i] = gp;  // g_i for now
-        }
-        for (int i = n_g - 3; i >= 1; --i) lrho[i] = lrho[i] - w[i] * lrho[i + 1];
-        const double alpha = -a.dx * a.dx / kEpsilon0;
-        const double u1 = lrho[1];
-        for (int i = 1; i <= n_g - 2; ++i) lrho[i] *= alpha;
-        // phi_0 = V enters directly and through f_1 -= V
-        a.dJdV[a.t] = lphi[0] - u1;
-    }
+    // u = A^-1 lambda_phi on the interior (A symmetric): the forward's solve
+    // with f = lambda_phi
+    scan_solve(lrho, lrho, n_g, tmp);
+    // phi_0 = V enters directly and through f_1 -= V
+    if (threadIdx.x == 0) a.dJdV[a.t] = lphi[0] - lrho[1];
     __syncthreads();
+    const double alpha = -a.dx * a.dx / kEpsilon0;
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) {
         const double f = ((q == 0 || q == n_g - 1) ? 2.0 : 1.0) * a.wf;
-        const double lr = lrho[q] * kECharge;  // rho = e (n_i - n_e)
+        const double lr = (q == 0 || q == n_g - 1 ? lrho[q] : alpha * lrho[q]) * kECharge;  // rho = e (n_i - n_e)
         a.ge[q] = -lr * f;
         double lh = a.ln_held[q] + lr;
         if (a.ion_step) {
@@ -569,7 +600,7 @@ int fwd_step(pic_adjoint *a, uint32_t t, bool record, bool accumulate, uint32_t 
     f.n_slots_e = a->n_slots_e;
     f.n_slots_i = a->n_slots_i;
     f.tape = a->tape;
-    k_adj_field<<<1, 256, 2 * n_g * sizeof(double), st>>>(f);
+    k_adj_field<<<1, 256, (2 * n_g + 512) * sizeof(double), st>>>(f);
     PIC_LAUNCHED("k_adj_field");
     const size_t lds = ((n_g * 4 + 15) / 16) * 16 + 2 * (size_t)n_g * 8;
     PushArgs pa{};
@@ -802,7 +833,7 @@ int backward(pic_adjoint *a, double *grad_cos, double *grad_sin, double *dJdV_ou
             k_adj_push_back<<<a->grid, 256, n_g * 4, st>>>(be);
             PIC_LAUNCHED("k_adj_push_back");
             BackFieldArgs bf{t, ion, n_g, a->p.dx, (double)a->k.wf, a->lE, a->ln_held, a->lobj, a->ge, a->gi, a->dJdV};
-            k_adj_field_back<<<1, 256, 3 * n_g * sizeof(double), st>>>(bf);
+            k_adj_field_back<<<1, 256, (3 * n_g + 512) * sizeof(double), st>>>(bf);
             PIC_LAUNCHED("k_adj_field_back");
             BackDepArgs bd{t, be.traj, a->lxe, a->death_e, a->n_slots_e, a->cap_e, a->ge, a->k.inv_dx, n_g};
             k_adj_deposit_back<<<a->grid, 256, n_g * 4, st>>>(bd);
@@ -858,8 +889,8 @@ int pic_adj_create(const pic_adj_config_t *c, pic_adjoint_t **out) {
     }
     const uint32_t n_sub = (uint32_t)c->params.n_sub;
     const uint32_t K = c->checkpoint ? c->checkpoint : std::min<uint32_t>(c->steps, 200u);
-    if (c->steps % n_sub || K % n_sub || c->t0 % n_sub || c->params.n_g < 3 || c->params.n_g > 4096) {
-        set_error("pic_adj_create: steps, checkpoint interval and t0 must be multiples of n_sub");
+    if (c->steps % n_sub || K % n_sub || c->t0 % n_sub || c->params.n_g < 3 || c->params.n_g > 2048) {
+        set_error("pic_adj_create: steps, checkpoint interval and t0 must be multiples of n_sub, n_g at most 2048");
         return PIC_ERR_ARG;
     }
     pic_adjoint *a = new pic_adjoint();

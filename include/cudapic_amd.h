@@ -339,7 +339,7 @@ int pic_ens_read_measurements(pic_ensemble_t *e, int inst, double *cumul_e, doub
  * solve).  Densities are accumulated in 64-bit fixed point, so every pass is
  * deterministic. */
 typedef struct pic_adj_config {
-    pic_params_t params;        /* derived; the waveform is wave_cos / wave_sin */
+    pic_params_t params;        /* derived (n_g <= 2048); the waveform is wave_cos / wave_sin */
     uint32_t capacity_e, capacity_i;
     uint32_t steps;             /* window length (a multiple of n_sub) */
     uint32_t checkpoint;        /* steps between checkpoints (multiple of n_sub; 0 = 200) */
