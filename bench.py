@@ -16,6 +16,13 @@ over a second window, for the roofline.  The CPU baseline times the
 reference's own eduPIC (compiled from source into oracle/_ref) on one core
 over one RF cycle of the reference case (~1e5 particles per species).
 
+The metric also names the batched and the adjoint figures: after the main
+measurement the line gains an "ensemble" object (64 discharges in one
+launch per stage against the same 64 run one after the other on the fused
+path, cudapic_amd.ens_bench's workload) and an "adjoint" object (backward
+pass / frozen-event forward pass of the 4 M-particle discharge,
+cudapic_amd.adjoint_bench); --no-extras skips both.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -64,6 +71,11 @@ def parse():
                     help="pick grid, particles per thread and replication with cudapic_amd.autotune "
                          "(cached per device and workload; the search runs before the warm-up)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-ensemble and adjoint figures of the metric")
+    ap.add_argument("--adjoint-steps", type=int, default=800,
+                    help="window of the adjoint cost figure (the differentiation configuration "
+                         "uses 8000: python -m cudapic_amd.adjoint_bench)")
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="particles per species of the CPU sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the streaming-copy calibration kernel on MIB MiB, 20 times")
@@ -243,7 +255,58 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.no_extras:
+        sim.close()
+        out["ensemble"] = ensemble_figure()
+        out["adjoint"] = adjoint_figure(args.adjoint_steps)
     print(json.dumps(out), flush=True)
+
+
+def ensemble_figure():
+    """The batched half of the metric: 64 discharges (V0 ~ U[100,400] V,
+    p ~ logU[3,30] Pa, 50 k - 2 M particles each) in one launch per stage
+    against the same 64 run one after the other on the fused path."""
+    import cudapic_amd as ca
+    from cudapic_amd import synthetic
+    from cudapic_amd.ens_bench import instances
+    v0, pres, cnt = instances(64)
+    ps = [ca.params(v0=float(v), pressure=float(p), weight=7.0e3 * 1.2e6 / float(c), precision=ca.PIC_PREC_FP64)
+          for v, p, c in zip(v0, pres, cnt)]
+    caps = [int(c * 1.5) + 4096 for c in cnt]
+    states = [synthetic.state(int(c), int(c * 1.05), seed=100 + i) for i, c in enumerate(cnt)]
+    print("bench: ensemble of 64 discharges", file=sys.stderr, flush=True)
+    with ca.Ensemble(ps, caps) as ens:
+        for i, (E, I) in enumerate(states):
+            ens.upload(i, 0, *E)
+            ens.upload(i, 1, *I)
+        ens.run(80)
+        ens.synchronize()
+        t = ens.time_steps(400)
+        ens.check()
+    ens_rate = t["particle_steps"] / (t["ms_total"] * 1e-3)
+    tot_ps, tot_ms = 0, 0.0
+    for i, (E, I) in enumerate(states):
+        with ca.Sim(ps[i], capacity=caps[i]) as s:
+            s.upload(0, *E)
+            s.upload(1, *I)
+            s.run(80)
+            s.synchronize()
+            r = s.time_steps(400)
+        tot_ps += r["particle_steps"]
+        tot_ms += r["ms_total"]
+    seq = tot_ps / (tot_ms * 1e-3)
+    return {"instances": 64, "steps": 400, "value": round(ens_rate, 1), "sequential_fused": round(seq, 1),
+            "speedup": round(ens_rate / seq, 3), "unit": "particle-steps/s"}
+
+
+def adjoint_figure(steps):
+    """The adjoint half of the metric: backward pass / frozen-event forward
+    pass of the 4 M-particle discharge (cudapic_amd.adjoint_bench)."""
+    from cudapic_amd import adjoint_bench
+    print(f"bench: adjoint, {steps}-step window", file=sys.stderr, flush=True)
+    r = adjoint_bench.measure(steps=steps)
+    return {k: r[k] for k in ("steps", "checkpoint", "forward_ms_per_step", "backward_ms_per_step",
+                              "backward_over_forward", "gradient_over_forward", "replay_equals_record")}
 
 
 if __name__ == "__main__":
