@@ -224,8 +224,8 @@ def main():
         traffic = traffic_from_profiles()
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "electron particle kernel: k_inplace (39 of 40 steps) / k_fused compaction "
-                              "(every 2*N_SUB steps); push+collide+deposit, timed with HIP events on the sim stream",
+                    "kernel": "electron particle kernel: k_inplace / k_fused compaction (every 800 steps); "
+                              "push+collide+deposit, timed with HIP events on the sim stream",
                     "kernel_avg_us": round(avg_ms * 1e3, 2), "electrons_per_launch": int(units),
                     "bytes_per_unit": ALG_BYTES}
     sim.close()

@@ -50,6 +50,7 @@ class PicSimConfig(C.Structure):
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
                                     "replicas", "measure", "ppt_inplace", "lds_replicas")],
+        ("compact_every", C.c_uint32),
     ]
 
 
@@ -60,6 +61,7 @@ class PicEnsembleConfig(C.Structure):
         ("capacity", C.POINTER(C.c_uint32)),
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas", "lds_replicas")],
+        ("compact_every", C.c_uint32),
     ]
 
 

@@ -51,8 +51,16 @@ struct pic_ensemble {
     unsigned long long steps = 0;
     bool need_prologue = true;
     bool use_graph = true;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_roles[2][6] = {};
+    uint32_t compact_every = 800;  // steps between compaction passes (a multiple of 4*N_SUB)
+    // captured 4*N_SUB-step graphs, keyed by the measure flag, whether the
+    // period starts with a compaction, and the roles at its start; each
+    // remembers the roles it leaves behind
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        int key[8] = {};
+        int after[6] = {};
+    } graphs[8];
+    int graph_next = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -147,7 +155,7 @@ EnsCompactJob compact_job(pic_ensemble *s, bool electrons, float *const *src, fl
 
 // One ensemble step.  Particles are updated in place (lost ones leave
 // holes, newborn electrons and ions are appended after the watermarks);
-// every 2*N_SUB steps a compaction pass after the step packs each species
+// every compact_every steps a compaction pass after the step packs each species
 // into the other array of its pair.  Electron watermarks alternate between
 // two counter words every step: the step reads one, the Poisson kernel
 // copies it into the other, newborns are appended there.
@@ -155,7 +163,7 @@ int step(pic_ensemble *s, bool measure) {
     hipStream_t st = s->stream;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
-    const bool comp_e = (t % (2u * (uint32_t)s->p0.n_sub)) == 0;
+    const bool comp_e = (t % s->compact_every) == 0;
     const bool comp_i = ion && comp_e;
     const int in_e = e_word(s), out_e = EC_E0 + (1 - s->pe);
     EnsPoissonJob pj = poisson_job(s);
@@ -242,10 +250,10 @@ int step(pic_ensemble *s, bool measure) {
 }
 
 void invalidate_graphs(pic_ensemble *s) {
-    for (int m = 0; m < 2; ++m)
-        if (s->graph[m]) {
-            (void)hipGraphExecDestroy(s->graph[m]);
-            s->graph[m] = nullptr;
+    for (auto &g : s->graphs)
+        if (g.exec) {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
         }
 }
 
@@ -267,13 +275,20 @@ void set_roles(pic_ensemble *s, const int *r) {
     s->ac = r[5];
 }
 
-// Capture 4*N_SUB steps (every role returns to its start value after that
-// many) into a graph for this measure flag.
-int capture(pic_ensemble *s, bool measure) {
-    const int m = measure ? 1 : 0;
+// The graph of the 4*N_SUB steps starting now (t a multiple of 4*N_SUB),
+// from the cache or captured; a period holds at most one compaction (at its
+// first step).
+int graph_for(pic_ensemble *s, bool measure, pic_ensemble::GraphSlot **out) {
+    int key[8];
+    key[0] = measure ? 1 : 0;
+    key[1] = (s->t % s->compact_every) == 0 ? 1 : 0;
+    get_roles(s, key + 2);
+    for (auto &g : s->graphs)
+        if (g.exec && memcmp(g.key, key, sizeof(key)) == 0) {
+            *out = &g;
+            return PIC_OK;
+        }
     const int period = 4 * s->p0.n_sub;
-    int roles[6];
-    get_roles(s, roles);
     const uint32_t t0 = s->t;
     const unsigned long long st0 = s->steps;
     hipGraph_t g = nullptr;
@@ -281,10 +296,13 @@ int capture(pic_ensemble *s, bool measure) {
     int rc = PIC_OK;
     for (int i = 0; i < period && rc == PIC_OK; ++i) rc = step(s, measure);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
-    // capture does not execute: restore the host mirror
+    // capture does not execute: remember the roles the period leaves, then
+    // restore the host mirror
+    int after[6];
+    get_roles(s, after);
     s->t = t0;
     s->steps = st0;
-    set_roles(s, roles);
+    set_roles(s, key + 2);
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -294,8 +312,21 @@ int capture(pic_ensemble *s, bool measure) {
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-    s->graph[m] = ex;
-    memcpy(s->graph_roles[m], roles, sizeof(roles));
+    pic_ensemble::GraphSlot *slot = nullptr;
+    for (auto &gs : s->graphs)
+        if (!gs.exec) {
+            slot = &gs;
+            break;
+        }
+    if (!slot) {
+        slot = &s->graphs[s->graph_next];
+        s->graph_next = (s->graph_next + 1) % 8;
+        (void)hipGraphExecDestroy(slot->exec);
+    }
+    slot->exec = ex;
+    memcpy(slot->key, key, sizeof(key));
+    memcpy(slot->after, after, sizeof(after));
+    *out = slot;
     return PIC_OK;
 }
 
@@ -305,20 +336,13 @@ int run(pic_ensemble *s, uint32_t n, bool measure) {
     const bool graphs = s->use_graph && (uint32_t)s->p0.n_t % period == 0;
     while (n > 0) {
         if (graphs && s->t % period == 0 && n >= period) {
-            const int m = measure ? 1 : 0;
-            int roles[6];
-            get_roles(s, roles);
-            if (s->graph[m] && memcmp(roles, s->graph_roles[m], sizeof(roles)) != 0) {
-                (void)hipGraphExecDestroy(s->graph[m]);
-                s->graph[m] = nullptr;
-            }
-            if (!s->graph[m]) TRY(capture(s, measure));
-            while (n >= period) {
-                PIC_HIP(hipGraphLaunch(s->graph[m], s->stream));
-                s->t = (s->t + period) % (uint32_t)s->p0.n_t;
-                s->steps += period;
-                n -= period;
-            }
+            pic_ensemble::GraphSlot *g = nullptr;
+            TRY(graph_for(s, measure, &g));
+            PIC_HIP(hipGraphLaunch(g->exec, s->stream));
+            set_roles(s, g->after);
+            s->t = (s->t + period) % (uint32_t)s->p0.n_t;
+            s->steps += period;
+            n -= period;
             continue;
         }
         TRY(step(s, measure));
@@ -411,6 +435,15 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
     sh.ppt_inplace = c->ppt > 0 ? c->ppt : 2;
     sh.lds_reps = std::max(0, c->lds_replicas);
+    {
+        const uint32_t period = 4u * (uint32_t)s->p0.n_sub;
+        s->compact_every = c->compact_every ? c->compact_every : std::max(period, 800u / period * period);
+        if (s->compact_every % period) {
+            delete s;
+            set_error("pic_ens_create: compact_every must be a multiple of 4*n_sub");
+            return PIC_ERR_ARG;
+        }
+    }
     // global density replicas: 1 or a multiple of 8 (read in groups of eight)
     sh.replicas = c->replicas > 0 ? c->replicas : 8;
     if (sh.replicas > 1) sh.replicas = std::min(64, (sh.replicas + 7) / 8 * 8);

@@ -7,7 +7,7 @@
 // A species is stored in segments (each dense from its start, with its own
 // live count).  Most steps update the particles in place (k_inplace): a
 // particle stays in its slot and collides with that slot's RNG stream, a
-// particle that leaves the gap becomes a hole (x = -1).  Every 2*N_SUB steps
+// particle that leaves the gap becomes a hole (x = -1).  Every compact_every steps
 // the compacting kernel (k_fused) squeezes the holes out into the other
 // array of the pair; there, as in the reference (main.rs:1785-1791), a
 // survivor collides with the RNG stream of the slot it lands in.  Electrons
@@ -774,7 +774,7 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
 // ============================================================ ensembles
 // The in-place particle step for an ensemble of independent discharges in
 // one persistent launch, and the compaction pass that squeezes out holes
-// every 2*N_SUB steps.  Every instance's slots are cut into tiles; a
+// every compact_every steps.  Every instance's slots are cut into tiles; a
 // per-instance tile prefix (in LDS) maps a global tile to its instance.
 // Each workgroup takes a contiguous range of global tiles, so it meets few
 // instance boundaries.  At a boundary it finishes the instance it leaves --

@@ -83,8 +83,17 @@ struct pic_sim {
     unsigned long long steps = 0;
     bool need_prologue = true;
     unsigned long long ref_meas[2] = {0, 0};
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_roles[2][5] = {};
+    // fused path: steps between compactions (a multiple of 4*N_SUB)
+    uint32_t compact_every = 800;
+    // captured 4*N_SUB-step graphs, keyed by the measure flag, whether the
+    // period starts with a compaction, and the roles at its start; each
+    // remembers the roles it leaves behind
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        int key[7] = {};
+        int after[5] = {};
+    } graphs[8];
+    int graph_next = 0;  // slot the next capture replaces when all are taken
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -271,7 +280,7 @@ int fused_prologue(pic_sim *s) {
     return PIC_OK;
 }
 
-// One fused step.  Every 2*N_SUB steps the particle kernels compact (src ->
+// One fused step.  Every compact_every steps the particle kernels compact (src ->
 // dst arrays, holes squeezed out, survivors take the RNG stream of their new
 // slot); the other steps update the particles in place.  The electron live
 // counters are double-buffered every step (the in-place kernel appends
@@ -280,7 +289,7 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     hipStream_t st = s->stream;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p.n_sub) == 0;
-    const bool comp_e = (t % (2u * (uint32_t)s->p.n_sub)) == 0;
+    const bool comp_e = (t % s->compact_every) == 0;
     const bool comp_i = ion && comp_e;
     // Poisson of this step: e density accumulated by the previous step,
     // ion density = the held one, or on ion steps the accumulated one
@@ -389,10 +398,10 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
 }
 
 void invalidate_graphs(pic_sim *s) {
-    for (int m = 0; m < 2; ++m)
-        if (s->graph[m]) {
-            (void)hipGraphExecDestroy(s->graph[m]);
-            s->graph[m] = nullptr;
+    for (auto &g : s->graphs)
+        if (g.exec) {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
         }
 }
 
@@ -405,27 +414,52 @@ int one_step(pic_sim *s, bool measure) {
     return PIC_OK;
 }
 
-// Capture 4*N_SUB fused steps (every role flag returns to its start value
-// after that many steps) into a graph for this measure flag.
-int capture_graph(pic_sim *s, bool measure) {
-    const int m = measure ? 1 : 0;
+void get_roles(const pic_sim *s, int *r) {
+    r[0] = s->ce;
+    r[1] = s->ci;
+    r[2] = s->cd;
+    r[3] = s->hi;
+    r[4] = s->pe;
+}
+
+void set_roles(pic_sim *s, const int *r) {
+    s->ce = r[0];
+    s->ci = r[1];
+    s->cd = r[2];
+    s->hi = r[3];
+    s->pe = r[4];
+}
+
+// The graph of the 4*N_SUB fused steps starting now (t a multiple of
+// 4*N_SUB): found in the cache by (measure, compaction at the first step,
+// roles) or captured.  A period holds at most one compaction (at its first
+// step), so the density, counter and ion-hold parities return to their
+// start values and only the array parities can differ at its end.
+int graph_for(pic_sim *s, bool measure, pic_sim::GraphSlot **out) {
+    int key[7];
+    key[0] = measure ? 1 : 0;
+    key[1] = (s->t % s->compact_every) == 0 ? 1 : 0;
+    get_roles(s, key + 2);
+    for (auto &g : s->graphs)
+        if (g.exec && memcmp(g.key, key, sizeof(key)) == 0) {
+            *out = &g;
+            return PIC_OK;
+        }
     const int period = 4 * s->p.n_sub;
-    int roles[5] = {s->ce, s->ci, s->cd, s->hi, s->pe};
-    uint32_t t0 = s->t;
-    unsigned long long steps0 = s->steps;
+    const uint32_t t0 = s->t;
+    const unsigned long long steps0 = s->steps;
     hipGraph_t g = nullptr;
     PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = PIC_OK;
     for (int i = 0; i < period && rc == PIC_OK; ++i) rc = one_step(s, measure);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
-    // capture does not execute: restore the host mirror
+    // capture does not execute: remember the roles the period leaves, then
+    // restore the host mirror
+    int after[5];
+    get_roles(s, after);
     s->t = t0;
     s->steps = steps0;
-    s->ce = roles[0];
-    s->ci = roles[1];
-    s->cd = roles[2];
-    s->hi = roles[3];
-    s->pe = roles[4];
+    set_roles(s, key + 2);
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -435,8 +469,21 @@ int capture_graph(pic_sim *s, bool measure) {
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-    s->graph[m] = ex;
-    memcpy(s->graph_roles[m], roles, sizeof(roles));
+    pic_sim::GraphSlot *slot = nullptr;
+    for (auto &gs : s->graphs)
+        if (!gs.exec) {
+            slot = &gs;
+            break;
+        }
+    if (!slot) {
+        slot = &s->graphs[s->graph_next];
+        s->graph_next = (s->graph_next + 1) % 8;
+        (void)hipGraphExecDestroy(slot->exec);
+    }
+    slot->exec = ex;
+    memcpy(slot->key, key, sizeof(key));
+    memcpy(slot->after, after, sizeof(after));
+    *out = slot;
     return PIC_OK;
 }
 
@@ -447,19 +494,13 @@ int run_steps(pic_sim *s, uint32_t n, bool measure) {
                         (uint32_t)s->p.n_t % period == 0;
     while (n > 0) {
         if (graphs && s->t % period == 0 && n >= period) {
-            const int m = measure ? 1 : 0;
-            int roles[5] = {s->ce, s->ci, s->cd, s->hi, s->pe};
-            if (s->graph[m] && memcmp(roles, s->graph_roles[m], sizeof(roles)) != 0) {
-                (void)hipGraphExecDestroy(s->graph[m]);
-                s->graph[m] = nullptr;
-            }
-            if (!s->graph[m]) TRY(capture_graph(s, measure));
-            while (n >= period) {
-                PIC_HIP(hipGraphLaunch(s->graph[m], s->stream));
-                s->t = (s->t + period) % (uint32_t)s->p.n_t;
-                s->steps += period;
-                n -= period;
-            }
+            pic_sim::GraphSlot *g = nullptr;
+            TRY(graph_for(s, measure, &g));
+            PIC_HIP(hipGraphLaunch(g->exec, s->stream));
+            set_roles(s, g->after);
+            s->t = (s->t + period) % (uint32_t)s->p.n_t;
+            s->steps += period;
+            n -= period;
             continue;
         }
         TRY(one_step(s, measure));
@@ -526,6 +567,15 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         return PIC_ERR_ARG;
     }
     s->f64 = s->prec == PIC_PREC_FP64;
+    {
+        const uint32_t period = 4u * (uint32_t)s->p.n_sub;
+        s->compact_every = c->compact_every ? c->compact_every : std::max(period, 800u / period * period);
+        if (s->compact_every % period) {
+            delete s;
+            set_error("pic_sim_create: compact_every must be a multiple of 4*n_sub");
+            return PIC_ERR_ARG;
+        }
+    }
     s->cap = c->capacity;
     // the fused path keeps each species in segments with their own counters;
     // the reference path in one (its stage kernels see a plain array)

@@ -13,7 +13,7 @@ from ._abi import PicEnsembleConfig, PicParams, call, lib
 class Ensemble:
     def __init__(self, params_list, capacities, seed_e=(0x12345678, 0x11112222, 0x22223333, 0x33334444),
                  seed_i=(0x44445555, 0x55556666, 0x66667777, 0x77778888), use_graph=True, grid_blocks=0,
-                 ppt=0, replicas=0, lds_replicas=0):
+                 ppt=0, replicas=0, lds_replicas=0, compact_every=0):
         n = len(params_list)
         self._params = (PicParams * n)(*params_list)
         self._caps = (C.c_uint32 * n)(*[int(c) for c in capacities])
@@ -25,6 +25,7 @@ class Ensemble:
         cfg.seed_i[:] = list(seed_i)
         cfg.use_graph, cfg.grid_blocks, cfg.ppt, cfg.replicas = int(use_graph), grid_blocks, ppt, replicas
         cfg.lds_replicas = lds_replicas
+        cfg.compact_every = int(compact_every)
         self.n = n
         self.n_g = params_list[0].n_g
         h = C.c_void_p()

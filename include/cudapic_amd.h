@@ -196,6 +196,9 @@ typedef struct pic_sim_config {
     int32_t lds_replicas;      /* fused path: at most this many LDS density
                                   histograms per workgroup (shared by its
                                   waves): 0 = one per wave as LDS allows */
+    uint32_t compact_every;    /* fused path: steps between the compactions that
+                                  squeeze out the holes left by lost particles
+                                  (a multiple of 4*n_sub; 0 = 800) */
 } pic_sim_config_t;
 
 typedef struct pic_sim pic_sim_t;
@@ -215,7 +218,7 @@ int pic_sim_download(pic_sim_t *s, int species, float *x, float *vx, float *vy, 
  * number of slots in use in each segment (counts[n_seg], may be NULL).
  * Slots in use q*seg_size + [0, counts[q]) hold the particles; on the fused
  * path a slot whose particle left the gap in an in-place step is a hole
- * (x < 0) until the next compacting step (every 2*N_SUB steps).
+ * (x < 0) until the next compacting step (every compact_every steps).
  * pic_sim_download returns the live particles segment by segment; the
  * fused path needs a few percent of headroom in `capacity` because segments
  * fill independently. */
@@ -277,7 +280,7 @@ int pic_sim_load(pic_sim_t *s, const char *path);
  * capacity.  No reference counterpart: the reference runs one discharge per
  * process; this is the batched mode of the same step.  Particles are
  * updated in the slots they occupy, as on the fused single-discharge path
- * (lost particles leave holes, a compaction pass every 2*N_SUB steps), so
+ * (lost particles leave holes, a compaction pass every compact_every steps), so
  * a particle's RNG stream is its slot's until the next compaction. */
 typedef struct pic_ensemble_config {
     int32_t n_instances;          /* 1..256 */
@@ -287,8 +290,9 @@ typedef struct pic_ensemble_config {
     /* use_graph: replay 4*N_SUB steps from a hipGraph; grid_blocks (0 = 4
      * per compute unit), ppt (particles per thread per tile, 1, 2 or 4; 0 =
      * 2) and replicas (global density replicas, 0 = 8) shape the particle
-     * kernels; lds_replicas as in pic_sim_config_t */
+     * kernels; lds_replicas and compact_every as in pic_sim_config_t */
     int32_t use_graph, grid_blocks, ppt, replicas, lds_replicas;
+    uint32_t compact_every;
 } pic_ensemble_config_t;
 
 typedef struct pic_ensemble pic_ensemble_t;
