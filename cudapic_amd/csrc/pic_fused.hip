@@ -116,15 +116,6 @@ __device__ __forceinline__ void add_diagnostics(unsigned long long *c, uint32_t 
 }
 
 template <typename Acc>
-__device__ __forceinline__ void deposit_lds(float *h, float x, const KPhys &k) {
-    const float pos = x * k.inv_dx;
-    const int q = cell_of(pos, k.n_g);
-    const float rem = pos - (float)q;
-    atomicAdd(&h[q], (1.0f - rem) * k.wf);
-    atomicAdd(&h[q + 1], rem * k.wf);
-}
-
-template <typename Acc>
 __device__ __forceinline__ void deposit_global(Acc *d, float x, const KPhys &k) {
     const float pos = x * k.inv_dx;
     const int q = cell_of(pos, k.n_g);
