@@ -125,14 +125,8 @@ def cpu_baseline(n_sample: int):
     E, I = synthetic.state(n_sample, int(n_sample * 1.05), seed=7)
     tmp = tempfile.mkdtemp(prefix="edupic_")
     try:
-        with open(os.path.join(tmp, "picdata.bin"), "wb") as f:
-            # C/eduPIC.cc:726-749 layout: Time, cycles_done, N_e (f64), e arrays, N_i, i arrays
-            np.array([0.0, 1.0, float(len(E[0]))], np.float64).tofile(f)
-            for a in E:
-                a.astype(np.float64).tofile(f)
-            np.array([float(len(I[0]))], np.float64).tofile(f)
-            for a in I:
-                a.astype(np.float64).tofile(f)
+        from cudapic_amd.edupic_io import write_picdata
+        write_picdata(os.path.join(tmp, "picdata.bin"), 0.0, 1, E, I)
         t0 = time.perf_counter()
         r = subprocess.run([exe, "1"], cwd=tmp, capture_output=True, text=True, timeout=600)
         wall = time.perf_counter() - t0

@@ -16,6 +16,10 @@
 //   --thermal-init (cudaPIC's thermal start, main.rs:1361-1376; default is
 //   eduPIC's start at rest, C/eduPIC.cc:283-294)   --seed S
 //   --load FILE --save FILE (binary checkpoint)    --out DIR
+//   --import-edupic FILE / --export-edupic FILE: eduPIC's picdata.bin
+//   (C/eduPIC.cc:726-781: Time, cycles_done, N_e as f64, then x, vx, vy, vz
+//   of the electrons as f64, N_i and the ions likewise); --int-headers for
+//   the C++ eduPIC's variant with int32 cycles_done, N_e and N_i
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -48,7 +52,8 @@ void usage() {
                  "usage: cudapic <num_cycles> [--measure <measurement_cycles>] [--path fused|reference]\n"
                  "       [--precision fp32|comp|fp64] [--no-graph] [--weight W] [--voltage V] [--pressure P]\n"
                  "       [--ng N] [--capacity N] [--n-init N] [--thermal-init] [--seed S]\n"
-                 "       [--load FILE] [--save FILE] [--out DIR]\n");
+                 "       [--load FILE] [--save FILE] [--out DIR]\n"
+                 "       [--import-edupic FILE] [--export-edupic FILE] [--int-headers]\n");
 }
 
 // xoshiro128+ for the seeded initial positions (the reference draws them from
@@ -90,6 +95,59 @@ void mkdirs(const std::string &path) {
 
 }  // namespace
 
+// eduPIC's particle data file (C/eduPIC.cc:726-781)
+struct EduState {
+    double time = 0.0, cycles = 0.0;
+    std::vector<float> sp[2][4];  // electrons, ions: x, vx, vy, vz
+};
+
+static bool read_edupic(const std::string &path, bool int_headers, EduState &st) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    auto header = [&](double &v) {
+        if (int_headers) {
+            int32_t i = 0;
+            if (std::fread(&i, 4, 1, f) != 1) return false;
+            v = i;
+            return true;
+        }
+        return std::fread(&v, 8, 1, f) == 1;
+    };
+    bool ok = std::fread(&st.time, 8, 1, f) == 1 && header(st.cycles);
+    for (int sp = 0; ok && sp < 2; ++sp) {
+        double n = 0.0;
+        ok = header(n) && n >= 0.0 && n < 4.3e9;
+        std::vector<double> buf(ok ? (size_t)n : 0);
+        for (int c = 0; ok && c < 4; ++c) {
+            ok = std::fread(buf.data(), 8, buf.size(), f) == buf.size();
+            st.sp[sp][c].assign(buf.begin(), buf.end());
+        }
+    }
+    std::fclose(f);
+    return ok;
+}
+
+static bool write_edupic(const std::string &path, bool int_headers, const EduState &st) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    auto header = [&](double v) {
+        if (int_headers) {
+            const int32_t i = (int32_t)v;
+            return std::fwrite(&i, 4, 1, f) == 1;
+        }
+        return std::fwrite(&v, 8, 1, f) == 1;
+    };
+    bool ok = std::fwrite(&st.time, 8, 1, f) == 1 && header(st.cycles);
+    for (int sp = 0; ok && sp < 2; ++sp) {
+        ok = header((double)st.sp[sp][0].size());
+        for (int c = 0; ok && c < 4; ++c) {
+            std::vector<double> buf(st.sp[sp][c].begin(), st.sp[sp][c].end());
+            ok = std::fwrite(buf.data(), 8, buf.size(), f) == buf.size();
+        }
+    }
+    return std::fclose(f) == 0 && ok;
+}
+
 int main(int argc, char **argv) {
     std::printf(">> cudaPIC (MI355X): starting...\n");
     if (argc < 2) {
@@ -103,7 +161,8 @@ int main(int argc, char **argv) {
     }
     bool measure = false, graph = true, thermal = false;
     long measurement_cycles = 0;
-    std::string path = "fused", precision = "fp64", load, save, out_root = "results";
+    std::string path = "fused", precision = "fp64", load, save, out_root = "results", import_ed, export_ed;
+    bool int_headers = false;
     double weight = 7.0e4, voltage = 250.0, pressure = 10.0;
     int n_g = 400;
     long capacity = 120000, n_init = 1000;
@@ -133,6 +192,9 @@ int main(int argc, char **argv) {
         else if (a == "--seed") seed = (uint32_t)std::strtoul(val().c_str(), nullptr, 10);
         else if (a == "--load") load = val();
         else if (a == "--save") save = val();
+        else if (a == "--import-edupic") import_ed = val();
+        else if (a == "--export-edupic") export_ed = val();
+        else if (a == "--int-headers") int_headers = true;
         else if (a == "--out") out_root = val();
         else {
             usage();
@@ -163,9 +225,28 @@ int main(int argc, char **argv) {
     pic_sim_t *sim = nullptr;
     CK(pic_sim_create(&cfg, &sim));
     const pic_params_t &p = cfg.params;
+    double ed_time = 0.0, ed_cycles = 0.0;
     if (!load.empty()) {
         CK(pic_sim_load(sim, load.c_str()));
         std::printf(">> cudaPIC: state loaded from %s\n", load.c_str());
+    } else if (!import_ed.empty()) {
+        EduState st;
+        if (!read_edupic(import_ed, int_headers, st)) {
+            std::fprintf(stderr, ">> cudaPIC: cannot read eduPIC particle data %s\n", import_ed.c_str());
+            return 1;
+        }
+        ed_time = st.time;
+        ed_cycles = st.cycles;
+        for (int sp = 0; sp < 2; ++sp) {
+            const auto &v = st.sp[sp];
+            const uint32_t n = (uint32_t)v[0].size();
+            CK(pic_sim_upload(sim, sp, v[0].data(), v[1].data(), v[2].data(), v[3].data(), n));
+        }
+        // the step of the RF cycle the saved time falls on
+        const long step = std::lround(st.time / p.dt_e);
+        CK(pic_sim_set_time(sim, (uint32_t)(((step % p.n_t) + p.n_t) % p.n_t)));
+        std::printf(">> cudaPIC: eduPIC data loaded: %zu electrons %zu ions, %.0f cycles completed before\n",
+                    st.sp[0][0].size(), st.sp[1][0].size(), st.cycles);
     } else {
         // main.rs:1652-1662: N_INIT electrons and ions at random positions
         Xo rng{{0x9E3779B9u + seed, 0x7F4A7C15u, 0x85EBCA6Bu, 0xC2B2AE35u}};
@@ -317,6 +398,20 @@ int main(int argc, char **argv) {
         }
         std::printf(">> cudaPIC: measurement data saved (density_avg, eepf, ifed, info) for %ld cycles\n",
                     measurement_cycles);
+    }
+    if (!export_ed.empty()) {
+        EduState st;
+        st.time = ed_time + (double)num_cycles * p.n_t * p.dt_e;
+        st.cycles = ed_cycles + (double)num_cycles;
+        const std::vector<float> *src[2][4] = {{&ex, &evx, &evy, &evz}, {&ix, &ivx, &ivy, &ivz}};
+        const uint32_t cnt[2] = {ne, ni};
+        for (int sp = 0; sp < 2; ++sp)
+            for (int c = 0; c < 4; ++c) st.sp[sp][c].assign(src[sp][c]->begin(), src[sp][c]->begin() + cnt[sp]);
+        if (!write_edupic(export_ed, int_headers, st)) {
+            std::fprintf(stderr, ">> cudaPIC: cannot write %s\n", export_ed.c_str());
+            return 1;
+        }
+        std::printf(">> cudaPIC: eduPIC data saved to %s: %u electrons %u ions\n", export_ed.c_str(), ne, ni);
     }
     if (!save.empty()) {
         CK(pic_sim_save(sim, save.c_str()));

@@ -48,3 +48,27 @@ def test_cli_run_writes_reference_outputs(tmp_path, path):
 def test_cli_usage_error():
     r = subprocess.run([CLI], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_cli_imports_and_exports_edupic_picdata(tmp_path):
+    """A converged reference-case state in eduPIC's picdata.bin goes in, one
+    cycle runs, and picdata.bin comes out with the cycle and time advanced."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_refcase_state import unpack
+    from cudapic_amd import edupic_io
+    E, I = unpack(np.load(os.path.join(ROOT, "tests", "golden", "refcase_state.npz")))
+    period = 1.0 / 13.56e6
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    edupic_io.write_picdata(src, 1500 * period, 1500, E, I)
+    r = subprocess.run([CLI, "1", "--capacity", "400000", "--import-edupic", str(src), "--export-edupic", str(dst),
+                        "--out", str(tmp_path / "res")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"eduPIC data loaded: {len(E[0])} electrons {len(I[0])} ions, 1500 cycles" in r.stdout
+    d = edupic_io.read_picdata(dst)
+    assert d["cycles"] == 1501 and abs(d["time"] - 1501 * period) < 1e-6 * period
+    # one cycle of a converged discharge changes the populations by ~1 %
+    assert abs(len(d["electrons"][0]) / len(E[0]) - 1.0) < 0.05
+    assert abs(len(d["ions"][0]) / len(I[0]) - 1.0) < 0.05
+    x = d["ions"][0]
+    assert (x >= 0).all() and (x <= 0.025).all()
