@@ -3,9 +3,11 @@
 
 Workload (the plain method at scale): the eduPIC argon CCP (25 mm, 10 Pa,
 350 K, 13.56 MHz, 250 V, N_G = 400, N_T = 4000, N_SUB = 20) with the
-superparticle weight lowered to 2e3 so that ~3.74 M electrons and ~3.93 M ions
+superparticle weight lowered to 2e3 so that ~3.8 M electrons and ~4.0 M ions
 are live (MAX_PARTICLES = 8 M per species, FP32 SoA), started from a
-plasma-like state drawn from the published converged density profiles.
+converged state: the engine's converged reference case (W = 7e4, 1500 RF
+cycles, shipped as cudapic_amd/data/refcase_state.npz) with every particle
+repeated 35 times, which is the same discharge at W = 2e3.
 
 A step is one RF time step (electrons every step, ions every N_SUB steps);
 a particle-step is one particle advanced through one of its own steps.
@@ -58,6 +60,9 @@ def parse():
     ap.add_argument("--n-i", type=int, default=3_930_000)
     ap.add_argument("--capacity", type=int, default=8_000_000)
     ap.add_argument("--weight", type=float, default=2.0e3)
+    ap.add_argument("--synthetic-start", action="store_true",
+                    help="start from the plasma-like synthetic state of --n-e / --n-i particles instead of "
+                         "the converged reference case replicated to the weight")
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--precision", choices=["fp32", "comp", "fp64"], default="fp64")
     ap.add_argument("--no-graph", action="store_true")
@@ -179,9 +184,17 @@ def main():
     path = ca.PIC_PATH_FUSED if args.path == "fused" else ca.PIC_PATH_REFERENCE
     p = ca.params(weight=args.weight, precision=prec)
     cu, ws, arch = ca.device_info()
-    print(f"bench: device {arch}, {cu} CUs, wave{ws}; building state "
-          f"{args.n_e} e + {args.n_i} i", file=sys.stderr, flush=True)
-    E, I = synthetic.state(args.n_e, args.n_i, seed=1)
+    print(f"bench: device {arch}, {cu} CUs, wave{ws}; building state", file=sys.stderr, flush=True)
+    reps = 7.0e4 / args.weight
+    if not args.synthetic_start and abs(reps - round(reps)) < 1e-9 and round(reps) >= 1:
+        # the converged reference case (W = 7e4), each particle repeated
+        # 7e4 / W times: the same discharge at this weight
+        E, I = synthetic.converged_state(int(round(reps)))
+        start = f"converged reference case x{int(round(reps))}"
+    else:
+        E, I = synthetic.state(args.n_e, args.n_i, seed=1)
+        start = "synthetic plasma-like state"
+    print(f"bench: start: {start}, {len(E[0])} e + {len(I[0])} i", file=sys.stderr, flush=True)
     shape = dict(grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
                  ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas)
     tuned = None
@@ -240,8 +253,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "argon CCP at scale: W=2e3, ~3.74M e + ~3.93M i, MAX=8M/species, "
-                               "N_G=400, N_T=4000, N_SUB=20 (plain method at scale)",
+        "config": {"workload": "argon CCP at scale: W=2e3, ~3.8M e + ~4.0M i, MAX=8M/species, "
+                               "N_G=400, N_T=4000, N_SUB=20 (plain method at scale)", "start": start,
                    "n_electrons_end": n_e, "n_ions_end": n_i, "path": args.path,
                    "graph": not args.no_graph, "poisson_and_density_flush": args.precision,
                    "step_hbm_frac_80B": round(step_frac, 4),

@@ -53,11 +53,8 @@ def test_cli_usage_error():
 def test_cli_imports_and_exports_edupic_picdata(tmp_path):
     """A converged reference-case state in eduPIC's picdata.bin goes in, one
     cycle runs, and picdata.bin comes out with the cycle and time advanced."""
-    import sys
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    from make_refcase_state import unpack
-    from cudapic_amd import edupic_io
-    E, I = unpack(np.load(os.path.join(ROOT, "tests", "golden", "refcase_state.npz")))
+    from cudapic_amd import edupic_io, synthetic
+    E, I = synthetic.converged_state()
     period = 1.0 / 13.56e6
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     edupic_io.write_picdata(src, 1500 * period, 1500, E, I)

@@ -3,9 +3,9 @@ N_G = 400) against eduPIC's published converged outputs
 (C/ReferenceCase_RawData: Figure9_conv, Figure10_density, Figure12_eepf,
 Figure13_ifed, kept as tests/golden/refcase_*.dat).
 
-The runs start from a converged state of this engine (tests/golden/
-refcase_state.npz, made by make_refcase_state.py: 1500 RF cycles from a
-synthetic state drawn from the published profiles), relax 20 cycles and
+The runs start from a converged state of this engine
+(cudapic_amd/data/refcase_state.npz: 1500 RF cycles from a synthetic state
+drawn from the published profiles), relax 20 cycles and
 measure 100 cycles in 10 batches; the batch means give the statistical
 error.  Measured on the MI355X when the tolerances were set (fused path):
 counts +0.9 % / +0.8 % of eduPIC's converged 107.0 k e / 112.5 k i,
@@ -26,10 +26,8 @@ RELAX, MEASURE, BATCHES = 20, 100, 10
 
 
 def _state():
-    import sys
-    sys.path.insert(0, GOLDEN)
-    from make_refcase_state import unpack
-    return unpack(np.load(os.path.join(GOLDEN, "refcase_state.npz")))
+    from cudapic_amd import synthetic
+    return synthetic.converged_state()
 
 
 def _published():
