@@ -63,6 +63,12 @@ struct PoissonJob {
     // from the waveform), pot0 of the next step written to pot0_next
     const double *pot0_in = nullptr;
     double *pot0_next = nullptr;
+    // with pot0_next: pot0 of every step t < n_t (launch_pot0_table), read
+    // instead of evaluating the waveform inside the solve
+    const double *pot0_tab = nullptr;
+    // when given, the ion node densities (n_g doubles, summed over the
+    // replicas of i_dens by an earlier solve or launch_sum_replicas) replace i_dens
+    const double *i_sum = nullptr;
     // the summed node densities the solve used (n_g doubles each), optional
     double *dens_e_out = nullptr, *dens_i_out = nullptr;
     int n_t = 4000, n_sub = 20;
@@ -82,6 +88,11 @@ struct PoissonJob {
 int launch_poisson(const PoissonJob &j, hipStream_t s);
 // *out = pot0 of the step *time (the waveform the Poisson kernel evaluates)
 int launch_pot0(const PoissonJob &j, double *out, hipStream_t s);
+// tab[t] = pot0 of step t, t < j.n_t
+int launch_pot0_table(const PoissonJob &j, double *tab, hipStream_t s);
+// out[k] = node k of dens (j.precision's type, j.replicas / j.rep_stride)
+// summed over the replicas exactly as the Poisson kernel sums it
+int launch_sum_replicas(const PoissonJob &j, const void *dens, double *out, hipStream_t s);
 
 // n_cap bounds the live count read from n_active (an overflowed counter can
 // exceed the array it counts)

@@ -332,6 +332,8 @@ struct PoissonArgs {
     const uint32_t *time;
     const double *pot0_in;
     double *pot0_next;
+    const double *pot0_tab;
+    const double *i_sum;
     double *dens_e_out, *dens_i_out;
     int n_t, n_sub, n_harm;
     double wave_cos[PIC_MAX_HARMONICS], wave_sin[PIC_MAX_HARMONICS];
@@ -352,48 +354,39 @@ struct PoissonArgs {
     uint32_t *advance_time;
 };
 
-// Node k of both densities summed over the global replicas (in replica
-// order).  The replica count is 1 or a multiple of 8; the loads of up to 16
-// replicas of both species are independent, so they cost one memory
-// latency (a group of 8 also loads replica reps-1 eight more times, from
-// cache, to keep one code path).
+// Node k of one density summed over the global replicas, in replica order.
+// The replica count is 1 or a multiple of 8; the loads of up to 16 replicas
+// are independent, so they cost one memory latency (a group of 8 also loads
+// replica reps-1 eight more times, from cache, to keep one code path).
 template <typename D>
-__device__ __forceinline__ void load_dens_pair(const D *ed, const D *id, int reps, int stride, int k,
-                                               double &ne, double &ni) {
-    if (reps == 1) {
-        ne = (double)ed[k];
-        ni = (double)id[k];
-        return;
-    }
-    double se = 0.0, si = 0.0;
+__device__ __forceinline__ double sum_replicas(const D *d, int reps, int stride, int k) {
+    if (reps == 1) return (double)d[k];
+    double sum = 0.0;
     for (int r0 = 0; r0 < reps; r0 += 16) {
-        D ve[16], vi[16];
+        D v[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int r = min(r0 + j, reps - 1);
-            ve[j] = ed[r * stride + k];
-            vi[j] = id[r * stride + k];
-        }
+        for (int j = 0; j < 16; ++j) v[j] = d[min(r0 + j, reps - 1) * stride + k];
 #pragma unroll
         for (int j = 0; j < 16; ++j)
-            if (r0 + j < reps) {
-                se += (double)ve[j];
-                si += (double)vi[j];
-            }
+            if (r0 + j < reps) sum += (double)v[j];
     }
-    ne = se;
-    ni = si;
+    return sum;
 }
 
 // Densities of the (up to 4) nodes of this lane, summed over the replicas.
+// With i_sum the ion densities are taken from it (n_g doubles that an
+// earlier solve summed from the same replicas: the same values, one load).
 template <typename D>
-__device__ __forceinline__ void load_node_densities(const PoissonGeom &pg, const D *ed, const D *id, double *dne,
-                                                    double *dni) {
+__device__ __forceinline__ void load_node_densities(const PoissonGeom &pg, const D *ed, const D *id,
+                                                    const double *i_sum, double *dne, double *dni) {
     const int k0 = threadIdx.x * pg.per;
 #pragma unroll
     for (int u = 0; u < kPoissonMaxPer; ++u) {
         dne[u] = dni[u] = 0.0;
-        if (u < pg.per && k0 + u < pg.n_g) load_dens_pair(ed, id, pg.reps, pg.rep_stride, k0 + u, dne[u], dni[u]);
+        if (u < pg.per && k0 + u < pg.n_g) {
+            dne[u] = sum_replicas(ed, pg.reps, pg.rep_stride, k0 + u);
+            dni[u] = i_sum ? i_sum[k0 + u] : sum_replicas(id, pg.reps, pg.rep_stride, k0 + u);
+        }
     }
 }
 
@@ -528,13 +521,17 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     uint32_t cnt_copy = 0;
     if (a.copy_cnt_dst && tid < a.copy_cnt_words) cnt_copy = a.copy_cnt_src[tid];
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
-    load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, dne, dni);
+    load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, a.i_sum, dne, dni);
+    double p0_next = 0.0;
     if (tid == 0) {
         double p0 = a.pot0;
         int ion = 1;
         if (a.time) {
             ion = (t % (uint32_t)a.n_sub) == 0;
             p0 = a.pot0_in ? p0_in : waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
+            // the next step's pot0 from the table: the load is in flight
+            // during the solve
+            if (a.pot0_tab && a.pot0_next) p0_next = a.pot0_tab[(t + 1u) % (uint32_t)a.n_t];
         }
         s_pot0 = p0;
         s_ion = ion;
@@ -565,12 +562,26 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     if (tid == 0 && a.advance_time) {
         const uint32_t tn = (t + 1u) % (uint32_t)a.n_t;
         *a.advance_time = tn;
-        if (a.pot0_next) *a.pot0_next = waveform(tn, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
+        if (a.pot0_next)
+            *a.pot0_next = a.pot0_tab ? p0_next : waveform(tn, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
     }
 }
 
 __global__ void k_pot0(const uint32_t *time, int n_t, int n_harm, PoissonArgs a, double *out) {
     *out = waveform(*time, n_t, n_harm, a.wave_cos, a.wave_sin);
+}
+
+// tab[t] = pot0 of step t for t < n_t (the values k_pot0 computes)
+__global__ void k_pot0_table(int n_t, int n_harm, PoissonArgs a, double *tab) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_t) tab[t] = waveform((uint32_t)t, n_t, n_harm, a.wave_cos, a.wave_sin);
+}
+
+// out[k] = node k of dens summed over the replicas, as the Poisson kernel sums it
+template <typename D>
+__global__ void k_sum_replicas(PoissonGeom g, const D *dens, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < g.n_g) out[k] = sum_replicas(dens, g.reps, g.rep_stride, k);
 }
 
 int launch_poisson(const PoissonJob &j, hipStream_t s) {
@@ -606,6 +617,8 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     a.time = j.time;
     a.pot0_in = j.pot0_in;
     a.pot0_next = j.pot0_next;
+    a.pot0_tab = j.pot0_tab;
+    a.i_sum = j.i_sum;
     a.dens_e_out = j.dens_e_out;
     a.dens_i_out = j.dens_i_out;
     a.n_t = j.n_t;
@@ -680,6 +693,40 @@ int launch_pot0(const PoissonJob &j, double *out, hipStream_t s) {
     return PIC_OK;
 }
 
+int launch_pot0_table(const PoissonJob &j, double *tab, hipStream_t s) {
+    if (!tab || j.n_t <= 0) {
+        set_error("pot0 table: needs n_t > 0 and an output");
+        return PIC_ERR_ARG;
+    }
+    PoissonArgs a;
+    memset(&a, 0, sizeof(a));
+    int n_harm = 0;
+    for (int h = 0; h < PIC_MAX_HARMONICS; ++h) {
+        a.wave_cos[h] = j.wave_cos[h];
+        a.wave_sin[h] = j.wave_sin[h];
+        if (j.wave_cos[h] != 0.0 || j.wave_sin[h] != 0.0) n_harm = h + 1;
+    }
+    k_pot0_table<<<(j.n_t + 255) / 256, 256, 0, s>>>(j.n_t, n_harm, a, tab);
+    PIC_LAUNCHED("k_pot0_table");
+    return PIC_OK;
+}
+
+int launch_sum_replicas(const PoissonJob &j, const void *dens, double *out, hipStream_t s) {
+    if (!dens || !out || j.n_g <= 0 || (j.replicas != 1 && (j.replicas % 8) != 0)) {
+        set_error("sum_replicas: bad arguments");
+        return PIC_ERR_ARG;
+    }
+    PoissonGeom g;
+    memset(&g, 0, sizeof(g));
+    g.n_g = j.n_g;
+    g.reps = std::max(1, j.replicas);
+    g.rep_stride = j.rep_stride > 0 ? j.rep_stride : j.n_g;
+    if (j.precision == PIC_PREC_FP64) k_sum_replicas<double><<<(j.n_g + 255) / 256, 256, 0, s>>>(g, (const double *)dens, out);
+    else k_sum_replicas<float><<<(j.n_g + 255) / 256, 256, 0, s>>>(g, (const float *)dens, out);
+    PIC_LAUNCHED("k_sum_replicas");
+    return PIC_OK;
+}
+
 // Ensemble: one workgroup per discharge, all discharges in one launch.
 struct EnsPoissonArgs {
     PoissonGeom g;
@@ -711,7 +758,7 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
     uint32_t t = 0;
     if (tid == 0) t = *a.time;
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
-    load_node_densities<D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, dne, dni);
+    load_node_densities<D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, nullptr, dne, dni);
     if (tid == 0) {
         s_ion = (t % (uint32_t)a.n_sub) == 0;
         const double wc = a.inst[b].v0, ws = 0.0;

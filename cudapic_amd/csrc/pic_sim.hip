@@ -67,6 +67,7 @@ struct pic_sim {
     // grid
     float *pot = nullptr, *efield = nullptr;
     double *grid_e = nullptr, *grid_i = nullptr;  // node densities of the last solve
+    double *pot0_tab = nullptr;                     // pot0 of every step of the period (n_t)
     void *e_dens[2] = {}, *i_dens[2] = {};  // replicas * stride entries of 8 bytes
     int dens_words = 0;                      // entries per buffer (replicas * stride)
     // measurement
@@ -268,12 +269,16 @@ int fused_prologue(pic_sim *s) {
         TRY(launch_deposit(s->ion[s->ci][0] + o, icnt(s, s->ci) + q, s->i_dens[1], s->f64, s->k, sh, st, s->seg));
     }
     // the device clock follows the host's; the pot0 of its step is computed
-    // by the device waveform (each Poisson solve then computes the next one)
+    // by the device waveform (each Poisson solve then reads the next one from
+    // the table of the period); the held ion density is summed over its
+    // replicas into grid_i, which the solves of steps without ions read
     PIC_HIP(hipMemcpyAsync(s->cnt + C_TIME, &s->t, sizeof(uint32_t), hipMemcpyHostToDevice, st));
     {
         PoissonJob pj = base_poisson(s);
         pj.time = s->cnt + C_TIME;
         TRY(launch_pot0(pj, (double *)(s->cnt + C_P0), st));
+        TRY(launch_pot0_table(pj, s->pot0_tab, st));
+        TRY(launch_sum_replicas(pj, s->i_dens[s->hi], s->grid_i, st));
     }
     PIC_HIP(hipStreamSynchronize(st));
     s->need_prologue = false;
@@ -301,6 +306,10 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     pj.advance_time = s->cnt + C_TIME;
     pj.pot0_in = (const double *)(s->cnt + C_P0);
     pj.pot0_next = (double *)(s->cnt + C_P0);
+    pj.pot0_tab = s->pot0_tab;
+    // between ion steps the held ion density is the one the last ion step's
+    // solve summed into grid_i
+    if (!ion) pj.i_sum = s->grid_i;
     if (ion) {
         pj.zero_b = s->i_dens[s->hi];  // old held buffer becomes the accumulator
         pj.zero_b_words = (int)dens_words32(s);
@@ -520,7 +529,7 @@ int free_all(pic_sim *s) {
         (void)hipFree(s->e_rng[c]);
         (void)hipFree(s->i_rng[c]);
     }
-    void *bufs[] = {s->cnt, s->pot, s->efield, s->grid_e, s->grid_i, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
+    void *bufs[] = {s->cnt, s->pot, s->efield, s->grid_e, s->grid_i, s->pot0_tab, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->cumul_e, s->cumul_i, s->eepf, s->ifed, s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
     for (int i = 0; i < 2; ++i)
@@ -635,6 +644,7 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     if ((rc = dalloc(&s->efield, s->p.n_g))) return fail(rc);
     if ((rc = dalloc(&s->grid_e, s->p.n_g))) return fail(rc);
     if ((rc = dalloc(&s->grid_i, s->p.n_g))) return fail(rc);
+    if ((rc = dalloc(&s->pot0_tab, (size_t)s->p.n_t))) return fail(rc);
     for (int q = 0; q < 2; ++q) {
         if ((rc = dalloc((double **)&s->e_dens[q], s->dens_words))) return fail(rc);
         if ((rc = dalloc((double **)&s->i_dens[q], s->dens_words))) return fail(rc);
