@@ -62,6 +62,7 @@ class PicEnsembleConfig(C.Structure):
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas", "lds_replicas")],
         ("compact_every", C.c_uint32),
+        ("group_slots", C.c_uint32), ("group_periods", C.c_uint32), ("group_streams", C.c_uint32),
     ]
 
 

@@ -1,7 +1,10 @@
 // pic_ensemble.hip -- ensembles of independent discharges on one GPU: every
 // stage of every instance in one launch (one Poisson workgroup per
 // instance, one persistent particle kernel per species over all instances),
-// the 4*N_SUB-step launch pattern captured into a hipGraph.
+// the 4*N_SUB-step launch pattern captured into a hipGraph.  On the graph
+// path the instances are run in groups sized to the last-level cache (the
+// MI355X's 256 MB Infinity Cache): a group runs a block of periods while
+// its particles stay resident, then the next group runs the same steps.
 //
 // Instances share the grid (N_G, gap), the time step (frequency, N_T,
 // N_SUB) and the precision; each has its own driving amplitude, gas
@@ -31,12 +34,15 @@ struct pic_ensemble {
     float *e[2][4] = {}, *ion[2][4] = {};
     uint32_t *e_rng[4] = {}, *i_rng[4] = {};
     uint32_t *cnt = nullptr;   // [E][EC_WORDS]
-    uint32_t *clock = nullptr; // device step index within the cycle
+    uint32_t *clock = nullptr; // [E] device step index within the cycle; group [g0, g1) uses clock[g0]
     uint32_t *live = nullptr;  // [2][E] live counts for the host
     void *e_dens[2] = {}, *i_dens[2] = {};
     int inst_stride = 0;       // density entries per instance
     float *pot = nullptr, *efield = nullptr;
     double *cumul_e = nullptr, *cumul_i = nullptr;
+    double *trig = nullptr;   // [n_t][2] cos, sin of the drive's phase
+    double *pot0 = nullptr;   // [E] pot0 of the step each instance solves next
+    double *i_sum = nullptr;  // [E][n_g] held ion densities summed over the replicas
     unsigned long long *counters = nullptr;  // [E][8]
     float *cs = nullptr, *stot_e = nullptr, *stot_i = nullptr;
     Shape sh;
@@ -52,14 +58,23 @@ struct pic_ensemble {
     bool need_prologue = true;
     bool use_graph = true;
     uint32_t compact_every = 800;  // steps between compaction passes (a multiple of 4*N_SUB)
-    // captured 4*N_SUB-step graphs, keyed by the measure flag, whether the
-    // period starts with a compaction, and the roles at its start; each
-    // remembers the roles it leaves behind
+    // instance groups [grp[g], grp[g+1]) and the periods a group runs at a time
+    std::vector<int> grp;
+    uint32_t group_periods = 5;
+    // streams the groups are dealt to (work[0] is the ensemble's stream;
+    // the others fork from it at a block's start and join it at its end)
+    std::vector<hipStream_t> work;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    // captured 4*N_SUB-step graphs of one group, keyed by the group, the
+    // measure flag, whether the period starts with a compaction, and the
+    // roles at its start; each remembers the roles it leaves behind
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
-        int key[8] = {};
+        int key[9] = {};
         int after[6] = {};
-    } graphs[8];
+    };
+    std::vector<GraphSlot> graphs;
     int graph_next = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -85,7 +100,7 @@ uint64_t dens_words32(pic_ensemble *s) { return (uint64_t)s->E * s->inst_stride 
 int e_word(const pic_ensemble *s) { return s->ac ? EC_C : EC_E0 + s->pe; }
 int i_word(const pic_ensemble *s) { return EC_I0 + s->ci; }
 
-EnsPoissonJob poisson_job(pic_ensemble *s) {
+EnsPoissonJob poisson_job(pic_ensemble *s, int g0, int g1) {
     const double E = 1.60217662e-19, EPS0 = 8.85418781e-12;
     EnsPoissonJob j;
     memset(&j, 0, sizeof(j));
@@ -98,18 +113,23 @@ EnsPoissonJob poisson_job(pic_ensemble *s) {
     j.g.inv_dx = s->p0.inv_dx;
     j.g.half_inv_dx = 0.5 * s->p0.inv_dx;
     j.g.half_dx_over_eps = s->p0.dx / (2.0 * EPS0);
-    j.inst = s->inst_d;
-    j.n_inst = s->E;
+    j.inst = s->inst_d + g0;
+    j.n_inst = g1 - g0;
     j.inst_stride = s->inst_stride;
-    j.time = s->clock;
+    j.time = s->clock + g0;
     j.n_t = s->p0.n_t;
     j.n_sub = s->p0.n_sub;
-    j.pot = s->pot;
-    j.efield = s->efield;
-    j.counters = s->counters;
-    j.cnt = s->cnt;
+    j.pot = s->pot + (size_t)g0 * s->p0.n_g;
+    j.efield = s->efield + (size_t)g0 * s->p0.n_g;
+    j.counters = s->counters + (size_t)g0 * 8;
+    j.cnt = s->cnt + (size_t)g0 * EC_WORDS;
     j.copy_src = j.copy_dst = j.zc0 = j.zc1 = -1;
     return j;
+}
+
+// density buffer b shifted to instance g0
+void *dens_at(const pic_ensemble *s, void *b, int g0) {
+    return (char *)b + (size_t)g0 * s->inst_stride * (s->f64 ? 8 : 4);
 }
 
 int prologue(pic_ensemble *s) {
@@ -130,88 +150,104 @@ int prologue(pic_ensemble *s) {
             TRY(launch_deposit(s->ion[s->ci][0] + in.i_off, c + i_word(s), id, s->f64, in.k, sh, st, in.i_cap));
         }
     }
-    PIC_HIP(hipMemcpyAsync(s->clock, &s->t, 4, hipMemcpyHostToDevice, st));
+    const std::vector<uint32_t> clk(s->E, s->t);
+    PIC_HIP(hipMemcpyAsync(s->clock, clk.data(), 4ull * s->E, hipMemcpyHostToDevice, st));
+    // the Poisson kernels read pot0 and, between ion steps, the summed held
+    // ion density; each writes the next step's pot0 and, on ion steps, the sums
+    TRY(launch_ens_pot0_init(s->inst_d, s->E, s->p0.n_t, s->t, s->trig, s->pot0, st));
+    {
+        const EnsPoissonJob pj = poisson_job(s, 0, s->E);
+        TRY(launch_sum_replicas_ens(pj.g, s->prec, s->i_dens[s->hi], s->inst_stride, s->E, s->i_sum, st));
+    }
     PIC_HIP(hipStreamSynchronize(st));
     s->need_prologue = false;
     return PIC_OK;
 }
 
-EnsCompactJob compact_job(pic_ensemble *s, bool electrons, float *const *src, float *const *dst, int c_in,
-                          int c_out) {
+EnsCompactJob compact_job(pic_ensemble *s, int g0, int g1, bool electrons, float *const *src, float *const *dst,
+                          int c_in, int c_out) {
     EnsCompactJob j;
     memset(&j, 0, sizeof(j));
     j.electrons = electrons;
-    j.inst = s->inst_d;
-    j.n_inst = s->E;
+    j.inst = s->inst_d + g0;
+    j.n_inst = g1 - g0;
     for (int q = 0; q < 4; ++q) {
         j.src[q] = src[q];
         j.dst[q] = dst[q];
     }
-    j.cnt = s->cnt;
+    j.cnt = s->cnt + (size_t)g0 * EC_WORDS;
     j.c_in = c_in;
     j.c_out = c_out;
     return j;
 }
 
-// One ensemble step.  Particles are updated in place (lost ones leave
-// holes, newborn electrons and ions are appended after the watermarks);
-// every compact_every steps a compaction pass after the step packs each species
-// into the other array of its pair.  Electron watermarks alternate between
-// two counter words every step: the step reads one, the Poisson kernel
-// copies it into the other, newborns are appended there.
-int step(pic_ensemble *s, bool measure) {
+// One ensemble step of the instances [g0, g1).  Particles are updated in
+// place (lost ones leave holes, newborn electrons and ions are appended
+// after the watermarks); every compact_every steps a compaction pass after
+// the step packs each species into the other array of its pair.  Electron
+// watermarks alternate between two counter words every step: the step
+// reads one, the Poisson kernel copies it into the other, newborns are
+// appended there.  The roles and the host clock are advanced by
+// end_step once every group has had the step.
+int step_range(pic_ensemble *s, bool measure, int g0, int g1) {
     hipStream_t st = s->stream;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
     const bool comp_e = (t % s->compact_every) == 0;
     const bool comp_i = ion && comp_e;
     const int in_e = e_word(s), out_e = EC_E0 + (1 - s->pe);
-    EnsPoissonJob pj = poisson_job(s);
-    pj.e_dens = s->e_dens[s->cd];
-    pj.i_dens = s->i_dens[ion ? 1 - s->hi : s->hi];
-    pj.zero_a = s->e_dens[1 - s->cd];
-    pj.zero_b = s->i_dens[s->hi];
+    int ci = s->ci;
+    EnsPoissonJob pj = poisson_job(s, g0, g1);
+    pj.e_dens = dens_at(s, s->e_dens[s->cd], g0);
+    pj.i_dens = dens_at(s, s->i_dens[ion ? 1 - s->hi : s->hi], g0);
+    // the next electron accumulator is cleared by this step's electron
+    // kernel (the buffer the solve reads now); the ion one here
+    pj.zero_b = dens_at(s, s->i_dens[s->hi], g0);
+    pj.pot0 = s->pot0 + g0;
+    pj.trig = s->trig;
+    if (ion) pj.i_sum_out = s->i_sum + (size_t)g0 * s->p0.n_g;
+    else pj.i_sum_in = s->i_sum + (size_t)g0 * s->p0.n_g;
     pj.copy_src = in_e;
     pj.copy_dst = out_e;
     if (comp_e) pj.zc0 = EC_C;
-    if (comp_i) pj.zc1 = EC_I0 + (1 - s->ci);
+    if (comp_i) pj.zc1 = EC_I0 + (1 - ci);
     if (measure) {
-        pj.cumul_e = s->cumul_e;
-        pj.cumul_i = s->cumul_i;
+        pj.cumul_e = s->cumul_e + (size_t)g0 * s->p0.n_g;
+        pj.cumul_i = s->cumul_i + (size_t)g0 * s->p0.n_g;
     }
     TRY(launch_poisson_ens(pj, st));
     const int i_acc = ion ? s->hi : 1 - s->hi;
     EnsStepJob base;
     memset(&base, 0, sizeof(base));
-    base.inst = s->inst_d;
-    base.n_inst = s->E;
+    base.inst = s->inst_d + g0;
+    base.n_inst = g1 - g0;
     base.k = s->inst_h[0].k;
-    base.efield = s->efield;
-    base.cnt = s->cnt;
+    base.efield = s->efield + (size_t)g0 * s->p0.n_g;
+    base.cnt = s->cnt + (size_t)g0 * EC_WORDS;
     base.inst_stride = s->inst_stride;
     base.dens_f64 = s->f64;
     base.cs = s->cs;
-    base.counters = s->counters;
+    base.counters = s->counters + (size_t)g0 * 8;
     base.n_t = s->p0.n_t;
     if (ion) {
         EnsStepJob fi = base;
         fi.electrons = false;
-        fi.x = s->ion[s->ci][0];
-        fi.vx = s->ion[s->ci][1];
-        fi.vy = s->ion[s->ci][2];
-        fi.vz = s->ion[s->ci][3];
+        fi.x = s->ion[ci][0];
+        fi.vx = s->ion[ci][1];
+        fi.vy = s->ion[ci][2];
+        fi.vz = s->ion[ci][3];
         fi.rng0 = s->i_rng[0];
         fi.rng1 = s->i_rng[1];
         fi.rng2 = s->i_rng[2];
         fi.rng3 = s->i_rng[3];
-        fi.c_in = i_word(s);
-        fi.dens = s->i_dens[i_acc];
+        fi.c_in = EC_I0 + ci;
+        fi.dens = dens_at(s, s->i_dens[i_acc], g0);
         fi.stot = s->stot_i;
         TRY(launch_inplace_ens(fi, s->sh, st));
         if (comp_i) {
             TRY(launch_compact_ens(
-                compact_job(s, false, s->ion[s->ci], s->ion[1 - s->ci], EC_I0 + s->ci, EC_I0 + (1 - s->ci)), st));
-            s->ci ^= 1;
+                compact_job(s, g0, g1, false, s->ion[ci], s->ion[1 - ci], EC_I0 + ci, EC_I0 + (1 - ci)), st));
+            ci ^= 1;
         }
     }
     EnsStepJob fe = base;
@@ -224,28 +260,44 @@ int step(pic_ensemble *s, bool measure) {
     fe.rng1 = s->e_rng[1];
     fe.rng2 = s->e_rng[2];
     fe.rng3 = s->e_rng[3];
-    fe.ix = s->ion[s->ci][0];
-    fe.ivx = s->ion[s->ci][1];
-    fe.ivy = s->ion[s->ci][2];
-    fe.ivz = s->ion[s->ci][3];
+    fe.ix = s->ion[ci][0];
+    fe.ivx = s->ion[ci][1];
+    fe.ivy = s->ion[ci][2];
+    fe.ivz = s->ion[ci][3];
     fe.c_in = in_e;
     fe.c_out = out_e;
-    fe.c_ion = i_word(s);
-    fe.dens = s->e_dens[1 - s->cd];
-    fe.ion_dens = s->i_dens[i_acc];
+    fe.c_ion = EC_I0 + ci;
+    fe.dens = dens_at(s, s->e_dens[1 - s->cd], g0);
+    fe.ion_dens = dens_at(s, s->i_dens[i_acc], g0);
     fe.stot = s->stot_e;
-    fe.time = s->clock;
+    fe.time = s->clock + g0;
+    fe.zero = (uint32_t *)dens_at(s, s->e_dens[s->cd], g0);
+    fe.zero_words = (uint32_t)((size_t)(g1 - g0) * s->inst_stride * (s->f64 ? 2 : 1));
     TRY(launch_inplace_ens(fe, s->sh, st));
-    if (comp_e) {
-        TRY(launch_compact_ens(compact_job(s, true, s->e[s->ce], s->e[1 - s->ce], out_e, EC_C), st));
-        s->ce ^= 1;
-    }
+    if (comp_e) TRY(launch_compact_ens(compact_job(s, g0, g1, true, s->e[s->ce], s->e[1 - s->ce], out_e, EC_C), st));
+    return PIC_OK;
+}
+
+// the roles and the host clock after a step
+void end_step(pic_ensemble *s) {
+    const uint32_t t = s->t;
+    const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
+    const bool comp_e = (t % s->compact_every) == 0;
+    if (ion && comp_e) s->ci ^= 1;
+    if (comp_e) s->ce ^= 1;
     s->ac = comp_e ? 1 : 0;
     s->pe ^= 1;
     s->cd ^= 1;
     if (ion) s->hi ^= 1;
     s->t = (s->t + 1) % (uint32_t)s->p0.n_t;
     s->steps++;
+}
+
+// one step of every instance, group by group (each group's kernels advance
+// its own device clock)
+int step(pic_ensemble *s, bool measure) {
+    for (size_t g = 0; g + 1 < s->grp.size(); ++g) TRY(step_range(s, measure, s->grp[g], s->grp[g + 1]));
+    end_step(s);
     return PIC_OK;
 }
 
@@ -275,27 +327,31 @@ void set_roles(pic_ensemble *s, const int *r) {
     s->ac = r[5];
 }
 
-// The graph of the 4*N_SUB steps starting now (t a multiple of 4*N_SUB),
-// from the cache or captured; a period holds at most one compaction (at its
-// first step).
-int graph_for(pic_ensemble *s, bool measure, pic_ensemble::GraphSlot **out) {
-    int key[8];
+// The graph of group g's 4*N_SUB steps starting now (t a multiple of
+// 4*N_SUB), from the cache or captured; a period holds at most one
+// compaction (at its first step).
+int graph_for(pic_ensemble *s, bool measure, int g, pic_ensemble::GraphSlot **out) {
+    int key[9];
     key[0] = measure ? 1 : 0;
     key[1] = (s->t % s->compact_every) == 0 ? 1 : 0;
     get_roles(s, key + 2);
-    for (auto &g : s->graphs)
-        if (g.exec && memcmp(g.key, key, sizeof(key)) == 0) {
-            *out = &g;
+    key[8] = g;
+    for (auto &gs : s->graphs)
+        if (gs.exec && memcmp(gs.key, key, sizeof(key)) == 0) {
+            *out = &gs;
             return PIC_OK;
         }
     const int period = 4 * s->p0.n_sub;
     const uint32_t t0 = s->t;
     const unsigned long long st0 = s->steps;
-    hipGraph_t g = nullptr;
+    hipGraph_t gr = nullptr;
     PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = PIC_OK;
-    for (int i = 0; i < period && rc == PIC_OK; ++i) rc = step(s, measure);
-    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    for (int i = 0; i < period && rc == PIC_OK; ++i) {
+        rc = step_range(s, measure, s->grp[g], s->grp[g + 1]);
+        end_step(s);
+    }
+    hipError_t e = hipStreamEndCapture(s->stream, &gr);
     // capture does not execute: remember the roles the period leaves, then
     // restore the host mirror
     int after[6];
@@ -304,13 +360,13 @@ int graph_for(pic_ensemble *s, bool measure, pic_ensemble::GraphSlot **out) {
     s->steps = st0;
     set_roles(s, key + 2);
     if (rc != PIC_OK) {
-        if (g) (void)hipGraphDestroy(g);
+        if (gr) (void)hipGraphDestroy(gr);
         return rc;
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     hipGraphExec_t ex = nullptr;
-    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     pic_ensemble::GraphSlot *slot = nullptr;
     for (auto &gs : s->graphs)
@@ -320,7 +376,7 @@ int graph_for(pic_ensemble *s, bool measure, pic_ensemble::GraphSlot **out) {
         }
     if (!slot) {
         slot = &s->graphs[s->graph_next];
-        s->graph_next = (s->graph_next + 1) % 8;
+        s->graph_next = (s->graph_next + 1) % (int)s->graphs.size();
         (void)hipGraphExecDestroy(slot->exec);
     }
     slot->exec = ex;
@@ -330,19 +386,54 @@ int graph_for(pic_ensemble *s, bool measure, pic_ensemble::GraphSlot **out) {
     return PIC_OK;
 }
 
+// Graph path: blocks of up to group_periods periods; each group runs the
+// whole block (its graphs back to back on the stream) before the next
+// group starts it from the same time and roles.
 int run(pic_ensemble *s, uint32_t n, bool measure) {
     if (s->need_prologue) TRY(prologue(s));
     const uint32_t period = 4u * (uint32_t)s->p0.n_sub;
     const bool graphs = s->use_graph && (uint32_t)s->p0.n_t % period == 0;
     while (n > 0) {
         if (graphs && s->t % period == 0 && n >= period) {
-            pic_ensemble::GraphSlot *g = nullptr;
-            TRY(graph_for(s, measure, &g));
-            PIC_HIP(hipGraphLaunch(g->exec, s->stream));
-            set_roles(s, g->after);
-            s->t = (s->t + period) % (uint32_t)s->p0.n_t;
-            s->steps += period;
-            n -= period;
+            const uint32_t block = std::min(s->group_periods, n / period);
+            const int n_groups = (int)s->grp.size() - 1;
+            const int ns = std::min((int)s->work.size(), n_groups);
+            int roles0[6], roles1[6];
+            get_roles(s, roles0);
+            const uint32_t t0 = s->t;
+            const unsigned long long st0 = s->steps;
+            if (ns > 1) {
+                PIC_HIP(hipEventRecord(s->fork, s->stream));
+                for (int q = 1; q < ns; ++q) PIC_HIP(hipStreamWaitEvent(s->work[q], s->fork, 0));
+            }
+            int rc = PIC_OK;
+            for (int g = 0; g < n_groups && rc == PIC_OK; ++g) {
+                set_roles(s, roles0);
+                s->t = t0;
+                for (uint32_t b = 0; b < block && rc == PIC_OK; ++b) {
+                    pic_ensemble::GraphSlot *gs = nullptr;
+                    rc = graph_for(s, measure, g, &gs);
+                    if (rc != PIC_OK) break;
+                    const hipError_t e = hipGraphLaunch(gs->exec, s->work[g % ns]);
+                    if (e != hipSuccess) {
+                        rc = hip_fail(e, "hipGraphLaunch");
+                        break;
+                    }
+                    set_roles(s, gs->after);
+                    s->t = (s->t + period) % (uint32_t)s->p0.n_t;
+                }
+                get_roles(s, roles1);
+            }
+            // the join is recorded whatever happened, so that the ensemble's
+            // stream orders everything launched on the others
+            for (int q = 1; q < ns; ++q) {
+                PIC_HIP(hipEventRecord(s->join[q], s->work[q]));
+                PIC_HIP(hipStreamWaitEvent(s->stream, s->join[q], 0));
+            }
+            if (rc != PIC_OK) return rc;
+            set_roles(s, roles1);
+            s->steps = st0 + (unsigned long long)block * period;
+            n -= block * period;
             continue;
         }
         TRY(step(s, measure));
@@ -385,6 +476,7 @@ int gather_live(pic_ensemble *s, int inst, int species, float *x, float *vx, flo
 }
 
 void free_all(pic_ensemble *s) {
+    for (size_t q = 1; q < s->work.size(); ++q) (void)hipStreamSynchronize(s->work[q]);
     invalidate_graphs(s);
     for (int q = 0; q < 2; ++q)
         for (int c = 0; c < 4; ++c) {
@@ -396,10 +488,16 @@ void free_all(pic_ensemble *s) {
         (void)hipFree(s->i_rng[c]);
     }
     void *bufs[] = {s->inst_d, s->cnt, s->clock, s->live, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
-                    s->pot, s->efield, s->cumul_e, s->cumul_i, s->counters, s->cs, s->stot_e, s->stot_i};
+                    s->pot, s->efield, s->cumul_e, s->cumul_i, s->i_sum, s->pot0, s->trig,
+                    s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
     for (int i = 0; i < 2; ++i)
         if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+    for (size_t q = 1; q < s->work.size(); ++q) {
+        (void)hipStreamDestroy(s->work[q]);
+        if (s->join[q]) (void)hipEventDestroy(s->join[q]);
+    }
+    if (s->fork) (void)hipEventDestroy(s->fork);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -480,6 +578,24 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
         return PIC_ERR_ARG;
     }
     s->e_total = s->i_total = (uint32_t)tot;
+    {
+        // groups of consecutive instances (capacities summed up to group_slots)
+        const uint64_t budget = c->group_slots ? c->group_slots : 9000000u;
+        s->group_periods = c->group_periods ? c->group_periods : 5u;
+        s->grp.push_back(0);
+        uint64_t acc = 0;
+        for (int i = 0; i < s->E; ++i) {
+            const uint64_t cap = s->inst_h[i].e_cap;
+            if (acc > 0 && acc + cap > budget) {
+                s->grp.push_back(i);
+                acc = 0;
+            }
+            acc += cap;
+        }
+        s->grp.push_back(s->E);
+        s->graphs.resize(8 * (s->grp.size() - 1));
+    }
+    const int n_streams = (int)std::min<uint32_t>(c->group_streams ? c->group_streams : 1u, 8u);
     auto fail = [&](int r) {
         free_all(s);
         delete s;
@@ -488,6 +604,20 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     int rc;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(hip_fail(hipGetLastError(), "hipStreamCreate"));
+    s->work.assign(1, s->stream);
+    s->join.assign(1, nullptr);
+    for (int q = 1; q < std::min(n_streams, (int)s->grp.size() - 1); ++q) {
+        hipStream_t w = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&w, hipStreamNonBlocking) != hipSuccess)
+            return fail(hip_fail(hipGetLastError(), "hipStreamCreate"));
+        s->work.push_back(w);
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            return fail(hip_fail(hipGetLastError(), "hipEventCreate"));
+        s->join.push_back(ev);
+    }
+    if (s->work.size() > 1 && hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "hipEventCreate"));
     for (int q = 0; q < 2; ++q)
         for (int k = 0; k < 4; ++k) {
             if ((rc = dalloc(&s->e[q][k], s->e_total))) return fail(rc);
@@ -499,7 +629,7 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     }
     if ((rc = dalloc(&s->inst_d, s->E))) return fail(rc);
     if ((rc = dalloc(&s->cnt, (size_t)s->E * EC_WORDS))) return fail(rc);
-    if ((rc = dalloc(&s->clock, 1))) return fail(rc);
+    if ((rc = dalloc(&s->clock, (size_t)s->E))) return fail(rc);
     if ((rc = dalloc(&s->live, 2 * (size_t)s->E))) return fail(rc);
     for (int q = 0; q < 2; ++q) {
         if ((rc = dalloc((double **)&s->e_dens[q], (size_t)s->E * s->inst_stride))) return fail(rc);
@@ -510,6 +640,9 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     if ((rc = dalloc(&s->efield, gridw))) return fail(rc);
     if ((rc = dalloc(&s->cumul_e, gridw))) return fail(rc);
     if ((rc = dalloc(&s->cumul_i, gridw))) return fail(rc);
+    if ((rc = dalloc(&s->i_sum, gridw))) return fail(rc);
+    if ((rc = dalloc(&s->pot0, (size_t)s->E))) return fail(rc);
+    if ((rc = dalloc(&s->trig, 2 * (size_t)s->p0.n_t))) return fail(rc);
     if ((rc = dalloc(&s->counters, (size_t)s->E * 8))) return fail(rc);
     if ((rc = dalloc(&s->cs, (size_t)PIC_N_CS * PIC_CS_RANGES))) return fail(rc);
     if ((rc = dalloc(&s->stot_e, PIC_CS_RANGES))) return fail(rc);

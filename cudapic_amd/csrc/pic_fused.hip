@@ -807,6 +807,7 @@ __global__ __launch_bounds__(256) void k_inplace_ens(EnsArgs A) {
     }
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     if (threadIdx.x == 0) s_ncoll = 0;
+    clear_slice(j.zero, j.zero_words);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = 0;

@@ -221,8 +221,23 @@ struct EnsPoissonJob {
     // counter words: c[copy_dst] = c[copy_src] (when copy_dst >= 0), then
     // c[zc0] = 0 and c[zc1] = 0 (each when >= 0)
     int copy_src, copy_dst, zc0, zc1;
+    // optional: pot0 of the step per instance (read; the next step's written,
+    // from trig, the cos / sin table of launch_ens_pot0_init); without it the
+    // kernel evaluates the waveform
+    double *pot0;
+    const double *trig;
+    // optional: summed ion densities [n_inst][n_g], read instead of i_dens
+    // (steps without ions) or written (ion steps)
+    const double *i_sum_in;
+    double *i_sum_out;
 };
 int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s);
+// trig = the phase table of n_t steps; pot0[b] = v0_b cos(2 pi t / n_t)
+int launch_ens_pot0_init(const EnsInst *inst, int n_inst, int n_t, uint32_t t, double *trig, double *pot0,
+                         hipStream_t s);
+// out[b][k] = node k of instance b's density (replicas summed as the Poisson kernel sums them)
+int launch_sum_replicas_ens(const PoissonGeom &g, int precision, const void *dens, int inst_stride, int n_inst,
+                            double *out, hipStream_t s);
 
 // In-place particle step of every instance in one launch (the ensemble
 // counterpart of the in-place FusedJob): pooled arrays updated in their
@@ -246,6 +261,8 @@ struct EnsStepJob {
     unsigned long long *counters;          // [n_inst][8]
     uint32_t *time;                        // advanced by one step (electron step)
     int n_t;
+    uint32_t *zero;                        // cleared while the kernel runs (optional)
+    uint32_t zero_words;
 };
 int launch_inplace_ens(const EnsStepJob &j, const Shape &sh, hipStream_t s);
 

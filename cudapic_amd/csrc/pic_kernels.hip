@@ -403,6 +403,14 @@ __device__ double waveform(uint32_t t, int n_t, int n_harm, const double *wc, co
     return p0;
 }
 
+// pot0 = v0 cos(2 pi t / n_t) from the tabulated cos / sin of step t, with
+// the arithmetic of waveform() for one harmonic with amplitudes (v0, 0)
+__device__ __forceinline__ double pot0_from_trig(double v0, double c, double s) {
+    double p0 = 0.0;
+    p0 += v0 * c + 0.0 * s;
+    return p0;
+}
+
 // solve_poisson_scan_f32 (main.rs:408-494) generalised to three
 // precisions and n_g <= 4096 (up to four nodes per lane), for one discharge
 // by one workgroup of 1024 lanes.  The tridiagonal Dirichlet system with
@@ -727,6 +735,55 @@ int launch_sum_replicas(const PoissonJob &j, const void *dens, double *out, hipS
     return PIC_OK;
 }
 
+// trig[2t], trig[2t+1] = cos, sin of the phase of step t, as waveform()
+// evaluates them for the first harmonic
+__global__ void k_trig_table(int n_t, double *trig) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_t) return;
+    const uint64_t m = (2ull * (uint64_t)t) % (2ull * (uint64_t)n_t);
+    double sv, cv;
+    sincospi((double)m / (double)n_t, &sv, &cv);
+    trig[2 * t] = cv;
+    trig[2 * t + 1] = sv;
+}
+
+__global__ void k_ens_pot0(const EnsInst *inst, int n_inst, const double *trig, uint32_t t, double *pot0) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n_inst) pot0[b] = pot0_from_trig(inst[b].v0, trig[2 * t], trig[2 * t + 1]);
+}
+
+template <typename D>
+__global__ void k_sum_replicas_ens(PoissonGeom g, const D *dens, int inst_stride, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (k < g.n_g) out[(size_t)b * g.n_g + k] = sum_replicas(dens + (size_t)b * inst_stride, g.reps, g.rep_stride, k);
+}
+
+int launch_ens_pot0_init(const EnsInst *inst, int n_inst, int n_t, uint32_t t, double *trig, double *pot0,
+                         hipStream_t s) {
+    if (!inst || n_inst <= 0 || n_t <= 0 || t >= (uint32_t)n_t || !trig || !pot0) {
+        set_error("ens_pot0_init: bad arguments");
+        return PIC_ERR_ARG;
+    }
+    k_trig_table<<<(n_t + 255) / 256, 256, 0, s>>>(n_t, trig);
+    k_ens_pot0<<<(n_inst + 255) / 256, 256, 0, s>>>(inst, n_inst, trig, t, pot0);
+    PIC_LAUNCHED("k_ens_pot0");
+    return PIC_OK;
+}
+
+int launch_sum_replicas_ens(const PoissonGeom &g, int precision, const void *dens, int inst_stride, int n_inst,
+                            double *out, hipStream_t s) {
+    if (!dens || !out || n_inst <= 0 || g.n_g <= 0 || (g.reps != 1 && (g.reps % 8) != 0)) {
+        set_error("sum_replicas_ens: bad arguments");
+        return PIC_ERR_ARG;
+    }
+    const dim3 grid((g.n_g + 255) / 256, n_inst);
+    if (precision == PIC_PREC_FP64) k_sum_replicas_ens<double><<<grid, 256, 0, s>>>(g, (const double *)dens, inst_stride, out);
+    else k_sum_replicas_ens<float><<<grid, 256, 0, s>>>(g, (const float *)dens, inst_stride, out);
+    PIC_LAUNCHED("k_sum_replicas_ens");
+    return PIC_OK;
+}
+
 // Ensemble: one workgroup per discharge, all discharges in one launch.
 struct EnsPoissonArgs {
     PoissonGeom g;
@@ -742,7 +799,12 @@ struct EnsPoissonArgs {
     int zero_words;  // 32-bit words per instance
     uint32_t *cnt;
     int copy_src, copy_dst, zc0, zc1;
+    double *pot0;            // [n_inst] pot0 of the step (read, then the next step's written)
+    const double *trig;      // [n_t][2] cos, sin of the drive's phase
+    const double *i_sum_in;  // [n_inst][n_g] summed ion densities (steps without ions)
+    double *i_sum_out;       // [n_inst][n_g] written (ion steps)
 };
+
 
 template <typename T, typename D>
 __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs a) {
@@ -756,13 +818,30 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
     const int n_g = a.g.n_g;
     const size_t off = (size_t)b * a.inst_stride;
     uint32_t t = 0;
-    if (tid == 0) t = *a.time;
+    double p0 = 0.0, v0 = 0.0;
+    if (tid == 0) {
+        t = *a.time;
+        if (a.pot0) {
+            p0 = a.pot0[b];
+            v0 = a.inst[b].v0;
+        }
+    }
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
-    load_node_densities<D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off, nullptr, dne, dni);
+    load_node_densities<D>(a.g, (const D *)a.e_dens + off, (const D *)a.i_dens + off,
+                           a.i_sum_in ? a.i_sum_in + (size_t)b * n_g : nullptr, dne, dni);
+    double nc = 0.0, ns = 0.0;
     if (tid == 0) {
         s_ion = (t % (uint32_t)a.n_sub) == 0;
-        const double wc = a.inst[b].v0, ws = 0.0;
-        s_pot0 = waveform(t, a.n_t, 1, &wc, &ws);
+        if (a.pot0) {
+            // the next step's cos / sin: the loads are in flight during the solve
+            const uint32_t tn = (t + 1u) % (uint32_t)a.n_t;
+            nc = a.trig[2 * tn];
+            ns = a.trig[2 * tn + 1];
+        } else {
+            const double wc = a.inst[b].v0, ws = 0.0;
+            p0 = waveform(t, a.n_t, 1, &wc, &ws);
+        }
+        s_pot0 = p0;
     }
     __syncthreads();
     const bool ion = s_ion != 0;
@@ -770,10 +849,18 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson_ens(EnsPoissonArgs 
                         a.pot + (size_t)b * n_g, a.efield + (size_t)b * n_g,
                         a.cumul_e ? a.cumul_e + (size_t)b * n_g : nullptr,
                         a.cumul_i ? a.cumul_i + (size_t)b * n_g : nullptr, pot_s, wtot);
+    if (a.i_sum_out) {
+        const int k0 = tid * a.g.per;
+#pragma unroll
+        for (int u = 0; u < kPoissonMaxPer; ++u)
+            if (u < a.g.per && k0 + u < n_g) a.i_sum_out[(size_t)b * n_g + k0 + u] = dni[u];
+    }
     const size_t zoff = (size_t)b * a.zero_words;
-    for (int w = tid; w < a.zero_words; w += blockDim.x) a.zero_a[zoff + w] = 0u;
-    if (ion)
+    if (a.zero_a)
+        for (int w = tid; w < a.zero_words; w += blockDim.x) a.zero_a[zoff + w] = 0u;
+    if (ion && a.zero_b)
         for (int w = tid; w < a.zero_words; w += blockDim.x) a.zero_b[zoff + w] = 0u;
+    if (tid == 0 && a.pot0) a.pot0[b] = pot0_from_trig(v0, nc, ns);
     uint32_t *c = a.cnt + (size_t)b * EC_WORDS;
     if (tid == 0) {
         if (a.copy_dst >= 0) c[a.copy_dst] = c[a.copy_src];
@@ -815,6 +902,14 @@ int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
     a.copy_dst = j.copy_dst;
     a.zc0 = j.zc0;
     a.zc1 = j.zc1;
+    a.pot0 = j.pot0;
+    a.trig = j.trig;
+    a.i_sum_in = j.i_sum_in;
+    a.i_sum_out = j.i_sum_out;
+    if (a.pot0 && !a.trig) {
+        set_error("poisson_ens: pot0 needs the phase table");
+        return PIC_ERR_ARG;
+    }
     switch (j.precision) {
     case PIC_PREC_FP32:
         a.zero_words = j.inst_stride;

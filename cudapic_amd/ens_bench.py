@@ -51,6 +51,11 @@ def main():
     ap.add_argument("--grid", type=int, default=0, help="particle-kernel workgroups (0 = 4 per compute unit)")
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread per tile (0 = 2)")
     ap.add_argument("--replicas", type=int, default=0, help="global density replicas (0 = 8)")
+    ap.add_argument("--group-slots", type=int, default=0,
+                    help="electron capacity per group of instances run a block at a time (0 = 9e6, "
+                         "4294967295 = one group)")
+    ap.add_argument("--group-periods", type=int, default=0, help="4*N_SUB-step periods per block (0 = 5)")
+    ap.add_argument("--group-streams", type=int, default=0, help="streams the groups are dealt to (0 = 1)")
     args = ap.parse_args()
     import cudapic_amd as ca
     from cudapic_amd import synthetic
@@ -63,7 +68,9 @@ def main():
     print(f"ens_bench: {args.instances} instances, {int(cnt.sum())} electrons in total "
           f"({int(cnt.min())}..{int(cnt.max())})", file=sys.stderr, flush=True)
 
-    ens = ca.Ensemble(ps, caps, grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas)
+    ens = ca.Ensemble(ps, caps, grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas,
+                      group_slots=args.group_slots, group_periods=args.group_periods,
+                      group_streams=args.group_streams)
     for i, (E, I) in enumerate(states):
         ens.upload(i, 0, *E)
         ens.upload(i, 1, *I)

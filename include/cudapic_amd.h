@@ -293,6 +293,18 @@ typedef struct pic_ensemble_config {
      * kernels; lds_replicas and compact_every as in pic_sim_config_t */
     int32_t use_graph, grid_blocks, ppt, replicas, lds_replicas;
     uint32_t compact_every;
+    /* Temporal blocking of the graph path: consecutive instances whose
+     * electron capacities sum to at most group_slots form a group (an
+     * instance above it is a group alone; 0 = 9,000,000, 0xffffffff = one
+     * group of all), and each group runs group_periods 4*N_SUB-step graphs
+     * (0 = 5) before the next one starts, so that a group's particles stay
+     * in the GPU's last-level cache from step to step.  The groups are
+     * dealt round-robin to group_streams streams (0 = 1; more overlap one
+     * group's Poisson solves with another's particle kernels, at the price
+     * of two groups sharing the cache).  Every
+     * instance still goes through the same steps; the order in which
+     * instances' steps execute is all that changes. */
+    uint32_t group_slots, group_periods, group_streams;
 } pic_ensemble_config_t;
 
 typedef struct pic_ensemble pic_ensemble_t;

@@ -27,13 +27,16 @@ def _source_slots(orc, po, ef, parts, name):
 
 
 # start steps: 1 updates electrons in place, 20 (= N_SUB) both species in
-# place, 0 both species in place followed by the compaction pass
+# place, 0 both species in place followed by the compaction pass; the
+# instances stepped as one group, or each as a group of its own (every
+# per-instance array then addressed from the group's first instance)
+@pytest.mark.parametrize("grouped", [False, True], ids=["one-group", "group-per-instance"])
 @pytest.mark.parametrize("t0", [0, 1, 20])
-def test_ensemble_step_per_instance_replay(ca, orc, t0):
+def test_ensemble_step_per_instance_replay(ca, orc, t0, grouped):
     specs = [(250.0, 10.0, 2.0e3, 60_000), (120.0, 4.0, 5.0e3, 30_000), (380.0, 25.0, 1.0e3, 90_000)]
     ps = [ca.params(v0=v, pressure=pr, weight=w, precision=2) for v, pr, w, _ in specs]
     caps = [int(n * 1.3) for *_, n in specs]
-    ens = ca.Ensemble(ps, caps, use_graph=False)
+    ens = ca.Ensemble(ps, caps, use_graph=False, group_slots=1 if grouped else 0xFFFFFFFF)
     ens.set_time(t0)
     states = []
     for i, (v, pr, w, n) in enumerate(specs):
@@ -102,8 +105,11 @@ def test_ensemble_step_per_instance_replay(ca, orc, t0):
 
 def test_ensemble_graph_replays_match_eager(ca):
     """Graph replays (4*N_SUB steps each) against the same steps run eagerly,
-    with eager work between the replays.  At 1e-9 Pa the discharges are
-    collisionless (collision probability ~1e-13 per particle-step), so both
+    with eager work between the replays: one group of all instances, and
+    each instance a group of its own that runs two periods before the
+    next one (temporal blocking), on one stream and dealt to two streams.
+    At 1e-9 Pa the discharges are
+    collisionless (collision probability ~1e-13 per particle-step), so all
     runs see the same fields up to the summation order of the density
     atomics.  The weights put both at the reference case's plasma density,
     so that the plasma shields the RF voltage and keeps its electrons."""
@@ -111,8 +117,10 @@ def test_ensemble_graph_replays_match_eager(ca):
     ps = [ca.params(v0=v, pressure=1e-9, weight=w, precision=2) for v, w, _ in specs]
     caps = [int(n * 1.5) for *_, n in specs]
     ens = []
-    for graph in (False, True):
-        e = ca.Ensemble(ps, caps, use_graph=graph)
+    for graph, slots, periods, streams in ((False, 0, 0, 0), (True, 0xFFFFFFFF, 0, 0), (True, 1, 2, 1),
+                                           (True, 1, 2, 2)):
+        e = ca.Ensemble(ps, caps, use_graph=graph, group_slots=slots, group_periods=periods,
+                        group_streams=streams)
         for i, (*_, n) in enumerate(specs):
             E, I = synthetic_state(n, int(n * 1.05), seed=7 + i)
             e.upload(i, 0, *E)
@@ -122,17 +130,19 @@ def test_ensemble_graph_replays_match_eager(ca):
         for e in ens:
             e.run(160)
             e.synchronize()
-        ca_, cb = ens[0].counts(), ens[1].counts()
-        print("rep", rep, "counts eager", ca_, "graph", cb, flush=True)
-        for i in range(len(specs)):
-            assert ca_[0][i] > 0.5 * specs[i][2]
-            assert abs(int(ca_[0][i]) - int(cb[0][i])) <= 2 and abs(int(ca_[1][i]) - int(cb[1][i])) <= 2
-            a = sort_rows(*ens[0].download(i, 0))
-            b = sort_rows(*ens[1].download(i, 0))
-            m = min(len(a), len(b))
-            assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
-            pa, pb = ens[0].grid(i)[0], ens[1].grid(i)[0]
-            assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
+        ca_ = ens[0].counts()
+        for other in ens[1:]:
+            cb = other.counts()
+            print("rep", rep, "counts eager", ca_, "graph", cb, flush=True)
+            for i in range(len(specs)):
+                assert ca_[0][i] > 0.5 * specs[i][2]
+                assert abs(int(ca_[0][i]) - int(cb[0][i])) <= 2 and abs(int(ca_[1][i]) - int(cb[1][i])) <= 2
+                a = sort_rows(*ens[0].download(i, 0))
+                b = sort_rows(*other.download(i, 0))
+                m = min(len(a), len(b))
+                assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
+                pa, pb = ens[0].grid(i)[0], other.grid(i)[0]
+                assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
         for e in ens:
             e.run(3)
             e.run(77)
