@@ -53,6 +53,9 @@ ENSEMBLE_SPACE = {
     "ppt": [2, 1, 4],
     "lds_replicas": [0, 2, 1],
     "replicas": [8, 16, 32],
+    # electron slots per group of instances run a block at a time
+    # (0xFFFFFFFF: one group of all)
+    "group_slots": [9_000_000, 6_000_000, 12_000_000, 0xFFFFFFFF],
 }
 
 
@@ -167,7 +170,7 @@ def sim_kwargs(cfg, cus):
 
 def ensemble_kwargs(cfg, cus):
     return {"grid_blocks": cfg["grid_per_cu"] * cus, "ppt": cfg["ppt"], "lds_replicas": cfg["lds_replicas"],
-            "replicas": cfg["replicas"]}
+            "replicas": cfg["replicas"], "group_slots": cfg["group_slots"]}
 
 
 def tune_ensemble(params_list, capacities, states, steps=80, warmup=80, passes=2, use_cache=True, log=None):
