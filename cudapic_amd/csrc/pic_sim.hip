@@ -496,26 +496,47 @@ int graph_for(pic_sim *s, bool measure, pic_sim::GraphSlot **out) {
     return PIC_OK;
 }
 
-int run_steps(pic_sim *s, uint32_t n, bool measure) {
+// n steps: whole 4*N_SUB-step periods from the graph cache, the rest
+// eagerly.  dry: only capture the graphs those periods will replay (up to
+// the first eager step), then restore the host state, so that a timed run
+// does not include graph capture.
+int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
     if (s->cfg.path == PIC_PATH_FUSED && s->need_prologue) TRY(fused_prologue(s));
     const uint32_t period = 4u * (uint32_t)s->p.n_sub;
     const bool graphs = s->cfg.use_graph && s->cfg.path == PIC_PATH_FUSED &&
                         (uint32_t)s->p.n_t % period == 0;
-    while (n > 0) {
+    int roles0[5];
+    get_roles(s, roles0);
+    const uint32_t t0 = s->t;
+    const unsigned long long steps0 = s->steps;
+    int rc = PIC_OK;
+    while (n > 0 && rc == PIC_OK) {
         if (graphs && s->t % period == 0 && n >= period) {
             pic_sim::GraphSlot *g = nullptr;
-            TRY(graph_for(s, measure, &g));
-            PIC_HIP(hipGraphLaunch(g->exec, s->stream));
+            if ((rc = graph_for(s, measure, &g)) != PIC_OK) break;
+            if (!dry) {
+                const hipError_t e = hipGraphLaunch(g->exec, s->stream);
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "hipGraphLaunch");
+                    break;
+                }
+            }
             set_roles(s, g->after);
             s->t = (s->t + period) % (uint32_t)s->p.n_t;
             s->steps += period;
             n -= period;
             continue;
         }
-        TRY(one_step(s, measure));
+        if (dry) break;
+        rc = one_step(s, measure);
         --n;
     }
-    return PIC_OK;
+    if (dry) {
+        set_roles(s, roles0);
+        s->t = t0;
+        s->steps = steps0;
+    }
+    return rc;
 }
 
 int free_all(pic_sim *s) {
@@ -956,6 +977,7 @@ int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_to
         (void)hipEventDestroy(w0);
         (void)hipEventDestroy(w1);
     } else {
+        TRY(run_steps(s, n_steps, measure != 0, true));
         PIC_HIP(hipEventRecord(s->ev[0], s->stream));
         TRY(run_steps(s, n_steps, measure != 0));
         PIC_HIP(hipEventRecord(s->ev[1], s->stream));
