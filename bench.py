@@ -272,7 +272,8 @@ def main():
 def ensemble_figure():
     """The batched half of the metric: 64 discharges (V0 ~ U[100,400] V,
     p ~ logU[3,30] Pa, 50 k - 2 M particles each) in one launch per stage
-    against the same 64 run one after the other on the fused path."""
+    per cache-sized group of instances, against the same 64 run one after
+    the other on the fused path."""
     import cudapic_amd as ca
     from cudapic_amd import synthetic
     from cudapic_amd.ens_bench import instances

@@ -13,7 +13,8 @@ at the reference case's physical density: a plasma of 50k particles at
 W = 7e3 cannot shield a few hundred volts and runs away into an ionisation
 avalanche within a few hundred steps.  The same instances, from
 the same states, are run once as one ensemble (one Poisson launch and one
-particle launch per species per step for all of them, hipGraph replay) and
+particle launch per species per step for each cache-sized group of
+instances, each group a block of hipGraph periods at a time) and
 once one after the other as single-discharge simulations: on the fused fast
 path with its own hipGraph, and on the reference's kernel sequence (push,
 compaction, collisions and deposit as separate launches, main.rs:1725-1808).  Both runs advance every instance by the same

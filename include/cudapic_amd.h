@@ -272,7 +272,7 @@ int pic_sim_save(pic_sim_t *s, const char *path);
 int pic_sim_load(pic_sim_t *s, const char *path);
 
 /* ------------------------------------------------------------------------
- * Ensembles of independent discharges (one launch per stage for all)
+ * Ensembles of independent discharges (one launch per stage per group of instances)
  * ---------------------------------------------------------------------- */
 
 /* Instances share n_g, n_t, n_sub, frequency, gap and precision (taken from
