@@ -276,7 +276,7 @@ __device__ __forceinline__ bool i_test(const KPhys &k, float stot, float g, Xo &
     return rv < pc;
 }
 
-__device__ __forceinline__ int i_scatter(const KPhys &k, float c0, float c1, float &vx, float &vy,
+__device__ __forceinline__ int i_scatter([[maybe_unused]] const KPhys &k, float c0, float c1, float &vx, float &vy,
                                          float &vz, float ax, float ay, float az, float g, Xo &s) {
     const float gx = vx - ax, gy = vy - ay, gz = vz - az;
     const float t0 = c0;

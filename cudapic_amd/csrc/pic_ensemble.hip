@@ -396,7 +396,7 @@ int run(pic_ensemble *s, uint32_t n, bool measure, bool dry = false) {
     const uint32_t period = 4u * (uint32_t)s->p0.n_sub;
     const bool graphs = s->use_graph && (uint32_t)s->p0.n_t % period == 0;
     if (dry) {
-        int roles0[6], roles1[6];
+        int roles0[6];
         get_roles(s, roles0);
         const uint32_t t0 = s->t;
         int rc = PIC_OK;
@@ -409,7 +409,6 @@ int run(pic_ensemble *s, uint32_t n, bool measure, bool dry = false) {
                 set_roles(s, gs->after);
                 s->t = (s->t + period) % (uint32_t)s->p0.n_t;
             }
-            get_roles(s, roles1);
         }
         set_roles(s, roles0);
         s->t = t0;

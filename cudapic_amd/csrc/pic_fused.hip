@@ -402,7 +402,6 @@ __global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
     extern __shared__ __align__(16) float sh[];
     __shared__ uint32_t s_tiles[kMaxSegments + 1], s_cnt[kMaxSegments];
     __shared__ uint32_t s_ncoll;
-    constexpr int NF = ELECTRON ? kCollFieldsE : kCollFieldsI;
     const int n_g = k.n_g;
     const int K = j.n_seg;
     float *ef = sh;
