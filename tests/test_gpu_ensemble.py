@@ -151,6 +151,42 @@ def test_ensemble_graph_replays_match_eager(ca):
         e.close()
 
 
+def test_ensemble_timed_window_equals_plain_run(ca):
+    """A timed window of a temporally blocked ensemble first captures every
+    group's graphs (a dry pass) and must leave the host state where a plain
+    run of the same steps leaves it (collisionless, as the graph test)."""
+    specs = [(250.0, 1.5e5, 40_000), (150.0, 3.0e5, 20_000), (300.0, 2.0e5, 30_000)]
+    ps = [ca.params(v0=v, pressure=1e-9, weight=w, precision=2) for v, w, _ in specs]
+    caps = [int(n * 1.5) for *_, n in specs]
+    ens = []
+    for _ in range(2):
+        e = ca.Ensemble(ps, caps, group_slots=80_000, group_periods=2)
+        for i, (*_, n) in enumerate(specs):
+            E, I = synthetic_state(n, int(n * 1.05), seed=17 + i)
+            e.upload(i, 0, *E)
+            e.upload(i, 1, *I)
+        ens.append(e)
+    r = ens[0].time_steps(240)
+    ens[1].run(240)
+    assert r["particle_steps"] > 0 and r["ms_total"] > 0
+    for extra in (0, 160):
+        for e in ens:
+            e.run(extra)
+            e.synchronize()
+        ca_, cb = ens[0].counts(), ens[1].counts()
+        for i in range(len(specs)):
+            assert abs(int(ca_[0][i]) - int(cb[0][i])) <= 2 and abs(int(ca_[1][i]) - int(cb[1][i])) <= 2
+            a = sort_rows(*ens[0].download(i, 0))
+            b = sort_rows(*ens[1].download(i, 0))
+            m = min(len(a), len(b))
+            assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
+            pa, pb = ens[0].grid(i)[0], ens[1].grid(i)[0]
+            assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
+    for e in ens:
+        e.check()
+        e.close()
+
+
 def test_ensemble_init_cycle_statistics(ca):
     """Four copies of the reference case from 1000 e + 1000 i at rest end the
     init cycle near eduPIC's published 888 e / 3345 i; a fifth instance at

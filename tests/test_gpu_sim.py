@@ -384,6 +384,43 @@ def test_graph_replays_match_eager(ca, orc):
         s.close()
 
 
+def test_timed_window_equals_plain_run(ca, orc):
+    """A timed window first captures the graphs it will replay (a dry pass
+    over the same periods) and must leave the host state where a plain run
+    of the same steps leaves it: collisionless FP64 runs, one timed and one
+    not, agree after the window and after more steps."""
+    p = ca.params(weight=2.0e3, precision=2)
+    po = orc.params(weight=2.0e3)
+    E, I = synthetic_state(100_000, 110_000, seed=52)
+    cs, se0, si0 = zero_tables(orc, po)
+    sims = []
+    for _ in range(2):
+        s = make_sim(ca, p, 200_000, 1, graph=True)
+        s.set_tables(cs, se0, si0)
+        s.upload(0, *E)
+        s.upload(1, *I)
+        sims.append(s)
+    r = sims[0].time_steps(880)
+    sims[1].run(880)
+    assert r["particle_steps"] > 0 and r["ms_total"] > 0
+    for extra in (0, 80):
+        for s in sims:
+            s.run(extra)
+            s.synchronize()
+        ce = [s.counts() for s in sims]
+        assert abs(ce[0][0] - ce[1][0]) <= 2 and abs(ce[0][1] - ce[1][1]) <= 2, ce
+        a = sort_rows(*sims[0].download(0))
+        b = sort_rows(*sims[1].download(0))
+        m = min(len(a), len(b))
+        assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
+        pa, pb = sims[0].grid()[0], sims[1].grid()[0]
+        assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
+    for s in sims:
+        assert s.time()[0] == 960
+        s.check()
+        s.close()
+
+
 def test_checkpoint_round_trip(ca, orc, tmp_path):
     p = ca.params(weight=2.0e3)
     E, I = synthetic_state(50_000, 52_000, seed=61)
