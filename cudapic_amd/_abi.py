@@ -117,6 +117,8 @@ SIGNATURES = {
     "pic_sim_read_grid": (I32, [P, P, P, P, P]),
     "pic_sim_read_measurements": (I32, [P, P, P, P, P, P]),
     "pic_sim_reset_measurements": (I32, [P]),
+    "pic_sim_set_xt": (I32, [P, U32]),
+    "pic_sim_read_xt": (I32, [P, P, C.POINTER(U32)]),
     "pic_sim_time_steps": (I32, [P, U32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(U32),
                                  C.POINTER(C.c_ulonglong)]),
     "pic_sim_save": (I32, [P, C.c_char_p]),

@@ -8,7 +8,11 @@
 //   particle_growth_<ts>.csv                       step,n_e,n_i every 100 cycles
 //   density_avg_<ts>.csv, eepf_<ts>.csv            with --measure
 // and, with --measure, ifed_<ts>.csv and info_<ts>.txt (electrode particle
-// fluxes and mean ion energies, the quantities of C/eduPIC.cc:1002-1008).
+// fluxes and mean ion energies, the quantities of C/eduPIC.cc:1002-1008);
+// with --measure and --xt BIN also eduPIC's spatio-temporal maps pot_xt.dat,
+// efield_xt.dat, ne_xt.dat, ni_xt.dat, je_xt.dat, ji_xt.dat, powere_xt.dat,
+// poweri_xt.dat, meanee_xt.dat, meanei_xt.dat, ioniz_xt.dat (N_G rows of
+// N_T / BIN columns, C/eduPIC.cc:847-919; eduPIC's BIN is 20).
 //
 // Options beyond the reference's compile-time constants:
 //   --path fused|reference   --precision fp32|comp|fp64   --no-graph
@@ -53,7 +57,7 @@ void usage() {
                  "       [--precision fp32|comp|fp64] [--no-graph] [--weight W] [--voltage V] [--pressure P]\n"
                  "       [--ng N] [--capacity N] [--n-init N] [--thermal-init] [--seed S]\n"
                  "       [--load FILE] [--save FILE] [--out DIR]\n"
-                 "       [--import-edupic FILE] [--export-edupic FILE] [--int-headers]\n");
+                 "       [--import-edupic FILE] [--export-edupic FILE] [--int-headers] [--xt BIN]\n");
 }
 
 // xoshiro128+ for the seeded initial positions (the reference draws them from
@@ -163,6 +167,7 @@ int main(int argc, char **argv) {
     long measurement_cycles = 0;
     std::string path = "fused", precision = "fp64", load, save, out_root = "results", import_ed, export_ed;
     bool int_headers = false;
+    uint32_t xt_bin = 0;
     double weight = 7.0e4, voltage = 250.0, pressure = 10.0;
     int n_g = 400;
     long capacity = 120000, n_init = 1000;
@@ -195,6 +200,7 @@ int main(int argc, char **argv) {
         else if (a == "--import-edupic") import_ed = val();
         else if (a == "--export-edupic") export_ed = val();
         else if (a == "--int-headers") int_headers = true;
+        else if (a == "--xt") xt_bin = (uint32_t)std::strtoul(val().c_str(), nullptr, 10);
         else if (a == "--out") out_root = val();
         else {
             usage();
@@ -276,6 +282,7 @@ int main(int argc, char **argv) {
     }
     std::printf(">> cudaPIC: running %ld cycles x%d steps (%s path)...\n", num_cycles, p.n_t, path.c_str());
     auto t0 = std::chrono::steady_clock::now();
+    if (measure && xt_bin) CK(pic_sim_set_xt(sim, xt_bin));
     if (measure) CK(pic_sim_reset_measurements(sim));
     for (long cycle = 0; cycle < num_cycles; ++cycle) {
         const bool in_window = measure && cycle >= measurement_start;
@@ -396,8 +403,27 @@ int main(int argc, char **argv) {
             std::fprintf(f, "measured time                         = %12.3e [s]\n", t_meas);
             std::fclose(f);
         }
-        std::printf(">> cudaPIC: measurement data saved (density_avg, eepf, ifed, info) for %ld cycles\n",
-                    measurement_cycles);
+        if (xt_bin) {
+            // save_all_xt (C/eduPIC.cc:905-919): N_G lines of N_XT values
+            uint32_t n_xt = 0;
+            CK(pic_sim_read_xt(sim, nullptr, &n_xt));
+            std::vector<double> xt((size_t)11 * p.n_g * n_xt);
+            CK(pic_sim_read_xt(sim, xt.data(), &n_xt));
+            const char *names[11] = {"pot", "efield", "ne", "ni", "je", "ji",
+                                     "powere", "poweri", "meanee", "meanei", "ioniz"};
+            for (int q = 0; q < 11; ++q) {
+                FILE *fx = std::fopen((dir + "/" + names[q] + "_xt.dat").c_str(), "w");
+                if (!fx) continue;
+                for (int k = 0; k < p.n_g; ++k) {
+                    for (uint32_t b = 0; b < n_xt; ++b)
+                        std::fprintf(fx, "%e  ", xt[((size_t)q * p.n_g + k) * n_xt + b]);
+                    std::fprintf(fx, "\n");
+                }
+                std::fclose(fx);
+            }
+        }
+        std::printf(">> cudaPIC: measurement data saved (density_avg, eepf, ifed, info%s) for %ld cycles\n",
+                    xt_bin ? ", xt" : "", measurement_cycles);
     }
     if (!export_ed.empty()) {
         EduState st;

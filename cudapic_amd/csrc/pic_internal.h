@@ -84,6 +84,12 @@ struct PoissonJob {
     uint32_t *zero_cnt_ion = nullptr; int zero_cnt_ion_words = 0;  // cleared on ion steps
     const uint32_t *copy_cnt_src = nullptr; uint32_t *copy_cnt_dst = nullptr; int copy_cnt_words = 0;
     uint32_t *advance_time = nullptr;  // == time: *time <- (*time + 1) % n_t after the solve
+    // XT maps (measurement steps, with cumul_e): pot, E, n_e, n_i of the step
+    // added to [4][n_g][xt_n] at time bin t / xt_bin; xt_count[bin] += 1
+    double *xt_grid = nullptr;
+    unsigned long long *xt_count = nullptr;
+    uint32_t xt_bin = 0;
+    int xt_n = 0;
 };
 int launch_poisson(const PoissonJob &j, hipStream_t s);
 // *out = pot0 of the step *time (the waveform the Poisson kernel evaluates)
@@ -133,6 +139,29 @@ struct CollideIJob {
 int launch_collide_i(const CollideIJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
 
 int launch_accum_density(const void *dens, bool f64, double *cumul, int n_g, hipStream_t s);
+// XT maps of one species (C/eduPIC.cc:558-575, 603-614): for every live
+// particle of a segmented array, with the mid-step velocity u = vx + 0.5
+// factor E(x) and the CIC weights c of its two nodes, adds c, c u, c energy
+// and (electrons) c sigma_ion(energy) |v| dt_e n_gas into xt[q][node][bin],
+// bin = t / bin_steps for the step t the clock held before the Poisson solve
+// of this step advanced it.  Electrons 4 quantities, ions 3.
+struct XtJob {
+    bool electrons;
+    const float *x, *vx, *vy, *vz;
+    const uint32_t *cnt;
+    int n_seg;
+    uint32_t seg_size;
+    const float *efield;
+    const float *sigma_ion;  // electrons: ionisation cross section [PIC_CS_RANGES]
+    float gas_density;
+    const uint32_t *clock;
+    int n_t;
+    uint32_t bin_steps;
+    int n_xt;
+    double *xt;
+};
+int launch_xt(const XtJob &j, const KPhys &k, hipStream_t s);
+
 int launch_eepf(const float *x, const float *vx, const float *vy, const float *vz,
                 const float *efield, const uint32_t *n_active, uint32_t *eepf, const KPhys &k,
                 const Shape &sh, hipStream_t s, uint32_t n_cap = 0xffffffffu);

@@ -352,6 +352,10 @@ struct PoissonArgs {
     uint32_t *copy_cnt_dst;
     int copy_cnt_words;
     uint32_t *advance_time;
+    double *xt_grid;
+    unsigned long long *xt_count;
+    uint32_t xt_bin;
+    int xt_n;
 };
 
 // Node k of one density summed over the global replicas, in replica order.
@@ -517,6 +521,7 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     T *wtot = pot_s + a.g.n_g;               // 17
     __shared__ double s_pot0;
     __shared__ int s_ion;
+    __shared__ uint32_t s_t;
     const int tid = threadIdx.x;
     // the clock, the step's pot0, the counters to copy and the densities are
     // loaded together
@@ -543,9 +548,27 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
         }
         s_pot0 = p0;
         s_ion = ion;
+        s_t = t;
     }
     __syncthreads();
     poisson_solve<T>(a.g, dne, dni, s_pot0, s_ion != 0, a.pot, a.efield, a.cumul_e, a.cumul_i, pot_s, wtot);
+    if (a.xt_grid && a.cumul_e) {
+        // XT maps of the grid quantities (C/eduPIC.cc:705-715); each lane
+        // reads back the pot and E it wrote
+        const uint32_t tb = s_t / a.xt_bin;
+        const size_t plane = (size_t)a.g.n_g * a.xt_n;
+        const int k0 = tid * a.g.per;
+#pragma unroll
+        for (int u = 0; u < kPoissonMaxPer; ++u)
+            if (u < a.g.per && k0 + u < a.g.n_g) {
+                const size_t o = (size_t)(k0 + u) * a.xt_n + tb;
+                a.xt_grid[o] += (double)a.pot[k0 + u];
+                a.xt_grid[plane + o] += (double)a.efield[k0 + u];
+                a.xt_grid[2 * plane + o] += dne[u];
+                a.xt_grid[3 * plane + o] += dni[u];
+            }
+        if (tid == 0) a.xt_count[tb] += 1ull;
+    }
     if (a.dens_e_out) {
         const int k0 = tid * a.g.per;
 #pragma unroll
@@ -627,6 +650,15 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     a.pot0_next = j.pot0_next;
     a.pot0_tab = j.pot0_tab;
     a.i_sum = j.i_sum;
+    a.xt_grid = j.xt_grid;
+    a.xt_count = j.xt_count;
+    a.xt_bin = j.xt_bin;
+    a.xt_n = j.xt_n;
+    if (a.xt_grid && (!a.time || !a.xt_count || a.xt_bin == 0 || a.xt_n <= 0 ||
+                      (uint32_t)a.xt_n * a.xt_bin != (uint32_t)j.n_t)) {
+        set_error("poisson: XT maps need the device clock and xt_bin * xt_n == n_t");
+        return PIC_ERR_ARG;
+    }
     a.dens_e_out = j.dens_e_out;
     a.dens_i_out = j.dens_i_out;
     a.n_t = j.n_t;
@@ -1366,6 +1398,72 @@ __global__ __launch_bounds__(256) void k_eepf(const float *__restrict__ x, const
     if (!__syncthreads_or(any)) return;
     for (int b = threadIdx.x; b < kEepfBins; b += blockDim.x)
         if (h[b]) atomicAdd(&eepf[b], h[b]);
+}
+
+template <bool ELECTRON>
+__global__ __launch_bounds__(256) void k_xt(XtJob j, KPhys k) {
+    extern __shared__ float acc[];  // [NQ][n_g]
+    constexpr int NQ = ELECTRON ? 4 : 3;
+    const int n_g = k.n_g;
+    for (int q = threadIdx.x; q < NQ * n_g; q += blockDim.x) acc[q] = 0.0f;
+    __syncthreads();
+    const float half_kick = 0.5f * (ELECTRON ? k.factor_e : k.factor_i);
+    const float to_ev = ELECTRON ? k.half_me_over_e : k.half_mi_over_e;
+    const float rate_f = k.dt_e * j.gas_density;
+    for (int sg = 0; sg < j.n_seg; ++sg) {
+        const uint32_t n = min(j.cnt[sg], j.seg_size);
+        const size_t base = (size_t)sg * j.seg_size;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const float x = j.x[base + i];
+            if (!(x >= 0.0f)) continue;  // a hole
+            const float pos = x * k.inv_dx;
+            const int p = cell_of(pos, n_g);
+            const float c2 = pos - (float)p, c1 = 1.0f - c2;
+            const float ex = c1 * j.efield[p] + c2 * j.efield[p + 1];
+            const float u = j.vx[base + i] + half_kick * ex;
+            const float vy = j.vy[base + i], vz = j.vz[base + i];
+            const float v2 = u * u + vy * vy + vz * vz;
+            const float en = to_ev * v2;
+            atomicAdd(&acc[p], c1);
+            atomicAdd(&acc[p + 1], c2);
+            atomicAdd(&acc[n_g + p], c1 * u);
+            atomicAdd(&acc[n_g + p + 1], c2 * u);
+            atomicAdd(&acc[2 * n_g + p], c1 * en);
+            atomicAdd(&acc[2 * n_g + p + 1], c2 * en);
+            if (ELECTRON) {
+                const float rate = j.sigma_ion[energy_index(en, k.de_cs)] * sqrtf(v2) * rate_f;
+                atomicAdd(&acc[3 * n_g + p], c1 * rate);
+                atomicAdd(&acc[3 * n_g + p + 1], c2 * rate);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t t = (*j.clock + (uint32_t)j.n_t - 1u) % (uint32_t)j.n_t;
+    const uint32_t tb = t / j.bin_steps;
+    for (int q = threadIdx.x; q < NQ * n_g; q += blockDim.x)
+        if (acc[q] != 0.0f) {
+            const int qq = q / n_g, node = q - qq * n_g;
+            atomicAdd(&j.xt[((size_t)qq * n_g + node) * j.n_xt + tb], (double)acc[q]);
+        }
+}
+
+int launch_xt(const XtJob &j, const KPhys &k, hipStream_t s) {
+    const int nq = j.electrons ? 4 : 3;
+    const size_t lds = (size_t)nq * k.n_g * sizeof(float);
+    if (!j.x || !j.vx || !j.vy || !j.vz || !j.cnt || !j.efield || !j.clock || !j.xt || j.n_seg < 1 ||
+        j.bin_steps == 0 || j.n_xt <= 0 || (j.electrons && !j.sigma_ion) || k.n_g < 2) {
+        set_error("xt: bad arguments");
+        return PIC_ERR_ARG;
+    }
+    if (lds > (size_t)device_lds_per_block()) {
+        set_error("xt: n_g too large for the per-workgroup accumulators");
+        return PIC_ERR_UNSUPPORTED;
+    }
+    const int grid = device_cus();
+    if (j.electrons) k_xt<true><<<grid, 256, lds, s>>>(j, k);
+    else k_xt<false><<<grid, 256, lds, s>>>(j, k);
+    PIC_LAUNCHED("k_xt");
+    return PIC_OK;
 }
 
 int launch_eepf(const float *x, const float *vx, const float *vy, const float *vz,

@@ -74,6 +74,14 @@ struct pic_sim {
     double *cumul_e = nullptr, *cumul_i = nullptr;
     uint32_t *eepf = nullptr, *ifed = nullptr;
     unsigned long long *counters = nullptr;
+    // XT maps (fused path, measurement steps): kXtPlanes raw planes of
+    // [n_g][xt_n] -- pot, E, n_e, n_i of the grid; electron weight, weighted
+    // mid-step velocity, energy and ionisation rate; ion weight, velocity,
+    // energy -- and the measured steps of each time bin
+    double *xt = nullptr;
+    unsigned long long *xt_count = nullptr;
+    uint32_t xt_bin = 0;
+    int xt_n = 0;
     // tables
     float *cs = nullptr, *stot_e = nullptr, *stot_i = nullptr;
     // bookkeeping
@@ -251,6 +259,8 @@ int step_reference(pic_sim *s, bool measure) {
 }
 
 // ---------------------------------------------------------------- fused path
+constexpr int kXtPlanes = 11;
+
 uint64_t dens_words32(pic_sim *s) { return (uint64_t)s->dens_words * (s->f64 ? 2 : 1); }
 
 // Deposit the current particles into the accumulators the next Poisson
@@ -330,8 +340,49 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
         pj.cumul_e = s->cumul_e;
         pj.cumul_i = s->cumul_i;
         pj.meas_steps = s->counters + 6;
+        if (s->xt) {
+            pj.xt_grid = s->xt;
+            pj.xt_count = s->xt_count;
+            pj.xt_bin = s->xt_bin;
+            pj.xt_n = s->xt_n;
+        }
     }
     TRY(launch_poisson(pj, st));
+    if (measure && s->xt) {
+        // particle XT quantities from the positions and velocities before
+        // this step's push, with this step's field (C/eduPIC.cc:558-575, 603-614)
+        const size_t plane = (size_t)s->p.n_g * s->xt_n;
+        XtJob xj;
+        memset(&xj, 0, sizeof(xj));
+        xj.n_seg = s->nseg;
+        xj.seg_size = s->seg;
+        xj.efield = s->efield;
+        xj.sigma_ion = s->cs + 2 * (size_t)PIC_CS_RANGES;
+        xj.gas_density = (float)s->p.gas_density;
+        xj.clock = s->cnt + C_TIME;
+        xj.n_t = s->p.n_t;
+        xj.bin_steps = s->xt_bin;
+        xj.n_xt = s->xt_n;
+        if (ion) {
+            XtJob xi = xj;
+            xi.electrons = false;
+            xi.x = s->ion[s->ci][0];
+            xi.vx = s->ion[s->ci][1];
+            xi.vy = s->ion[s->ci][2];
+            xi.vz = s->ion[s->ci][3];
+            xi.cnt = icnt(s, s->ci);
+            xi.xt = s->xt + 8 * plane;
+            TRY(launch_xt(xi, s->k, st));
+        }
+        xj.electrons = true;
+        xj.x = s->e[s->ce][0];
+        xj.vx = s->e[s->ce][1];
+        xj.vy = s->e[s->ce][2];
+        xj.vz = s->e[s->ce][3];
+        xj.cnt = ecnt(s, s->pe);
+        xj.xt = s->xt + 4 * plane;
+        TRY(launch_xt(xj, s->k, st));
+    }
     const int i_acc = ion ? s->hi : 1 - s->hi;  // accumulator for the next ion step
     FusedJob base;
     memset(&base, 0, sizeof(base));
@@ -550,6 +601,8 @@ int free_all(pic_sim *s) {
         (void)hipFree(s->e_rng[c]);
         (void)hipFree(s->i_rng[c]);
     }
+    (void)hipFree(s->xt);
+    (void)hipFree(s->xt_count);
     void *bufs[] = {s->cnt, s->pot, s->efield, s->grid_e, s->grid_i, s->pot0_tab, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->cumul_e, s->cumul_i, s->eepf, s->ifed, s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
@@ -924,6 +977,90 @@ int pic_sim_reset_measurements(pic_sim_t *s) {
     PIC_HIP(hipMemset(s->ifed, 0, sizeof(uint32_t) * 2 * PIC_N_IFED));
     PIC_HIP(hipMemset(s->counters, 0, sizeof(unsigned long long) * kCounterLanes * kMeasCounters));
     s->ref_meas[0] = s->ref_meas[1] = 0;
+    if (s->xt) {
+        PIC_HIP(hipMemset(s->xt, 0, sizeof(double) * kXtPlanes * s->p.n_g * s->xt_n));
+        PIC_HIP(hipMemset(s->xt_count, 0, sizeof(unsigned long long) * s->xt_n));
+    }
+    return PIC_OK;
+}
+
+int pic_sim_set_xt(pic_sim_t *s, uint32_t bin_steps) {
+    if (!s) return PIC_ERR_ARG;
+    if (bin_steps && (s->cfg.path != PIC_PATH_FUSED || (uint32_t)s->p.n_t % bin_steps != 0)) {
+        set_error("pic_sim_set_xt: XT maps are measured on the fused path, with bin_steps dividing n_t");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    invalidate_graphs(s);  // measurement graphs gain or lose the XT launches
+    (void)hipFree(s->xt);
+    (void)hipFree(s->xt_count);
+    s->xt = nullptr;
+    s->xt_count = nullptr;
+    s->xt_bin = 0;
+    s->xt_n = 0;
+    if (!bin_steps) return PIC_OK;
+    const int n = (int)((uint32_t)s->p.n_t / bin_steps);
+    int rc;
+    if ((rc = dalloc(&s->xt, (size_t)kXtPlanes * s->p.n_g * n)) != PIC_OK) return rc;
+    if ((rc = dalloc(&s->xt_count, (size_t)n)) != PIC_OK) return rc;
+    s->xt_bin = bin_steps;
+    s->xt_n = n;
+    return PIC_OK;
+}
+
+int pic_sim_read_xt(pic_sim_t *s, double *out, uint32_t *n_xt) {
+    if (!s) return PIC_ERR_ARG;
+    if (n_xt) *n_xt = (uint32_t)s->xt_n;
+    if (!out) return PIC_OK;
+    if (!s->xt) {
+        set_error("pic_sim_read_xt: XT maps are not enabled (pic_sim_set_xt)");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(s->stream));
+    const int n_g = s->p.n_g, nx = s->xt_n;
+    const size_t plane = (size_t)n_g * nx;
+    std::vector<double> raw(kXtPlanes * plane);
+    std::vector<unsigned long long> cnt(nx);
+    PIC_HIP(hipMemcpy(raw.data(), s->xt, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+    PIC_HIP(hipMemcpy(cnt.data(), s->xt_count, sizeof(unsigned long long) * nx, hipMemcpyDeviceToHost));
+    // norm_all_xt (C/eduPIC.cc:864-903), with the measured steps of each
+    // bin in place of cycles * N_BIN
+    const double e = 1.60217662e-19;
+    const double rate_norm = s->p.weight / (s->p.area * s->p.dx) / s->p.dt_e;
+    enum { POT, EF, NE, NI, CE, UE, EE, IZ, CI, UI, EI };
+    enum { O_POT, O_EF, O_NE, O_NI, O_JE, O_JI, O_PE, O_PI, O_EE, O_EI, O_IZ };
+    for (int i = 0; i < n_g; ++i)
+        for (int jb = 0; jb < nx; ++jb) {
+            const size_t o = (size_t)i * nx + jb;
+            auto r = [&](int q) { return raw[q * plane + o]; };
+            auto w = [&](int q, double v) { out[q * plane + o] = v; };
+            const double f1 = cnt[jb] ? 1.0 / (double)cnt[jb] : 0.0;
+            const double f2 = cnt[jb] ? rate_norm / (double)cnt[jb] : 0.0;
+            const double pot = r(POT) * f1, ef = r(EF) * f1, ne = r(NE) * f1, ni = r(NI) * f1;
+            double je = 0.0, ee = 0.0, iz = 0.0, ji = 0.0, ei = 0.0;
+            if (r(CE) > 0.0) {
+                const double ue = r(UE) / r(CE);
+                je = -ue * ne * e;
+                ee = r(EE) / r(CE);
+                iz = r(IZ) * f2;
+            }
+            if (r(CI) > 0.0) {
+                const double ui = r(UI) / r(CI);
+                ji = ui * ni * e;
+                ei = r(EI) / r(CI);
+            }
+            w(O_POT, pot);
+            w(O_EF, ef);
+            w(O_NE, ne);
+            w(O_NI, ni);
+            w(O_JE, je);
+            w(O_JI, ji);
+            w(O_PE, je * ef);
+            w(O_PI, ji * ef);
+            w(O_EE, ee);
+            w(O_EI, ei);
+            w(O_IZ, iz);
+        }
     return PIC_OK;
 }
 

@@ -141,6 +141,22 @@ class Sim:
     def reset_measurements(self):
         call("pic_sim_reset_measurements", self.h)
 
+    # eduPIC's XT file names (C/eduPIC.cc:905-919), in pic_sim_read_xt's order
+    XT_NAMES = ("pot", "efield", "ne", "ni", "je", "ji", "powere", "poweri", "meanee", "meanei", "ioniz")
+
+    def set_xt(self, bin_steps=20):
+        """Measure the spatio-temporal maps in time bins of bin_steps steps
+        (0 switches them off)."""
+        call("pic_sim_set_xt", self.h, int(bin_steps))
+
+    def xt(self):
+        """The normalised XT maps: {name: array[n_g, n_t / bin_steps]}."""
+        n = C.c_uint32()
+        call("pic_sim_read_xt", self.h, None, C.byref(n))
+        out = np.zeros((len(self.XT_NAMES), self.p.n_g, n.value), np.float64)
+        call("pic_sim_read_xt", self.h, out.ctypes.data, C.byref(n))
+        return dict(zip(self.XT_NAMES, out))
+
     def save(self, path):
         call("pic_sim_save", self.h, str(path).encode())
 

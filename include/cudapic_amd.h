@@ -258,6 +258,18 @@ int pic_sim_read_grid(pic_sim_t *s, float *pot, float *efield, double *e_dens, d
 int pic_sim_read_measurements(pic_sim_t *s, double *cumul_e, double *cumul_i, uint32_t *eepf,
                               uint32_t *ifed, unsigned long long *counters);
 int pic_sim_reset_measurements(pic_sim_t *s);
+/* Spatio-temporal (XT) maps of eduPIC's measurement mode (C/eduPIC.cc:126-145,
+ * 558-575, 603-614, 705-715), fused path only: on every measured step the
+ * grid quantities and the CIC-weighted mid-step particle quantities are
+ * added into time bins of bin_steps steps (bin_steps divides n_t; eduPIC
+ * uses 20).  0 switches them off.  No reference counterpart in cudaPIC,
+ * which measures densities and the EEPF only. */
+int pic_sim_set_xt(pic_sim_t *s, uint32_t bin_steps);
+/* The 11 maps normalised as norm_all_xt (C/eduPIC.cc:864-903) with each
+ * bin's measured steps: out[q][k][b], q in eduPIC's file order pot, efield,
+ * ne, ni, je, ji, powere (heate), poweri (heati), meanee, meanei, ioniz;
+ * k the node, b < *n_xt = n_t / bin_steps the time bin.  out NULL: *n_xt only. */
+int pic_sim_read_xt(pic_sim_t *s, double *out, uint32_t *n_xt);
 
 /* Timing: run n_steps and report wall milliseconds (HIP events on the sim
  * stream) and the summed duration of the dominant particle kernel

@@ -45,6 +45,28 @@ def test_cli_run_writes_reference_outputs(tmp_path, path):
     assert "state loaded" in r2.stdout
 
 
+def test_cli_writes_xt_maps(tmp_path):
+    """--xt 20 writes eduPIC's eleven spatio-temporal maps (N_G rows of
+    N_T / 20 time bins) for the measured cycle; averaged over the bins, the
+    density maps are the density_avg profiles of the same cycle (every bin
+    holds 20 steps and one ion step)."""
+    out = tmp_path / "results"
+    r = subprocess.run([CLI, "2", "--measure", "1", "--xt", "20", "--out", str(out)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = glob.glob(str(out / "*"))[0]
+    for name in ("pot", "efield", "ne", "ni", "je", "ji", "powere", "poweri", "meanee", "meanei", "ioniz"):
+        a = np.loadtxt(os.path.join(d, name + "_xt.dat"))
+        assert a.shape == (400, 200), (name, a.shape)
+        assert np.isfinite(a).all()
+    avg = np.loadtxt(glob.glob(os.path.join(d, "density_avg_*"))[0], delimiter=",", skiprows=1)
+    for name, col in (("ne", 1), ("ni", 2)):
+        m = np.loadtxt(os.path.join(d, name + "_xt.dat"))
+        assert (m >= 0).all() and avg[:, col].sum() > 0
+        # %e keeps 7 significant digits
+        np.testing.assert_allclose(m.mean(axis=1), avg[:, col], rtol=2e-6, atol=1e-6 * avg[:, col].max())
+
+
 def test_cli_usage_error():
     r = subprocess.run([CLI], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "usage" in r.stderr

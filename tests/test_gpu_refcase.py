@@ -1,7 +1,9 @@
 """Full runs of the eduPIC reference case (250 V, 10 Pa, 350 K, W = 7e4,
 N_G = 400) against eduPIC's published converged outputs
 (C/ReferenceCase_RawData: Figure9_conv, Figure10_density, Figure12_eepf,
-Figure13_ifed, kept as tests/golden/refcase_*.dat).
+Figure13_ifed, kept as tests/golden/refcase_*.dat, and the XT maps of
+Figure11a-k reduced to blocks of 10 nodes x 5 time bins,
+tests/golden/refcase_xt_coarse.npz, made by make_refcase_xt.py).
 
 The runs start from a converged state of this engine
 (cudapic_amd/data/refcase_state.npz: 1500 RF cycles from a synthetic state
@@ -11,7 +13,12 @@ error.  Measured on the MI355X when the tolerances were set (fused path):
 counts +0.9 % / +0.8 % of eduPIC's converged 107.0 k e / 112.5 k i,
 density profiles within 1.5 % (L2) and 2.2 % (max, of the peak), EEPF
 within 1.5 % (mean |log ratio| over 0-20 eV), IFED mean energies within
-0.5 %, electron and ion fluxes at the electrodes balanced to 0.03 %.  The
+0.5 %, electron and ion fluxes at the electrodes balanced to 0.03 %, XT
+maps (block means, L2) pot 0.15 %, E 0.24 %, n_e / n_i 1.5 %, j_e 1.1 %,
+j_i 0.6 %, electron / ion power 1.9 % / 0.8 %, mean ion energy 0.7 %,
+ionisation rate 1.8 %, mean electron energy 0.85 % where n_e is above 5 %
+of its peak (in the expanded sheath a block holds a handful of electrons,
+and eduPIC writes 0 where a bin saw none).  The
 tolerances are about 3x those figures: the batch standard error of the
 centre density is 0.1-0.3 % of the peak, and what remains is the published
 data's own noise and run-to-run spread."""
@@ -36,6 +43,29 @@ def _published():
     return d[:, 1], d[:, 2], conv[-100:, 1].mean(), conv[-100:, 2].mean()
 
 
+# L2 of block-mean differences relative to the published block means, about
+# 3x what the MI355X measured
+XT_TOL = {"pot": 0.005, "efield": 0.008, "ne": 0.05, "ni": 0.05, "je": 0.035, "ji": 0.02, "powere": 0.06,
+          "poweri": 0.025, "meanei": 0.02, "ioniz": 0.055}
+
+
+def _check_xt(xt):
+    pub = np.load(os.path.join(GOLDEN, "refcase_xt_coarse.npz"))
+    bn, bb = (int(v) for v in pub["block"])
+    for name, m in xt.items():
+        n, t = m.shape
+        ours = m.reshape(n // bn, bn, t // bb, bb).mean(axis=(1, 3))
+        ref = pub[name]
+        if name == "meanee":
+            sel = pub["ne"] > 0.05 * pub["ne"].max()
+            dev = np.linalg.norm((ours - ref)[sel]) / np.linalg.norm(ref[sel])
+            assert dev < 0.03, (name, dev)
+            continue
+        dev = np.linalg.norm(ours - ref) / np.linalg.norm(ref)
+        assert dev < XT_TOL[name], (name, dev)
+        assert np.corrcoef(ours.ravel(), ref.ravel())[0, 1] > 0.999, name
+
+
 def _check_density(ne, ni, ne_pub, ni_pub, se_e=None, se_i=None):
     for ours, pub, se in ((ne, ne_pub, se_e), (ni, ni_pub, se_i)):
         dev = ours - pub
@@ -55,6 +85,7 @@ def test_reference_case_converged_statistics():
         sim.upload(0, *E)
         sim.upload(1, *I)
         sim.run(4000 * RELAX)
+        sim.set_xt(20)  # eduPIC's N_BIN
         sim.reset_measurements()
         prev, ne_b, ni_b = None, [], []
         for b in range(BATCHES):
@@ -72,6 +103,8 @@ def test_reference_case_converged_statistics():
         sim.check()
         n_e, n_i = sim.counts()
         m = sim.measurements()
+        xt = sim.xt()
+    _check_xt(xt)
     c = m["counters"]
     assert c["e_meas_steps"] == 4000 * MEASURE and c["i_meas_steps"] == 200 * MEASURE
     # particle numbers of the converged state
