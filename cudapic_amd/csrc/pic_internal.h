@@ -159,7 +159,10 @@ struct XtJob {
     uint32_t bin_steps;
     int n_xt;
     double *xt;
+    float *stage;  // per-workgroup partial maps, xt_stage_floats(n_g) floats
 };
+// staging floats launch_xt needs for n_g nodes
+size_t xt_stage_floats(int n_g);
 int launch_xt(const XtJob &j, const KPhys &k, hipStream_t s);
 
 int launch_eepf(const float *x, const float *vx, const float *vy, const float *vz,

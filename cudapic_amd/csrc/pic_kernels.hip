@@ -1400,17 +1400,29 @@ __global__ __launch_bounds__(256) void k_eepf(const float *__restrict__ x, const
         if (h[b]) atomicAdd(&eepf[b], h[b]);
 }
 
+// Fixed-point scales of the XT partial sums (exact 64-bit integer sums in
+// LDS: float LDS atomics run about ten times slower than integer ones on
+// gfx950): CIC weight 2^32, weighted velocity 2^16 per m/s, weighted energy
+// 2^28 per eV, weighted ionisation probability 2^44.  A workgroup sums at
+// most the particles of its share of one segment.
+__device__ constexpr float kXtScale[4] = {4294967296.0f, 65536.0f, 268435456.0f, 17592186044416.0f};
+
 template <bool ELECTRON>
 __global__ __launch_bounds__(256) void k_xt(XtJob j, KPhys k) {
-    extern __shared__ float acc[];  // [NQ][n_g]
+    extern __shared__ unsigned long long acc[];  // [NQ][n_g], two's complement
     constexpr int NQ = ELECTRON ? 4 : 3;
     const int n_g = k.n_g;
-    for (int q = threadIdx.x; q < NQ * n_g; q += blockDim.x) acc[q] = 0.0f;
+    for (int q = threadIdx.x; q < NQ * n_g; q += blockDim.x) acc[q] = 0ull;
     __syncthreads();
     const float half_kick = 0.5f * (ELECTRON ? k.factor_e : k.factor_i);
     const float to_ev = ELECTRON ? k.half_me_over_e : k.half_mi_over_e;
     const float rate_f = k.dt_e * j.gas_density;
-    for (int sg = 0; sg < j.n_seg; ++sg) {
+    auto add = [&](int q, int node, float v) {
+        atomicAdd(&acc[q * n_g + node], (unsigned long long)__float2ll_rn(v * kXtScale[q]));
+    };
+    {
+        // blockIdx.y: the segment; the x workgroups of a segment stride over it
+        const int sg = blockIdx.y;
         const uint32_t n = min(j.cnt[sg], j.seg_size);
         const size_t base = (size_t)sg * j.seg_size;
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1424,33 +1436,54 @@ __global__ __launch_bounds__(256) void k_xt(XtJob j, KPhys k) {
             const float vy = j.vy[base + i], vz = j.vz[base + i];
             const float v2 = u * u + vy * vy + vz * vz;
             const float en = to_ev * v2;
-            atomicAdd(&acc[p], c1);
-            atomicAdd(&acc[p + 1], c2);
-            atomicAdd(&acc[n_g + p], c1 * u);
-            atomicAdd(&acc[n_g + p + 1], c2 * u);
-            atomicAdd(&acc[2 * n_g + p], c1 * en);
-            atomicAdd(&acc[2 * n_g + p + 1], c2 * en);
+            add(0, p, c1);
+            add(0, p + 1, c2);
+            add(1, p, c1 * u);
+            add(1, p + 1, c2 * u);
+            add(2, p, c1 * en);
+            add(2, p + 1, c2 * en);
             if (ELECTRON) {
                 const float rate = j.sigma_ion[energy_index(en, k.de_cs)] * sqrtf(v2) * rate_f;
-                atomicAdd(&acc[3 * n_g + p], c1 * rate);
-                atomicAdd(&acc[3 * n_g + p + 1], c2 * rate);
+                add(3, p, c1 * rate);
+                add(3, p + 1, c2 * rate);
             }
         }
     }
     __syncthreads();
+    // the workgroup's partial maps, reduced over the workgroups by k_xt_reduce
+    // (atomics from every workgroup onto the same few thousand map entries
+    // would serialise)
+    float *st = j.stage + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * NQ * n_g;
+    for (int q = threadIdx.x; q < NQ * n_g; q += blockDim.x)
+        st[q] = (float)((double)(long long)acc[q] / (double)kXtScale[q / n_g]);
+}
+
+// map entry e of the bin of this step += the sum of the workgroups'
+// partials: workgroup (x, y) sums slice y of the partials for 256 entries
+// (coalesced over the entries) and adds it with one atomic per entry
+constexpr int kXtBlocksPerCu = 8, kXtSlices = 32;
+__global__ __launch_bounds__(256) void k_xt_reduce(XtJob j, int nq, int n_g, int parts) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = nq * n_g;
+    if (e >= n) return;
+    const int w0 = (int)((int64_t)parts * blockIdx.y / gridDim.y);
+    const int w1 = (int)((int64_t)parts * (blockIdx.y + 1) / gridDim.y);
+    double sum = 0.0;
+#pragma unroll 8
+    for (int w = w0; w < w1; ++w) sum += (double)j.stage[(size_t)w * n + e];
+    if (sum == 0.0) return;
     const uint32_t t = (*j.clock + (uint32_t)j.n_t - 1u) % (uint32_t)j.n_t;
     const uint32_t tb = t / j.bin_steps;
-    for (int q = threadIdx.x; q < NQ * n_g; q += blockDim.x)
-        if (acc[q] != 0.0f) {
-            const int qq = q / n_g, node = q - qq * n_g;
-            atomicAdd(&j.xt[((size_t)qq * n_g + node) * j.n_xt + tb], (double)acc[q]);
-        }
+    const int q = e / n_g, node = e - q * n_g;
+    atomicAdd(&j.xt[((size_t)q * n_g + node) * j.n_xt + tb], sum);
 }
+
+size_t xt_stage_floats(int n_g) { return (size_t)device_cus() * kXtBlocksPerCu * 4 * n_g; }
 
 int launch_xt(const XtJob &j, const KPhys &k, hipStream_t s) {
     const int nq = j.electrons ? 4 : 3;
-    const size_t lds = (size_t)nq * k.n_g * sizeof(float);
-    if (!j.x || !j.vx || !j.vy || !j.vz || !j.cnt || !j.efield || !j.clock || !j.xt || j.n_seg < 1 ||
+    const size_t lds = (size_t)nq * k.n_g * sizeof(unsigned long long);
+    if (!j.x || !j.vx || !j.vy || !j.vz || !j.cnt || !j.efield || !j.clock || !j.xt || !j.stage || j.n_seg < 1 ||
         j.bin_steps == 0 || j.n_xt <= 0 || (j.electrons && !j.sigma_ion) || k.n_g < 2) {
         set_error("xt: bad arguments");
         return PIC_ERR_ARG;
@@ -1459,10 +1492,16 @@ int launch_xt(const XtJob &j, const KPhys &k, hipStream_t s) {
         set_error("xt: n_g too large for the per-workgroup accumulators");
         return PIC_ERR_UNSUPPORTED;
     }
-    const int grid = device_cus();
+    // several workgroups per compute unit to hide the gathers' latency,
+    // shared out over the segments (each a y row of the grid); the partial
+    // maps fit xt_stage_floats
+    const int gx = std::max(1, device_cus() * kXtBlocksPerCu / j.n_seg);
+    const dim3 grid(gx, j.n_seg);
     if (j.electrons) k_xt<true><<<grid, 256, lds, s>>>(j, k);
     else k_xt<false><<<grid, 256, lds, s>>>(j, k);
     PIC_LAUNCHED("k_xt");
+    k_xt_reduce<<<dim3((nq * k.n_g + 255) / 256, kXtSlices), 256, 0, s>>>(j, nq, k.n_g, gx * j.n_seg);
+    PIC_LAUNCHED("k_xt_reduce");
     return PIC_OK;
 }
 

@@ -80,6 +80,7 @@ struct pic_sim {
     // energy -- and the measured steps of each time bin
     double *xt = nullptr;
     unsigned long long *xt_count = nullptr;
+    float *xt_stage = nullptr;  // per-workgroup partial maps of the particle pass
     uint32_t xt_bin = 0;
     int xt_n = 0;
     // tables
@@ -363,6 +364,7 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
         xj.n_t = s->p.n_t;
         xj.bin_steps = s->xt_bin;
         xj.n_xt = s->xt_n;
+        xj.stage = s->xt_stage;
         if (ion) {
             XtJob xi = xj;
             xi.electrons = false;
@@ -603,6 +605,7 @@ int free_all(pic_sim *s) {
     }
     (void)hipFree(s->xt);
     (void)hipFree(s->xt_count);
+    (void)hipFree(s->xt_stage);
     void *bufs[] = {s->cnt, s->pot, s->efield, s->grid_e, s->grid_i, s->pot0_tab, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->cumul_e, s->cumul_i, s->eepf, s->ifed, s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
@@ -994,8 +997,10 @@ int pic_sim_set_xt(pic_sim_t *s, uint32_t bin_steps) {
     invalidate_graphs(s);  // measurement graphs gain or lose the XT launches
     (void)hipFree(s->xt);
     (void)hipFree(s->xt_count);
+    (void)hipFree(s->xt_stage);
     s->xt = nullptr;
     s->xt_count = nullptr;
+    s->xt_stage = nullptr;
     s->xt_bin = 0;
     s->xt_n = 0;
     if (!bin_steps) return PIC_OK;
@@ -1003,6 +1008,7 @@ int pic_sim_set_xt(pic_sim_t *s, uint32_t bin_steps) {
     int rc;
     if ((rc = dalloc(&s->xt, (size_t)kXtPlanes * s->p.n_g * n)) != PIC_OK) return rc;
     if ((rc = dalloc(&s->xt_count, (size_t)n)) != PIC_OK) return rc;
+    if ((rc = dalloc(&s->xt_stage, xt_stage_floats(s->p.n_g))) != PIC_OK) return rc;
     s->xt_bin = bin_steps;
     s->xt_n = n;
     return PIC_OK;
