@@ -1,12 +1,16 @@
 """The adjoint kernels (pic_adj_*) against the FP64 CPU adjoint of the same
 frozen-event model on the GPU's own event tape, against central finite
-differences (h = 1e-2 V) of the FP64 forward of that model on the same tape
+differences (h = 1e-3 V) of the FP64 forward of that model on the same tape
 (frozen events: common random numbers by construction), and the determinism
 the checkpointed recomputation relies on.
 
 The window is 400 steps (20 ion steps, two checkpoint segments).  Measured
-on the MI355X: the GPU gradient agrees with the FP64 CPU adjoint to 5e-4 and
-with the central differences to 4e-4.  Central differences of the GPU's own
+on the MI355X: the GPU gradient agrees with the FP64 CPU adjoint to 5.4e-4,
+and the CPU adjoint with central differences at h = 1e-3 V to 2e-5; at
+h = 1e-2 V the differences themselves are 0.3-1.3 % off the model's exact
+gradient (the frozen-event model is only piecewise smooth: a larger step
+moves particles across cells and clamps), so that step is not used.
+Central differences of the GPU's own
 FP32 forward are not used: a 1e-2 V change of the drive moves an ion by
 ~1e-10 m in a window, below the 2e-9 m resolution of an f32 position in a
 25 mm gap, so they measure rounding (3-13 % off at h = 1e-2 V).  Beyond ~500
@@ -102,11 +106,11 @@ def test_gradient_matches_fp64_cpu_adjoint(run):
 
 def test_gradient_matches_central_differences(run):
     """Central differences of the frozen-event model (FP64 forward on the
-    GPU's tape) at h = 1e-2 V, within 1 % of the GPU adjoint."""
+    GPU's tape) at h = 1e-3 V, within 1 % of the GPU adjoint."""
     m, state, tape = _cpu_case(run)
     wc, ws, target = run["wc"], run["ws"], run["target"]
     ggc, ggs, _ = run["grads"]
-    h = 1e-2
+    h = 1e-3
     scale = max(np.abs(ggs).max(), np.abs(ggc).max())
     for kind, k in (("sin", 0), ("sin", 1), ("cos", 0), ("sin", 6)):
         wp, wm = (ws.copy(), ws.copy()) if kind == "sin" else (wc.copy(), wc.copy())
