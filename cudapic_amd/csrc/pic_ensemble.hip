@@ -600,18 +600,31 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     }
     s->e_total = s->i_total = (uint32_t)tot;
     {
-        // groups of consecutive instances (capacities summed up to group_slots)
+        // groups of consecutive instances: as many as a greedy cut at
+        // group_slots electron slots needs, with the cuts then placed where
+        // the capacities are shared out most evenly (no oversized group and
+        // small remainder)
         const uint64_t budget = c->group_slots ? c->group_slots : 9000000u;
         s->group_periods = c->group_periods ? c->group_periods : 5u;
-        s->grp.push_back(0);
+        std::vector<uint64_t> pre(s->E + 1, 0);
+        int n_groups = 1;
         uint64_t acc = 0;
         for (int i = 0; i < s->E; ++i) {
             const uint64_t cap = s->inst_h[i].e_cap;
+            pre[i + 1] = pre[i] + cap;
             if (acc > 0 && acc + cap > budget) {
-                s->grp.push_back(i);
+                ++n_groups;
                 acc = 0;
             }
             acc += cap;
+        }
+        s->grp.push_back(0);
+        for (int g = 1; g < n_groups; ++g) {
+            const double target = (double)pre[s->E] * g / n_groups;
+            int best = s->grp.back() + 1;
+            for (int i = best; i <= s->E - (n_groups - g); ++i)
+                if (std::fabs((double)pre[i] - target) < std::fabs((double)pre[best] - target)) best = i;
+            s->grp.push_back(best);
         }
         s->grp.push_back(s->E);
         s->graphs.resize(8 * (s->grp.size() - 1));
