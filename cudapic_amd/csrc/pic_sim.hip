@@ -993,6 +993,10 @@ int pic_sim_set_xt(pic_sim_t *s, uint32_t bin_steps) {
         set_error("pic_sim_set_xt: XT maps are measured on the fused path, with bin_steps dividing n_t");
         return PIC_ERR_ARG;
     }
+    if (bin_steps && (size_t)4 * s->p.n_g * sizeof(unsigned long long) > (size_t)device_lds_per_block()) {
+        set_error("pic_sim_set_xt: the per-workgroup XT accumulators (32 B per node) exceed the LDS of a workgroup");
+        return PIC_ERR_UNSUPPORTED;
+    }
     PIC_HIP(hipStreamSynchronize(s->stream));
     invalidate_graphs(s);  // measurement graphs gain or lose the XT launches
     (void)hipFree(s->xt);

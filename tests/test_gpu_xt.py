@@ -94,3 +94,21 @@ def test_xt_is_a_fused_path_measurement(ca):
     with ca.Sim(p, capacity=10_000, path=ca.PIC_PATH_REFERENCE) as sim:
         with pytest.raises(Exception):
             sim.set_xt(20)
+
+
+
+def test_xt_on_the_finest_grid(ca):
+    """N_G = 4000 (128 KB of LDS accumulators per workgroup) still measures."""
+    p = ca.params(weight=2.0e3, n_g=4000, precision=2)
+    E, I = synthetic_state(30_000, 31_500, seed=93)
+    with ca.Sim(p, capacity=80_000) as sim:
+        sim.set_xt(20)
+        sim.upload(0, *E)
+        sim.upload(1, *I)
+        sim.run(40, measure=True)
+        m = sim.xt()
+        assert m["ne"].shape == (4000, 200)
+        for name, a in m.items():
+            assert np.isfinite(a).all(), name
+        assert np.all(m["ne"][1000:3000, :2] > 0) and np.all(m["meanee"][1000:3000, :2] > 0)
+        assert not np.any(m["ne"][:, 2:])
