@@ -1009,10 +1009,18 @@ int pic_sim_set_xt(pic_sim_t *s, uint32_t bin_steps) {
     s->xt_n = 0;
     if (!bin_steps) return PIC_OK;
     const int n = (int)((uint32_t)s->p.n_t / bin_steps);
-    int rc;
-    if ((rc = dalloc(&s->xt, (size_t)kXtPlanes * s->p.n_g * n)) != PIC_OK) return rc;
-    if ((rc = dalloc(&s->xt_count, (size_t)n)) != PIC_OK) return rc;
-    if ((rc = dalloc(&s->xt_stage, xt_stage_floats(s->p.n_g))) != PIC_OK) return rc;
+    int rc = dalloc(&s->xt, (size_t)kXtPlanes * s->p.n_g * n);
+    if (rc == PIC_OK) rc = dalloc(&s->xt_count, (size_t)n);
+    if (rc == PIC_OK) rc = dalloc(&s->xt_stage, xt_stage_floats(s->p.n_g));
+    if (rc != PIC_OK) {  // leave the maps off, not half allocated
+        (void)hipFree(s->xt);
+        (void)hipFree(s->xt_count);
+        (void)hipFree(s->xt_stage);
+        s->xt = nullptr;
+        s->xt_count = nullptr;
+        s->xt_stage = nullptr;
+        return rc;
+    }
     s->xt_bin = bin_steps;
     s->xt_n = n;
     return PIC_OK;
