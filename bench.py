@@ -181,8 +181,9 @@ def main():
         print(json.dumps({"copy_probe_MiB": args.copy_probe, "GB/s": round(copy_bandwidth(args.copy_probe), 1)}))
         return
     # the ensemble figure first, in a child process, before this process
-    # allocates anything on the GPU
-    ens_fig = None if args.no_extras else ensemble_figure()
+    # allocates anything on the GPU; a failing extra is reported in its own
+    # field and does not take the headline line with it
+    ens_fig = None if args.no_extras else _extra(ensemble_figure)
     prec = {"fp32": ca.PIC_PREC_FP32, "comp": ca.PIC_PREC_COMP, "fp64": ca.PIC_PREC_FP64}[args.precision]
     path = ca.PIC_PATH_FUSED if args.path == "fused" else ca.PIC_PATH_REFERENCE
     p = ca.params(weight=args.weight, precision=prec)
@@ -268,8 +269,16 @@ def main():
     if not args.no_extras:
         sim.close()
         out["ensemble"] = ens_fig
-        out["adjoint"] = adjoint_figure(args.adjoint_steps)
+        out["adjoint"] = _extra(adjoint_figure, args.adjoint_steps)
     print(json.dumps(out), flush=True)
+
+
+def _extra(fn, *a):
+    try:
+        return fn(*a)
+    except Exception as e:  # reported in the JSON line instead
+        print(f"bench: {fn.__name__} failed: {e}", file=sys.stderr, flush=True)
+        return {"error": f"{type(e).__name__}: {e}"[:500]}
 
 
 def ensemble_figure():
