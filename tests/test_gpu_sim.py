@@ -394,7 +394,7 @@ def test_timed_window_equals_plain_run(ca, orc):
     E, I = synthetic_state(100_000, 110_000, seed=52)
     cs, se0, si0 = zero_tables(orc, po)
     sims = []
-    for _ in range(2):
+    for _ in range(3):
         s = make_sim(ca, p, 200_000, 1, graph=True)
         s.set_tables(cs, se0, si0)
         s.upload(0, *E)
@@ -402,19 +402,26 @@ def test_timed_window_equals_plain_run(ca, orc):
         sims.append(s)
     r = sims[0].time_steps(880)
     sims[1].run(880)
+    # the electron kernel timed with events around each eager launch, from a
+    # step that is not on a period boundary
+    sims[2].run(3)
+    k = sims[2].time_steps(877, kernel_events=True)
     assert r["particle_steps"] > 0 and r["ms_total"] > 0
+    assert k["kernel_launches"] == 877 and 0 < k["ms_kernel"] < k["ms_total"]
     for extra in (0, 80):
         for s in sims:
             s.run(extra)
             s.synchronize()
         ce = [s.counts() for s in sims]
-        assert abs(ce[0][0] - ce[1][0]) <= 2 and abs(ce[0][1] - ce[1][1]) <= 2, ce
-        a = sort_rows(*sims[0].download(0))
         b = sort_rows(*sims[1].download(0))
-        m = min(len(a), len(b))
-        assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
-        pa, pb = sims[0].grid()[0], sims[1].grid()[0]
-        assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
+        pb = sims[1].grid()[0]
+        for other in (0, 2):
+            assert abs(ce[other][0] - ce[1][0]) <= 2 and abs(ce[other][1] - ce[1][1]) <= 2, ce
+            a = sort_rows(*sims[other].download(0))
+            m = min(len(a), len(b))
+            assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
+            pa = sims[other].grid()[0]
+            assert np.abs(pa - pb).max() < 1e-3 * np.abs(pa).max()
     for s in sims:
         assert s.time()[0] == 960
         s.check()
