@@ -187,6 +187,39 @@ def test_ensemble_timed_window_equals_plain_run(ca):
         e.close()
 
 
+def test_ensemble_with_an_empty_instance(ca):
+    """An instance without particles between two populated ones, each a
+    group of its own on the graph path and all in one group eagerly: the
+    empty one stays empty with a zero field apart from the drive, and the
+    others are unaffected by it."""
+    specs = [(250.0, 1.5e5, 20_000), (250.0, 1.5e5, 0), (150.0, 3.0e5, 20_000)]
+    ps = [ca.params(v0=v, pressure=1e-9, weight=w, precision=2) for v, w, _ in specs]
+    caps = [max(4096, int(n * 1.5)) for *_, n in specs]
+    runs = []
+    for graph, slots in ((True, 1), (False, 0xFFFFFFFF)):
+        e = ca.Ensemble(ps, caps, use_graph=graph, group_slots=slots)
+        for i, (*_, n) in enumerate(specs):
+            if n:
+                E, I = synthetic_state(n, int(n * 1.05), seed=41 + i)
+                e.upload(i, 0, *E)
+                e.upload(i, 1, *I)
+        e.run(160)
+        e.synchronize()
+        e.check()
+        ne, ni = e.counts()
+        assert ne[1] == 0 and ni[1] == 0
+        assert ne[0] > 10_000 and ne[2] > 10_000
+        pot, ef = e.grid(1)
+        assert np.isfinite(pot).all() and np.isfinite(ef).all()
+        # no space charge: the potential is the linear drop of the drive
+        lin = np.linspace(pot[0], 0.0, len(pot))
+        assert np.abs(pot - lin).max() < 1e-4 * max(1.0, abs(pot[0]))
+        runs.append((ne.copy(), ni.copy()))
+        e.close()
+    for i in (0, 2):
+        assert abs(int(runs[0][0][i]) - int(runs[1][0][i])) <= 2
+
+
 def test_ensemble_init_cycle_statistics(ca):
     """Four copies of the reference case from 1000 e + 1000 i at rest end the
     init cycle near eduPIC's published 888 e / 3345 i; a fifth instance at

@@ -97,6 +97,21 @@ def test_xt_is_a_fused_path_measurement(ca):
 
 
 
+def test_xt_of_an_empty_discharge(ca):
+    """No particles: the grid maps hold the vacuum potential and field, the
+    particle maps stay zero."""
+    p = ca.params(weight=2.0e3, precision=2)
+    with ca.Sim(p, capacity=10_000) as sim:
+        sim.set_xt(20)
+        sim.upload(0, *[np.zeros(0, np.float32)] * 4)
+        sim.upload(1, *[np.zeros(0, np.float32)] * 4)
+        sim.run(40, measure=True)
+        m = sim.xt()
+        for name in ("ne", "ni", "je", "ji", "meanee", "meanei", "ioniz", "powere", "poweri"):
+            assert not np.any(m[name]), name
+        assert np.abs(m["pot"][0, :2]).max() > 0 and np.isfinite(m["efield"]).all()
+
+
 def test_xt_on_the_finest_grid(ca):
     """N_G = 4000 (128 KB of LDS accumulators per workgroup) still measures."""
     p = ca.params(weight=2.0e3, n_g=4000, precision=2)
