@@ -83,11 +83,20 @@ F64 = C.c_double
 PP = C.POINTER(PicParams)
 PS = C.POINTER(PicSpecies)
 
+
+class PicReport(C.Structure):
+    _fields_ = (
+        [(n, C.c_double) for n in ("density_center", "plasma_freq", "debye_length", "coll_freq_e", "coll_freq_i",
+                                    "wp_dt_e", "dx_over_debye", "nu_max_e_dt_e", "nu_max_i_dt_i", "cfl_energy")]
+        + [("conditions_ok", C.c_int32), ("pad", C.c_int32)]
+    )
+
 # name -> (restype, argtypes); every function in include/cudapic_amd.h
 SIGNATURES = {
     "pic_params_default": (None, [PP]),
     "pic_params_derive": (I32, [PP]),
     "pic_cross_sections": (I32, [F64, P, P, P]),
+    "pic_report": (I32, [PP, P, P, F64, F64, F64, U64, U64, U32, U32, C.POINTER(PicReport)]),
     "pic_get_density": (I32, [P, P, P, I32, PP, P]),
     "pic_solve_poisson": (I32, [P, P, F64, I32, P, P, PP, P]),
     "pic_solve_poisson_flexible": (I32, [P, I32, F32, F32, F32, I32, P, P]),

@@ -380,18 +380,69 @@ int main(int argc, char **argv) {
             }
             std::fclose(f);
         }
+        // XT maps first: the report's <j*E> power densities average them
+        uint32_t n_xt = 0;
+        std::vector<double> xt;
+        if (xt_bin) {
+            CK(pic_sim_read_xt(sim, nullptr, &n_xt));
+            xt.resize((size_t)11 * p.n_g * n_xt);
+            CK(pic_sim_read_xt(sim, xt.data(), &n_xt));
+        }
         // C/eduPIC.cc:1002-1008: fluxes from the electrode absorption counters
         // over the measured time (the counters run over the whole run)
         const double t_meas = (double)measurement_cycles / p.freq;
         const double t_run = (double)num_cycles / p.freq;
         const double wa = p.weight / p.area;
+        // C/eduPIC.cc:926-978: plasma characteristics and the stability and
+        // accuracy conditions.  The mean electron energy in the EEPF window
+        // comes from the EEPF counts at their bin centres (eduPIC sums the
+        // energies themselves: at most DE_EEPF / 2 = 0.025 eV apart, and
+        // electrons above the histogram's 100 eV are not in it).
+        double cnt_c = 0.0, sum_c = 0.0;
+        for (int b = 0; b < PIC_N_EEPF; ++b) {
+            cnt_c += eepf[b];
+            sum_c += eepf[b] * (b + 0.5) * 0.05;
+        }
+        pic_report_t rep{};
+        {
+            std::vector<float> cs((size_t)PIC_N_CS * PIC_CS_RANGES), se(PIC_CS_RANGES), si(PIC_CS_RANGES);
+            CK(pic_cross_sections(p.gas_density, cs.data(), se.data(), si.data()));
+            CK(pic_report(&p, se.data(), si.data(), n_steps_e > 0 ? ce[p.n_g / 2] / n_steps_e : 0.0,
+                          cnt_c > 0 ? sum_c / cnt_c : 0.0, t_run, cnt[4], cnt[5], ne, ni, &rep));
+        }
         f = std::fopen((dir + "/info_" + ts + ".txt").c_str(), "w");
         if (f) {
+            const char *rule = "--------------------------------------------------------------------------------\n";
             std::fprintf(f, "cudaPIC (MI355X) run report\n");
             std::fprintf(f, "cycles run                            = %12ld\n", num_cycles);
             std::fprintf(f, "cycles measured                       = %12ld\n", measurement_cycles);
-            std::fprintf(f, "Electron density @ center             = %12.3e [m^{-3}]\n",
-                         n_steps_e > 0 ? ce[p.n_g / 2] / n_steps_e : 0.0);
+            std::fprintf(f, "%s", rule);
+            std::fprintf(f, "Plasma characteristics:\n");
+            std::fprintf(f, "Electron density @ center             = %12.3e [m^{-3}]\n", rep.density_center);
+            std::fprintf(f, "Plasma frequency @ center             = %12.3e [rad/s]\n", rep.plasma_freq);
+            std::fprintf(f, "Debye length @ center                 = %12.3e [m]\n", rep.debye_length);
+            std::fprintf(f, "Electron collision frequency          = %12.3e [1/s]\n", rep.coll_freq_e);
+            std::fprintf(f, "Ion collision frequency               = %12.3e [1/s]\n", rep.coll_freq_i);
+            std::fprintf(f, "%s", rule);
+            std::fprintf(f, "Stability and accuracy conditions:\n");
+            std::fprintf(f, "Plasma frequency @ center * DT_E      = %12.3f (OK if less than 0.20)\n", rep.wp_dt_e);
+            std::fprintf(f, "DX / Debye length @ center            = %12.3f (OK if less than 1.00)\n",
+                         rep.dx_over_debye);
+            std::fprintf(f, "Max. electron coll. frequency * DT_E  = %12.3f (OK if less than 0.05)\n",
+                         rep.nu_max_e_dt_e);
+            std::fprintf(f, "Max. ion coll. frequency * DT_I       = %12.3f (OK if less than 0.05)\n",
+                         rep.nu_max_i_dt_i);
+            std::fprintf(f, "%s", rule);
+            if (!rep.conditions_ok) {
+                std::fprintf(f, "** STABILITY AND ACCURACY CONDITION(S) VIOLATED - REFINE SIMULATION SETTINGS! **\n");
+                std::fprintf(f, "%s", rule);
+                std::printf(">> cudaPIC: ERROR: STABILITY AND ACCURACY CONDITION(S) VIOLATED (see info_%s.txt)\n",
+                            ts.c_str());
+            } else {
+                std::fprintf(f, "Max e- energy for CFL condition       = %12.3f [eV]\n", rep.cfl_energy);
+                std::fprintf(f, "%s", rule);
+            }
+            std::fprintf(f, "Particle characteristics at the electrodes:\n");
             std::fprintf(f, "Ion flux at powered electrode         = %12.3e [m^{-2} s^{-1}]\n", cnt[2] * wa / t_run);
             std::fprintf(f, "Ion flux at grounded electrode        = %12.3e [m^{-2} s^{-1}]\n", cnt[3] * wa / t_run);
             std::fprintf(f, "Mean ion energy at powered electrode  = %12.3e [eV]\n", mp);
@@ -401,14 +452,26 @@ int main(int argc, char **argv) {
             std::fprintf(f, "Electron collisions                   = %12llu\n", cnt[4]);
             std::fprintf(f, "Ion collisions                        = %12llu\n", cnt[5]);
             std::fprintf(f, "measured time                         = %12.3e [s]\n", t_meas);
+            if (xt_bin) {
+                // C/eduPIC.cc:1012-1024: <j*E> averaged over the XT map
+                double pe = 0.0, pi = 0.0;
+                const size_t m = (size_t)p.n_g * n_xt;
+                for (size_t k = 0; k < m; ++k) {
+                    pe += xt[6 * m + k];
+                    pi += xt[7 * m + k];
+                }
+                pe /= (double)m;
+                pi /= (double)m;
+                std::fprintf(f, "%s", rule);
+                std::fprintf(f, "Absorbed power calculated as <j*E>:\n");
+                std::fprintf(f, "Electron power density (average)      = %12.3e [W m^{-3}]\n", pe);
+                std::fprintf(f, "Ion power density (average)           = %12.3e [W m^{-3}]\n", pi);
+                std::fprintf(f, "Total power density(average)          = %12.3e [W m^{-3}]\n", pe + pi);
+            }
             std::fclose(f);
         }
         if (xt_bin) {
             // save_all_xt (C/eduPIC.cc:905-919): N_G lines of N_XT values
-            uint32_t n_xt = 0;
-            CK(pic_sim_read_xt(sim, nullptr, &n_xt));
-            std::vector<double> xt((size_t)11 * p.n_g * n_xt);
-            CK(pic_sim_read_xt(sim, xt.data(), &n_xt));
             const char *names[11] = {"pot", "efield", "ne", "ni", "je", "ji",
                                      "powere", "poweri", "meanee", "meanei", "ioniz"};
             for (int q = 0; q < 11; ++q) {

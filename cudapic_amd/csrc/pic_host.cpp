@@ -60,6 +60,43 @@ extern "C" int pic_cross_sections(double gas_density, float *cs, float *stot_e, 
     return PIC_OK;
 }
 
+// check_and_save_info (C/eduPIC.cc:926-978) and its max_electron_coll_freq /
+// max_ion_coll_freq (:252-276): the collision frequency maxima run over the
+// table energies i * DE_CS with the electron mass and the Ar+/Ar reduced
+// mass respectively.
+extern "C" int pic_report(const pic_params_t *p, const float *stot_e, const float *stot_i, double density_center,
+                          double mean_energy_center, double sim_time, unsigned long long n_e_coll,
+                          unsigned long long n_i_coll, uint32_t n_e, uint32_t n_i, pic_report_t *out) {
+    if (!p || !stot_e || !stot_i || !out || !(sim_time > 0.0)) return PIC_ERR_ARG;
+    const double e_charge = 1.60217662e-19, e_mass = 9.10938356e-31, ar_mass = 6.63352090e-26;
+    const double epsilon0 = 8.85418781e-12, mu_arar = ar_mass / 2.0, de_cs = 0.001;
+    pic_report_t r{};
+    r.density_center = density_center;
+    r.plasma_freq = e_charge * std::sqrt(density_center / epsilon0 / e_mass);
+    const double kt = 2.0 * mean_energy_center * e_charge / 3.0;
+    r.debye_length = std::sqrt(epsilon0 * kt / density_center) / e_charge;
+    r.coll_freq_e = (double)n_e_coll / sim_time / (double)n_e;
+    r.coll_freq_i = (double)n_i_coll / sim_time / (double)n_i;
+    double nu_e = 0.0, nu_i = 0.0;
+    for (int i = 0; i < PIC_CS_RANGES; ++i) {
+        const double e = i * de_cs;
+        nu_e = std::fmax(nu_e, std::sqrt(2.0 * e * e_charge / e_mass) * (double)stot_e[i]);
+        nu_i = std::fmax(nu_i, std::sqrt(2.0 * e * e_charge / mu_arar) * (double)stot_i[i]);
+    }
+    r.wp_dt_e = r.plasma_freq * p->dt_e;
+    r.dx_over_debye = p->dx / r.debye_length;
+    r.nu_max_e_dt_e = nu_e * p->dt_e;
+    r.nu_max_i_dt_i = nu_i * p->dt_i;
+    const double v_max = p->dx / p->dt_e;
+    r.cfl_energy = 0.5 * e_mass * v_max * v_max / e_charge;
+    // eduPIC fails a condition only when its value exceeds the limit, so a NaN
+    // (no electrons measured at the centre) passes there and here
+    r.conditions_ok = !(r.wp_dt_e > 0.2) && !(r.dx_over_debye > 1.0) && !(r.nu_max_e_dt_e > 0.05) &&
+                      !(r.nu_max_i_dt_i > 0.05);
+    *out = r;
+    return PIC_OK;
+}
+
 extern "C" int pic_abi_layout(uint64_t out[4]) {
     if (!out) return PIC_ERR_ARG;
     out[0] = sizeof(pic_params_t);

@@ -60,6 +60,25 @@ def cross_sections(gas_density: float):
     return cs, se, si
 
 
+def report(p, density_center, mean_energy_center, sim_time, n_e_coll, n_i_coll, n_e, n_i, tables=None):
+    """eduPIC's run report (C/eduPIC.cc:926-978): plasma characteristics at
+    the centre and the stability and accuracy conditions, as a dict.
+    `tables` = (sigma_tot_e, sigma_tot_i) of cross_sections(), computed if
+    omitted.  Host only."""
+    from ._abi import PicReport
+    if tables is None:
+        _, se, si = cross_sections(p.gas_density)
+    else:
+        se, si = (np.ascontiguousarray(t, np.float32) for t in tables)
+    r = PicReport()
+    call("pic_report", C.byref(p), se.ctypes.data, si.ctypes.data, float(density_center),
+         float(mean_energy_center), float(sim_time), int(n_e_coll), int(n_i_coll), int(n_e), int(n_i),
+         C.byref(r))
+    out = {n: getattr(r, n) for n, _ in PicReport._fields_ if n != "pad"}
+    out["conditions_ok"] = bool(out["conditions_ok"])
+    return out
+
+
 def get_density(x, density, n_active, p, density_f64=False, stream=None):
     call("pic_get_density", _p(x), _p(n_active), _p(density), int(density_f64), C.byref(p), stream)
 

@@ -92,6 +92,31 @@ typedef struct pic_species {
  * i backward) and the total macroscopic cross sections (main.rs:1378-1444). */
 int pic_cross_sections(double gas_density, float *cs, float *stot_e, float *stot_i);
 
+/* eduPIC's run report (C/eduPIC.cc:926-978, check_and_save_info; no cudaPIC
+ * counterpart): plasma characteristics at the centre of the gap and the four
+ * stability and accuracy conditions.  Inputs: the mean measured electron
+ * density at node N_G/2 [m^-3], the mean electron energy in the EEPF window
+ * [eV], the simulated time of the run [s], the collisions counted in it and
+ * the live particle counts; stot_e / stot_i are pic_cross_sections' totals.
+ * Host only.  conditions_ok is 0 if any condition is violated. */
+typedef struct pic_report {
+    double density_center;   /* m^-3 */
+    double plasma_freq;      /* electron plasma frequency at the centre [rad/s] */
+    double debye_length;     /* [m], from kT = 2/3 of the mean energy */
+    double coll_freq_e;      /* collisions / time / N_e [1/s] */
+    double coll_freq_i;
+    double wp_dt_e;          /* plasma_freq * DT_E      (OK below 0.20) */
+    double dx_over_debye;    /* DX / debye_length       (OK below 1.00) */
+    double nu_max_e_dt_e;    /* max_E v sigma_tot_e * DT_E (OK below 0.05) */
+    double nu_max_i_dt_i;    /* max_E g sigma_tot_i * DT_I (OK below 0.05) */
+    double cfl_energy;       /* 0.5 m_e (DX/DT_E)^2 [eV] */
+    int32_t conditions_ok;
+    int32_t pad;
+} pic_report_t;
+int pic_report(const pic_params_t *p, const float *stot_e, const float *stot_i, double density_center,
+               double mean_energy_center, double sim_time, unsigned long long n_e_coll,
+               unsigned long long n_i_coll, uint32_t n_e, uint32_t n_i, pic_report_t *out);
+
 /* ------------------------------------------------------------------------
  * Stage entry points (one per reference kernel)
  * ---------------------------------------------------------------------- */

@@ -1,6 +1,7 @@
 """The C++ host CLI: the reference's run interface and result files
 (Rust-CUDA/src/main.rs:1498-1627)."""
 import glob
+import re
 import os
 import subprocess
 
@@ -65,6 +66,25 @@ def test_cli_writes_xt_maps(tmp_path):
         assert (m >= 0).all() and avg[:, col].sum() > 0
         # %e keeps 7 significant digits
         np.testing.assert_allclose(m.mean(axis=1), avg[:, col], rtol=2e-6, atol=1e-6 * avg[:, col].max())
+    # the run report (C/eduPIC.cc:926-1025): the table-only lines as the
+    # reference's eduPIC prints them for the reference case, the centre
+    # density of density_avg and the <j*E> averages of the power maps
+    info = open(glob.glob(os.path.join(d, "info_*"))[0]).read()
+
+    def val(label):
+        m = re.search(re.escape(label) + r"\s*=\s*(\S+)", info)
+        assert m, (label, info)
+        return m.group(1)
+    assert val("Max. electron coll. frequency * DT_E") == "0.013"
+    assert val("Max. ion coll. frequency * DT_I") == "0.020"
+    assert val("Max e- energy for CFL condition") == "32.834"
+    assert "VIOLATED" not in info
+    assert abs(float(val("Electron density @ center")) / avg[200, 1] - 1.0) < 1e-3
+    assert float(val("Debye length @ center")) > 0 and float(val("Electron collision frequency")) > 0
+    for label, name in (("Electron power density (average)", "powere"), ("Ion power density (average)", "poweri")):
+        m = np.loadtxt(os.path.join(d, name + "_xt.dat"))
+        # %12.3e keeps 4 figures; the map's %e rounding moves the mean by far less
+        assert abs(float(val(label)) - m.mean()) <= 1e-3 * abs(m.mean()) + 1e-6 * np.abs(m).mean(), label
 
 
 def test_cli_usage_error():
