@@ -141,6 +141,28 @@ class Sim:
     def reset_measurements(self):
         call("pic_sim_reset_measurements", self.h)
 
+    def report(self, steps=None):
+        """eduPIC's run report (C/eduPIC.cc:926-978) from this simulation's
+        accumulators: the centre electron density averaged over the measured
+        steps, the centre mean energy from the EEPF counts at their bin
+        centres, and collision frequencies over `steps` steps (default: every
+        step run; pass the steps since reset_measurements() after a reset,
+        which also clears the collision counters).  A dict as
+        kernels.report returns; the collision maxima use the tables of the
+        discharge's gas density."""
+        from . import kernels
+        m = self.measurements()
+        c = m["counters"]
+        n_meas = c["e_meas_steps"]
+        density = m["cumul_e"][self.p.n_g // 2] / n_meas if n_meas else 0.0
+        h = m["eepf"].astype(np.float64)
+        mean_energy = float(h @ ((np.arange(PIC_N_EEPF) + 0.5) * 0.05) / h.sum()) if h.sum() else 0.0
+        if steps is None:
+            steps = self.time()[1]
+        n_e, n_i = self.counts()
+        return kernels.report(self.p, density, mean_energy, steps * self.p.dt_e, c["e_collisions"],
+                              c["i_collisions"], n_e, n_i)
+
     # eduPIC's XT file names (C/eduPIC.cc:905-919), in pic_sim_read_xt's order
     XT_NAMES = ("pot", "efield", "ne", "ni", "je", "ji", "powere", "poweri", "meanee", "meanei", "ioniz")
 
