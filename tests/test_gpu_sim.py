@@ -54,6 +54,8 @@ STARTS = [(0, 0), (1, 0), (1, 1), (1, 20)]
 # fused-path segment counts: the default for the test capacities (a few) and
 # the 32 of a multi-million-particle run
 SEGMENTS = [0, 32]
+# (path, t0, segments); the reference path keeps one segment
+CASES = [(path, t0, seg) for path, t0 in STARTS for seg in SEGMENTS if path or not seg]
 
 
 def _pot0(t0):
@@ -75,12 +77,9 @@ def _check_slot_draws(orc, rng0, rng1, slots, cnt, draws):
         np.testing.assert_array_equal(rng1[w][free], rng0[w][free])
 
 
-@pytest.mark.parametrize("path,t0", STARTS)
+@pytest.mark.parametrize("path,t0,segments", CASES)
 @pytest.mark.parametrize("prec", [0, 2])
-@pytest.mark.parametrize("segments", SEGMENTS)
 def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec, segments):
-    if path == 0 and segments:
-        pytest.skip("the reference path keeps one segment")
     p = ca.params(weight=2.0e3, precision=prec)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 300_000, 320_000, 400_000
@@ -258,11 +257,8 @@ def _check_new_ions(gi, pushed_ion_x, born_x, ions, check_v):
                 assert np.linalg.norm(v - ions[x]) <= 1e-3 * 270 * 6
 
 
-@pytest.mark.parametrize("path,t0", STARTS)
-@pytest.mark.parametrize("segments", SEGMENTS)
+@pytest.mark.parametrize("path,t0,segments", CASES)
 def test_step_with_collisions_slot_replay(ca, orc, path, t0, segments):
-    if path == 0 and segments:
-        pytest.skip("the reference path keeps one segment")
     p = ca.params(weight=2.0e3, precision=2)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 200_000, 210_000, 300_000
