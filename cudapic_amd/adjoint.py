@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import PIC_ELECTRONS, PIC_IONS, PicAdjConfig, call, lib
+from ._abi import PicAdjConfig, call, lib
 
 N_HARM = 16
 

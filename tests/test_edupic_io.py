@@ -3,7 +3,6 @@ trip in both header variants, and the reference's own eduPIC (compiled from
 C/eduPIC.cc into oracle/_ref) reads a file this package wrote, runs a cycle
 and writes one this package reads back."""
 import os
-import shutil
 import subprocess
 
 import numpy as np

@@ -23,7 +23,6 @@ def test_cli_run_writes_reference_outputs(tmp_path, path):
     assert "final particles" in r.stdout
     d = glob.glob(str(out / "*"))
     assert len(d) == 1
-    files = {os.path.basename(f).split("_")[0] for f in glob.glob(d[0] + "/*")}
     names = sorted(os.path.basename(f) for f in glob.glob(d[0] + "/*"))
     assert any(n.startswith("0003_") and n.endswith("_electrons.csv") for n in names)
     assert any(n.startswith("0003_") and n.endswith("_ions.csv") for n in names)

@@ -5,7 +5,7 @@ whole runs reproduce the reference's init-cycle statistics per instance."""
 import numpy as np
 import pytest
 
-from helpers import SEED_E, SEED_I, assert_same_multiset, sort_rows, synthetic_state
+from helpers import assert_same_multiset, sort_rows, synthetic_state
 from test_gpu_sim import _check_new_ions, _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
