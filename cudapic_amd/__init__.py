@@ -13,10 +13,11 @@ from .params import params
 from .sim import Sim
 from .ensemble import Ensemble
 from .adjoint import Adjoint
+from .step import StepState
 from . import kernels
 
 __all__ = [
-    "DeviceBuffer", "Sim", "Ensemble", "Adjoint", "kernels", "params", "lib", "synchronize", "device_info", "PicError",
+    "DeviceBuffer", "Sim", "StepState", "Ensemble", "Adjoint", "kernels", "params", "lib", "synchronize", "device_info", "PicError",
     "PIC_ELECTRONS", "PIC_IONS", "PIC_PATH_FUSED", "PIC_PATH_REFERENCE",
     "PIC_PREC_FP32", "PIC_PREC_COMP", "PIC_PREC_FP64",
 ]

@@ -74,6 +74,26 @@ class PicAdjConfig(C.Structure):
     ]
 
 
+class PicState(C.Structure):
+    """pic_state_t: the reference's GpuSimState buffers (device pointers)."""
+    _fields_ = (
+        [(n, C.c_void_p) for n in ("e_x", "e_vx", "e_vy", "e_vz", "i_x", "i_vx", "i_vy", "i_vz",
+                                    "n_electrons", "n_ions", "efield", "pot", "e_density", "i_density",
+                                    "cs", "sigma_tot_e", "sigma_tot_i",
+                                    "rng_e0", "rng_e1", "rng_e2", "rng_e3", "rng_i0", "rng_i1", "rng_i2", "rng_i3",
+                                    "tmp_x", "tmp_vx", "tmp_vy", "tmp_vz", "tmp_i_x", "tmp_i_vx", "tmp_i_vy",
+                                    "tmp_i_vz", "cumul_e_density", "cumul_i_density", "eepf_counts")]
+        + [("capacity", C.c_uint32), ("pad", C.c_uint32)]
+    )
+
+
+class PicStepConfig(C.Structure):
+    _fields_ = [("compact_every", C.c_uint32),
+                *[(n, C.c_int32) for n in ("grid_blocks", "ppt", "ppt_inplace", "lds_replicas", "pad")]]
+
+
+PIC_STEP_MEASURE, PIC_STEP_RESET = 1, 2
+
 P = C.c_void_p
 U32 = C.c_uint32
 U64 = C.c_uint64
@@ -156,12 +176,20 @@ SIGNATURES = {
     "pic_adj_tape_sizes": (I32, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                  C.POINTER(U32)]),
     "pic_adj_download_tape": (I32, [P] + [P] * 16),
+    "pic_init": (I32, [PP, C.POINTER(PicStepConfig), C.POINTER(P)]),
+    "pic_destroy": (I32, [P]),
+    "pic_step": (I32, [P, C.POINTER(PicState), U32, I32, P]),
+    "pic_step_graph": (I32, [P, C.POINTER(PicState), U32, U32, I32, P]),
+    "pic_step_diagnostics": (I32, [P, P, P, P, P]),
     "pic_malloc": (I32, [C.POINTER(P), U64]),
     "pic_free": (I32, [P]),
     "pic_memcpy_htod": (I32, [P, P, U64]),
     "pic_memcpy_dtoh": (I32, [P, P, U64]),
     "pic_memset": (I32, [P, I32, U64]),
     "pic_device_synchronize": (I32, []),
+    "pic_stream_create": (I32, [C.POINTER(P)]),
+    "pic_stream_destroy": (I32, [P]),
+    "pic_stream_synchronize": (I32, [P]),
     "pic_stream_copy": (I32, [P, P, U64, P]),
     "pic_last_error": (C.c_char_p, []),
     "pic_device_info": (I32, [C.POINTER(I32), C.POINTER(I32), C.c_char_p, I32]),
@@ -209,7 +237,7 @@ def call(name: str, *args):
 
 def layout_matches() -> bool:
     """True if the ctypes structures have the C sizes the library reports."""
-    out = (U64 * 4)()
+    out = (U64 * 6)()
     lib().pic_abi_layout(out)
     return tuple(out) == (C.sizeof(PicParams), C.sizeof(PicSpecies), C.sizeof(PicSimConfig),
-                          C.sizeof(PicEnsembleConfig))
+                          C.sizeof(PicEnsembleConfig), C.sizeof(PicState), C.sizeof(PicStepConfig))

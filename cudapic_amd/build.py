@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "libcudapic_amd.so")
 CLI = os.path.join(HERE, "cudapic")
 ARCH = os.environ.get("PIC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["pic_kernels.hip", "pic_fused.hip", "pic_sim.hip", "pic_ensemble.hip", "pic_adjoint.hip", "pic_host.cpp"]
+SOURCES = ["pic_kernels.hip", "pic_fused.hip", "pic_sim.hip", "pic_step.hip", "pic_ensemble.hip", "pic_adjoint.hip",
+           "pic_host.cpp"]
 HEADERS = ["pic_device.h", "pic_internal.h"]
 
 # -ffp-contract=off: the only fused multiply-adds are the explicit fmaf()
@@ -54,7 +55,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = True) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

@@ -397,7 +397,8 @@ int main(int argc, char **argv) {
         // accuracy conditions.  The mean electron energy in the EEPF window
         // comes from the EEPF counts at their bin centres (eduPIC sums the
         // energies themselves: at most DE_EEPF / 2 = 0.025 eV apart, and
-        // electrons above the histogram's 100 eV are not in it).
+        // electrons above the histogram's 100 eV are not in it).  An empty
+        // window gives NaN, as eduPIC's 0/0, which passes its conditions.
         double cnt_c = 0.0, sum_c = 0.0;
         for (int b = 0; b < PIC_N_EEPF; ++b) {
             cnt_c += eepf[b];
@@ -408,7 +409,7 @@ int main(int argc, char **argv) {
             std::vector<float> cs((size_t)PIC_N_CS * PIC_CS_RANGES), se(PIC_CS_RANGES), si(PIC_CS_RANGES);
             CK(pic_cross_sections(p.gas_density, cs.data(), se.data(), si.data()));
             CK(pic_report(&p, se.data(), si.data(), n_steps_e > 0 ? ce[p.n_g / 2] / n_steps_e : 0.0,
-                          cnt_c > 0 ? sum_c / cnt_c : 0.0, t_run, cnt[4], cnt[5], ne, ni, &rep));
+                          cnt_c > 0 ? sum_c / cnt_c : std::nan(""), t_run, cnt[4], cnt[5], ne, ni, &rep));
         }
         f = std::fopen((dir + "/info_" + ts + ".txt").c_str(), "w");
         if (f) {

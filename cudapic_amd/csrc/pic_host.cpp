@@ -97,11 +97,13 @@ extern "C" int pic_report(const pic_params_t *p, const float *stot_e, const floa
     return PIC_OK;
 }
 
-extern "C" int pic_abi_layout(uint64_t out[4]) {
+extern "C" int pic_abi_layout(uint64_t out[6]) {
     if (!out) return PIC_ERR_ARG;
     out[0] = sizeof(pic_params_t);
     out[1] = sizeof(pic_species_t);
     out[2] = sizeof(pic_sim_config_t);
     out[3] = sizeof(pic_ensemble_config_t);
+    out[4] = sizeof(pic_state_t);
+    out[5] = sizeof(pic_step_config_t);
     return PIC_OK;
 }

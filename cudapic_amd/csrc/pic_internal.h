@@ -84,6 +84,11 @@ struct PoissonJob {
     uint32_t *zero_cnt_ion = nullptr; int zero_cnt_ion_words = 0;  // cleared on ion steps
     const uint32_t *copy_cnt_src = nullptr; uint32_t *copy_cnt_dst = nullptr; int copy_cnt_words = 0;
     uint32_t *advance_time = nullptr;  // == time: *time <- (*time + 1) % n_t after the solve
+    // ion steps only: *copy_cnt_ion_dst = *copy_cnt_ion_src as read before any clear
+    const uint32_t *copy_cnt_ion_src = nullptr; uint32_t *copy_cnt_ion_dst = nullptr;
+    // >= 0: the step index (instead of reading *time); with pot0_tab the
+    // step's pot0 is pot0_tab[t_value] (pot0_in is ignored)
+    int64_t t_value = -1;
     // XT maps (measurement steps, with cumul_e): pot, E, n_e, n_i of the step
     // added to [4][n_g][xt_n] at time bin t / xt_bin; xt_count[bin] += 1
     double *xt_grid = nullptr;

@@ -29,22 +29,26 @@ int hip_fail(hipError_t e, const char *what) {
 }
 
 namespace {
-// One integer attribute of the current device, cached per device; `dflt`
-// if the query fails.
+// One integer attribute of the current device, cached per device (a fixed
+// table: launch paths that must not allocate query it too); `dflt` if the
+// query fails.
 int device_attr(hipDeviceAttribute_t attr, int slot, int dflt) {
+    constexpr int kDevices = 64;
     static std::mutex mu;
-    static std::vector<int> cache;
+    static int cache[kDevices][4] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return dflt;
-    std::lock_guard<std::mutex> lk(mu);
-    const size_t at = (size_t)dev * 4 + slot;
-    if (cache.size() <= at) cache.resize(at + 1, 0);
-    if (cache[at] == 0) {
-        int v = 0;
+    int v = 0;
+    if (dev < 0 || dev >= kDevices) {
         if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) v = dflt;
-        cache[at] = v;
+        return v;
     }
-    return cache[at];
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache[dev][slot] == 0) {
+        if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) v = dflt;
+        cache[dev][slot] = v;
+    }
+    return cache[dev][slot];
 }
 }  // namespace
 
@@ -131,56 +135,82 @@ int launch_copy_u32(uint32_t *dst, const uint32_t *src, hipStream_t s, uint32_t 
 }
 
 // ============================================================ deposition
-// get_density (main.rs:356-406).  Each wave owns an LDS histogram replica
-// (ds_add_f32 atomics; replicas cut same-bin conflicts between waves), the
-// workgroup sums its replicas once and flushes each non-zero node with one
-// global atomic (f32 or f64), spread over `replicas` global copies.
+// get_density (main.rs:356-406).  Each wave adds into an LDS histogram
+// replica (fixed point, integer LDS atomics); the workgroup sums its
+// replicas once and flushes each non-zero node with one global atomic (f32
+// or f64), spread over `replicas` global copies.
+//
+// One workgroup of 1024 lanes per compute unit (16 waves, four per SIMD)
+// and 16-byte loads of four positions, two in flight per lane per
+// iteration: the kernel reads 4 B per particle, so it is bound by how many
+// loads are in flight (one wave per SIMD with scalar loads reached 0.9 TB/s
+// at 3.8 M particles), not by the LDS atomics.  Fewer workgroups keep the
+// flush's same-address global atomics few.
+constexpr int kDepositBlock = 1024;
+constexpr int kDepositPerLane = 8;  // positions per lane per iteration (two float4)
+
+__device__ __forceinline__ void deposit_one(uint32_t *h, float x, float inv_dx, int n_g, float scale) {
+    if (x < 0.0f) return;  // a hole
+    const float pos = x * inv_dx;
+    const int q = cell_of(pos, n_g);
+    const float rem = pos - (float)q;
+    const uint32_t u2 = (uint32_t)(rem * scale + 0.5f);
+    atomicAdd(&h[q], (uint32_t)scale - u2);
+    atomicAdd(&h[q + 1], u2);
+}
+
 template <typename Acc>
-__global__ __launch_bounds__(256) void k_deposit(const float *__restrict__ x,
-                                                 const uint32_t *__restrict__ n_active,
-                                                 uint32_t n_cap, Acc *__restrict__ dens, int reps,
-                                                 int rep_stride, float inv_dx, float wf, int n_g,
-                                                 int lds_reps) {
-    // Fixed-point LDS histograms (integer LDS atomics run at full rate,
-    // float ones at a fraction of it): a particle adds exactly `scale` units
-    // split by its CIC weights.  The scale is the largest power of two (at
-    // most 2^20) for which a replica cannot overflow: its waves deposit at
-    // most iters * 4 * 64 particles each.
+__global__ __launch_bounds__(kDepositBlock) void k_deposit(const float *__restrict__ x,
+                                                           const uint32_t *__restrict__ n_active,
+                                                           uint32_t n_cap, Acc *__restrict__ dens, int reps,
+                                                           int rep_stride, float inv_dx, float wf, int n_g,
+                                                           int lds_reps) {
+    // Fixed-point LDS histograms: a particle adds exactly `scale` units split
+    // by its CIC weights.  The scale is the largest power of two (at most
+    // 2^20) for which a replica cannot overflow: its waves deposit at most
+    // iters * kDepositPerLane * 64 particles each.
     extern __shared__ uint32_t hs[];
     const int wave = threadIdx.x / kWave;
     for (int k = threadIdx.x; k < lds_reps * n_g; k += blockDim.x) hs[k] = 0u;
     __syncthreads();
     uint32_t *h = hs + (wave % lds_reps) * n_g;
     const uint32_t n = min(*n_active, n_cap);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint64_t iters = ((uint64_t)n + 4ull * stride - 1) / (4ull * stride);
+    // 16-byte loads need a 16-byte aligned start; a misaligned array takes
+    // its first (4 - offset) positions one at a time
+    const uint32_t lead = min(n, (uint32_t)((16u - ((uintptr_t)x & 15u)) & 15u) / 4u);
+    const uint32_t n4 = (n - lead) / 4u;          // float4 groups after the lead
+    const float4 *x4 = reinterpret_cast<const float4 *>(x + lead);
+    const uint32_t stride = gridDim.x * blockDim.x;  // float4 groups per half-iteration
+    const uint64_t iters = ((uint64_t)n4 + 2ull * stride - 1) / (2ull * stride) + 1;
     const uint64_t waves_per_rep = (blockDim.x / kWave + lds_reps - 1) / lds_reps;
-    const uint64_t bound = iters * 4u * kWave * waves_per_rep + 1u;
+    // per lane: kDepositPerLane positions an iteration, plus one of the
+    // lead and one of the tail
+    const uint64_t bound = (iters * (uint64_t)kDepositPerLane + 2u) * kWave * waves_per_rep + 1u;
     int sh_bits = 20;
     while (sh_bits > 0 && (bound << sh_bits) >= (1ull << 32)) --sh_bits;
     const float scale = (float)(1u << sh_bits);
-    bool any = false;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 4 * stride) {
-        float xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            uint32_t idx = i + u * stride;
-            xv[u] = idx < n ? x[idx] : 0.0f;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t g = gid; g < n4; g += 2u * stride) {
+        const float4 a = x4[g];
+        const bool second = g + stride < n4;
+        const float4 b = x4[second ? g + stride : g];
+        deposit_one(h, a.x, inv_dx, n_g, scale);
+        deposit_one(h, a.y, inv_dx, n_g, scale);
+        deposit_one(h, a.z, inv_dx, n_g, scale);
+        deposit_one(h, a.w, inv_dx, n_g, scale);
+        if (second) {
+            deposit_one(h, b.x, inv_dx, n_g, scale);
+            deposit_one(h, b.y, inv_dx, n_g, scale);
+            deposit_one(h, b.z, inv_dx, n_g, scale);
+            deposit_one(h, b.w, inv_dx, n_g, scale);
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (i + u * stride < n && xv[u] >= 0.0f) {  // x < 0: a hole
-                const float pos = xv[u] * inv_dx;
-                const int q = cell_of(pos, n_g);
-                const float rem = pos - (float)q;
-                const uint32_t u2 = (uint32_t)(rem * scale + 0.5f);
-                atomicAdd(&h[q], (uint32_t)scale - u2);
-                atomicAdd(&h[q + 1], u2);
-            }
-        }
-        any = true;
     }
-    if (!__syncthreads_or(any)) return;
+    // the misaligned lead and the tail of fewer than four positions
+    const uint32_t tail0 = lead + 4u * n4;
+    if (gid < lead) deposit_one(h, x[gid], inv_dx, n_g, scale);
+    if (gid < n - tail0) deposit_one(h, x[tail0 + gid], inv_dx, n_g, scale);
+    __syncthreads();
+    if ((uint64_t)blockIdx.x * blockDim.x >= (uint64_t)n4 && blockIdx.x != 0) return;  // saw nothing
     Acc *out = dens + (blockIdx.x % reps) * rep_stride;
     const Acc unit = (Acc)wf / (Acc)scale;
     for (int k = threadIdx.x; k < n_g; k += blockDim.x) {
@@ -197,14 +227,11 @@ __global__ __launch_bounds__(256) void k_deposit(const float *__restrict__ x,
 int launch_deposit(const float *x, const uint32_t *n_active, void *dens, bool f64, const KPhys &k,
                    const Shape &sh0, hipStream_t s, uint32_t n_cap) {
     Shape sh = resolve(sh0);
-    const int block = 256;
+    const int block = kDepositBlock;
     const int lr = lds_replicas(block, k.n_g, 0);
     const size_t lds = (size_t)lr * k.n_g * sizeof(uint32_t);
     const int stride = sh.rep_stride > 0 ? sh.rep_stride : k.n_g;
-    // one workgroup per compute unit: each flushes n_g atomics into the
-    // (possibly single) global density, so fewer workgroups means less
-    // same-address contention at the flush
-    const int grid = std::max(1, std::min(device_cus(), sh.grid * sh.block / block / 2));
+    const int grid = device_cus();
     if (f64)
         k_deposit<double><<<grid, block, lds, s>>>(x, n_active, n_cap, (double *)dens, sh.replicas, stride,
                                                    k.inv_dx, k.wf, k.n_g, lr);
@@ -352,6 +379,9 @@ struct PoissonArgs {
     uint32_t *copy_cnt_dst;
     int copy_cnt_words;
     uint32_t *advance_time;
+    const uint32_t *copy_cnt_ion_src;
+    uint32_t *copy_cnt_ion_dst;
+    int64_t t_value;
     double *xt_grid;
     unsigned long long *xt_count;
     uint32_t xt_bin;
@@ -527,21 +557,26 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     // loaded together
     uint32_t t = 0;
     double p0_in = 0.0;
-    if (tid == 0 && a.time) {
+    if (tid == 0 && a.t_value >= 0) {
+        t = (uint32_t)a.t_value;
+        if (a.pot0_tab) p0_in = a.pot0_tab[t];
+    } else if (tid == 0 && a.time) {
         t = *a.time;
         if (a.pot0_in) p0_in = *a.pot0_in;
     }
-    uint32_t cnt_copy = 0;
+    uint32_t cnt_copy = 0, cnt_copy_ion = 0;
     if (a.copy_cnt_dst && tid < a.copy_cnt_words) cnt_copy = a.copy_cnt_src[tid];
+    if (a.copy_cnt_ion_dst && tid == 0) cnt_copy_ion = *a.copy_cnt_ion_src;
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
     load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, a.i_sum, dne, dni);
     double p0_next = 0.0;
     if (tid == 0) {
         double p0 = a.pot0;
         int ion = 1;
-        if (a.time) {
+        if (a.t_value >= 0 || a.time) {
             ion = (t % (uint32_t)a.n_sub) == 0;
-            p0 = a.pot0_in ? p0_in : waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
+            const bool have = a.t_value >= 0 ? a.pot0_tab != nullptr : a.pot0_in != nullptr;
+            p0 = have ? p0_in : waveform(t, a.n_t, a.n_harm, a.wave_cos, a.wave_sin);
             // the next step's pot0 from the table: the load is in flight
             // during the solve
             if (a.pot0_tab && a.pot0_next) p0_next = a.pot0_tab[(t + 1u) % (uint32_t)a.n_t];
@@ -585,6 +620,7 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     if (a.zero_cnt && tid < a.zero_cnt_words) a.zero_cnt[tid] = 0u;
     if (s_ion && a.zero_cnt_ion && tid < a.zero_cnt_ion_words) a.zero_cnt_ion[tid] = 0u;
     if (a.copy_cnt_dst && tid < a.copy_cnt_words) a.copy_cnt_dst[tid] = cnt_copy;
+    if (s_ion && a.copy_cnt_ion_dst && tid == 0) *a.copy_cnt_ion_dst = cnt_copy_ion;
     if (tid == (int)blockDim.x - 1 && a.meas_steps && a.cumul_e) {
         a.meas_steps[0] += 1ull;
         if (s_ion) a.meas_steps[1] += 1ull;
@@ -691,6 +727,13 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     a.copy_cnt_dst = j.copy_cnt_dst;
     a.copy_cnt_words = j.copy_cnt_words;
     a.advance_time = j.advance_time;
+    a.copy_cnt_ion_src = j.copy_cnt_ion_src;
+    a.copy_cnt_ion_dst = j.copy_cnt_ion_dst;
+    a.t_value = j.t_value;
+    if (j.t_value >= j.n_t || (j.copy_cnt_ion_dst && !j.copy_cnt_ion_src)) {
+        set_error("poisson: step index beyond the cycle or an ion counter copy without a source");
+        return PIC_ERR_ARG;
+    }
     switch (j.precision) {
     case PIC_PREC_FP32: {
         size_t lds = (j.n_g + 17) * sizeof(float);
@@ -1056,12 +1099,20 @@ __device__ __forceinline__ uint32_t tile_reserve(uint32_t wave_count, uint32_t *
     return s_base[buf] + s_wave[buf * 16 + wave];
 }
 
-// move_and_compact (main.rs:566-678) on wave64: each thread takes PPT
-// particles of a tile (strided by the workgroup size, so every load is
-// coalesced), survivors are ranked with 64-bit ballots and v_mbcnt, and the
-// tile makes one global atomic.  The efield array is staged in LDS.
-// Lost particles are counted per electrode (one atomic per workgroup) and,
-// for ions, binned by energy (IFED, C/eduPIC.cc:625-667).
+// move_and_compact (main.rs:566-678) on wave64.  A persistent grid of one
+// 1024-lane workgroup per compute unit walks tiles of 1024 * PPT particles;
+// each thread takes PPT particles of a tile (strided by the workgroup size,
+// so every load is coalesced), survivors are ranked with 64-bit ballots and
+// v_mbcnt, and the tile reserves its output range with one global atomic.
+// The next tile's loads are issued before the current tile's reservation
+// and stores, so the reservations -- same-address atomics on the single
+// alive counter, which serialise in the memory-side atomic unit -- overlap
+// with the streaming instead of forming a phase of their own.  The efield
+// array is staged in LDS.  Lost particles are counted per electrode (one
+// atomic per workgroup) and, for ions, binned by energy (IFED,
+// C/eduPIC.cc:625-667).
+constexpr int kMoveBlock = 1024;
+
 // Sum v over the workgroup and add the total to *dst with one atomic: the
 // counters of these kernels are single words, and thousands of waves adding
 // to one address serialise in the memory-side atomic unit.  Every thread of
@@ -1078,7 +1129,7 @@ __device__ void wg_atomic_add(unsigned long long *dst, uint32_t v) {
 }
 
 template <int PPT>
-__global__ __launch_bounds__(256) void k_move_compact(MoveArgs a) {
+__global__ __launch_bounds__(kMoveBlock) void k_move_compact(MoveArgs a) {
     extern __shared__ float ef[];  // n_g
     __shared__ uint32_t s_wave[32], s_base[2];
     __shared__ uint32_t s_ifed[2 * kIfedBins];
@@ -1091,32 +1142,44 @@ __global__ __launch_bounds__(256) void k_move_compact(MoveArgs a) {
     const uint32_t tile_sz = blockDim.x * PPT;
     const uint32_t n_tiles = (n + tile_sz - 1) / tile_sz;
     uint32_t lost_pow = 0, lost_gnd = 0;
+    // loads of one tile; lanes past the count read slot 0 (the count test
+    // is applied where the values are used, so no wait sits at the issue)
+    auto load = [&](uint32_t tile, float (&x)[PPT], float (&vx)[PPT], float (&vy)[PPT], float (&vz)[PPT]) {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const uint32_t i0 = tile * tile_sz + threadIdx.x + j * blockDim.x;
+            const uint32_t i = tile < n_tiles && i0 < n ? i0 : 0u;
+            x[j] = a.sx[i];
+            vx[j] = a.svx[i];
+            vy[j] = a.svy[i];
+            vz[j] = a.svz[i];
+        }
+    };
+    float cx[PPT], cvx[PPT], cvy[PPT], cvz[PPT];
+    if (n_tiles > blockIdx.x) load(blockIdx.x, cx, cvx, cvy, cvz);
     int buf = 0;
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x, buf ^= 1) {
-        float nx[PPT], nvx[PPT], vy[PPT], vz[PPT];
+        float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
+        load(tile + gridDim.x, px, pvx, pvy, pvz);
+        float nx[PPT], nvx[PPT];
         bool keep[PPT];
         uint32_t off[PPT];
         const uint32_t base = tile * tile_sz + threadIdx.x;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-            const uint32_t i = base + j * blockDim.x;
             keep[j] = false;
-            if (i < n) {
-                const float x = a.sx[i];
-                const float e = field_at(ef, x, a.inv_dx, a.n_g);
-                nvx[j] = __builtin_fmaf(a.factor, e, a.svx[i]);
-                nx[j] = __builtin_fmaf(nvx[j], a.dt, x);
+            nx[j] = nvx[j] = 0.0f;
+            if (base + j * blockDim.x < n) {
+                const float e = field_at(ef, cx[j], a.inv_dx, a.n_g);
+                nvx[j] = __builtin_fmaf(a.factor, e, cvx[j]);
+                nx[j] = __builtin_fmaf(nvx[j], a.dt, cx[j]);
                 keep[j] = nx[j] >= 0.0f && nx[j] <= a.gap;
-                if (keep[j]) {
-                    vy[j] = a.svy[i];
-                    vz[j] = a.svz[i];
-                } else {
+                if (!keep[j]) {
                     const bool pw = nx[j] < 0.0f;
                     lost_pow += pw;
                     lost_gnd += !pw;
                     if (do_ifed) {
-                        const float y = a.svy[i], z = a.svz[i];
-                        const float en = a.half_m_over_e * (nvx[j] * nvx[j] + y * y + z * z);
+                        const float en = a.half_m_over_e * (nvx[j] * nvx[j] + cvy[j] * cvy[j] + cvz[j] * cvz[j]);
                         const int b = (int)en;
                         if (en >= 0.0f && b < kIfedBins) atomicAdd(&s_ifed[(pw ? 0 : 1) * kIfedBins + b], 1u);
                     }
@@ -1137,9 +1200,16 @@ __global__ __launch_bounds__(256) void k_move_compact(MoveArgs a) {
                 const uint32_t slot = wbase + off[j];
                 a.dx[slot] = nx[j];
                 a.dvx[slot] = nvx[j];
-                a.dvy[slot] = vy[j];
-                a.dvz[slot] = vz[j];
+                a.dvy[slot] = cvy[j];
+                a.dvz[slot] = cvz[j];
             }
+        }
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            cx[j] = px[j];
+            cvx[j] = pvx[j];
+            cvy[j] = pvy[j];
+            cvz[j] = pvz[j];
         }
     }
     if (a.abs_counts) {
@@ -1177,8 +1247,8 @@ int launch_move_compact(const MoveJob &j, const KPhys &k, const Shape &sh0, hipS
     a.inv_dx = k.inv_dx;
     a.gap = k.gap;
     const size_t lds = (size_t)k.n_g * sizeof(float);
-    const int block = 256;
-    const int grid = std::max(1, sh.grid * sh.block / block);
+    const int block = kMoveBlock;
+    const int grid = device_cus();
     switch (sh.ppt) {
     case 1: k_move_compact<1><<<grid, block, lds, s>>>(a); break;
     case 2: k_move_compact<2><<<grid, block, lds, s>>>(a); break;
@@ -1910,6 +1980,21 @@ int pic_memset(void *dst, int value, uint64_t bytes) {
 }
 int pic_device_synchronize(void) {
     PIC_HIP(hipDeviceSynchronize());
+    return PIC_OK;
+}
+int pic_stream_create(pic_stream_t *out) {
+    if (!out) return PIC_ERR_ARG;
+    hipStream_t st = nullptr;
+    PIC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *out = st;
+    return PIC_OK;
+}
+int pic_stream_destroy(pic_stream_t stream) {
+    if (stream) PIC_HIP(hipStreamDestroy((hipStream_t)stream));
+    return PIC_OK;
+}
+int pic_stream_synchronize(pic_stream_t stream) {
+    PIC_HIP(hipStreamSynchronize((hipStream_t)stream));
     return PIC_OK;
 }
 int pic_device_info(int *compute_units, int *wave_size, char *arch, int arch_len) {

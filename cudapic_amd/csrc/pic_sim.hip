@@ -45,7 +45,7 @@ constexpr int kDefaultSegments = 32;
 constexpr uint32_t kSlotsPerDefaultSegment = 131072;
 constexpr int kMeasCounters = kCounterWords;
 constexpr uint32_t kCkptMagic = 0x43504943u;  // "CIPC"
-constexpr uint32_t kCkptVersion = 1;
+constexpr uint32_t kCkptVersion = 2;
 }  // namespace
 
 struct pic_sim {
@@ -684,9 +684,10 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     // the fused kernels hold ~110 VGPRs (4 waves per SIMD): a persistent grid
     // of 4 workgroups of 256 lanes per compute unit is fully resident
     sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
-    // compaction reserves output slots with one atomic per tile on a single
-    // counter on the reference path: 8 particles per lane halves those
-    sh.ppt = c->ppt > 0 ? c->ppt : (c->path == PIC_PATH_FUSED ? 4 : 8);
+    // particles per lane of a compacting tile (on the reference path a tile
+    // of 1024 lanes reserves its output with one atomic on the single live
+    // counter, and 4 keeps several tiles per workgroup to overlap them)
+    sh.ppt = c->ppt > 0 ? c->ppt : 4;
     sh.ppt_inplace = c->ppt_inplace > 0 ? c->ppt_inplace : 2;
     sh.lds_reps = std::max(0, c->lds_replicas);
     // global density replicas: 1, or a multiple of 8 (the Poisson kernel
@@ -1153,53 +1154,62 @@ int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_to
     return PIC_OK;
 }
 
+// Checkpoint file (version 2): header u32[8] = magic, version, n_g, t,
+// n_e, n_i, 0, 0; steps done (u64); per species the n live particles (x,
+// vx, vy, vz as f32[n] each) and the four RNG words of each particle's slot
+// in the same order (u32[n] each); then the measurements (cumul_e, cumul_i
+// f64[n_g], EEPF u32[2000], IFED u32[400], counters u64[8]).  A load puts
+// the particles into the first slots of its segments and gives each one
+// the RNG stream it had, so every particle continues its own sequence of
+// draws.  The continued run is the same discharge but not the same bits:
+// slots freed by lost particles and the streams newborns take differ, and
+// the next Poisson solve deposits every ion present (ions born since the
+// last ion step included), where an uninterrupted run adds those at the
+// next ion step.
 int pic_sim_save(pic_sim_t *s, const char *path) {
     if (!s || !path) return PIC_ERR_ARG;
     PIC_HIP(hipStreamSynchronize(s->stream));
+    uint32_t ne = 0, ni = 0;
+    int rc = pic_sim_counts(s, &ne, &ni);
+    if (rc) return rc;
     FILE *f = fopen(path, "wb");
     if (!f) {
         set_error(std::string("cannot open ") + path);
         return PIC_ERR_IO;
     }
-    uint32_t ne = 0, ni = 0;
-    int rc = pic_sim_counts(s, &ne, &ni);
-    if (rc) {
+    auto fail = [&](int r) {
         fclose(f);
-        return rc;
-    }
-    uint32_t hdr[8] = {kCkptMagic, kCkptVersion, (uint32_t)s->p.n_g, s->t, ne, ni, s->slots, 0};
+        return r;
+    };
+    uint32_t hdr[8] = {kCkptMagic, kCkptVersion, (uint32_t)s->p.n_g, s->t, ne, ni, 0, 0};
     fwrite(hdr, sizeof(hdr), 1, f);
     fwrite(&s->steps, sizeof(s->steps), 1, f);
-    std::vector<float> buf(std::max<uint32_t>(std::max(ne, ni), 1));
-    std::vector<uint32_t> w(s->slots);
+    std::vector<uint32_t> w[4];
+    for (auto &v : w) v.resize(s->slots);
     for (int sp = 0; sp < 2; ++sp) {
         const uint32_t n = sp == 0 ? ne : ni;
         std::vector<float> a[4];
         for (auto &v : a) v.resize(std::max<uint32_t>(n, 1));
-        uint32_t got = 0;
-        rc = pic_sim_download(s, sp, a[0].data(), a[1].data(), a[2].data(), a[3].data(), n, &got);
-        if (rc) {
-            fclose(f);
-            return rc;
+        std::vector<uint32_t> slots(std::max<uint32_t>(n, 1)), g(std::max<uint32_t>(n, 1));
+        uint32_t got = 0, got_s = 0;
+        if ((rc = pic_sim_download(s, sp, a[0].data(), a[1].data(), a[2].data(), a[3].data(), n, &got)) ||
+            (rc = pic_sim_download_slots(s, sp, slots.data(), n, &got_s)) ||
+            (rc = pic_sim_rng_download(s, sp, w[0].data(), w[1].data(), w[2].data(), w[3].data(), s->slots)))
+            return fail(rc);
+        if (got != n || got_s != n) {
+            set_error("checkpoint: the live count changed while saving");
+            return fail(PIC_ERR_IO);
         }
         for (auto &v : a) fwrite(v.data(), sizeof(float), n, f);
         for (int q = 0; q < 4; ++q) {
-            uint32_t *src = (sp == 0 ? s->e_rng : s->i_rng)[q];
-            if (hipMemcpy(w.data(), src, 4ull * s->slots, hipMemcpyDeviceToHost) != hipSuccess) {
-                fclose(f);
-                return hip_fail(hipGetLastError(), "ckpt rng");
-            }
-            fwrite(w.data(), 4, s->slots, f);
+            for (uint32_t j = 0; j < n; ++j) g[j] = w[q][slots[j]];
+            fwrite(g.data(), 4, n, f);
         }
     }
     std::vector<double> ce(s->p.n_g), cii(s->p.n_g);
     std::vector<uint32_t> ee(PIC_N_EEPF), ifd(2 * PIC_N_IFED);
     unsigned long long cnt[8];
-    rc = pic_sim_read_measurements(s, ce.data(), cii.data(), ee.data(), ifd.data(), cnt);
-    if (rc) {
-        fclose(f);
-        return rc;
-    }
+    if ((rc = pic_sim_read_measurements(s, ce.data(), cii.data(), ee.data(), ifd.data(), cnt))) return fail(rc);
     fwrite(ce.data(), 8, s->p.n_g, f);
     fwrite(cii.data(), 8, s->p.n_g, f);
     fwrite(ee.data(), 4, PIC_N_EEPF, f);
@@ -1232,8 +1242,16 @@ int pic_sim_load(pic_sim_t *s, const char *path) {
         return bad("not a checkpoint of this version");
     if ((int)hdr[2] != s->p.n_g) return bad("grid size differs");
     if (hdr[4] > s->cap || hdr[5] > s->cap) return bad("particle count above capacity");
-    if (fread(&steps, sizeof(steps), 1, f) != 1) return bad("truncated");
-    const uint32_t ckcap = hdr[6];
+    // the header's counts must account for the whole file before anything
+    // is allocated from them
+    if (fseek(f, 0, SEEK_END) != 0) return bad("cannot seek");
+    const long size = ftell(f);
+    const uint64_t want = sizeof(hdr) + sizeof(steps) + 32ull * ((uint64_t)hdr[4] + hdr[5]) +
+                          16ull * s->p.n_g + 4ull * (PIC_N_EEPF + 2 * PIC_N_IFED) + 64ull;
+    if (size < 0 || (uint64_t)size != want) return bad("file size does not match its header");
+    if (fseek(f, sizeof(hdr), SEEK_SET) != 0 || fread(&steps, sizeof(steps), 1, f) != 1) return bad("truncated");
+    std::vector<uint32_t> w[4];
+    for (auto &v : w) v.resize(s->slots);
     for (int sp = 0; sp < 2; ++sp) {
         const uint32_t n = sp == 0 ? hdr[4] : hdr[5];
         std::vector<float> a[4];
@@ -1241,19 +1259,28 @@ int pic_sim_load(pic_sim_t *s, const char *path) {
             v.resize(std::max<uint32_t>(n, 1));
             if (fread(v.data(), sizeof(float), n, f) != n) return bad("truncated particles");
         }
+        std::vector<uint32_t> g[4];
+        for (auto &v : g) {
+            v.resize(std::max<uint32_t>(n, 1));
+            if (fread(v.data(), 4, n, f) != n) return bad("truncated rng");
+        }
         int rc = pic_sim_upload(s, sp, a[0].data(), a[1].data(), a[2].data(), a[3].data(), n);
+        std::vector<uint32_t> slots(std::max<uint32_t>(n, 1));
+        uint32_t got = 0;
+        if (!rc) rc = pic_sim_download_slots(s, sp, slots.data(), n, &got);
+        if (!rc) rc = pic_sim_rng_download(s, sp, w[0].data(), w[1].data(), w[2].data(), w[3].data(), s->slots);
+        if (!rc && got != n) {
+            set_error("checkpoint: the upload did not place every particle");
+            rc = PIC_ERR_IO;
+        }
         if (rc) {
             fclose(f);
             return rc;
         }
-        std::vector<uint32_t> w[4];
-        for (auto &v : w) {
-            v.resize(ckcap);
-            if (fread(v.data(), 4, ckcap, f) != ckcap) return bad("truncated rng");
-        }
-        const uint32_t nr = std::min(ckcap, s->slots);
-        rc = pic_sim_rng_upload(s, sp, w[0].data(), w[1].data(), w[2].data(), w[3].data(), nr);
-        if (rc) {
+        // each particle gets the stream it had; free slots keep theirs
+        for (int q = 0; q < 4; ++q)
+            for (uint32_t j = 0; j < n; ++j) w[q][slots[j]] = g[q][j];
+        if ((rc = pic_sim_rng_upload(s, sp, w[0].data(), w[1].data(), w[2].data(), w[3].data(), s->slots))) {
             fclose(f);
             return rc;
         }

@@ -156,7 +156,8 @@ class Sim:
         n_meas = c["e_meas_steps"]
         density = m["cumul_e"][self.p.n_g // 2] / n_meas if n_meas else 0.0
         h = m["eepf"].astype(np.float64)
-        mean_energy = float(h @ ((np.arange(PIC_N_EEPF) + 0.5) * 0.05) / h.sum()) if h.sum() else 0.0
+        # an empty EEPF window: NaN, as eduPIC's 0/0 (which passes its conditions)
+        mean_energy = float(h @ ((np.arange(PIC_N_EEPF) + 0.5) * 0.05) / h.sum()) if h.sum() else float("nan")
         if steps is None:
             steps = self.time()[1]
         n_e, n_i = self.counts()

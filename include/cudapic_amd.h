@@ -213,7 +213,7 @@ typedef struct pic_sim_config {
     int32_t block_size;        /* particle kernels: 0 = tuned default */
     int32_t grid_blocks;       /* persistent grid: 0 = tuned default */
     int32_t ppt;               /* particles per thread per tile, compacting
-                                  steps: 0 = default (4 fused, 8 reference) */
+                                  steps: 1, 2, 4 or 8; 0 = 4 */
     int32_t replicas;          /* global density replicas: 0 = default */
     int32_t measure;           /* allocate measurement buffers */
     int32_t ppt_inplace;       /* particles per thread per tile, in-place
@@ -307,6 +307,80 @@ int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_to
  * counters, held densities, time, measurements). */
 int pic_sim_save(pic_sim_t *s, const char *path);
 int pic_sim_load(pic_sim_t *s, const char *path);
+
+/* ------------------------------------------------------------------------
+ * The RF step on caller-owned buffers (the reference's GpuSimState and the
+ * loop body of main.rs:1725-1808), on the fused kernels
+ * ---------------------------------------------------------------------- */
+
+/* The device buffers of the reference's GpuSimState (main.rs:132-185), owned
+ * by the caller; every particle and RNG array has `capacity` slots.  A step
+ * updates the particles in their slots; a particle that leaves the gap
+ * becomes a hole (x = -1) below *n_electrons / *n_ions until the next
+ * compacting step (t % compact_every == 0, an ion step: both species are
+ * compacted into the tmp arrays and the struct's pointers swapped, as the
+ * reference's host swaps its handles at main.rs:1764-1767, 1780-1782).  The
+ * counts are therefore watermarks of the slots in use; readers skip x < 0.
+ * The RNG words stay with their slots (main.rs:1764-1767 swaps x and v only). */
+typedef struct pic_state {
+    float *e_x, *e_vx, *e_vy, *e_vz;
+    float *i_x, *i_vx, *i_vy, *i_vz;
+    uint32_t *n_electrons, *n_ions;  /* device words */
+    float *efield, *pot;             /* n_g each, written by every step */
+    /* optional (both or neither): the f64 node densities each step's Poisson
+     * solve used; state the next steps read, like the particles (NULL keeps
+     * them in the context) */
+    double *e_density, *i_density;
+    const float *cs, *sigma_tot_e, *sigma_tot_i;  /* as pic_cross_sections */
+    uint32_t *rng_e0, *rng_e1, *rng_e2, *rng_e3;
+    uint32_t *rng_i0, *rng_i1, *rng_i2, *rng_i3;
+    float *tmp_x, *tmp_vx, *tmp_vy, *tmp_vz;          /* electrons' compaction target */
+    /* ions' compaction target: no reference counterpart (the reference
+     * shares tmp_* between species; the fused step compacts both in one step) */
+    float *tmp_i_x, *tmp_i_vx, *tmp_i_vy, *tmp_i_vz;
+    double *cumul_e_density, *cumul_i_density;        /* PIC_STEP_MEASURE: n_g each */
+    uint32_t *eepf_counts;                            /* PIC_STEP_MEASURE: PIC_N_EEPF */
+    uint32_t capacity;                                /* MAX_PARTICLES */
+    uint32_t pad;
+} pic_state_t;
+
+enum pic_step_flags {
+    PIC_STEP_MEASURE = 1, /* accumulate densities and the EEPF (main.rs:1746-1756, 1802-1807) */
+    PIC_STEP_RESET = 2    /* deposit the current particles first: required on the first step
+                             and after the caller changes particles, counts or the step order */
+};
+
+/* Launch shape of the fused kernels (0 = default, as pic_sim_config_t);
+ * compact_every is a multiple of 4*n_sub (0 = 800). */
+typedef struct pic_step_config {
+    uint32_t compact_every;
+    int32_t grid_blocks, ppt, ppt_inplace, lds_replicas;
+    int32_t pad;
+} pic_step_config_t;
+
+typedef struct pic_ctx pic_ctx_t;
+
+/* The workspace of the fused step for one discharge (density accumulators,
+ * frozen watermarks, device clock, pot0 table, overflow flag, diagnostic
+ * counters) and a cache of captured period graphs.  params derived;
+ * n_t a multiple of 2*n_sub, at most 32768; cfg may be NULL. */
+int pic_init(const pic_params_t *p, const pic_step_config_t *cfg, pic_ctx_t **out);
+int pic_destroy(pic_ctx_t *ctx);
+/* One RF step at cycle index t (0 <= t < n_t; the reference's t,
+ * main.rs:1741), asynchronous on `stream`; consecutive calls pass
+ * consecutive t.  Updates *s when the step compacts. */
+int pic_step(pic_ctx_t *ctx, pic_state_t *s, uint32_t t, int flags, pic_stream_t stream);
+/* n_steps steps from t: whole 4*n_sub-step periods (from t a multiple of
+ * 4*n_sub) replayed from captured hipGraphs, the rest as pic_step.  Graphs
+ * need a non-null stream (NULL runs every step eagerly) and are keyed by the
+ * state's buffers. */
+int pic_step_graph(pic_ctx_t *ctx, pic_state_t *s, uint32_t t, uint32_t n_steps, int flags,
+                   pic_stream_t stream);
+/* Synchronises `stream`; then (each may be NULL) counters[8] as
+ * pic_sim_read_measurements since pic_init, ifed[2*200] of the absorbed
+ * ions, and the overflow flag (1: appends beyond capacity were dropped). */
+int pic_step_diagnostics(pic_ctx_t *ctx, unsigned long long counters[8], uint32_t *ifed, uint32_t *overflow,
+                         pic_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Ensembles of independent discharges (one launch per stage per group of instances)
@@ -439,6 +513,10 @@ int pic_memcpy_htod(void *dst, const void *src, uint64_t bytes);
 int pic_memcpy_dtoh(void *dst, const void *src, uint64_t bytes);
 int pic_memset(void *dst, int value, uint64_t bytes);
 int pic_device_synchronize(void);
+/* A non-blocking stream of the default priority (for pic_step_graph) and its release. */
+int pic_stream_create(pic_stream_t *out);
+int pic_stream_destroy(pic_stream_t stream);
+int pic_stream_synchronize(pic_stream_t stream);
 /* Kernel copy of `bytes` (16-byte aligned) with 16-byte vector accesses: the
  * reference point for HBM traffic counters and achievable bandwidth (no
  * reference counterpart). */
@@ -448,9 +526,9 @@ const char *pic_last_error(void);
 /* Device properties used for launch shapes: compute units, wave size. */
 int pic_device_info(int *compute_units, int *wave_size, char *arch, int arch_len);
 /* sizeof(pic_params_t), sizeof(pic_species_t), sizeof(pic_sim_config_t),
- * sizeof(pic_ensemble_config_t), so that a binding can verify its struct
- * layout (no reference counterpart). */
-int pic_abi_layout(uint64_t out[4]);
+ * sizeof(pic_ensemble_config_t), sizeof(pic_state_t), sizeof(pic_step_config_t),
+ * so that a binding can verify its struct layout (no reference counterpart). */
+int pic_abi_layout(uint64_t out[6]);
 
 #ifdef __cplusplus
 }

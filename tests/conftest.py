@@ -11,6 +11,13 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
     config.addinivalue_line("markers", "slow: longer CPU statistical test")
+    # Build the CPU oracle (and, where the reference sources are mounted, the
+    # reference's own eduPIC under oracle/_ref) before collection: the tests
+    # that compare against eduPIC are skipped only where it cannot be built,
+    # not because nothing built it yet.
+    if os.path.isdir("/root/reference"):
+        from oracle import oracle
+        oracle.build()
 
 
 @pytest.fixture(scope="session")

@@ -88,3 +88,24 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_abi, "_LIB", None)
     with pytest.raises(RuntimeError, match="must be built"):
         _abi.lib()
+
+
+def test_step_entry_points_reject_bad_arguments_before_the_gpu():
+    """pic_init validates the discharge (n_t a multiple of 2*n_sub, n_g in
+    range) and pic_step / pic_step_graph a null context before any HIP call."""
+    import cudapic_amd as ca
+    from cudapic_amd import _abi
+    lib = _abi.lib()
+    h = C.c_void_p()
+    p = ca.params()
+    p.n_t = 4010                      # not a multiple of 2 * n_sub = 40
+    assert lib.pic_init(C.byref(p), None, C.byref(h)) == _abi.PIC_ERR_ARG
+    p = ca.params()
+    p.n_g = 5000
+    assert lib.pic_init(C.byref(p), None, C.byref(h)) == _abi.PIC_ERR_ARG
+    cfg = _abi.PicStepConfig(compact_every=100)   # not a multiple of 4 * n_sub
+    assert lib.pic_init(C.byref(ca.params()), C.byref(cfg), C.byref(h)) == _abi.PIC_ERR_ARG
+    st = _abi.PicState()
+    assert lib.pic_step(None, C.byref(st), 0, 0, None) == _abi.PIC_ERR_ARG
+    assert lib.pic_step_graph(None, C.byref(st), 0, 80, 0, None) == _abi.PIC_ERR_ARG
+    assert b"null" in lib.pic_last_error()

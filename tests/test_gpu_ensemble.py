@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from helpers import assert_same_multiset, sort_rows, synthetic_state
-from test_gpu_sim import _check_new_ions, _push_ref, _replay_species, _split_ions
+from replay import _check_new_ions, _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
 

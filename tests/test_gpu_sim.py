@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from helpers import SEED_E, SEED_I, assert_same_multiset, sort_rows, synthetic_state
+from replay import _check_new_ions, _check_slot_draws, _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
 PATHS = [0, 1]   # PIC_PATH_REFERENCE, PIC_PATH_FUSED
@@ -39,14 +40,6 @@ def zero_tables(orc, po):
     return cs, np.zeros_like(se), np.zeros_like(si)
 
 
-def _push_ref(orc, po, efield, parts, species):
-    x, vx, vy, vz = parts
-    if species == "e":
-        return orc.move_and_compact(efield, x, vx, vy, vz, po.factor_e, po.dt_e_f, po)
-    return orc.move_and_compact(efield, x, vx, vy, vz, po.factor_i, po.dt_i_f, po,
-                                mass_over_2e=np.float32(0.5 * 6.63352090e-26 / 1.60217662e-19))
-
-
 # start steps: 0 is an ion step on both paths and a compacting step on the
 # fused path; on the fused path 1 updates electrons in place (no ion step)
 # and 20 (= N_SUB) updates both species in place
@@ -60,21 +53,6 @@ CASES = [(path, t0, seg) for path, t0 in STARTS for seg in SEGMENTS if path or n
 
 def _pot0(t0):
     return float(np.float32(250.0 * np.cos(2.0 * np.pi * t0 / 4000.0)))
-
-
-def _check_slot_draws(orc, rng0, rng1, slots, cnt, draws):
-    """Every occupied slot advanced its stream `draws` times, every other slot
-    is untouched."""
-    s4 = np.zeros(4, np.uint32)
-    for j in slots[:: max(1, cnt // 2000)]:
-        s4[:] = [rng0[w][j] for w in range(4)]
-        for _ in range(draws):
-            orc.xo_next(s4)
-        assert [rng1[w][j] for w in range(4)] == s4.tolist()
-    free = np.ones(len(rng0[0]), bool)
-    free[slots] = False
-    for w in range(4):
-        np.testing.assert_array_equal(rng1[w][free], rng0[w][free])
 
 
 @pytest.mark.parametrize("path,t0,segments", CASES)
@@ -129,132 +107,6 @@ def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec, segments):
     assert sim.counts() == (ne, ni)
     sim.check()
     sim.close()
-
-
-def _replay_species(orc, po, stot, cs, parts, efield, got, slots, rng0, rng1, name, missing=0,
-                    src_slots=None):
-    """Match every GPU particle to its source by its pushed position and
-    replay its collision with the RNG words of its slot.
-
-    f32 positions repeat (a few hundred coincidences in 2e5 particles), so the
-    check runs over positions held by exactly one pushed particle.  There a
-    single GPU particle is a survivor (replayed); two are an ionisation: the
-    parent's slot drew its numbers, the newborn's slot (or staging entry,
-    slot -1) did not.  Returns the positions at which electrons were born
-    (one entry per newborn) and the ion velocities of the checked pairs.
-    `missing`: pushed particles whose GPU rows were left out of `got`.
-    `src_slots`: for a step that compacted after colliding, the slot whose
-    stream pushed source p used (its slot before the compaction); the parent
-    of a pair is then the one with the replayed velocity."""
-    cnt, pushed, _ = _push_ref(orc, po, efield, parts, name)
-    px, pvx, pvy, pvz = pushed
-    ux, uidx, ucnt = np.unique(px, return_index=True, return_counts=True)
-    gx, gvx, gvy, gvz = got
-    n = len(gx)
-    gux, ginv, gcnt = np.unique(gx, return_inverse=True, return_counts=True)
-    k = np.searchsorted(ux, gux)
-    k = np.minimum(k, len(ux) - 1)
-    in_pushed = ux[k] == gux
-    assert in_pushed.all(), "a GPU particle has no pushed source"
-    mo = ucnt[k]                      # pushed sources per GPU position
-    born = gcnt - mo                  # newborns per GPU position
-    assert (born >= 0).all()
-    assert born.sum() == n - cnt + missing
-
-    def moved(j):
-        sl = slots[j]
-        return sl >= 0 and any(rng1[w][sl] != rng0[w][sl] for w in range(4))
-
-    def replay(src, slot):
-        v = np.array([pvx[src], pvy[src], pvz[src]], np.float32)
-        st = np.array([rng0[w][slot] for w in range(4)], np.uint32)
-        if name == "e":
-            kind, vo, so, sev, siv = orc.collide_e_one(stot, cs, v, st, po)
-            return kind, v, vo, so, sev, siv
-        kind, vo, so = orc.collide_i_one(stot, cs, v, st, po)
-        return kind, v, vo, so, None, None
-
-    def rng_slot(j, g):
-        return src_slots[uidx[k[g]]] if src_slots is not None else slots[j]
-
-    single = (gcnt == 1) & (mo == 1)
-    checked = mism = 0
-    step = max(1, n // 20000)
-    for j in range(0, n, step):
-        g = ginv[j]
-        if not single[g]:
-            continue
-        rs = rng_slot(j, g)
-        kind, v, vo, so, _, _ = replay(uidx[k[g]], rs)
-        checked += 1
-        if so.tolist() != [rng1[w][rs] for w in range(4)]:
-            mism += 1
-            continue
-        gv = np.array([gvx[j], gvy[j], gvz[j]], np.float64)
-        assert np.linalg.norm(gv - vo) <= 1e-4 * max(np.linalg.norm(v), 2000.0), (name, j, gv, vo)
-    assert checked > 0.9 * (n // step) - 0.02 * n
-    assert mism <= max(2, checked // 10000)
-    ions = {}
-    if name == "e":
-        pair_groups = np.nonzero((gcnt == 2) & (mo == 1))[0]
-        members = {}
-        for j in np.nonzero(np.isin(ginv, pair_groups))[0]:
-            members.setdefault(ginv[j], []).append(j)
-        for g, pair in members.items():
-            if src_slots is None:
-                par = [j for j in pair if moved(j)]
-                kid = [j for j in pair if not moved(j)]
-                assert len(par) == 1 and len(kid) == 1, (gux[g], pair)
-                kind, v, vo, so, sev, siv = replay(uidx[k[g]], slots[par[0]])
-            else:
-                rs = src_slots[uidx[k[g]]]
-                kind, v, vo, so, sev, siv = replay(uidx[k[g]], rs)
-                assert so.tolist() == [rng1[w][rs] for w in range(4)]
-                gv = [np.array([gvx[j], gvy[j], gvz[j]], np.float64) for j in pair]
-                swap = (np.linalg.norm(gv[1] - vo) + np.linalg.norm(gv[0] - sev)
-                        < np.linalg.norm(gv[0] - vo) + np.linalg.norm(gv[1] - sev))
-                par, kid = ([pair[1]], [pair[0]]) if swap else ([pair[0]], [pair[1]])
-            assert kind == 3
-            tol = 1e-4 * max(np.linalg.norm(v), 2000.0)
-            assert np.linalg.norm(np.array([gvx[par[0]], gvy[par[0]], gvz[par[0]]]) - vo) <= tol
-            assert np.linalg.norm(np.array([gvx[kid[0]], gvy[kid[0]], gvz[kid[0]]]) - sev) <= tol
-            ions[float(gux[g])] = siv
-        assert len(members) >= 0.8 * born.sum()
-    born_x = np.repeat(gux, born)
-    return born_x, ions
-
-
-def _split_ions(gi, old_x):
-    """Separate the GPU ions into the old ones (positions old_x) and the
-    ionisation products.  f32 positions can coincide, so this works on
-    multisets: returns the positions of the new ions (the multiset
-    difference), a mask of the GPU ions that are certainly old (their
-    position is held by old ions only), the number of old ions left out of
-    that mask and the ambiguous positions."""
-    ux, uc = np.unique(old_x, return_counts=True)
-    gx, gc = np.unique(gi[0], return_counts=True)
-    k = np.clip(np.searchsorted(ux, gx), 0, max(0, len(ux) - 1))
-    have = np.where(ux[k] == gx, uc[k], 0) if len(ux) else np.zeros_like(gc)
-    surplus = gc - have
-    assert (surplus >= 0).all(), "an old ion is missing"
-    amb_sel = (surplus > 0) & (have > 0)
-    amb = gx[amb_sel]
-    old_mask = np.isin(gi[0], old_x) & ~np.isin(gi[0], amb)
-    return np.repeat(gx, surplus), old_mask, int(have[amb_sel].sum()), amb
-
-
-def _check_new_ions(gi, pushed_ion_x, born_x, ions, check_v):
-    """Ions that are not pushed survivors are the ionisation products: one
-    per newborn electron, at the same position, with a thermal velocity."""
-    new_x = _split_ions(gi, pushed_ion_x)[0]
-    np.testing.assert_array_equal(np.sort(new_x), np.sort(born_x))
-    new = ~np.isin(gi[0], pushed_ion_x)
-    if check_v:
-        for j in np.nonzero(new)[0]:
-            x = float(gi[0][j])
-            if x in ions:
-                v = np.array([gi[1][j], gi[2][j], gi[3][j]])
-                assert np.linalg.norm(v - ions[x]) <= 1e-3 * 270 * 6
 
 
 @pytest.mark.parametrize("path,t0,segments", CASES)
@@ -434,7 +286,8 @@ def test_checkpoint_round_trip(ca, orc, tmp_path):
     path = os.path.join(tmp_path, "state.ckpt")
     sim.save(path)
     e0, i0 = sim.download(0), sim.download(1)
-    r0 = sim.rng_download(0)
+    r0 = [sim.rng_download(s) for s in (0, 1)]
+    slots0 = [sim.slots(s) for s in (0, 1)]
     m0 = sim.measurements()
     sim2 = make_sim(ca, p, 100_000, 1)
     sim2.load(path)
@@ -442,8 +295,12 @@ def test_checkpoint_round_trip(ca, orc, tmp_path):
         np.testing.assert_array_equal(a, b)
     for a, b in zip(sim2.download(1), i0):
         np.testing.assert_array_equal(a, b)
-    for a, b in zip(sim2.rng_download(0), r0):
-        np.testing.assert_array_equal(a, b)
+    # every particle keeps its RNG stream, although the load packs the
+    # particles into other slots (the holes of the saved run are gone)
+    for s in (0, 1):
+        r1, slots1 = sim2.rng_download(s), sim2.slots(s)
+        for w in range(4):
+            np.testing.assert_array_equal(r1[w][slots1], r0[s][w][slots0[s]])
     m1 = sim2.measurements()
     np.testing.assert_array_equal(m1["cumul_e"], m0["cumul_e"])
     np.testing.assert_array_equal(m1["eepf"], m0["eepf"])
@@ -454,6 +311,44 @@ def test_checkpoint_round_trip(ca, orc, tmp_path):
     assert sim2.counts()[0] > 0
     sim.close()
     sim2.close()
+
+
+def test_checkpoint_continuation_and_corrupt_files(ca, orc, tmp_path):
+    """A collisionless FP64 run saved and resumed continues as the run that
+    was not interrupted (up to the summation order of the density atomics);
+    truncated or padded files are refused before anything is allocated."""
+    p = ca.params(weight=2.0e3, precision=2)
+    po = orc.params(weight=2.0e3)
+    cs, se0, si0 = zero_tables(orc, po)
+    E, I = synthetic_state(60_000, 63_000, seed=62)
+    sims = [make_sim(ca, p, 100_000, 1) for _ in range(2)]
+    path = os.path.join(tmp_path, "c.ckpt")
+    try:
+        for s in sims:
+            s.set_tables(cs, se0, si0)
+        sims[0].upload(0, *E)
+        sims[0].upload(1, *I)
+        sims[0].run(57)
+        sims[0].save(path)
+        sims[1].load(path)
+        for s in sims:
+            s.run(103)
+        ca_, cb = sims[0].counts(), sims[1].counts()
+        assert abs(ca_[0] - cb[0]) <= 2 and abs(ca_[1] - cb[1]) <= 2, (ca_, cb)
+        for sp in (0, 1):
+            a = sort_rows(*sims[0].download(sp))
+            b = sort_rows(*sims[1].download(sp))
+            m = min(len(a), len(b))
+            assert np.abs(a[:m, 0] - b[:m, 0]).max() < 1e-6
+        raw = open(path, "rb").read()
+        for bad in (raw[:-8], raw + b"\0" * 8, raw[:100]):
+            with open(path, "wb") as f:
+                f.write(bad)
+            with pytest.raises(ca.PicError, match="status 5"):
+                sims[1].load(path)
+    finally:
+        for s in sims:
+            s.close()
 
 
 def test_capacity_overflow_is_reported(ca, orc):

@@ -100,5 +100,11 @@ def test_condition_flags(ca):
     # no electrons measured at the centre: NaN passes, as in eduPIC
     r = ca.kernels.report(p, 0.0, float("nan"), 1e-6, 1, 1, 1, 1, tables=(se, si))
     assert math.isnan(r["debye_length"]) and r["conditions_ok"]
+    # a centre density but an empty EEPF window: the mean energy is eduPIC's
+    # 0/0 = NaN (not 0 eV, which would make the Debye length 0 and DX / Debye
+    # infinite, a violation eduPIC does not report)
+    r = ca.kernels.report(p, 7.4e15, float("nan"), 1e-6, 1, 1, 1, 1, tables=(se, si))
+    assert math.isnan(r["debye_length"]) and math.isnan(r["dx_over_debye"]) and r["conditions_ok"]
+    assert r["plasma_freq"] > 0
     with pytest.raises(ca.PicError):
         ca.kernels.report(p, 1e15, 3.0, 0.0, 1, 1, 1, 1, tables=(se, si))
