@@ -181,6 +181,7 @@ SIGNATURES = {
     "pic_step": (I32, [P, C.POINTER(PicState), U32, I32, P]),
     "pic_step_graph": (I32, [P, C.POINTER(PicState), U32, U32, I32, P]),
     "pic_step_diagnostics": (I32, [P, P, P, P, P]),
+    "pic_step_noalloc": (I32, [PP, C.POINTER(PicState), U32, I32, P]),
     "pic_malloc": (I32, [C.POINTER(P), U64]),
     "pic_free": (I32, [P]),
     "pic_memcpy_htod": (I32, [P, P, U64]),

@@ -7,8 +7,9 @@
 // field for -- the next step's density accumulators (replicas, two
 // parities per species), the frozen watermarks of the in-place kernels, the
 // device clock and the pot0 of the period, the overflow flag and the
-// diagnostic counters -- lives in a workspace that pic_init allocates
-// (with a cache of captured period graphs).
+// diagnostic counters -- lives in a workspace.  pic_init allocates it (with
+// a cache of captured period graphs); pic_step_noalloc carves it out of a
+// fixed device array instead, for callers that cannot allocate.
 //
 // Counter protocol (one segment, the caller's arrays): *n_electrons is the
 // watermark the electron kernel appends newborns after; the Poisson solve of
@@ -22,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "pic_internal.h"
 
@@ -363,6 +365,39 @@ int steps_eager(const StepCtx &x, pic_state_t *s, uint32_t t, uint32_t n, int fl
     return PIC_OK;
 }
 
+constexpr size_t kScratchBytes = 1536u * 1024u;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// ------------------------------------------------------------ no-allocation workspace
+// The fixed device array behind pic_step_noalloc (each device holds its own
+// copy until the process ends).  Its name is a host-side stand-in, not a
+// device address: host code uses only the per-device address
+// hipGetSymbolAddress returns.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+int scratch_ptr(uint8_t **out) {
+    static std::mutex mu;
+    static uint8_t *per_dev[kMaxDevices] = {};
+    int dev = 0;
+    PIC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("pic_step_noalloc: device index beyond the table of workspaces");
+        return PIC_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!per_dev[dev]) {
+        void *p = nullptr;
+        PIC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        per_dev[dev] = (uint8_t *)p;
+    }
+    *out = per_dev[dev];
+    return PIC_OK;
+}
+
 }  // namespace
 
 struct pic_ctx {
@@ -547,6 +582,25 @@ int pic_step_diagnostics(pic_ctx_t *c, unsigned long long counters[8], uint32_t 
     if (ifed) PIC_HIP(hipMemcpy(ifed, c->x.ifed, 2 * PIC_N_IFED * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (overflow) PIC_HIP(hipMemcpy(overflow, c->x.cnt + SC_OVERFLOW, sizeof(uint32_t), hipMemcpyDeviceToHost));
     return PIC_OK;
+}
+
+int pic_step_noalloc(const pic_params_t *p, pic_state_t *s, uint32_t t, int flags, pic_stream_t stream) {
+    int rc = check_params(p);
+    if (rc != PIC_OK) return rc;
+    if ((rc = check_state(s, flags)) != PIC_OK) return rc;
+    if (t >= (uint32_t)p->n_t) {
+        set_error("pic_step_noalloc: t beyond the RF cycle");
+        return PIC_ERR_ARG;
+    }
+    if (step_layout(*p).total > kScratchBytes) {
+        set_error("pic_step_noalloc: n_g / n_t need more than the 1.5 MiB no-allocation workspace");
+        return PIC_ERR_ARG;
+    }
+    uint8_t *ws = nullptr;
+    if ((rc = scratch_ptr(&ws)) != PIC_OK) return rc;
+    StepCtx x;
+    if ((rc = make_ctx(*p, nullptr, ws, false, &x)) != PIC_OK) return rc;
+    return steps_eager(x, s, t, 1, flags, (hipStream_t)stream);
 }
 
 }  // extern "C"

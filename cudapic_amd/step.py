@@ -166,6 +166,11 @@ class StepState:
     def step(self, t, measure=False, reset=False):
         call("pic_step", self.ctx, C.byref(self.st), int(t), self._flags(measure, reset), self.stream)
 
+    def step_noalloc(self, t, measure=False, reset=False):
+        """pic_step_noalloc: the same step on the library's fixed workspace."""
+        call("pic_step_noalloc", C.byref(self.p), C.byref(self.st), int(t), self._flags(measure, reset),
+             self.stream)
+
     def run(self, t, n_steps, measure=False, reset=False):
         call("pic_step_graph", self.ctx, C.byref(self.st), int(t), int(n_steps), self._flags(measure, reset),
              self.stream)

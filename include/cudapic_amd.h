@@ -381,6 +381,19 @@ int pic_step_graph(pic_ctx_t *ctx, pic_state_t *s, uint32_t t, uint32_t n_steps,
  * ions, and the overflow flag (1: appends beyond capacity were dropped). */
 int pic_step_diagnostics(pic_ctx_t *ctx, unsigned long long counters[8], uint32_t *ifed, uint32_t *overflow,
                          pic_stream_t stream);
+/* pic_step without a context, for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  The workspace is a fixed 1.5 MiB
+ * device array per GPU, enough for every discharge pic_init accepts
+ * (n_g <= 4096, n_t <= 32768: 1.32 MiB; the reference case uses 0.15 MiB;
+ * a larger need returns PIC_ERR_ARG); the call allocates nothing and uses
+ * the default launch shape.  The array
+ * keeps the last call's state: consecutive steps of one discharge pass
+ * consecutive t, the first with PIC_STEP_RESET, and switching discharges on
+ * a GPU needs PIC_STEP_RESET again; calls that overlap on one GPU race.  No
+ * diagnostic counters or IFED are kept; a capacity overflow shows as a
+ * count above capacity. */
+int pic_step_noalloc(const pic_params_t *p, pic_state_t *s, uint32_t t, int flags, pic_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Ensembles of independent discharges (one launch per stage per group of instances)
