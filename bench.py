@@ -23,7 +23,11 @@ measurement the line gains an "ensemble" object (64 discharges in one
 launch per stage against the same 64 run one after the other on the fused
 path, cudapic_amd.ens_bench's workload) and an "adjoint" object (backward
 pass / frozen-event forward pass of the 4 M-particle discharge,
-cudapic_amd.adjoint_bench); --no-extras skips both.
+cudapic_amd.adjoint_bench).  The roofline gains an "hbm_resident" object
+(the same step at 4x the particles, beyond the Infinity Cache) and the copy
+cross-checks; "drop_in" gives the same workload through the two entry points
+a reference host binds (the whole step on its own buffers, and the
+reference's stage kernels).  --no-extras skips them all.
 
 Prints one JSON line.
 """
@@ -67,6 +71,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp32", "comp", "fp64"], default="fp64")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=0, help="lanes per workgroup of the fused kernels (128/256/512)")
     ap.add_argument("--ppt", type=int, default=0, help="particles per thread, compacting steps")
     ap.add_argument("--ppt-inplace", type=int, default=0, help="particles per thread, in-place steps")
     ap.add_argument("--replicas", type=int, default=0)
@@ -84,28 +89,37 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="particles per species of the CPU sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the streaming-copy calibration kernel on MIB MiB, 20 times")
+    ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth cross-checks")
     return ap.parse_args()
 
 
-def copy_bandwidth(mib: int, reps: int = 20) -> float:
-    """GB/s (read + write) of pic_stream_copy over mib MiB: the achievable
-    HBM rate on this device, and the calibration kernel of the traffic
-    counters."""
+def copy_bandwidth(mib: int, reps: int = 20) -> dict:
+    """GB/s over mib MiB of three streams: pic_stream_copy (read + write, the
+    calibration kernel of the traffic counters), the runtime's device-to-
+    device hipMemcpyAsync (read + write) and the read-only pic_stream_read:
+    what this device reaches from HBM, measured three ways."""
     import cudapic_amd as ca
     from cudapic_amd._abi import call
     n = mib * (1 << 20)
     a = ca.DeviceBuffer.zeroed(n // 4, np.uint32)
     b = ca.DeviceBuffer.zeroed(n // 4, np.uint32)
-    call("pic_stream_copy", b.ptr, a.ptr, n, None)
-    ca.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        call("pic_stream_copy", b.ptr, a.ptr, n, None)
-    ca.synchronize()
-    dt = time.perf_counter() - t0
-    a.free()
-    b.free()
-    return 2.0 * n * reps / dt / 1e9
+    acc = ca.DeviceBuffer.zeroed(1, np.uint64)
+
+    def rate(launch, bytes_moved):
+        launch()
+        ca.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            launch()
+        ca.synchronize()
+        return round(bytes_moved * reps / (time.perf_counter() - t0) / 1e9, 1)
+
+    out = {"copy_kernel": rate(lambda: call("pic_stream_copy", b.ptr, a.ptr, n, None), 2.0 * n),
+           "hip_memcpy_d2d": rate(lambda: call("pic_memcpy_dtod_async", b.ptr, a.ptr, n, None), 2.0 * n),
+           "read_kernel": rate(lambda: call("pic_stream_read", a.ptr, n, acc.ptr, None), 1.0 * n)}
+    for buf in (a, b, acc):
+        buf.free()
+    return out
 
 
 def traffic_from_profiles():
@@ -143,9 +157,10 @@ def cpu_baseline(n_sample: int):
         shutil.rmtree(tmp, ignore_errors=True)
     ne0, ni0 = len(E[0]), len(I[0])
     ps = 0.5 * (ne0 + ne1) * 4000 + 0.5 * (ni0 + ni1) * 200
-    return {"value": ps / wall, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
-            "sample": f"eduPIC C (g++ -O3 -march=x86-64-v3), 1 RF cycle (4000 steps) from "
-                      f"{ne0} e + {ni0} i of the reference case at W=7e4 ({wall:.1f} s wall)"}
+    host = os.cpu_count()
+    return {"value": ps / wall, "unit": "particle-steps/s", "cores": 1, "host_cores": host, "kind": "reference",
+            "sample": f"eduPIC C (g++ -O3 -march=x86-64-v3, serial: 1 of {host} host cores), 1 RF cycle "
+                      f"(4000 steps) from {ne0} e + {ni0} i of the reference case at W=7e4 ({wall:.1f} s wall)"}
 
 
 def port_baseline():
@@ -164,8 +179,8 @@ def port_baseline():
         ps += sim.n_e + (sim.n_i if t % 20 == 0 else 0)
         sim.step(t)
     wall = time.perf_counter() - t0
-    return {"value": ps / wall, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle step, 400 steps from 50000 e + 52500 i ({wall:.1f} s wall)"}
+    return {"value": ps / wall, "unit": "particle-steps/s", "cores": 1, "host_cores": os.cpu_count(),
+            "kind": "port", "sample": f"CPU oracle step, 400 steps from 50000 e + 52500 i ({wall:.1f} s wall)"}
 
 
 def main():
@@ -178,12 +193,16 @@ def main():
     from cudapic_amd import synthetic
 
     if args.copy_probe:
-        print(json.dumps({"copy_probe_MiB": args.copy_probe, "GB/s": round(copy_bandwidth(args.copy_probe), 1)}))
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "GB/s": copy_bandwidth(args.copy_probe)}))
         return
-    # the ensemble figure first, in a child process, before this process
-    # allocates anything on the GPU; a failing extra is reported in its own
-    # field and does not take the headline line with it
+    # the ensemble figure and the HBM-resident roofline first, in child
+    # processes, before this process allocates anything on the GPU; a failing
+    # extra is reported in its own field and does not take the headline line
+    # with it
     ens_fig = None if args.no_extras else _extra(ensemble_figure)
+    hbm_fig = None if (args.no_extras or args.path != "fused") else _extra(hbm_resident_figure)
+    drop_in = None if args.no_extras else {"caller_owned_step": _extra(caller_owned_step_figure),
+                                           "reference_stage_kernels": _extra(reference_path_figure)}
     prec = {"fp32": ca.PIC_PREC_FP32, "comp": ca.PIC_PREC_COMP, "fp64": ca.PIC_PREC_FP64}[args.precision]
     path = ca.PIC_PATH_FUSED if args.path == "fused" else ca.PIC_PATH_REFERENCE
     p = ca.params(weight=args.weight, precision=prec)
@@ -200,7 +219,7 @@ def main():
         start = "synthetic plasma-like state"
     print(f"bench: start: {start}, {len(E[0])} e + {len(I[0])} i", file=sys.stderr, flush=True)
     shape = dict(grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
-                 ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas)
+                 ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas, block_size=args.block_size)
     tuned = None
     if args.autotune and path == ca.PIC_PATH_FUSED:
         from cudapic_amd import autotune
@@ -215,6 +234,17 @@ def main():
     sim.synchronize()
     sim.check()
     print("bench: warmup done", file=sys.stderr, flush=True)
+    # which of the timed steps replay a captured graph: whole 4*N_SUB-step
+    # periods that start on the period (the rest launch eagerly)
+    graph_steps = 0
+    if not args.no_graph and path == ca.PIC_PATH_FUSED:
+        period, t_in, left = 4 * p.n_sub, sim.time()[0], args.steps
+        while left > 0:
+            if t_in % period == 0 and left >= period:
+                graph_steps += period
+                t_in, left = (t_in + period) % p.n_t, left - period
+            else:
+                t_in, left = (t_in + 1) % p.n_t, left - 1
     # timed region (one GPU: the barrier is the stream synchronisation)
     sim.synchronize()
     t = sim.time_steps(args.steps)
@@ -240,8 +270,13 @@ def main():
                     "kernel_avg_us": round(avg_ms * 1e3, 2), "electrons_per_launch": int(units),
                     "bytes_per_unit": ALG_BYTES}
     sim.close()
-    if roofline is not None:
-        roofline["copy_kernel_GBs"] = round(copy_bandwidth(1024), 1)
+    if roofline is not None and not args.no_copy:
+        bw = copy_bandwidth(1024)
+        roofline["copy_kernel_GBs"] = bw["copy_kernel"]
+        roofline["hip_memcpy_d2d_GBs"] = bw["hip_memcpy_d2d"]
+        roofline["read_kernel_GBs"] = bw["read_kernel"]
+    if roofline is not None and hbm_fig is not None:
+        roofline["hbm_resident"] = hbm_fig
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
     step_frac = value * ALG_BYTES / (HBM_PEAK_GBS * 1e9)
     out = {
@@ -257,10 +292,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "argon CCP at scale: W=2e3, ~3.8M e + ~4.0M i, MAX=8M/species, "
-                               "N_G=400, N_T=4000, N_SUB=20 (plain method at scale)", "start": start,
+        "config": {"workload": f"argon CCP at scale: W={args.weight:g}, {n_e / 1e6:.2f}M e + {n_i / 1e6:.2f}M i, "
+                               f"MAX={args.capacity / 1e6:g}M/species, N_G=400, N_T=4000, N_SUB=20 "
+                               "(plain method at scale)", "start": start,
                    "n_electrons_end": n_e, "n_ions_end": n_i, "path": args.path,
-                   "graph": not args.no_graph, "poisson_and_density_flush": args.precision,
+                   "graph": graph_steps > 0, "graph_steps": graph_steps,
+                   "eager_steps": args.steps - graph_steps, "poisson_and_density_flush": args.precision,
                    "step_hbm_frac_80B": round(step_frac, 4),
                    **({"autotuned": tuned["config"]} if tuned else {})},
         "roofline": roofline,
@@ -270,6 +307,7 @@ def main():
         sim.close()
         out["ensemble"] = ens_fig
         out["adjoint"] = _extra(adjoint_figure, args.adjoint_steps)
+        out["drop_in"] = drop_in
     print(json.dumps(out), flush=True)
 
 
@@ -291,14 +329,62 @@ def ensemble_figure():
     changes where the ensemble's arrays land, which alone moves the
     cache-blocked ensemble by 3-5 %."""
     print("bench: ensemble of 64 discharges (child process)", file=sys.stderr, flush=True)
-    r = subprocess.run([sys.executable, "-m", "cudapic_amd.ens_bench", "--sequential-path", "fused"],
+    r = subprocess.run([sys.executable, "-m", "cudapic_amd.ens_bench", "--sequential-path", "fused", "--one-group"],
                        cwd=ROOT, capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         raise RuntimeError(f"ens_bench failed ({r.returncode}):\n{r.stderr[-2000:]}")
     d = json.loads(r.stdout.strip().splitlines()[-1])
     return {"instances": d["instances"], "steps": d["steps"], "value": d["ensemble"]["value"],
             "sequential_fused": d["sequential"]["value"], "speedup": d["speedup"], "unit": d["unit"],
+            "one_group_value": d["one_group"]["value"], "one_group_speedup": d["one_group_speedup"],
+            "grouping": "cache-sized groups of instances, each a block of 5 graph periods at a time; "
+                        "one_group: every instance in each launch, no temporal blocking",
             "process": "child (cudapic_amd.ens_bench)"}
+
+
+def hbm_resident_figure():
+    """The same step at 4x the particles (W = 500: the converged reference
+    case repeated 140 times, ~15 M electrons + ~16 M ions, 2.6 GB of
+    electron state), whose working set does not fit the 256 MB Infinity
+    Cache: the electron kernel's roofline fraction streaming from HBM.
+    Measured in a child process before this process touches the GPU."""
+    print("bench: HBM-resident roofline at 4x the particles (child process)", file=sys.stderr, flush=True)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--capacity", "32000000", "--weight", "500",
+                        "--steps", "400", "--warmup", "80", "--kernel-steps", "400", "--no-extras",
+                        "--no-cpu-baseline", "--no-copy"], cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"HBM-resident run failed ({r.returncode}):\n{r.stderr[-2000:]}")
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    rf = d["roofline"]
+    return {"value": d["value"], "electrons_per_launch": rf["electrons_per_launch"],
+            "kernel_avg_us": rf["kernel_avg_us"], "achieved": rf["achieved"], "frac": rf["frac"],
+            "workload": d["config"]["workload"], "process": "child (bench.py --capacity 32000000 --weight 500)"}
+
+
+def caller_owned_step_figure():
+    """The same workload through the entry point a reference host binds with
+    its own GpuSimState buffers (pic_step_graph, cudapic_amd.step_bench),
+    in a child process."""
+    print("bench: caller-owned step (child process)", file=sys.stderr, flush=True)
+    r = subprocess.run([sys.executable, "-m", "cudapic_amd.step_bench", "--steps", "4000", "--warmup", "400"],
+                       cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"step_bench failed ({r.returncode}):\n{r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def reference_path_figure():
+    """The same workload on the reference's stage order, one kernel per stage
+    (the per-kernel drop-ins of main.rs:352-1358), in a child process."""
+    print("bench: reference stage order (child process)", file=sys.stderr, flush=True)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--path", "reference", "--steps", "400",
+                        "--warmup", "40", "--kernel-steps", "0", "--no-extras", "--no-cpu-baseline", "--no-copy"],
+                       cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference path failed ({r.returncode}):\n{r.stderr[-2000:]}")
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": d["value"], "unit": d["unit"], "steps": d["steps"], "ms_per_step": d["ms_per_step"],
+            "entry_points": "pic_get_density, pic_solve_poisson, pic_move_and_compact, pic_check_collisions_e/_i"}
 
 
 def adjoint_figure(steps):

@@ -63,6 +63,7 @@ class PicEnsembleConfig(C.Structure):
         *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas", "lds_replicas")],
         ("compact_every", C.c_uint32),
         ("group_slots", C.c_uint32), ("group_periods", C.c_uint32), ("group_streams", C.c_uint32),
+        ("block_size", C.c_int32),
     ]
 
 
@@ -192,6 +193,8 @@ SIGNATURES = {
     "pic_stream_destroy": (I32, [P]),
     "pic_stream_synchronize": (I32, [P]),
     "pic_stream_copy": (I32, [P, P, U64, P]),
+    "pic_stream_read": (I32, [P, U64, P, P]),
+    "pic_memcpy_dtod_async": (I32, [P, P, U64, P]),
     "pic_last_error": (C.c_char_p, []),
     "pic_device_info": (I32, [C.POINTER(I32), C.POINTER(I32), C.c_char_p, I32]),
     "pic_abi_layout": (I32, [C.POINTER(U64)]),

@@ -6,8 +6,9 @@
 The fused path's speed depends on a handful of launch parameters whose best
 values move with the device, the particle count and the grid size:
 
-  grid_blocks   persistent workgroups of the particle kernels (2, 4 or 8 per
-                compute unit)
+  block_size    lanes per workgroup of the particle kernels (128, 256, 512)
+  lanes_per_cu  persistent lanes per compute unit (512, 1024, 2048; the grid
+                is lanes_per_cu / block_size workgroups per compute unit)
   ppt_inplace   particles per lane per tile of the in-place kernel (1, 2, 4)
   ppt           particles per lane per tile of the compacting kernel (2, 4)
   lds_replicas  LDS density histograms per workgroup (1, 2, 4; the waves of
@@ -16,8 +17,6 @@ values move with the device, the particle count and the grid size:
                 (8, 16, 32)
   segments      segments per species (8, 16, 32; the single-discharge path)
 
-The workgroup size of the fused kernels is fixed at 256 lanes (four waves):
-their collision list, LDS layout and register budget are laid out for it.
 The tuner measures real steps: for each candidate it builds the simulation,
 uploads the same starting state, runs a warm-up (which captures the step
 graph) and times whole 4*N_SUB-step graph periods with HIP events.  The
@@ -41,7 +40,8 @@ import os
 import sys
 
 SIM_SPACE = {
-    "grid_per_cu": [4, 2, 8],
+    "block_size": [256, 128, 512],
+    "lanes_per_cu": [1024, 512, 2048],
     "ppt_inplace": [2, 1, 4],
     "lds_replicas": [0, 2, 1],
     "replicas": [8, 16, 32],
@@ -49,7 +49,8 @@ SIM_SPACE = {
     "ppt": [4, 2],
 }
 ENSEMBLE_SPACE = {
-    "grid_per_cu": [4, 2, 8],
+    "block_size": [256, 128, 512],
+    "lanes_per_cu": [1024, 512, 2048],
     "ppt": [2, 1, 4],
     "lds_replicas": [0, 2, 1],
     "replicas": [8, 16, 32],
@@ -162,15 +163,19 @@ def tune_sim(p, capacity, electrons, ions, steps=80, warmup=80, passes=2, use_ca
     return entry
 
 
+def _grid(cfg, cus):
+    return max(1, cfg["lanes_per_cu"] // cfg["block_size"]) * cus
+
+
 def sim_kwargs(cfg, cus):
-    return {"grid_blocks": cfg["grid_per_cu"] * cus, "ppt_inplace": cfg["ppt_inplace"],
+    return {"block_size": cfg["block_size"], "grid_blocks": _grid(cfg, cus), "ppt_inplace": cfg["ppt_inplace"],
             "lds_replicas": cfg["lds_replicas"], "replicas": cfg["replicas"], "segments": cfg["segments"],
             "ppt": cfg["ppt"]}
 
 
 def ensemble_kwargs(cfg, cus):
-    return {"grid_blocks": cfg["grid_per_cu"] * cus, "ppt": cfg["ppt"], "lds_replicas": cfg["lds_replicas"],
-            "replicas": cfg["replicas"], "group_slots": cfg["group_slots"]}
+    return {"block_size": cfg["block_size"], "grid_blocks": _grid(cfg, cus), "ppt": cfg["ppt"],
+            "lds_replicas": cfg["lds_replicas"], "replicas": cfg["replicas"], "group_slots": cfg["group_slots"]}
 
 
 def tune_ensemble(params_list, capacities, states, steps=80, warmup=80, passes=2, use_cache=True, log=None):

@@ -550,8 +550,13 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     s->f64 = s->prec == PIC_PREC_FP64;
     s->use_graph = c->use_graph != 0;
     Shape sh;
-    sh.block = 256;
-    sh.grid = c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
+    sh.block = c->block_size > 0 ? c->block_size : 256;
+    if (!particle_block_ok(sh.block)) {
+        delete s;
+        set_error("pic_ens_create: block_size must be 0 (256), 128, 256 or 512");
+        return PIC_ERR_ARG;
+    }
+    sh.grid = c->grid_blocks > 0 ? c->grid_blocks : default_particle_grid(device_cus(), sh.block);
     sh.ppt_inplace = c->ppt > 0 ? c->ppt : 2;
     sh.lds_reps = std::max(0, c->lds_replicas);
     {

@@ -223,7 +223,7 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
 // same-address atomics serialise in the memory-side atomic units, and one
 // counter for 4 M particles was the bottleneck of the whole step.
 template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
-__global__ __launch_bounds__(256) void k_fused(FusedArgs A) {
+__global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
     const FusedJob &j = A.j;
     const KPhys &k = A.k;
     extern __shared__ __align__(16) float sh[];
@@ -396,7 +396,7 @@ constexpr int kCollFieldsE = CF_BIN + 1;
 constexpr int kCollFieldsI = CF_AZ + 1;
 
 template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
-__global__ __launch_bounds__(256) void k_inplace(FusedArgs A) {
+__global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     const FusedJob &j = A.j;
     const KPhys &k = A.k;
     extern __shared__ __align__(16) float sh[];
@@ -727,13 +727,17 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
         return PIC_ERR_ARG;
     }
     Shape sh = resolve(sh0);
+    if (!particle_block_ok(sh.block)) {
+        set_error("fused: workgroup size must be 128, 256 or 512");
+        return PIC_ERR_ARG;
+    }
     FusedArgs a;
     a.j = j;
     a.k = k;
-    const int block = 256;
+    const int block = sh.block;
     const bool measure = j.electrons && j.eepf != nullptr;
     const int aux_words = j.electrons ? (measure ? kEepfBins : 0) : 2 * kIfedBins;
-    const int grid = std::max(1, sh.grid * sh.block / block);
+    const int grid = std::max(1, sh.grid);
     // LDS per workgroup: what the resident workgroups of a compute unit can
     // share (2 KB kept for the static arrays), at most what one may allocate.
     // It holds the field, the EEPF / IFED bins, the collision list of the
@@ -786,7 +790,7 @@ struct EnsArgs {
 };
 
 template <bool ELECTRON, int PPT, typename Acc>
-__global__ __launch_bounds__(256) void k_inplace_ens(EnsArgs A) {
+__global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
     const EnsStepJob &j = A.j;
     extern __shared__ __align__(16) float sh[];
     __shared__ uint32_t s_tiles[kMaxEnsemble + 1], s_cnt[kMaxEnsemble], s_off[kMaxEnsemble];
@@ -1190,11 +1194,15 @@ int launch_inplace_ens(const EnsStepJob &j, const Shape &sh0, hipStream_t s) {
         return PIC_ERR_ARG;
     }
     Shape sh = resolve(sh0);
+    if (!particle_block_ok(sh.block)) {
+        set_error("ensemble: workgroup size must be 128, 256 or 512");
+        return PIC_ERR_ARG;
+    }
     EnsArgs a;
     a.j = j;
     const int n_g = j.k.n_g;
-    const int block = 256;
-    const int grid = std::max(1, sh.grid * sh.block / block);
+    const int block = sh.block;
+    const int grid = std::max(1, sh.grid);
     // LDS as in the single-discharge kernel: the field, the collision list
     // and up to one histogram replica per wave in what the resident
     // workgroups of a compute unit can share

@@ -32,6 +32,14 @@ int device_lds_per_block();  // LDS bytes one workgroup may allocate
 
 KPhys make_kphys(const pic_params_t &p);
 
+// Workgroup sizes the persistent particle kernels are built for (their
+// launch bounds allow up to kMaxParticleBlock lanes; the default is 256).
+constexpr int kMaxParticleBlock = 512;
+inline bool particle_block_ok(int b) { return b == 128 || b == 256 || b == 512; }
+// Persistent workgroups by default: 1024 lanes per compute unit (4 waves
+// per SIMD, what the fused kernels' ~112 VGPRs allow resident).
+inline int default_particle_grid(int cus, int block) { return cus * (1024 / block); }
+
 // Launch shape of the persistent particle kernels.
 struct Shape {
     int block = 256;   // threads per workgroup (multiple of 64, <= 1024)

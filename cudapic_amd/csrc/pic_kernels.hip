@@ -1944,6 +1944,25 @@ __global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ s
         dst[i] = src[i];
 }
 
+// Read-only stream (16-byte loads, four in flight per lane), summed so the
+// loads cannot be dropped: the read half of the bandwidth on its own.
+__global__ __launch_bounds__(256) void k_stream_read(const uint4 *__restrict__ src, uint64_t n16,
+                                                     unsigned long long *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t acc = 0;
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        acc += (a.x ^ a.y ^ a.z ^ a.w) + (b.x ^ b.y ^ b.z ^ b.w) + (c.x ^ c.y ^ c.z ^ c.w) + (d.x ^ d.y ^ d.z ^ d.w);
+    }
+    for (; i < n16; i += stride) {
+        const uint4 a = src[i];
+        acc += a.x ^ a.y ^ a.z ^ a.w;
+    }
+    acc = wave_sum(acc);
+    if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
 int pic_stream_copy(void *dst, const void *src, uint64_t bytes, pic_stream_t stream) {
     if (!dst || !src || (bytes % 16) != 0 || ((uintptr_t)dst % 16) != 0 || ((uintptr_t)src % 16) != 0) {
         set_error("pic_stream_copy: pointers and size must be 16-byte aligned");
@@ -1954,6 +1973,24 @@ int pic_stream_copy(void *dst, const void *src, uint64_t bytes, pic_stream_t str
     const int grid = (int)std::min<uint64_t>((n16 + 255) / 256, (uint64_t)device_cus() * 16);
     k_stream_copy<<<grid, 256, 0, S(stream)>>>((const uint4 *)src, (uint4 *)dst, n16);
     PIC_LAUNCHED("k_stream_copy");
+    return PIC_OK;
+}
+
+int pic_stream_read(const void *src, uint64_t bytes, unsigned long long *sum, pic_stream_t stream) {
+    if (!src || !sum || (bytes % 16) != 0 || ((uintptr_t)src % 16) != 0) {
+        set_error("pic_stream_read: pointer and size must be 16-byte aligned");
+        return PIC_ERR_ARG;
+    }
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return PIC_OK;
+    const int grid = (int)std::min<uint64_t>((n16 + 255) / 256, (uint64_t)device_cus() * 16);
+    k_stream_read<<<grid, 256, 0, S(stream)>>>((const uint4 *)src, n16, sum);
+    PIC_LAUNCHED("k_stream_read");
+    return PIC_OK;
+}
+
+int pic_memcpy_dtod_async(void *dst, const void *src, uint64_t bytes, pic_stream_t stream) {
+    if (bytes) PIC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
     return PIC_OK;
 }
 

@@ -681,9 +681,16 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     s->slots = s->seg * (uint32_t)s->nseg;
     Shape sh;
     sh.block = c->block_size > 0 ? c->block_size : 256;
+    if (!particle_block_ok(sh.block) || (c->path != PIC_PATH_FUSED && sh.block != 256)) {
+        delete s;
+        set_error("pic_sim_create: block_size must be 0 (256), 128, 256 or 512, and 0 or 256 on the reference "
+                  "path (its stage kernels have fixed shapes)");
+        return PIC_ERR_ARG;
+    }
     // the fused kernels hold ~110 VGPRs (4 waves per SIMD): a persistent grid
-    // of 4 workgroups of 256 lanes per compute unit is fully resident
-    sh.grid = c->grid_blocks > 0 ? c->grid_blocks : (c->path == PIC_PATH_FUSED ? device_cus() * 4 : 0);
+    // of 1024 lanes per compute unit is fully resident
+    sh.grid = c->grid_blocks > 0 ? c->grid_blocks
+                                 : (c->path == PIC_PATH_FUSED ? default_particle_grid(device_cus(), sh.block) : 0);
     // particles per lane of a compacting tile (on the reference path a tile
     // of 1024 lanes reserves its output with one atomic on the single live
     // counter, and 4 keeps several tiles per workgroup to overlap them)

@@ -122,7 +122,7 @@ int make_ctx(const pic_params_t &p, const pic_step_config_t *c, uint8_t *ws, boo
     x->compact_every = compact_every(p, c);
     Shape sh;
     sh.block = 256;
-    sh.grid = c && c->grid_blocks > 0 ? c->grid_blocks : device_cus() * 4;
+    sh.grid = c && c->grid_blocks > 0 ? c->grid_blocks : default_particle_grid(device_cus(), sh.block);
     sh.ppt = c && c->ppt > 0 ? c->ppt : 4;
     sh.ppt_inplace = c && c->ppt_inplace > 0 ? c->ppt_inplace : 2;
     sh.lds_reps = c ? std::max(0, c->lds_replicas) : 0;

@@ -56,6 +56,9 @@ def main():
                          "4294967295 = one group)")
     ap.add_argument("--group-periods", type=int, default=0, help="4*N_SUB-step periods per block (0 = 5)")
     ap.add_argument("--group-streams", type=int, default=0, help="streams the groups are dealt to (0 = 1)")
+    ap.add_argument("--one-group", action="store_true",
+                    help="also time the ensemble as one group of all instances (one launch per stage "
+                         "and step over every instance, no temporal blocking)")
     args = ap.parse_args()
     import cudapic_amd as ca
     from cudapic_amd import synthetic
@@ -68,21 +71,28 @@ def main():
     print(f"ens_bench: {args.instances} instances, {int(cnt.sum())} electrons in total "
           f"({int(cnt.min())}..{int(cnt.max())})", file=sys.stderr, flush=True)
 
-    ens = ca.Ensemble(ps, caps, grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas,
-                      group_slots=args.group_slots, group_periods=args.group_periods,
-                      group_streams=args.group_streams)
-    for i, (E, I) in enumerate(states):
-        ens.upload(i, 0, *E)
-        ens.upload(i, 1, *I)
-    ens.run(args.warmup)
-    ens.synchronize()
-    ens.check()
-    t = ens.time_steps(args.steps)
-    ens.check()
-    ens_rate = t["particle_steps"] / (t["ms_total"] * 1e-3)
-    ens_ms = t["ms_total"]
-    ens.close()
+    def ensemble(group_slots):
+        ens = ca.Ensemble(ps, caps, grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas,
+                          group_slots=group_slots, group_periods=args.group_periods,
+                          group_streams=args.group_streams)
+        for i, (E, I) in enumerate(states):
+            ens.upload(i, 0, *E)
+            ens.upload(i, 1, *I)
+        ens.run(args.warmup)
+        ens.synchronize()
+        ens.check()
+        t = ens.time_steps(args.steps)
+        ens.check()
+        ens.close()
+        return t["particle_steps"] / (t["ms_total"] * 1e-3), t["ms_total"]
+
+    ens_rate, ens_ms = ensemble(args.group_slots)
     print(f"ens_bench: ensemble {args.steps} steps in {ens_ms:.1f} ms", file=sys.stderr, flush=True)
+    one = None
+    if args.one_group:
+        r1, ms1 = ensemble(0xFFFFFFFF)
+        one = {"value": round(r1, 1), "ms_total": round(ms1, 2)}
+        print(f"ens_bench: ensemble as one group {args.steps} steps in {ms1:.1f} ms", file=sys.stderr, flush=True)
 
     def sequential(path):
         tot_ps, tot_ms = 0, 0.0
@@ -115,6 +125,8 @@ def main():
         "ensemble": {"value": round(ens_rate, 1), "ms_total": round(ens_ms, 2)},
         "sequential": seq,
         "speedup": round(ens_rate / seq["value"], 3) if seq else None,
+        "one_group": one,
+        "one_group_speedup": round(one["value"] / seq["value"], 3) if (one and seq) else None,
         "sequential_reference_path": seq_ref,
         "speedup_vs_reference_path": round(ens_rate / seq_ref["value"], 3) if seq_ref else None,
         "config": {"v0": "U[100,400] V", "pressure": "logU[3,30] Pa",

@@ -210,7 +210,10 @@ typedef struct pic_sim_config {
     uint32_t seed_i[4];
     int32_t path;              /* enum pic_path */
     int32_t use_graph;         /* fused path: capture 4*N_SUB steps into a hipGraph */
-    int32_t block_size;        /* particle kernels: 0 = tuned default */
+    int32_t block_size;        /* lanes per workgroup of the fused particle
+                                  kernels: 128, 256 or 512 (0 = 256); other
+                                  values, and any but 0 / 256 on the reference
+                                  path, are rejected */
     int32_t grid_blocks;       /* persistent grid: 0 = tuned default */
     int32_t ppt;               /* particles per thread per tile, compacting
                                   steps: 1, 2, 4 or 8; 0 = 4 */
@@ -411,8 +414,8 @@ typedef struct pic_ensemble_config {
     const pic_params_t *params;   /* [n_instances], derived */
     const uint32_t *capacity;     /* [n_instances] slots per species */
     uint32_t seed_e[4], seed_i[4];
-    /* use_graph: replay 4*N_SUB steps from a hipGraph; grid_blocks (0 = 4
-     * per compute unit), ppt (particles per thread per tile, 1, 2 or 4; 0 =
+    /* use_graph: replay 4*N_SUB steps from a hipGraph; grid_blocks (0 = 1024
+     * lanes per compute unit), ppt (particles per thread per tile, 1, 2 or 4; 0 =
      * 2) and replicas (global density replicas, 0 = 8) shape the particle
      * kernels; lds_replicas and compact_every as in pic_sim_config_t */
     int32_t use_graph, grid_blocks, ppt, replicas, lds_replicas;
@@ -429,6 +432,8 @@ typedef struct pic_ensemble_config {
      * instance still goes through the same steps; the order in which
      * instances' steps execute is all that changes. */
     uint32_t group_slots, group_periods, group_streams;
+    /* lanes per workgroup of the particle kernel: 128, 256 or 512 (0 = 256) */
+    int32_t block_size;
 } pic_ensemble_config_t;
 
 typedef struct pic_ensemble pic_ensemble_t;
@@ -534,6 +539,11 @@ int pic_stream_synchronize(pic_stream_t stream);
  * reference point for HBM traffic counters and achievable bandwidth (no
  * reference counterpart). */
 int pic_stream_copy(void *dst, const void *src, uint64_t bytes, pic_stream_t stream);
+/* Read-only counterpart (16-byte loads; *sum += a digest of the data, so
+ * the loads happen), and the runtime's own device-to-device copy: the two
+ * cross-checks of the copy kernel's bandwidth (no reference counterpart). */
+int pic_stream_read(const void *src, uint64_t bytes, unsigned long long *sum, pic_stream_t stream);
+int pic_memcpy_dtod_async(void *dst, const void *src, uint64_t bytes, pic_stream_t stream);
 /* Text of the last error on this thread. */
 const char *pic_last_error(void);
 /* Device properties used for launch shapes: compute units, wave size. */

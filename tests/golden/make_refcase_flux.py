@@ -15,9 +15,10 @@ batch.  eduPIC seeds its Mersenne twister from std::random_device
 
     python tests/golden/make_refcase_flux.py [--chains 6] [--workdir DIR]
 
-Writes tests/golden/refcase_flux.json: per batch and quantity the values,
-and their mean and standard error over all batches.  Takes ~30 min on 6
-cores of a 2.7 GHz Xeon.
+Writes tests/golden/refcase_flux.json: per quantity the batch values (run
+by run: the first BATCHES belong to the first chain), and their mean and
+standard error over all batches.  Takes ~45 min on 6 cores of a 2.7 GHz
+Xeon.
 """
 from __future__ import annotations
 

@@ -109,3 +109,23 @@ def test_step_entry_points_reject_bad_arguments_before_the_gpu():
     assert lib.pic_step(None, C.byref(st), 0, 0, None) == _abi.PIC_ERR_ARG
     assert lib.pic_step_graph(None, C.byref(st), 0, 80, 0, None) == _abi.PIC_ERR_ARG
     assert b"null" in lib.pic_last_error()
+
+
+def test_unsupported_workgroup_sizes_are_rejected_before_the_gpu():
+    """block_size is honoured by the fused particle kernels (128, 256, 512)
+    and rejected otherwise -- never silently ignored."""
+    import cudapic_amd as ca
+    from cudapic_amd import _abi
+    lib = _abi.lib()
+    h = C.c_void_p()
+    for path, block in ((1, 300), (1, 1024), (0, 512)):
+        cfg = _abi.PicSimConfig()
+        lib.pic_sim_config_default(C.byref(cfg))
+        cfg.path, cfg.block_size = path, block
+        assert lib.pic_sim_create(C.byref(cfg), C.byref(h)) == _abi.PIC_ERR_ARG, (path, block)
+        assert b"block_size" in lib.pic_last_error()
+    p = (_abi.PicParams * 1)(ca.params())
+    caps = (C.c_uint32 * 1)(1000)
+    ec = _abi.PicEnsembleConfig(n_instances=1, params=p, capacity=caps, block_size=192)
+    assert lib.pic_ens_create(C.byref(ec), C.byref(h)) == _abi.PIC_ERR_ARG
+    assert b"block_size" in lib.pic_last_error()

@@ -48,15 +48,15 @@ def test_tune_sim_small_discharge(tmp_path, monkeypatch):
     import cudapic_amd as ca
     from helpers import synthetic_state
     monkeypatch.setenv("CUDAPIC_AUTOTUNE_CACHE", str(tmp_path / "cache.json"))
-    monkeypatch.setattr(autotune, "SIM_SPACE", {"grid_per_cu": [4, 2], "ppt_inplace": [2, 1],
-                                                "lds_replicas": [0], "replicas": [8],
+    monkeypatch.setattr(autotune, "SIM_SPACE", {"block_size": [256, 512], "lanes_per_cu": [1024, 512],
+                                                "ppt_inplace": [2, 1], "lds_replicas": [0], "replicas": [8],
                                                 "segments": [0], "ppt": [4]})
     p = ca.params(weight=2.0e3, precision=ca.PIC_PREC_FP64)
     E, I = synthetic_state(200_000, 210_000, seed=5)
     res = autotune.tune_sim(p, 400_000, E, I, steps=80, warmup=80, passes=1)
-    assert res["rate"] > 1e9 and res["evaluations"] == 3
+    assert res["rate"] > 1e9 and res["evaluations"] == 4
     cfg = res["config"]
-    assert cfg["ppt_inplace"] in (1, 2) and cfg["grid_blocks"] > 0
+    assert cfg["ppt_inplace"] in (1, 2) and cfg["grid_blocks"] > 0 and cfg["block_size"] in (256, 512)
     # cached: the second call measures nothing
     again = autotune.tune_sim(p, 400_000, E, I)
     assert again == json.loads(json.dumps(res))

@@ -4,7 +4,8 @@ differences (h = 1e-3 V) of the FP64 forward of that model on the same tape
 (frozen events: common random numbers by construction), and the determinism
 the checkpointed recomputation relies on.
 
-The window is 400 steps (20 ion steps, two checkpoint segments).  Measured
+The window is 400 steps (20 ion steps, two checkpoint segments), at 20 k +
+21 k and at 200 k + 210 k particles.  Measured
 on the MI355X: the GPU gradient agrees with the FP64 CPU adjoint to 5.4e-4,
 and the CPU adjoint with central differences at h = 1e-3 V to 2e-5; at
 h = 1e-2 V the differences themselves are 0.3-1.3 % off the model's exact
@@ -35,16 +36,14 @@ def _waveform(seed=3):
     return wc, ws
 
 
-@pytest.fixture(scope="module")
-def run():
+def _make_run(n, seed):
     import cudapic_amd as ca
     from cudapic_amd import synthetic
-    n = 20_000
     p = ca.params(weight=7.0e4 * 110_000 / n)
     wc, ws = _waveform()
     for h in range(16):
         p.wave_cos[h], p.wave_sin[h] = wc[h], ws[h]
-    E, I = synthetic.state(n, int(n * 1.05), seed=21)
+    E, I = synthetic.state(n, int(n * 1.05), seed=seed)
     adj = ca.Adjoint(p, 2 * n, 2 * n, STEPS, checkpoint=CHECKPOINT)
     adj.upload(0, *E)
     adj.upload(1, *I)
@@ -57,9 +56,23 @@ def run():
     adj.set_target(target)
     j0, nbar0 = adj.forward(wc, ws, nbar=True)
     grads = adj.backward()
-    yield {"ca": ca, "p": p, "adj": adj, "E": E, "I": I, "wc": wc, "ws": ws, "target": target,
-           "j_rec": j_rec, "j0": j0, "nbar0": nbar0, "grads": grads}
-    adj.close()
+    return {"ca": ca, "p": p, "adj": adj, "E": E, "I": I, "wc": wc, "ws": ws, "target": target,
+            "j_rec": j_rec, "j0": j0, "nbar0": nbar0, "grads": grads}
+
+
+@pytest.fixture(scope="module")
+def run():
+    r = _make_run(20_000, 21)
+    yield r
+    r["adj"].close()
+
+
+@pytest.fixture(scope="module")
+def run_large():
+    """Ten times the particles: 200 k electrons + 210 k ions."""
+    r = _make_run(200_000, 22)
+    yield r
+    r["adj"].close()
 
 
 def test_replay_is_bit_exact(run):
@@ -111,7 +124,6 @@ def test_gradient_matches_central_differences(run):
     wc, ws, target = run["wc"], run["ws"], run["target"]
     ggc, ggs, _ = run["grads"]
     h = 1e-3
-    scale = max(np.abs(ggs).max(), np.abs(ggc).max())
     for kind, k in (("sin", 0), ("sin", 1), ("cos", 0), ("sin", 6)):
         wp, wm = (ws.copy(), ws.copy()) if kind == "sin" else (wc.copy(), wc.copy())
         wp[k] += h
@@ -125,4 +137,40 @@ def test_gradient_matches_central_differences(run):
                   - oa.forward(m, state, tape, wm, ws, target, STEPS)[0]) / (2 * h)
             g = ggc[k]
         print(kind, k, "fd", fd, "adjoint", g)
-        assert abs(fd - g) <= 1e-2 * abs(g) + 1e-3 * scale, (kind, k, fd, g)
+        assert abs(fd - g) <= 1e-2 * abs(g), (kind, k, fd, g)
+
+
+def _fd(m, state, tape, wc, ws, target, kind, k, h):
+    wp, wm = (ws.copy(), ws.copy()) if kind == "sin" else (wc.copy(), wc.copy())
+    wp[k] += h
+    wm[k] -= h
+    if kind == "sin":
+        return (oa.forward(m, state, tape, wc, wp, target, STEPS)[0]
+                - oa.forward(m, state, tape, wc, wm, target, STEPS)[0]) / (2 * h)
+    return (oa.forward(m, state, tape, wp, ws, target, STEPS)[0]
+            - oa.forward(m, state, tape, wm, ws, target, STEPS)[0]) / (2 * h)
+
+
+def test_large_window_gradient_matches_cpu_adjoint_and_differences(run_large):
+    """200 k + 210 k particles, the same 400-step window: the GPU gradient
+    against the FP64 CPU adjoint on the GPU's tape, and against central
+    differences at h = 1e-3 V within 1 % (purely relative).  The
+    configuration's h = 1e-2 V is evaluated too and printed: the frozen-event
+    model is piecewise smooth, so a 1e-2 V step is not always within 1 %."""
+    run = run_large
+    m, state, tape = _cpu_case(run)
+    wc, ws, target = run["wc"], run["ws"], run["target"]
+    J, gc, gs, dv, _ = oa.gradient(m, state, tape, wc, ws, target, STEPS)
+    ggc, ggs, gdv = run["grads"]
+    print("J gpu", run["j0"], "cpu", J)
+    assert J > 0 and abs(run["j0"] / J - 1.0) < 1e-3
+    scale = max(np.abs(gs).max(), np.abs(gc).max())
+    assert np.abs(ggs - gs).max() < 5e-3 * scale and np.abs(ggc - gc).max() < 5e-3 * scale
+    assert np.linalg.norm(gdv - dv) < 1e-2 * np.linalg.norm(dv)
+    for kind, k in (("sin", 0), ("cos", 0), ("sin", 1)):
+        g = ggs[k] if kind == "sin" else ggc[k]
+        fd3 = _fd(m, state, tape, wc, ws, target, kind, k, 1e-3)
+        fd2 = _fd(m, state, tape, wc, ws, target, kind, k, 1e-2)
+        print(f"{kind}_{k}: adjoint {g:.6e} fd(1e-3) {fd3:.6e} ({fd3 / g - 1:+.2e}) "
+              f"fd(1e-2) {fd2:.6e} ({fd2 / g - 1:+.2e})")
+        assert abs(fd3 - g) <= 1e-2 * abs(g), (kind, k, fd3, g)

@@ -110,3 +110,31 @@ def test_cli_imports_and_exports_edupic_picdata(tmp_path):
     assert abs(len(d["ions"][0]) / len(I[0]) - 1.0) < 0.05
     x = d["ions"][0]
     assert (x >= 0).all() and (x <= 0.025).all()
+
+
+def test_cli_reference_case_ten_cycles():
+    """The plain eduPIC reference case from the CLI: 1000 electrons + 1000
+    ions at rest, 10 RF cycles, 8 seeds.  The final counts are compared
+    with the CPU oracle's (the C restatement of the reference step, 16
+    seeds, tests/golden/oracle_conv10.json) within 3.5 combined standard
+    errors of the means, and with eduPIC's published run (Figure9_conv.dat,
+    cycle 10: 6091 electrons, 9444 ions) within 3.5 seed spreads of a single
+    run."""
+    import json
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "oracle_conv10.json")))
+    counts = []
+    for seed in range(8):
+        r = subprocess.run([CLI, "10", "--seed", str(seed)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"final particles: (\d+) electrons, (\d+) ions", r.stdout)
+        assert m, r.stdout
+        counts.append((int(m.group(1)), int(m.group(2))))
+    c = np.array(counts, np.float64)
+    print("10 cycles, 8 seeds:", counts)
+    for k, (name, pub) in enumerate((("e", 6091.0), ("i", 9444.0))):
+        mean, sd = c[:, k].mean(), c[:, k].std(ddof=1)
+        o_mean, o_sd = ref[f"mean_{name}"][-1], ref[f"sd_{name}"][-1]
+        se = np.hypot(sd / np.sqrt(len(c)), o_sd / np.sqrt(ref["seeds"]))
+        print(f"{name}: engine {mean:.0f} +- {sd:.0f}, oracle {o_mean:.0f} +- {o_sd:.0f}, published {pub:.0f}")
+        assert abs(mean - o_mean) <= 3.5 * se, (name, mean, o_mean, se)
+        assert abs(mean - pub) <= 3.5 * np.hypot(o_sd, o_sd / np.sqrt(len(c))), (name, mean, pub)

@@ -111,11 +111,24 @@ def test_collisionless_step_stage_by_stage(ca, orc, path, t0, prec, segments):
 
 @pytest.mark.parametrize("path,t0,segments", CASES)
 def test_step_with_collisions_slot_replay(ca, orc, path, t0, segments):
+    _collision_step_replay(ca, orc, path, t0, segments=segments)
+
+
+@pytest.mark.parametrize("block", [128, 512])
+@pytest.mark.parametrize("t0", [0, 1, 20])
+def test_fused_workgroup_sizes_slot_replay(ca, orc, block, t0):
+    """The fused kernels at 128 and 512 lanes per workgroup (the autotuner's
+    other sizes): the same per-slot replay as at 256, at a compacting step,
+    an electron-only in-place step and an in-place ion step."""
+    _collision_step_replay(ca, orc, 1, t0, segments=32, block_size=block)
+
+
+def _collision_step_replay(ca, orc, path, t0, **sim_kw):
     p = ca.params(weight=2.0e3, precision=2)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 200_000, 210_000, 300_000
     E, I = synthetic_state(n_e, n_i, seed=41 + t0, te_ev=6.0)
-    sim = make_sim(ca, p, cap, path, segments=segments)
+    sim = make_sim(ca, p, cap, path, **sim_kw)
     sim.set_time(t0)
     cs, se, si = orc.cross_sections(po.gas_density)
     sim.upload(0, *E)
