@@ -120,6 +120,9 @@ int launch_deposit(const float *x, const uint32_t *n_active, void *dens, bool f6
 int launch_fill(void *ptr, uint32_t value, uint64_t words, hipStream_t s);
 // *dst = src ? min(*src, clamp) : 0
 int launch_copy_u32(uint32_t *dst, const uint32_t *src, hipStream_t s, uint32_t clamp = 0xffffffffu);
+// *d0 = min(*s0, clamp) and *d1 = min(*s1, clamp) in one launch
+int launch_copy2_u32(uint32_t *d0, const uint32_t *s0, uint32_t *d1, const uint32_t *s1, hipStream_t s,
+                     uint32_t clamp = 0xffffffffu);
 
 struct MoveJob {
     const float *efield;

@@ -134,6 +134,22 @@ int launch_copy_u32(uint32_t *dst, const uint32_t *src, hipStream_t s, uint32_t 
     return PIC_OK;
 }
 
+// Two counter copies in one launch (both sources read before either store).
+__global__ void k_copy2_u32(uint32_t *d0, const uint32_t *s0, uint32_t *d1, const uint32_t *s1, uint32_t clamp) {
+    if (threadIdx.x == 0) {
+        const uint32_t a = min(*s0, clamp), b = min(*s1, clamp);
+        *d0 = a;
+        *d1 = b;
+    }
+}
+
+int launch_copy2_u32(uint32_t *d0, const uint32_t *s0, uint32_t *d1, const uint32_t *s1, hipStream_t s,
+                     uint32_t clamp) {
+    k_copy2_u32<<<1, 64, 0, s>>>(d0, s0, d1, s1, clamp);
+    PIC_LAUNCHED("k_copy2_u32");
+    return PIC_OK;
+}
+
 // ============================================================ deposition
 // get_density (main.rs:356-406).  Each wave adds into an LDS histogram
 // replica (fixed point, integer LDS atomics); the workgroup sums its
@@ -1260,6 +1276,24 @@ int launch_move_compact(const MoveJob &j, const KPhys &k, const Shape &sh0, hipS
 }
 
 // ============================================================ collisions
+// Both stage collision kernels (check_collisions_e / _i with collision_e /
+// _i, main.rs:971-1292) run as persistent workgroups over tiles of
+// 256 * kCollidePPT slots, the next tile's loads in flight while the
+// current one is tested.  The collision test only lists a colliding particle
+// (its slot, velocity, RNG state after the test draw and energy bin; ions
+// also the partner atom's velocity and the relative speed) in LDS; after the
+// last tile the workgroup scatters its list 64 entries per wave pass, one
+// entry per lane.  With ~0.5 % of electrons colliding per step, a wave that
+// ran the long scattering code for its one or two colliding lanes in every
+// tile spent most of the kernel there.  Each particle's stream continues
+// from the listed state, so its draws are those of the sequential order.  A
+// full list (far above the tens of entries a workgroup sees) scatters the
+// excess in place.  Ionisations append the new electron and ion with one
+// atomic per species per wave (lanes ranked by v_mbcnt).
+constexpr int kCollideBlock = 256;
+constexpr int kCollidePPT = 2;
+constexpr uint32_t kStageList = 256;
+
 struct CollEArgs {
     const float *stot_e, *cs;
     const uint32_t *n_active;
@@ -1273,70 +1307,162 @@ struct CollEArgs {
     KPhys k;
 };
 
-// check_collisions_e (main.rs:971-1165).  The loop is wave-uniform so that
-// the ionisation appends can be aggregated: one atomic per species per wave,
-// lanes ranked by v_mbcnt.
-__global__ __launch_bounds__(256) void k_collide_e(CollEArgs a) {
+// The ionisation products of the lanes with kind == kIonisation: the new
+// electron at (*alive_e)++ (position of the parent, velocity se), the new
+// ion at (*alive_i)++ (velocity si); appends past the capacities are
+// dropped and flagged.
+__device__ __forceinline__ void stage_ionisation(const CollEArgs &a, int kind, uint32_t parent,
+                                                 const float *se, const float *si) {
+    const uint64_t sp = ballot64(kind == kIonisation);
+    if (!sp) return;
+    const int leader = __builtin_ctzll(sp);
+    uint32_t be = 0, bi = 0;
+    if ((int)lane_id() == leader) {
+        const uint32_t cnt = (uint32_t)__popcll(sp);
+        be = atomicAdd(a.alive_e, cnt);
+        bi = atomicAdd(a.alive_i, cnt);
+    }
+    be = __shfl(be, leader, kWave);
+    bi = __shfl(bi, leader, kWave);
+    if (kind != kIonisation) return;
+    const uint32_t o = mbcnt64(sp);
+    const float xi = a.x[parent];
+    const uint32_t je = be + o, ji = bi + o;
+    if (je < a.cap_e) {
+        a.x[je] = xi;
+        a.vx[je] = se[0];
+        a.vy[je] = se[1];
+        a.vz[je] = se[2];
+    } else {
+        *a.overflow = 1u;
+    }
+    if (ji < a.cap_i) {
+        a.ix[ji] = xi;
+        a.ivx[ji] = si[0];
+        a.ivy[ji] = si[1];
+        a.ivz[ji] = si[2];
+    } else {
+        *a.overflow = 1u;
+    }
+}
+
+__global__ __launch_bounds__(kCollideBlock) void k_collide_e(CollEArgs a) {
+    // list fields: slot, vx, vy, vz, 4 RNG words, bin
+    __shared__ uint32_t list[9 * kStageList];
+    __shared__ uint32_t s_n;
+    const KPhys &k = a.k;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
     const uint32_t n = min(*a.n_active, a.cap_e);
-    const uint32_t lane = lane_id();
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t tile_sz = blockDim.x * kCollidePPT;
+    const uint32_t n_tiles = (n + tile_sz - 1) / tile_sz;
     uint32_t ncoll = 0;
-    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(kWave - 1)); i0 < n;
-         i0 += stride) {
-        const uint32_t i = i0 + lane;
+    auto load = [&](uint32_t tile, float (&vx)[kCollidePPT], float (&vy)[kCollidePPT], float (&vz)[kCollidePPT],
+                    Xo (&st)[kCollidePPT]) {
+#pragma unroll
+        for (int u = 0; u < kCollidePPT; ++u) {
+            const uint32_t i0 = tile * tile_sz + threadIdx.x + u * blockDim.x;
+            const uint32_t i = tile < n_tiles && i0 < n ? i0 : 0u;
+            vx[u] = a.vx[i];
+            vy[u] = a.vy[i];
+            vz[u] = a.vz[i];
+            st[u] = rng_load(a.r0, a.r1, a.r2, a.r3, i);
+        }
+    };
+    float vx[kCollidePPT], vy[kCollidePPT], vz[kCollidePPT];
+    Xo st[kCollidePPT];
+    if (n_tiles > blockIdx.x) load(blockIdx.x, vx, vy, vz, st);
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        float pvx[kCollidePPT], pvy[kCollidePPT], pvz[kCollidePPT];
+        Xo pst[kCollidePPT];
+        load(tile + gridDim.x, pvx, pvy, pvz, pst);
+#pragma unroll
+        for (int u = 0; u < kCollidePPT; ++u) {
+            const uint32_t i = tile * tile_sz + threadIdx.x + u * blockDim.x;
+            const bool valid = i < n;
+            bool hit = false;
+            int idx = 0;
+            if (valid) {
+                float speed;
+                idx = e_index(k, vx[u], vy[u], vz[u], speed);
+                hit = e_test(k, a.stot_e[idx], speed, st[u]);
+                rng_store(a.r0, a.r1, a.r2, a.r3, i, st[u]);
+            }
+            const uint64_t m = ballot64(hit);
+            if (!m) continue;
+            const int leader = __builtin_ctzll(m);
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&s_n, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, kWave);
+            const uint32_t pos = base + mbcnt64(m);
+            const bool listed = hit && pos < kStageList;
+            if (listed) {
+                uint32_t *e = list + pos;
+                e[0 * kStageList] = i;
+                e[1 * kStageList] = __float_as_uint(vx[u]);
+                e[2 * kStageList] = __float_as_uint(vy[u]);
+                e[3 * kStageList] = __float_as_uint(vz[u]);
+                e[4 * kStageList] = st[u].s0;
+                e[5 * kStageList] = st[u].s1;
+                e[6 * kStageList] = st[u].s2;
+                e[7 * kStageList] = st[u].s3;
+                e[8 * kStageList] = (uint32_t)idx;
+            }
+            const bool local = hit && !listed;
+            if (ballot64(local)) {  // a full list: scatter here
+                int kind = kNone;
+                float se[3], si[3];
+                if (local) {
+                    float w = vx[u], y = vy[u], z = vz[u];
+                    Xo s2 = st[u];
+                    kind = e_scatter(k, a.cs[idx], a.cs[kCsRanges + idx], a.cs[2 * kCsRanges + idx], w, y, z, s2,
+                                     se, si);
+                    rng_store(a.r0, a.r1, a.r2, a.r3, i, s2);
+                    a.vx[i] = w;
+                    a.vy[i] = y;
+                    a.vz[i] = z;
+                    ++ncoll;
+                }
+                stage_ionisation(a, kind, i, se, si);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCollidePPT; ++u) {
+            vx[u] = pvx[u];
+            vy[u] = pvy[u];
+            vz[u] = pvz[u];
+            st[u] = pst[u];
+        }
+    }
+    __syncthreads();
+    const uint32_t n_list = min(s_n, kStageList);
+    const uint32_t wave = threadIdx.x / kWave;
+    for (uint32_t b0 = wave * kWave; b0 < n_list; b0 += blockDim.x) {
+        const uint32_t e = b0 + lane_id();
         int kind = kNone;
         float se[3], si[3];
-        if (i < n) {
-            float vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-            Xo s = rng_load(a.r0, a.r1, a.r2, a.r3, i);
-            kind = collide_electron(a.stot_e, a.cs, a.k, vx, vy, vz, s, se, si);
-            rng_store(a.r0, a.r1, a.r2, a.r3, i, s);
-            if (kind != kNone) {
-                a.vx[i] = vx;
-                a.vy[i] = vy;
-                a.vz[i] = vz;
-                ++ncoll;
-            }
+        uint32_t slot = 0;
+        if (e < n_list) {
+            const uint32_t *q = list + e;
+            slot = q[0];
+            float w = __uint_as_float(q[1 * kStageList]), y = __uint_as_float(q[2 * kStageList]),
+                  z = __uint_as_float(q[3 * kStageList]);
+            Xo s2{q[4 * kStageList], q[5 * kStageList], q[6 * kStageList], q[7 * kStageList]};
+            const int b = (int)q[8 * kStageList];
+            kind = e_scatter(k, a.cs[b], a.cs[kCsRanges + b], a.cs[2 * kCsRanges + b], w, y, z, s2, se, si);
+            rng_store(a.r0, a.r1, a.r2, a.r3, slot, s2);
+            a.vx[slot] = w;
+            a.vy[slot] = y;
+            a.vz[slot] = z;
+            ++ncoll;
         }
-        const uint64_t sp = ballot64(kind == kIonisation);
-        if (sp) {
-            const int leader = __builtin_ctzll(sp);
-            uint32_t be = 0, bi = 0;
-            if ((int)lane == leader) {
-                const uint32_t cnt = (uint32_t)__popcll(sp);
-                be = atomicAdd(a.alive_e, cnt);
-                bi = atomicAdd(a.alive_i, cnt);
-            }
-            be = __shfl(be, leader, kWave);
-            bi = __shfl(bi, leader, kWave);
-            if (kind == kIonisation) {
-                const uint32_t o = mbcnt64(sp);
-                const float xi = a.x[i];
-                const uint32_t je = be + o, ji = bi + o;
-                if (je < a.cap_e) {
-                    a.x[je] = xi;
-                    a.vx[je] = se[0];
-                    a.vy[je] = se[1];
-                    a.vz[je] = se[2];
-                } else {
-                    *a.overflow = 1u;
-                }
-                if (ji < a.cap_i) {
-                    a.ix[ji] = xi;
-                    a.ivx[ji] = si[0];
-                    a.ivy[ji] = si[1];
-                    a.ivz[ji] = si[2];
-                } else {
-                    *a.overflow = 1u;
-                }
-            }
-        }
+        stage_ionisation(a, kind, slot, se, si);
     }
     if (a.n_coll) wg_atomic_add(a.n_coll, ncoll);
 }
 
 int launch_collide_e(const CollideEJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
-    Shape sh = resolve(sh0);
+    (void)sh0;
     CollEArgs a;
     a.stot_e = j.stot_e;
     a.cs = j.cs;
@@ -1360,9 +1486,8 @@ int launch_collide_e(const CollideEJob &j, const KPhys &k, const Shape &sh0, hip
     a.overflow = j.overflow;
     a.n_coll = j.n_coll;
     a.k = k;
-    const int block = 256;
-    const int grid = std::max(1, sh.grid * sh.block / block);
-    k_collide_e<<<grid, block, 0, s>>>(a);
+    // a persistent grid of four 256-lane workgroups per compute unit
+    k_collide_e<<<device_cus() * 4, kCollideBlock, 0, s>>>(a);
     PIC_LAUNCHED("k_collide_e");
     return PIC_OK;
 }
@@ -1377,28 +1502,112 @@ struct CollIArgs {
     KPhys k;
 };
 
-// check_collisions_i (main.rs:1167-1292)
-__global__ __launch_bounds__(256) void k_collide_i(CollIArgs a) {
+__global__ __launch_bounds__(kCollideBlock) void k_collide_i(CollIArgs a) {
+    // list fields: slot, vx, vy, vz, 4 RNG words, bin, atom velocity (3), relative speed
+    __shared__ uint32_t list[13 * kStageList];
+    __shared__ uint32_t s_n;
+    const KPhys &k = a.k;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
     const uint32_t n = min(*a.n_active, a.n_cap);
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t tile_sz = blockDim.x * kCollidePPT;
+    const uint32_t n_tiles = (n + tile_sz - 1) / tile_sz;
     uint32_t ncoll = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-        Xo s = rng_load(a.r0, a.r1, a.r2, a.r3, i);
-        const int kind = collide_ion(a.stot_i, a.cs, a.k, vx, vy, vz, s);
-        rng_store(a.r0, a.r1, a.r2, a.r3, i, s);
-        if (kind != kNone) {
-            a.vx[i] = vx;
-            a.vy[i] = vy;
-            a.vz[i] = vz;
-            ++ncoll;
+    auto load = [&](uint32_t tile, float (&vx)[kCollidePPT], float (&vy)[kCollidePPT], float (&vz)[kCollidePPT],
+                    Xo (&st)[kCollidePPT]) {
+#pragma unroll
+        for (int u = 0; u < kCollidePPT; ++u) {
+            const uint32_t i0 = tile * tile_sz + threadIdx.x + u * blockDim.x;
+            const uint32_t i = tile < n_tiles && i0 < n ? i0 : 0u;
+            vx[u] = a.vx[i];
+            vy[u] = a.vy[i];
+            vz[u] = a.vz[i];
+            st[u] = rng_load(a.r0, a.r1, a.r2, a.r3, i);
         }
+    };
+    float vx[kCollidePPT], vy[kCollidePPT], vz[kCollidePPT];
+    Xo st[kCollidePPT];
+    if (n_tiles > blockIdx.x) load(blockIdx.x, vx, vy, vz, st);
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        float pvx[kCollidePPT], pvy[kCollidePPT], pvz[kCollidePPT];
+        Xo pst[kCollidePPT];
+        load(tile + gridDim.x, pvx, pvy, pvz, pst);
+#pragma unroll
+        for (int u = 0; u < kCollidePPT; ++u) {
+            const uint32_t i = tile * tile_sz + threadIdx.x + u * blockDim.x;
+            const bool valid = i < n;
+            bool hit = false;
+            int idx = 0;
+            float ax = 0.0f, ay = 0.0f, az = 0.0f, g = 0.0f;
+            if (valid) {
+                idx = i_index(k, vx[u], vy[u], vz[u], st[u], ax, ay, az, g);
+                hit = i_test(k, a.stot_i[idx], g, st[u]);
+                rng_store(a.r0, a.r1, a.r2, a.r3, i, st[u]);
+            }
+            const uint64_t m = ballot64(hit);
+            if (!m) continue;
+            const int leader = __builtin_ctzll(m);
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&s_n, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, kWave);
+            const uint32_t pos = base + mbcnt64(m);
+            if (hit && pos < kStageList) {
+                uint32_t *e = list + pos;
+                e[0 * kStageList] = i;
+                e[1 * kStageList] = __float_as_uint(vx[u]);
+                e[2 * kStageList] = __float_as_uint(vy[u]);
+                e[3 * kStageList] = __float_as_uint(vz[u]);
+                e[4 * kStageList] = st[u].s0;
+                e[5 * kStageList] = st[u].s1;
+                e[6 * kStageList] = st[u].s2;
+                e[7 * kStageList] = st[u].s3;
+                e[8 * kStageList] = (uint32_t)idx;
+                e[9 * kStageList] = __float_as_uint(ax);
+                e[10 * kStageList] = __float_as_uint(ay);
+                e[11 * kStageList] = __float_as_uint(az);
+                e[12 * kStageList] = __float_as_uint(g);
+            } else if (hit) {  // a full list: scatter here
+                float w = vx[u], y = vy[u], z = vz[u];
+                Xo s2 = st[u];
+                i_scatter(k, a.cs[3 * kCsRanges + idx], a.cs[4 * kCsRanges + idx], w, y, z, ax, ay, az, g, s2);
+                rng_store(a.r0, a.r1, a.r2, a.r3, i, s2);
+                a.vx[i] = w;
+                a.vy[i] = y;
+                a.vz[i] = z;
+                ++ncoll;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCollidePPT; ++u) {
+            vx[u] = pvx[u];
+            vy[u] = pvy[u];
+            vz[u] = pvz[u];
+            st[u] = pst[u];
+        }
+    }
+    __syncthreads();
+    const uint32_t n_list = min(s_n, kStageList);
+    for (uint32_t e = threadIdx.x; e < n_list; e += blockDim.x) {
+        const uint32_t *q = list + e;
+        const uint32_t slot = q[0];
+        float w = __uint_as_float(q[1 * kStageList]), y = __uint_as_float(q[2 * kStageList]),
+              z = __uint_as_float(q[3 * kStageList]);
+        Xo s2{q[4 * kStageList], q[5 * kStageList], q[6 * kStageList], q[7 * kStageList]};
+        const int b = (int)q[8 * kStageList];
+        i_scatter(k, a.cs[3 * kCsRanges + b], a.cs[4 * kCsRanges + b], w, y, z, __uint_as_float(q[9 * kStageList]),
+                  __uint_as_float(q[10 * kStageList]), __uint_as_float(q[11 * kStageList]),
+                  __uint_as_float(q[12 * kStageList]), s2);
+        rng_store(a.r0, a.r1, a.r2, a.r3, slot, s2);
+        a.vx[slot] = w;
+        a.vy[slot] = y;
+        a.vz[slot] = z;
+        ++ncoll;
     }
     if (a.n_coll) wg_atomic_add(a.n_coll, ncoll);
 }
 
 int launch_collide_i(const CollideIJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
-    Shape sh = resolve(sh0);
+    (void)sh0;
     CollIArgs a;
     a.stot_i = j.stot_i;
     a.cs = j.cs;
@@ -1413,9 +1622,7 @@ int launch_collide_i(const CollideIJob &j, const KPhys &k, const Shape &sh0, hip
     a.r3 = j.i.rng3;
     a.n_coll = j.n_coll;
     a.k = k;
-    const int block = 256;
-    const int grid = std::max(1, sh.grid * sh.block / block);
-    k_collide_i<<<grid, block, 0, s>>>(a);
+    k_collide_i<<<device_cus() * 4, kCollideBlock, 0, s>>>(a);
     PIC_LAUNCHED("k_collide_i");
     return PIC_OK;
 }

@@ -206,6 +206,10 @@ int step_reference(pic_sim *s, bool measure) {
     pj.i_dens = s->i_dens[0];
     const double p0 = host_pot0(s->p, s->t);
     pj.pot0 = s->prec == PIC_PREC_FP64 ? p0 : (double)(float)p0;
+    // the alive counter of the electron compaction is cleared by the solve
+    // (main.rs:1758), which runs just before it: one launch fewer per step
+    pj.zero_cnt = alive;
+    pj.zero_cnt_words = 1;
     TRY(launch_poisson(pj, st));
     // main.rs:1746-1756
     if (measure) {
@@ -216,8 +220,7 @@ int step_reference(pic_sim *s, bool measure) {
             s->ref_meas[1]++;
         }
     }
-    // main.rs:1758-1768: electrons
-    TRY(launch_copy_u32(alive, nullptr, st));
+    // main.rs:1758-1768: electrons (alive cleared by the solve above)
     MoveJob mj{s->efield, s->e[s->ce][0], s->e[s->ce][1], s->e[s->ce][2], s->e[s->ce][3],
                s->e[1 - s->ce][0], s->e[1 - s->ce][1], s->e[1 - s->ce][2], s->e[1 - s->ce][3],
                alive, ne, s->k.factor_e, s->k.dt_e, 0.0f, s->counters + 0, nullptr, s->cap};
@@ -236,16 +239,14 @@ int step_reference(pic_sim *s, bool measure) {
         TRY(launch_move_compact(mi, s->k, s->sh, st));
         s->ci ^= 1;
         ni = icnt(s, s->ci);
-        TRY(launch_copy_u32(ni, alive, st, s->cap));
-        TRY(launch_copy_u32(alive, ne, st));
+        TRY(launch_copy2_u32(ni, alive, alive, ne, st, s->cap));
     }
     // main.rs:1785-1792
     CollideEJob cj{s->stot_e, s->cs, ne, species(s, PIC_ELECTRONS, s->ce), species(s, PIC_IONS, s->ci),
                    alive, ni, s->cnt + C_OVERFLOW, s->counters + 4};
     TRY(launch_collide_e(cj, s->k, s->sh, st));
     // appends past the capacity were dropped (and flagged): clamp the counts
-    TRY(launch_copy_u32(ne, alive, st, s->cap));
-    TRY(launch_copy_u32(ni, ni, st, s->cap));
+    TRY(launch_copy2_u32(ne, alive, ni, ni, st, s->cap));
     // main.rs:1794-1800
     if (ion) {
         CollideIJob ij{s->stot_i, s->cs, ni, species(s, PIC_IONS, s->ci), s->counters + 5};
