@@ -17,7 +17,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpic_oracle.so")
+# PIC_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of `make -C oracle sanitize`, for tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("PIC_ORACLE_LIB") or os.path.join(HERE, "libpic_oracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 CS_RANGES = 1_000_000
 N_EEPF = 2000
