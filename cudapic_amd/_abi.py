@@ -50,7 +50,7 @@ class PicSimConfig(C.Structure):
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
                                     "replicas", "measure", "ppt_inplace", "lds_replicas")],
-        ("compact_every", C.c_uint32),
+        ("compact_every", C.c_uint32), ("rng_defer", C.c_uint32),
     ]
 
 
@@ -90,7 +90,8 @@ class PicState(C.Structure):
 
 class PicStepConfig(C.Structure):
     _fields_ = [("compact_every", C.c_uint32),
-                *[(n, C.c_int32) for n in ("grid_blocks", "ppt", "ppt_inplace", "lds_replicas", "pad")]]
+                *[(n, C.c_int32) for n in ("grid_blocks", "ppt", "ppt_inplace", "lds_replicas")],
+                ("rng_defer", C.c_uint32)]
 
 
 PIC_STEP_MEASURE, PIC_STEP_RESET = 1, 2
@@ -183,6 +184,7 @@ SIGNATURES = {
     "pic_step_graph": (I32, [P, C.POINTER(PicState), U32, U32, I32, P]),
     "pic_step_diagnostics": (I32, [P, P, P, P, P]),
     "pic_step_noalloc": (I32, [PP, C.POINTER(PicState), U32, I32, P]),
+    "pic_step_lagged": (I32, [P, C.POINTER(PicState), U32, I32, U32, U32, P]),
     "pic_malloc": (I32, [C.POINTER(P), U64]),
     "pic_free": (I32, [P]),
     "pic_memcpy_htod": (I32, [P, P, U64]),

@@ -58,6 +58,31 @@ __device__ __forceinline__ uint32_t xo_next(Xo &s) {
     return result;
 }
 
+// n state transitions of xo_next, outputs discarded.
+__device__ __forceinline__ void xo_skip(Xo &s, uint32_t n) {
+    for (uint32_t r = 0; r < n; ++r) (void)xo_next(s);
+}
+
+// The inverse transition: the state one draw earlier.  Forward, with t =
+// s1 << 9: s0' = s0 ^ s3 ^ s1, s1' = s1 ^ s2 ^ s0, s2' = s2 ^ s0 ^ t,
+// s3' = rotl(s3 ^ s1, 11).  So s3 ^ s1 = rotr(s3', 11), s0 follows from s0',
+// s1' ^ s2' = s1 ^ (s1 << 9), which y ^ (y << 9) ^ (y << 18) ^ (y << 27)
+// inverts, and s2, s3 follow.
+__device__ __forceinline__ void xo_prev(Xo &s) {
+    const uint32_t d1 = rotl32(s.s3, 21);
+    const uint32_t a = s.s0 ^ d1;
+    const uint32_t y = s.s1 ^ s.s2;
+    const uint32_t b = y ^ (y << 9) ^ (y << 18) ^ (y << 27);
+    s.s2 = s.s1 ^ b ^ a;
+    s.s3 = d1 ^ b;
+    s.s0 = a;
+    s.s1 = b;
+}
+
+__device__ __forceinline__ void xo_back(Xo &s, uint32_t n) {
+    for (uint32_t r = 0; r < n; ++r) xo_prev(s);
+}
+
 // main.rs:883-885: the top 23 bits as the mantissa of [1,2), minus 1.
 __device__ __forceinline__ float u32_to_real(uint32_t x) {
     return __uint_as_float(0x3F800000u | (x >> 9)) - 1.0f;

@@ -18,8 +18,13 @@
 //
 // HBM traffic per in-place particle-step: read x, v and the four RNG words
 // (32 B), write x, vx and the RNG words (24 B), vy and vz only after a
-// collision.  The field, the histograms and the cross-section gathers stay
-// in LDS / L2.
+// collision.  Electrons defer the RNG write-back: within a run of in-place
+// steps a live slot that does not collide keeps its stored words, which
+// fall one draw behind its stream per step, and catches up on its next
+// load (the deferred draws cost a few ALU operations, the spared store
+// 16 B); the run's last step writes every slot's exact words again, so
+// outside a run the words are the reference's.  The field, the histograms
+// and the cross-section gathers stay in LDS / L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -159,7 +164,9 @@ __device__ uint32_t tile_reserve_f(uint32_t wave_count, uint32_t *s_wave, uint32
 // Where an electron step puts its ionisation products: the new electron
 // after the watermark *e_cnt of the electron region that starts at slot
 // e_base (e_cap slots), the new ion after *i_cnt of the ion region at
-// i_base (i_cap slots).
+// i_base (i_cap slots).  e_back > 0 (deferred RNG write-back): the new
+// electron's slot becomes live with its RNG words e_back draws behind its
+// stream, so they are rewound (er0..er3: the electron RNG words).
 struct Appends {
     float *ex, *evx, *evy, *evz;
     uint32_t *e_cnt;
@@ -167,6 +174,8 @@ struct Appends {
     float *ix, *ivx, *ivy, *ivz;
     uint32_t *i_cnt;
     uint32_t i_base, i_cap;
+    uint32_t *er0, *er1, *er2, *er3;
+    uint32_t e_back;
 };
 
 // Apply an electron collision's products: the new electron (into the LDS
@@ -197,6 +206,11 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
         a.evx[es] = se[0];
         a.evy[es] = se[1];
         a.evz[es] = se[2];
+        if (a.e_back) {
+            Xo r = rng_load(a.er0, a.er1, a.er2, a.er3, es);
+            xo_back(r, a.e_back);
+            rng_store(a.er0, a.er1, a.er2, a.er3, es, r);
+        }
         deposit_fix(h, px, k, scale);
         if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
     } else {
@@ -373,7 +387,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
 // watermark (input count cnt_in, copied into cnt_out before the launch),
 // so they are neither pushed nor collision-tested until the next step
 // (main.rs:992).  Per particle the traffic is read x, v, RNG (32 B), write
-// x, vx, RNG (24 B) and vy, vz only after a collision.
+// x, vx (8 B), the RNG words (16 B) only on a step that leaves them exact
+// (FusedJob::rng_next_lag == 0) and vy, vz only after a collision.
 //
 // Collisions are rare (well under 1% of particle-steps) and their
 // scattering code is long: a wave that ran it for its one or two colliding
@@ -446,8 +461,13 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     float *const X = (float *)j.sx, *const VX = (float *)j.svx, *const VY = (float *)j.svy,
                  *const VZ = (float *)j.svz;
     Acc *const i_dens_g = (Acc *)j.ion_dens + (blockIdx.x % A.reps) * A.rep_stride;
+    // deferred RNG write-back (electrons): the words of live slots are `lag`
+    // draws behind their streams on entry and are left `lag_out` behind
+    const uint32_t lag = ELECTRON ? j.rng_lag : 0u;
+    const uint32_t lag_out = ELECTRON ? j.rng_next_lag : 0u;
     const Appends app{X,        VX,       VY,       VZ,       j.cnt_out + ko, ko * j.seg_size,     j.seg_size,
-                      j.ix,     j.ivx,    j.ivy,    j.ivz,    j.ion_cnt + ko, ko * j.ion_seg_size, j.ion_seg_size};
+                      j.ix,     j.ivx,    j.ivy,    j.ivz,    j.ion_cnt + ko, ko * j.ion_seg_size, j.ion_seg_size,
+                      j.rng0,   j.rng1,   j.rng2,   j.rng3,   lag_out};
     uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
     bool overflow = false;
 
@@ -498,6 +518,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             valid[u] = u * blockDim.x < avail;
             if (!valid[u]) x[u] = -1.0f;  // past the segment's count
         }
+        // a live slot's stream catches up with the draws it was spared
+        if (lag) {
+#pragma unroll
+            for (int u = 0; u < PPT; ++u)
+                if (x[u] >= 0.0f) xo_skip(st[u], lag);
+        }
         // (2) push; in-gap particles bin their energy and gather stot
         float nx[PPT], spd[PPT], sig[PPT];
         float ax[PPT], ay[PPT], az[PPT];  // ions: velocity of the collision partner
@@ -531,12 +557,14 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
         uint32_t pslot0, pavail;
         load_tile(tile_next, pslot0, pavail, px, pvx, pvy, pvz, pst);
         // (3) collision tests, write-back, deposits; colliding particles
-        // are listed; lost ones leave a hole.  Every slot of the tile is
-        // written back (x, vx, RNG words; a colliding particle's values are
-        // overwritten by its scattering after the last tile, holes and lost
-        // particles write their unchanged RNG words): stores that skip
+        // are listed; lost ones leave a hole.  Every slot of the tile writes
+        // x and vx back, and on a step that leaves the RNG words exact
+        // (lag_out == 0) also its RNG words (a colliding particle's values
+        // are overwritten by its scattering after the last tile, holes and
+        // lost particles write their unchanged RNG words): stores that skip
         // lanes leave partly written lines, which cost a read-modify-write
-        // in memory.
+        // in memory.  Otherwise only a particle lost in this step writes its
+        // RNG words (when they were behind): a hole's words are exact.
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t i = slot0 + u * blockDim.x;
@@ -549,7 +577,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             if (valid[u]) {
                 X[i] = in[u] ? nx[u] : -1.0f;
                 VX[i] = vx[u];
-                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
+                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
             }
             if (!in[u] && x[u] >= 0.0f) {  // lost in this step
                 const bool pw = nx[u] < 0.0f;
@@ -604,6 +633,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                         else
                             kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
                                              ay[u], az[u], spd[u], s);
+                        xo_back(s, lag_out);
                         rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
                         VX[i] = w;
                         VY[i] = y;
@@ -660,6 +690,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                                  __uint_as_float(q[CF_AX * kCollPerWg]), __uint_as_float(q[CF_AY * kCollPerWg]),
                                  __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]), s);
             }
+            xo_back(s, lag_out);
             rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
             VX[slot] = w;
             VY[slot] = y;
@@ -724,6 +755,14 @@ static void launch_ppt(const FusedArgs &a, bool f64, bool inplace, int ppt, int 
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_t s) {
     if (j.n_seg < 1 || j.n_seg > kMaxSegments || j.seg_size == 0) {
         set_error("fused: 1 to 64 segments of non-zero size");
+        return PIC_ERR_ARG;
+    }
+    // a live slot that does not collide keeps its words: one draw further
+    // behind after the step, or exact if the step writes them all back
+    const bool deferred = j.rng_lag || j.rng_next_lag;
+    if (deferred && (!j.electrons || !j.inplace || j.rng_lag >= kMaxRngLag ||
+                     (j.rng_next_lag != 0 && j.rng_next_lag != j.rng_lag + 1))) {
+        set_error("fused: deferred RNG write-back is for in-place electron steps, next lag 0 or lag + 1");
         return PIC_ERR_ARG;
     }
     Shape sh = resolve(sh0);

@@ -39,6 +39,14 @@ inline bool particle_block_ok(int b) { return b == 128 || b == 256 || b == 512; 
 // Persistent workgroups by default: 1024 lanes per compute unit (4 waves
 // per SIMD, what the fused kernels' ~112 VGPRs allow resident).
 inline int default_particle_grid(int cus, int block) { return cus * (1024 / block); }
+// Default steps per run of deferred electron RNG write-backs: 8, or the
+// largest power of two below it that divides 4*n_sub (a captured period
+// must end with a write-back).
+inline uint32_t default_rng_defer(int n_sub) {
+    uint32_t d = 8;
+    while (d > 1 && (4u * (uint32_t)n_sub) % d) d >>= 1;
+    return d;
+}
 
 // Launch shape of the persistent particle kernels.
 struct Shape {
@@ -226,8 +234,18 @@ struct FusedJob {
     // density buffer the step's Poisson solve has read; optional)
     uint32_t *zero;
     uint32_t zero_words;
+    // Deferred RNG write-back (in-place electron steps only; 0 and 0 on
+    // every other launch): on entry the RNG words of each live slot are
+    // rng_lag draws behind its stream (holes and slots past the watermark
+    // are exact); the launch leaves them rng_next_lag draws behind (0: every
+    // slot exact).  A live particle that does not collide then writes no
+    // RNG words; a colliding one and a newborn store their state rewound by
+    // rng_next_lag draws.
+    uint32_t rng_lag, rng_next_lag;
 };
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
+// Largest lag rng_lag / rng_next_lag may take.
+constexpr uint32_t kMaxRngLag = 64;
 // *out += number of x >= 0 in the first cnt[q] slots of each segment q
 int launch_count_live(const float *x, const uint32_t *cnt, int n_seg, uint32_t seg_size, uint32_t *out,
                       hipStream_t s);

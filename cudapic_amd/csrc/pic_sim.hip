@@ -95,6 +95,11 @@ struct pic_sim {
     unsigned long long ref_meas[2] = {0, 0};
     // fused path: steps between compactions (a multiple of 4*N_SUB)
     uint32_t compact_every = 800;
+    // fused path: deferred electron RNG write-back -- steps per run (divides
+    // 4*N_SUB) and the draws the stored words of live slots are behind at
+    // the next step (0 between calls)
+    uint32_t rng_defer = 8;
+    uint32_t e_lag = 0;
     // captured 4*N_SUB-step graphs, keyed by the measure flag, whether the
     // period starts with a compaction, and the roles at its start; each
     // remembers the roles it leaves behind
@@ -301,13 +306,25 @@ int fused_prologue(pic_sim *s) {
 // dst arrays, holes squeezed out, survivors take the RNG stream of their new
 // slot); the other steps update the particles in place.  The electron live
 // counters are double-buffered every step (the in-place kernel appends
-// newborns after a frozen copy of the watermark).
-int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEvent_t e1) {
+// newborns after a frozen copy of the watermark).  The electrons' RNG words
+// are written back in runs of rng_defer steps aligned to multiples of
+// rng_defer in the cycle; the step before a compaction, the last step of the
+// cycle and the last step of a call (`last`) write them all back, so that
+// every call and every captured period starts and ends with exact words.
+int step_fused(pic_sim *s, bool measure, bool last, bool time_kernel, hipEvent_t e0, hipEvent_t e1) {
     hipStream_t st = s->stream;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p.n_sub) == 0;
     const bool comp_e = (t % s->compact_every) == 0;
     const bool comp_i = ion && comp_e;
+    const uint32_t lag = s->e_lag;
+    const uint32_t lag_out = (last || comp_e || (t + 1u) % s->rng_defer == 0 || t + 1u == (uint32_t)s->p.n_t)
+                                 ? 0u
+                                 : lag + 1u;
+    if (comp_e && lag) {
+        set_error("pic_sim: internal: a compacting step starts behind on RNG words");
+        return PIC_ERR_ARG;
+    }
     // Poisson of this step: e density accumulated by the previous step,
     // ion density = the held one, or on ion steps the accumulated one
     PoissonJob pj = base_poisson(s);
@@ -450,9 +467,12 @@ int step_fused(pic_sim *s, bool measure, bool time_kernel, hipEvent_t e0, hipEve
     // the density this step's solve read is the next step's accumulator
     fe.zero = (uint32_t *)s->e_dens[s->cd];
     fe.zero_words = (uint32_t)dens_words32(s);
+    fe.rng_lag = lag;
+    fe.rng_next_lag = lag_out;
     if (time_kernel) PIC_HIP(hipEventRecord(e0, st));
     TRY(launch_fused(fe, s->k, s->sh, st));
     if (time_kernel) PIC_HIP(hipEventRecord(e1, st));
+    s->e_lag = lag_out;
     if (comp_e) s->ce ^= 1;
     s->pe ^= 1;
     s->cd ^= 1;
@@ -468,8 +488,8 @@ void invalidate_graphs(pic_sim *s) {
         }
 }
 
-int one_step(pic_sim *s, bool measure) {
-    int rc = s->cfg.path == PIC_PATH_FUSED ? step_fused(s, measure, false, nullptr, nullptr)
+int one_step(pic_sim *s, bool measure, bool last) {
+    int rc = s->cfg.path == PIC_PATH_FUSED ? step_fused(s, measure, last, false, nullptr, nullptr)
                                            : step_reference(s, measure);
     if (rc != PIC_OK) return rc;
     s->t = (s->t + 1) % (uint32_t)s->p.n_t;
@@ -511,17 +531,22 @@ int graph_for(pic_sim *s, bool measure, pic_sim::GraphSlot **out) {
     const int period = 4 * s->p.n_sub;
     const uint32_t t0 = s->t;
     const unsigned long long steps0 = s->steps;
+    if (s->e_lag) {
+        set_error("pic_sim: internal: a period starts behind on RNG words");
+        return PIC_ERR_ARG;
+    }
     hipGraph_t g = nullptr;
     PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = PIC_OK;
-    for (int i = 0; i < period && rc == PIC_OK; ++i) rc = one_step(s, measure);
+    for (int i = 0; i < period && rc == PIC_OK; ++i) rc = one_step(s, measure, false);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     // capture does not execute: remember the roles the period leaves, then
-    // restore the host mirror
+    // restore the host mirror (a period starts and ends with exact RNG words)
     int after[5];
     get_roles(s, after);
     s->t = t0;
     s->steps = steps0;
+    s->e_lag = 0;
     set_roles(s, key + 2);
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -582,7 +607,7 @@ int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
             continue;
         }
         if (dry) break;
-        rc = one_step(s, measure);
+        rc = one_step(s, measure, n == 1);
         --n;
     }
     if (dry) {
@@ -660,6 +685,12 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         if (s->compact_every % period) {
             delete s;
             set_error("pic_sim_create: compact_every must be a multiple of 4*n_sub");
+            return PIC_ERR_ARG;
+        }
+        s->rng_defer = c->rng_defer ? c->rng_defer : default_rng_defer(s->p.n_sub);
+        if (s->rng_defer > kMaxRngLag || period % s->rng_defer) {
+            delete s;
+            set_error("pic_sim_create: rng_defer must divide 4*n_sub and be at most 64");
             return PIC_ERR_ARG;
         }
     }
@@ -1121,7 +1152,7 @@ int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_to
             const uint32_t c = std::min(chunk, n_steps - done);
             PIC_HIP(hipEventRecord(w0, s->stream));
             for (uint32_t i = 0; i < c; ++i) {
-                TRY(step_fused(s, measure != 0, true, ev[2 * i], ev[2 * i + 1]));
+                TRY(step_fused(s, measure != 0, done + i + 1 == n_steps, true, ev[2 * i], ev[2 * i + 1]));
                 s->t = (s->t + 1) % (uint32_t)s->p.n_t;
                 s->steps++;
             }

@@ -74,6 +74,7 @@ struct StepCtx {
     int prec = PIC_PREC_FP64;
     bool f64 = true;
     uint32_t compact_every = 800;
+    uint32_t rng_defer = 8;  // steps per deferred RNG write-back run
     uint32_t *cnt = nullptr;
     void *e_dens[2] = {}, *i_dens[2] = {};
     double *grid_e = nullptr, *grid_i = nullptr, *tab = nullptr;
@@ -108,6 +109,11 @@ int check_config(const pic_params_t &p, const pic_step_config_t *c) {
         set_error("pic_step: compact_every must be a multiple of 4*n_sub");
         return PIC_ERR_ARG;
     }
+    const uint32_t period = 4u * (uint32_t)p.n_sub;
+    if (c && c->rng_defer && (c->rng_defer > kMaxRngLag || period % c->rng_defer)) {
+        set_error("pic_step: rng_defer must divide 4*n_sub and be at most 64");
+        return PIC_ERR_ARG;
+    }
     return PIC_OK;
 }
 
@@ -120,6 +126,7 @@ int make_ctx(const pic_params_t &p, const pic_step_config_t *c, uint8_t *ws, boo
     x->prec = p.precision;
     x->f64 = p.precision == PIC_PREC_FP64;
     x->compact_every = compact_every(p, c);
+    x->rng_defer = c && c->rng_defer ? c->rng_defer : default_rng_defer(p.n_sub);
     Shape sh;
     sh.block = 256;
     sh.grid = c && c->grid_blocks > 0 ? c->grid_blocks : default_particle_grid(device_cus(), sh.block);
@@ -233,13 +240,28 @@ void swap4(float *&a0, float *&a1, float *&a2, float *&a3, float *&b0, float *&b
     std::swap(a3, b3);
 }
 
+// Deferred RNG write-back of the electrons: the lag (draws the stored RNG
+// words of live slots are behind their streams) after step t, given the lag
+// of step t.  Runs of rng_defer steps, aligned to multiples of rng_defer in
+// the cycle; the step before a compaction, the last step of the cycle and
+// the last step of a call write the words back (lag 0), so every call
+// starts and ends with exact words and every captured period (t a multiple
+// of 4*n_sub, which rng_defer divides) starts and ends at lag 0.
+uint32_t next_lag(const StepCtx &x, uint32_t t, uint32_t lag, bool last) {
+    if (last || t % x.compact_every == 0 || (t + 1u) % x.rng_defer == 0 || t + 1u == (uint32_t)x.p.n_t) return 0;
+    return lag + 1u;
+}
+
 // One RF step at cycle index t (main.rs:1725-1808): Poisson from the
 // accumulated densities, then the ion kernel (ion steps) and the electron
 // kernel, in place or compacting (t % compact_every == 0, both species).
 // by_value: the solve takes t and pot0_tab[t] from the arguments (eager
 // calls); otherwise from the device clock (captured periods, replayed at
-// any t of the same phase).  Both advance the clock.
-int step_one(const StepCtx &x, pic_state_t &s, uint32_t t, bool measure, bool by_value, hipStream_t st) {
+// any t of the same phase).  Both advance the clock.  lag / lag_out: the
+// electrons' deferred RNG write-back on entry and exit (0 on compacting
+// steps).
+int step_one(const StepCtx &x, pic_state_t &s, uint32_t t, bool measure, bool by_value, hipStream_t st,
+             uint32_t lag = 0, uint32_t lag_out = 0) {
     const bool ion = (t % (uint32_t)x.p.n_sub) == 0;
     const bool comp_e = (t % x.compact_every) == 0;
     const bool comp_i = ion && comp_e;
@@ -345,6 +367,12 @@ int step_one(const StepCtx &x, pic_state_t &s, uint32_t t, bool measure, bool by
     fe.eepf = measure ? s.eepf_counts : nullptr;
     fe.zero = (uint32_t *)x.e_dens[cd];
     fe.zero_words = (uint32_t)x.dens_words32;
+    if (comp_e && (lag || lag_out)) {
+        set_error("pic_step: a compacting step starts and ends with exact RNG words");
+        return PIC_ERR_ARG;
+    }
+    fe.rng_lag = lag;
+    fe.rng_next_lag = lag_out;
     if ((rc = launch_fused(fe, x.k, x.sh, st)) != PIC_OK) return rc;
     // main.rs:1764-1767
     if (comp_e) swap4(s.e_x, s.e_vx, s.e_vy, s.e_vz, s.tmp_x, s.tmp_vx, s.tmp_vy, s.tmp_vz);
@@ -357,9 +385,12 @@ int steps_eager(const StepCtx &x, pic_state_t *s, uint32_t t, uint32_t n, int fl
         const int rc = prologue(x, *s, t, st);
         if (rc != PIC_OK) return rc;
     }
+    uint32_t lag = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const int rc = step_one(x, *s, t, measure, true, st);
+        const uint32_t lag_out = next_lag(x, t, lag, i + 1 == n);
+        const int rc = step_one(x, *s, t, measure, true, st, lag, lag_out);
         if (rc != PIC_OK) return rc;
+        lag = lag_out;
         t = (t + 1u) % (uint32_t)x.p.n_t;
     }
     return PIC_OK;
@@ -439,7 +470,12 @@ int period_graph(pic_ctx *c, const pic_state_t &s, uint32_t t, bool measure, hip
     hipGraph_t g = nullptr;
     PIC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = PIC_OK;
-    for (uint32_t i = 0; i < period && rc == PIC_OK; ++i) rc = step_one(c->x, tmp, t + i, measure, false, st);
+    uint32_t lag = 0;
+    for (uint32_t i = 0; i < period && rc == PIC_OK; ++i) {
+        const uint32_t lag_out = next_lag(c->x, t + i, lag, false);
+        rc = step_one(c->x, tmp, t + i, measure, false, st, lag, lag_out);
+        lag = lag_out;
+    }
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -543,8 +579,15 @@ int pic_step_graph(pic_ctx_t *c, pic_state_t *s, uint32_t t, uint32_t n_steps, i
         PoissonJob pj = poisson_job(c->x, *s);
         if ((rc = launch_pot0(pj, (double *)(c->x.cnt + SC_P0), st)) != PIC_OK) return rc;
     }
+    uint32_t lag = 0;  // deferred RNG write-back of the eager steps
     while (n_steps > 0) {
         if (st && t % period == 0 && n_steps >= period) {
+            // the step before t wrote the RNG words back (4*n_sub is a
+            // multiple of rng_defer), as the captured period assumes
+            if (lag) {
+                set_error("pic_step_graph: internal: a period starts behind on RNG words");
+                return PIC_ERR_ARG;
+            }
             hipGraphExec_t ex = nullptr;
             if ((rc = period_graph(c, *s, t, measure, st, &ex)) != PIC_OK) return rc;
             const hipError_t e = hipGraphLaunch(ex, st);
@@ -557,11 +600,34 @@ int pic_step_graph(pic_ctx_t *c, pic_state_t *s, uint32_t t, uint32_t n_steps, i
             n_steps -= period;
             continue;
         }
-        if ((rc = step_one(c->x, *s, t, measure, true, st)) != PIC_OK) return rc;
+        const uint32_t lag_out = next_lag(c->x, t, lag, n_steps == 1);
+        if ((rc = step_one(c->x, *s, t, measure, true, st, lag, lag_out)) != PIC_OK) return rc;
+        lag = lag_out;
         t = (t + 1u) % (uint32_t)c->p.n_t;
         --n_steps;
     }
     return PIC_OK;
+}
+
+int pic_step_lagged(pic_ctx_t *c, pic_state_t *s, uint32_t t, int flags, uint32_t rng_lag, uint32_t rng_lag_out,
+                    pic_stream_t stream) {
+    if (!c) {
+        set_error("pic_step_lagged: null context");
+        return PIC_ERR_ARG;
+    }
+    int rc = check_state(s, flags);
+    if (rc != PIC_OK) return rc;
+    if (t >= (uint32_t)c->p.n_t) {
+        set_error("pic_step_lagged: t beyond the RF cycle");
+        return PIC_ERR_ARG;
+    }
+    if (rng_lag >= kMaxRngLag || (rng_lag_out != 0 && rng_lag_out != rng_lag + 1)) {
+        set_error("pic_step_lagged: lag below 64, lag_out 0 or lag + 1");
+        return PIC_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if ((flags & PIC_STEP_RESET) && (rc = prologue(c->x, *s, t, st)) != PIC_OK) return rc;
+    return step_one(c->x, *s, t, (flags & PIC_STEP_MEASURE) != 0, true, st, rng_lag, rng_lag_out);
 }
 
 int pic_step_diagnostics(pic_ctx_t *c, unsigned long long counters[8], uint32_t *ifed, uint32_t *overflow,

@@ -13,7 +13,7 @@ from ._abi import (PIC_ELECTRONS, PIC_IONS, PIC_N_EEPF, PIC_N_IFED, PIC_PATH_FUS
 class Sim:
     def __init__(self, p, capacity=120_000, path=PIC_PATH_FUSED, use_graph=True, seed_e=None,
                  seed_i=None, segments=0, block_size=0, grid_blocks=0, ppt=0, replicas=0,
-                 ppt_inplace=0, lds_replicas=0, compact_every=0):
+                 ppt_inplace=0, lds_replicas=0, compact_every=0, rng_defer=0):
         cfg = PicSimConfig()
         lib().pic_sim_config_default(C.byref(cfg))
         cfg.params = p
@@ -29,6 +29,7 @@ class Sim:
         cfg.ppt_inplace = ppt_inplace
         cfg.lds_replicas = lds_replicas
         cfg.compact_every = int(compact_every)
+        cfg.rng_defer = int(rng_defer)
         self.p = p
         self.cfg = cfg
         self.capacity = int(capacity)

@@ -29,7 +29,7 @@ _TI = ("tmp_i_x", "tmp_i_vx", "tmp_i_vy", "tmp_i_vz")
 
 class StepState:
     def __init__(self, p, capacity, seed_e=SEED_E, seed_i=SEED_I, tables=None, stream=True,
-                 compact_every=0, grid_blocks=0, ppt=0, ppt_inplace=0, lds_replicas=0, context=True):
+                 compact_every=0, grid_blocks=0, ppt=0, ppt_inplace=0, lds_replicas=0, rng_defer=0, context=True):
         """tables: (cs, stot_e, stot_i) host arrays (default: pic_cross_sections
         of the discharge).  stream: run on a stream of its own (needed for
         graphs) or on the null stream.  context: create the pic_init context
@@ -83,7 +83,7 @@ class StepState:
         self.ctx = None
         if context:
             cfg = PicStepConfig(compact_every=compact_every, grid_blocks=grid_blocks, ppt=ppt,
-                                ppt_inplace=ppt_inplace, lds_replicas=lds_replicas)
+                                ppt_inplace=ppt_inplace, lds_replicas=lds_replicas, rng_defer=rng_defer)
             h = C.c_void_p()
             call("pic_init", C.byref(p), C.byref(cfg), C.byref(h))
             self.ctx = h.value
@@ -170,6 +170,12 @@ class StepState:
         """pic_step_noalloc: the same step on the library's fixed workspace."""
         call("pic_step_noalloc", C.byref(self.p), C.byref(self.st), int(t), self._flags(measure, reset),
              self.stream)
+
+    def step_lagged(self, t, lag, lag_out, measure=False, reset=False):
+        """pic_step_lagged: one step whose live electron slots' RNG words are
+        `lag` draws behind on entry and are left `lag_out` behind."""
+        call("pic_step_lagged", self.ctx, C.byref(self.st), int(t), self._flags(measure, reset), int(lag),
+             int(lag_out), self.stream)
 
     def run(self, t, n_steps, measure=False, reset=False):
         call("pic_step_graph", self.ctx, C.byref(self.st), int(t), int(n_steps), self._flags(measure, reset),

@@ -227,6 +227,9 @@ typedef struct pic_sim_config {
     uint32_t compact_every;    /* fused path: steps between the compactions that
                                   squeeze out the holes left by lost particles
                                   (a multiple of 4*n_sub; 0 = 800) */
+    uint32_t rng_defer;        /* fused path: steps per run of deferred electron
+                                  RNG write-backs (as pic_step_config_t; the
+                                  words are exact whenever a call returns) */
 } pic_sim_config_t;
 
 typedef struct pic_sim pic_sim_t;
@@ -358,7 +361,12 @@ enum pic_step_flags {
 typedef struct pic_step_config {
     uint32_t compact_every;
     int32_t grid_blocks, ppt, ppt_inplace, lds_replicas;
-    int32_t pad;
+    uint32_t rng_defer;  /* electrons' deferred RNG write-back: inside one
+                            call a live slot that does not collide stores no
+                            RNG words for up to this many consecutive steps
+                            (must divide 4*n_sub, at most 64; 1 = write back
+                            every step; 0 = 8).  Every call returns with the
+                            reference's exact words. */
 } pic_step_config_t;
 
 typedef struct pic_ctx pic_ctx_t;
@@ -397,6 +405,16 @@ int pic_step_diagnostics(pic_ctx_t *ctx, unsigned long long counters[8], uint32_
  * diagnostic counters or IFED are kept; a capacity overflow shows as a
  * count above capacity. */
 int pic_step_noalloc(const pic_params_t *p, pic_state_t *s, uint32_t t, int flags, pic_stream_t stream);
+/* One pic_step with the electrons' deferred RNG write-back made explicit,
+ * for tests that check the deferral one step at a time.  No reference
+ * counterpart.  On entry the RNG words of every live electron slot
+ * (x >= 0, below *n_electrons) are rng_lag draws behind the slot's stream;
+ * holes and slots past the watermark hold exact words.  On return the live
+ * slots (newborns included) are rng_lag_out draws behind, holes and
+ * unused slots exact.  rng_lag < 64; rng_lag_out 0 or rng_lag + 1; both 0
+ * at a compacting step.  pic_step is pic_step_lagged(.., 0, 0, ..). */
+int pic_step_lagged(pic_ctx_t *ctx, pic_state_t *s, uint32_t t, int flags, uint32_t rng_lag,
+                    uint32_t rng_lag_out, pic_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Ensembles of independent discharges (one launch per stage per group of instances)

@@ -78,6 +78,7 @@ def _load():
         "or_params_derive": (None, [P, C.c_int, C.c_int, C.c_int] + [C.c_double] * 7),
         "or_xo_next": (C.c_uint32, [_u32p]),
         "or_xo_jump": (None, [_u32p]),
+        "or_xo_skip_soa": (None, [_u32p, _u32p, _u32p, _u32p, C.c_uint32, _u32p]),
         "or_u32_to_real": (C.c_float, [C.c_uint32]),
         "or_seed_streams": (None, [_u32p, C.c_uint32, _u32p, _u32p, _u32p, _u32p]),
         "or_three_normal": (None, [_u32p, C.c_float, _f32p]),
@@ -133,6 +134,16 @@ def xo_next(state: np.ndarray) -> int:
 
 def xo_jump(state: np.ndarray) -> None:
     lib().or_xo_jump(state)
+
+
+def xo_skip_soa(words, counts):
+    """The SoA streams (four uint32 arrays) advanced by counts[j] draws each
+    (new arrays)."""
+    out = [np.ascontiguousarray(w, np.uint32).copy() for w in words]
+    k = np.ascontiguousarray(counts, np.uint32)
+    assert all(len(w) == len(k) for w in out)
+    lib().or_xo_skip_soa(*out, len(k), k)
+    return out
 
 
 def u32_to_real(x: int) -> float:

@@ -107,6 +107,19 @@ void or_xo_jump(uint32_t s[4]) {
     memcpy(s, acc, sizeof(acc));
 }
 
+/* Test support, no reference counterpart: the SoA streams s0..s3[j]
+ * (j < n) each advanced by k[j] draws of or_xo_next. */
+void or_xo_skip_soa(uint32_t *s0, uint32_t *s1, uint32_t *s2, uint32_t *s3, uint32_t n, const uint32_t *k) {
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t st[4] = {s0[j], s1[j], s2[j], s3[j]};
+        for (uint32_t r = 0; r < k[j]; ++r) (void)or_xo_next(st);
+        s0[j] = st[0];
+        s1[j] = st[1];
+        s2[j] = st[2];
+        s3[j] = st[3];
+    }
+}
+
 /* main.rs:883-885: 23 high bits into the mantissa of 1.0, minus 1. */
 float or_u32_to_real(uint32_t x) {
     union { uint32_t u; float f; } c;

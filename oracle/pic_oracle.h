@@ -44,6 +44,7 @@ void or_params_derive(or_params *p, int n_g, int n_t, int n_sub, double freq,
 /* xoshiro128+ 1.0 (main.rs:860-885, xorshiro128c/xoshiro128p.c:36-55) */
 uint32_t or_xo_next(uint32_t s[4]);
 void     or_xo_jump(uint32_t s[4]);
+void     or_xo_skip_soa(uint32_t *s0, uint32_t *s1, uint32_t *s2, uint32_t *s3, uint32_t n, const uint32_t *k);
 float    or_u32_to_real(uint32_t x);
 void     or_seed_streams(const uint32_t master[4], uint32_t n, uint32_t *s0,
                          uint32_t *s1, uint32_t *s2, uint32_t *s3);

@@ -43,3 +43,17 @@ def assert_same_multiset(got, want):
     w = sort_rows(*want)
     assert g.shape == w.shape, (g.shape, w.shape)
     np.testing.assert_array_equal(g.view(np.uint32), w.view(np.uint32))
+
+
+def no_ionisation_tables(orc, po):
+    """The reference case's tables with the ionisation cross section set to
+    zero (and the electron totals summed again as the reference sums them,
+    f64 sum of the f32 entries times the gas density): electrons collide
+    elastically or excite, so no particle is created and a run's slot
+    contents do not depend on the order of the newborns' appends."""
+    cs, _, si = orc.cross_sections(po.gas_density)
+    cs = cs.copy()
+    r = len(si)
+    cs[2 * r:3 * r] = 0.0
+    se = ((cs[:r].astype(np.float64) + cs[r:2 * r].astype(np.float64)) * po.gas_density).astype(np.float32)
+    return cs, se, si

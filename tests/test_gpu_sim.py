@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import SEED_E, SEED_I, assert_same_multiset, sort_rows, synthetic_state
+from helpers import SEED_E, SEED_I, assert_same_multiset, no_ionisation_tables, sort_rows, synthetic_state
 from replay import _check_new_ions, _check_slot_draws, _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
@@ -287,6 +287,49 @@ def test_timed_window_equals_plain_run(ca, orc):
         assert s.time()[0] == 960
         s.check()
         s.close()
+
+
+def test_deferred_rng_write_back_equals_every_step(ca, orc):
+    """The simulation object (segments, captured periods, timed eager steps)
+    with the default deferred electron RNG write-back and with a write-back
+    every step (rng_defer = 1).  Without ionisation the slot contents are
+    deterministic, so after pic_sim_run (eager steps, a captured period,
+    eager steps) and pic_sim_time_steps with kernel events (eager steps that
+    end off a write-back boundary) every slot -- particles and RNG words --
+    must be equal (up to 1e-4 of the slots, for the last bit that the density
+    flush's floating-point atomics may move in an f32 field value)."""
+    p = ca.params(weight=2.0e3, precision=2)
+    po = orc.params(weight=2.0e3)
+    E, I = synthetic_state(200_000, 210_000, seed=93, te_ev=6.0)
+    tables = no_ionisation_tables(orc, po)
+    out = []
+    for defer in (1, 8):
+        s = ca.Sim(p, capacity=300_000, rng_defer=defer)
+        try:
+            s.set_tables(*tables)
+            s.set_time(1)
+            s.upload(0, *E)
+            s.upload(1, *I)
+            print(f"pic_sim rng_defer={defer}: 170 steps run, 37 timed", flush=True)
+            s.run(170, measure=True)
+            s.time_steps(37, kernel_events=True)
+            s.synchronize()
+            s.check()
+            out.append((s.counts(), s.download(0) + s.rng_download(0), s.download(1) + s.rng_download(1),
+                        s.measurements()["counters"]))
+        finally:
+            s.close()
+    (na, ea, ia, ca_), (nb, eb, ib, cb) = out
+    assert na == nb
+    assert ca_["e_collisions"] > 10_000 and ca_["i_collisions"] > 0, ca_
+    for a, b in ((ea, eb), (ia, ib)):
+        n = len(a[0])
+        assert [len(u) for u in a] == [len(v) for v in b]
+        differ = np.zeros(len(a[4]), bool)
+        for w in range(4):
+            differ[:n] |= np.asarray(a[w]).view(np.uint32) != np.asarray(b[w]).view(np.uint32)
+            differ |= a[4 + w] != b[4 + w]
+        assert differ.sum() <= max(2, n // 10_000), (differ.sum(), n)
 
 
 def test_checkpoint_round_trip(ca, orc, tmp_path):

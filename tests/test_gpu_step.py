@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from helpers import assert_same_multiset, sort_rows, synthetic_state
+from helpers import assert_same_multiset, no_ionisation_tables, sort_rows, synthetic_state
 from replay import _check_new_ions, _push_ref, _replay_species, _split_ions
 
 pytestmark = pytest.mark.gpu
@@ -28,20 +28,42 @@ def ca():
     return cudapic_amd
 
 
-@pytest.mark.parametrize("t0", [0, 1, 20])
-def test_one_step_vs_oracle(ca, orc, t0):
+@pytest.mark.parametrize("t0,lag,lag_out", [(0, None, 0), (1, None, 0), (20, None, 0),
+                                             (1, 3, 4), (21, 5, 0), (20, 2, 3), (41, 0, 1)])
+def test_one_step_vs_oracle(ca, orc, t0, lag, lag_out):
+    """lag None: pic_step.  Otherwise pic_step_lagged, the step inside a run
+    of deferred electron RNG write-backs: the stored words of the live
+    electron slots start `lag` draws behind their streams and end `lag_out`
+    behind.  The test brings them to their streams' positions with the
+    oracle's generator (live slots only; holes and unused slots must hold
+    exact words) and replays the step against them."""
     p = ca.params(weight=2.0e3, precision=2)
     po = orc.params(weight=2.0e3)
     n_e, n_i, cap = 200_000, 210_000, 300_000
-    E, I = synthetic_state(n_e, n_i, seed=61 + t0, te_ev=6.0)
+    E, I = synthetic_state(n_e, n_i, seed=61 + t0 + 7 * (lag or 0), te_ev=6.0)
+    if lag is not None:
+        # 2000 of them next to the walls, moving into them: lost in the step
+        r = np.random.default_rng(t0)
+        near = r.uniform(0.0, 5e-6, 2000).astype(np.float32)
+        E[0][:1000] = near[:1000]
+        E[1][:1000] = -2e6
+        E[0][1000:2000] = np.float32(0.025) - near[1000:]
+        E[1][1000:2000] = 2e6
     cs, se, si = orc.cross_sections(po.gas_density)
     with ca.StepState(p, cap, tables=(cs, se, si)) as st:
         st.upload(0, *E)
         st.upload(1, *I)
         tmp_e, tmp_i = st.st.tmp_x, st.st.tmp_i_x
         rng_e0, rng_i0 = st.rng_download(0), st.rng_download(1)
-        print(f"pic_step one step t0={t0}", flush=True)
-        st.step(t0, measure=True, reset=True)
+        print(f"pic_step one step t0={t0} lag={lag} lag_out={lag_out}", flush=True)
+        if lag is None:
+            st.step(t0, measure=True, reset=True)
+        else:
+            st.step_lagged(t0, lag, lag_out, measure=True, reset=True)
+            # every uploaded electron is live: its stream is `lag` draws on
+            k0 = np.zeros(cap, np.uint32)
+            k0[:n_e] = lag
+            rng_e0 = orc.xo_skip_soa(rng_e0, k0)
         pot, ef, ed, idn = st.grid()
         for got, parts in ((ed, E), (idn, I)):
             want = orc.deposit(parts[0], po)
@@ -57,6 +79,20 @@ def test_one_step_vs_oracle(ca, orc, t0):
         ge, slots_e = st.download(0)
         gi, slots_i = st.download(1)
         rng_e1, rng_i1 = st.rng_download(0), st.rng_download(1)
+        if lag is not None:
+            # live slots (newborns included) are `lag_out` behind; holes and
+            # the slots past the watermark hold exact words
+            k1 = np.zeros(cap, np.uint32)
+            k1[slots_e] = lag_out
+            rng_e1 = orc.xo_skip_soa(rng_e1, k1)
+            x_slots = st.slots_in_use(0)[0]
+            holes = np.nonzero(x_slots < 0.0)[0]
+            unused = np.arange(len(x_slots), cap)
+            assert len(holes) >= 1900
+            for w in range(4):
+                # a particle lost in the step drew nothing: its stream's words before the step
+                np.testing.assert_array_equal(rng_e1[w][holes], rng_e0[w][holes])
+                np.testing.assert_array_equal(rng_e1[w][unused], rng_e0[w][unused])
         born_x, ions = _replay_species(orc, po, se, cs, E, ef, ge, slots_e, rng_e0, rng_e1, "e")
         assert len(born_x) > 0
         ion_step = t0 % 20 == 0
@@ -180,6 +216,44 @@ def test_eager_and_graph_steps_match_sim_run(ca, orc):
         sim.close()
         eager.close()
         graph.close()
+
+
+def test_deferred_rng_write_back_equals_every_step(ca, orc):
+    """213 steps from t = 1 of pic_step_graph (eager steps, a captured period,
+    eager steps ending off a write-back boundary) with the default deferred
+    RNG write-back (runs of 8 steps) and with a write-back every step
+    (rng_defer = 1): the same collisions from the same streams, so every
+    slot below the watermark -- x, v and the four RNG words -- must be equal
+    when the calls return.  Without ionisation the runs are deterministic
+    (the density flush's floating-point atomics can only change the last bit
+    of a sum, which may move an f32 field value in rare cases: up to 1e-4 of
+    the slots may differ)."""
+    p = ca.params(weight=2.0e3, precision=2)
+    po = orc.params(weight=2.0e3)
+    E, I = synthetic_state(200_000, 210_000, seed=91, te_ev=6.0)
+    tables = no_ionisation_tables(orc, po)
+    runs = []
+    for defer in (1, 8):
+        with ca.StepState(p, 300_000, tables=tables, rng_defer=defer) as st:
+            st.upload(0, *E)
+            st.upload(1, *I)
+            print(f"deferred write-back: rng_defer={defer}, 213 steps", flush=True)
+            st.run(1, 213, measure=True, reset=True)
+            st.synchronize()
+            counters, _, overflow = st.diagnostics()
+            n = (st.count(0), st.count(1))
+            runs.append((n, st.slots_in_use(0), st.rng_download(0, n[0]), st.slots_in_use(1),
+                         st.rng_download(1, n[1]), counters, overflow))
+    (na, ea, rea, ia, ria, ca_, ova), (nb, eb, reb, ib, rib, cb, ovb) = runs
+    assert na == nb and ova == ovb == 0
+    assert ca_[4] > 10_000   # electron collisions happened (and ion ones: index 5)
+    assert ca_[5] > 0
+    for a, b, ra, rb, n in ((ea, eb, rea, reb, na[0]), (ia, ib, ria, rib, na[1])):
+        differ = np.zeros(n, bool)
+        for u, v in zip(a + ra, b + rb):
+            differ |= np.asarray(u).view(np.uint32) != np.asarray(v).view(np.uint32)
+        assert differ.sum() <= max(2, n // 10_000), (differ.sum(), n)
+    assert (np.asarray(ea[0]) < 0).sum() > 0   # holes were made and kept exact words
 
 
 def test_reset_restarts_from_new_particles(ca, orc):
