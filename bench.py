@@ -79,6 +79,8 @@ def parse():
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--segments", type=int, default=0)
     ap.add_argument("--lds-replicas", type=int, default=0)
+    ap.add_argument("--rng-defer", type=int, default=0,
+                    help="steps per run of deferred electron RNG write-backs (0 = 8; 1 = every step)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick grid, particles per thread and replication with cudapic_amd.autotune "
                          "(cached per device and workload; the search runs before the warm-up)")
@@ -230,7 +232,8 @@ def main():
         start = "synthetic plasma-like state"
     print(f"bench: start: {start}, {len(E[0])} e + {len(I[0])} i", file=sys.stderr, flush=True)
     shape = dict(grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
-                 ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas, block_size=args.block_size)
+                 ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas, block_size=args.block_size,
+                 rng_defer=args.rng_defer)
     tuned = None
     if args.autotune and path == ca.PIC_PATH_FUSED:
         from cudapic_amd import autotune
@@ -280,6 +283,14 @@ def main():
                               "push+collide+deposit, timed with HIP events on the sim stream",
                     "kernel_avg_us": round(avg_ms * 1e3, 2), "electrons_per_launch": int(units),
                     "bytes_per_unit": ALG_BYTES}
+        # what the in-place electron kernel must move per electron: read x,
+        # v and the RNG words (32 B), write x, vx (8 B) and the RNG words
+        # once per run of deferred write-backs
+        defer = shape.get("rng_defer") or 8
+        compulsory = 40.0 + 16.0 / defer
+        roofline["rng_defer"] = defer
+        roofline["compulsory_bytes_per_unit"] = round(compulsory, 2)
+        roofline["compulsory_frac"] = round(compulsory * units / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     sim.close()
     if roofline is not None and not args.no_copy:
         bw = copy_bandwidth(1024)

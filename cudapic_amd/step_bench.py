@@ -18,12 +18,12 @@ import sys
 import time
 
 
-def measure(steps=4000, warmup=400, capacity=8_000_000, weight=2.0e3):
+def measure(steps=4000, warmup=400, capacity=8_000_000, weight=2.0e3, rng_defer=0):
     import cudapic_amd as ca
     from cudapic_amd import synthetic
     p = ca.params(weight=weight, precision=ca.PIC_PREC_FP64)
     E, I = synthetic.converged_state(int(round(7.0e4 / weight)))
-    with ca.StepState(p, capacity) as st:
+    with ca.StepState(p, capacity, rng_defer=rng_defer) as st:
         st.upload(0, *E)
         st.upload(1, *I)
         del E, I
@@ -41,15 +41,18 @@ def measure(steps=4000, warmup=400, capacity=8_000_000, weight=2.0e3):
     ps = 0.5 * (n0[0] + n1[0]) * steps + 0.5 * (n0[1] + n1[1]) * ion_steps
     return {"value": round(ps / wall, 1), "unit": "particle-steps/s", "steps": steps,
             "ms_per_step": round(wall * 1e3 / steps, 5), "n_electrons": n1[0], "n_ions": n1[1],
-            "overflow": overflow, "entry_point": "pic_step_graph on caller-owned buffers"}
+            "overflow": overflow, "rng_defer": rng_defer or 8,
+            "entry_point": "pic_step_graph on caller-owned buffers"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--rng-defer", type=int, default=0,
+                    help="steps per run of deferred electron RNG write-backs (0 = the default, 8)")
     a = ap.parse_args()
-    r = measure(a.steps, a.warmup)
+    r = measure(a.steps, a.warmup, rng_defer=a.rng_defer)
     print(f"step_bench: {a.steps} steps, {r['ms_per_step']} ms per step", file=sys.stderr, flush=True)
     print(json.dumps(r), flush=True)
 
