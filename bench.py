@@ -248,17 +248,13 @@ def main():
     sim.synchronize()
     sim.check()
     print("bench: warmup done", file=sys.stderr, flush=True)
-    # which of the timed steps replay a captured graph: whole 4*N_SUB-step
-    # periods that start on the period (the rest launch eagerly)
+    # the timed window replays captured graphs only: whole 4*N_SUB-step
+    # periods and the partial periods it starts or ends with, all captured
+    # by a dry pass before the window opens (fused path with graphs, n_t a
+    # multiple of 4*N_SUB); otherwise every step launches eagerly
     graph_steps = 0
-    if not args.no_graph and path == ca.PIC_PATH_FUSED:
-        period, t_in, left = 4 * p.n_sub, sim.time()[0], args.steps
-        while left > 0:
-            if t_in % period == 0 and left >= period:
-                graph_steps += period
-                t_in, left = (t_in + period) % p.n_t, left - period
-            else:
-                t_in, left = (t_in + 1) % p.n_t, left - 1
+    if not args.no_graph and path == ca.PIC_PATH_FUSED and p.n_t % (4 * p.n_sub) == 0:
+        graph_steps = args.steps
     # timed region (one GPU: the barrier is the stream synchronisation)
     sim.synchronize()
     t = sim.time_steps(args.steps)

@@ -100,14 +100,17 @@ struct pic_sim {
     // the next step (0 between calls)
     uint32_t rng_defer = 8;
     uint32_t e_lag = 0;
-    // captured 4*N_SUB-step graphs, keyed by the measure flag, whether the
-    // period starts with a compaction, and the roles at its start; each
-    // remembers the roles it leaves behind
+    // captured runs of steps inside one 4*N_SUB-step period (whole periods,
+    // and the partial ones a timed window starts or ends with), keyed by the
+    // measure flag, whether the run starts with a compaction, the roles at
+    // its start, and its offset in the period and length; each remembers the
+    // roles it leaves behind
+    static constexpr int kGraphSlots = 16;
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
-        int key[7] = {};
+        int key[9] = {};
         int after[5] = {};
-    } graphs[8];
+    } graphs[kGraphSlots];
     int graph_next = 0;  // slot the next capture replaces when all are taken
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -513,35 +516,44 @@ void set_roles(pic_sim *s, const int *r) {
     s->pe = r[4];
 }
 
-// The graph of the 4*N_SUB fused steps starting now (t a multiple of
-// 4*N_SUB): found in the cache by (measure, compaction at the first step,
-// roles) or captured.  A period holds at most one compaction (at its first
-// step), so the density, counter and ion-hold parities return to their
-// start values and only the array parities can differ at its end.
-int graph_for(pic_sim *s, bool measure, pic_sim::GraphSlot **out) {
-    int key[7];
+// The graph of `len` fused steps from now, inside one 4*N_SUB-step period
+// ((t % 4*N_SUB) + len <= 4*N_SUB): found in the cache by (measure,
+// compaction at the first step, roles, offset in the period, len) or, when
+// `capture` is set, captured (otherwise *out = nullptr).  A period holds at
+// most one compaction (at its first step), so over a whole period the
+// density, counter and ion-hold parities return to their start values and
+// only the array parities can differ at its end.  A run starts with exact
+// RNG words (it starts a call or a period) and its last step writes them
+// back (it ends a period, where the write-back runs are aligned, or ends
+// the call).
+int graph_for(pic_sim *s, bool measure, uint32_t len, bool capture, pic_sim::GraphSlot **out) {
+    const uint32_t period = 4u * (uint32_t)s->p.n_sub;
+    int key[9];
     key[0] = measure ? 1 : 0;
     key[1] = (s->t % s->compact_every) == 0 ? 1 : 0;
     get_roles(s, key + 2);
+    key[7] = (int)(s->t % period);
+    key[8] = (int)len;
+    *out = nullptr;
     for (auto &g : s->graphs)
         if (g.exec && memcmp(g.key, key, sizeof(key)) == 0) {
             *out = &g;
             return PIC_OK;
         }
-    const int period = 4 * s->p.n_sub;
+    if (!capture) return PIC_OK;
     const uint32_t t0 = s->t;
     const unsigned long long steps0 = s->steps;
     if (s->e_lag) {
-        set_error("pic_sim: internal: a period starts behind on RNG words");
+        set_error("pic_sim: internal: a captured run starts behind on RNG words");
         return PIC_ERR_ARG;
     }
     hipGraph_t g = nullptr;
     PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = PIC_OK;
-    for (int i = 0; i < period && rc == PIC_OK; ++i) rc = one_step(s, measure, false);
+    for (uint32_t i = 0; i < len && rc == PIC_OK; ++i) rc = one_step(s, measure, i + 1 == len);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
-    // capture does not execute: remember the roles the period leaves, then
-    // restore the host mirror (a period starts and ends with exact RNG words)
+    // capture does not execute: remember the roles the run leaves, then
+    // restore the host mirror (a run starts and ends with exact RNG words)
     int after[5];
     get_roles(s, after);
     s->t = t0;
@@ -565,7 +577,7 @@ int graph_for(pic_sim *s, bool measure, pic_sim::GraphSlot **out) {
         }
     if (!slot) {
         slot = &s->graphs[s->graph_next];
-        s->graph_next = (s->graph_next + 1) % 8;
+        s->graph_next = (s->graph_next + 1) % pic_sim::kGraphSlots;
         (void)hipGraphExecDestroy(slot->exec);
     }
     slot->exec = ex;
@@ -575,10 +587,12 @@ int graph_for(pic_sim *s, bool measure, pic_sim::GraphSlot **out) {
     return PIC_OK;
 }
 
-// n steps: whole 4*N_SUB-step periods from the graph cache, the rest
-// eagerly.  dry: only capture the graphs those periods will replay (up to
-// the first eager step), then restore the host state, so that a timed run
-// does not include graph capture.
+// n steps in runs that end at period boundaries: whole 4*N_SUB-step periods
+// from the graph cache (captured on first use), partial runs from the cache
+// when a timed window captured them, the rest eagerly.  dry: only capture
+// the graphs of every run of the n steps (partial ones included), then
+// restore the host state, so that a timed window replays graphs only and
+// does not include their capture.
 int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
     if (s->cfg.path == PIC_PATH_FUSED && s->need_prologue) TRY(fused_prologue(s));
     const uint32_t period = 4u * (uint32_t)s->p.n_sub;
@@ -590,21 +604,24 @@ int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
     const unsigned long long steps0 = s->steps;
     int rc = PIC_OK;
     while (n > 0 && rc == PIC_OK) {
-        if (graphs && s->t % period == 0 && n >= period) {
+        if (graphs && s->e_lag == 0) {
+            const uint32_t len = std::min(n, period - s->t % period);
             pic_sim::GraphSlot *g = nullptr;
-            if ((rc = graph_for(s, measure, &g)) != PIC_OK) break;
-            if (!dry) {
-                const hipError_t e = hipGraphLaunch(g->exec, s->stream);
-                if (e != hipSuccess) {
-                    rc = hip_fail(e, "hipGraphLaunch");
-                    break;
+            if ((rc = graph_for(s, measure, len, dry || len == period, &g)) != PIC_OK) break;
+            if (g) {
+                if (!dry) {
+                    const hipError_t e = hipGraphLaunch(g->exec, s->stream);
+                    if (e != hipSuccess) {
+                        rc = hip_fail(e, "hipGraphLaunch");
+                        break;
+                    }
                 }
+                set_roles(s, g->after);
+                s->t = (s->t + len) % (uint32_t)s->p.n_t;
+                s->steps += len;
+                n -= len;
+                continue;
             }
-            set_roles(s, g->after);
-            s->t = (s->t + period) % (uint32_t)s->p.n_t;
-            s->steps += period;
-            n -= period;
-            continue;
         }
         if (dry) break;
         rc = one_step(s, measure, n == 1);
@@ -614,6 +631,7 @@ int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
         set_roles(s, roles0);
         s->t = t0;
         s->steps = steps0;
+        s->e_lag = 0;
     }
     return rc;
 }

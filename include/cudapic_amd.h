@@ -304,7 +304,11 @@ int pic_sim_read_xt(pic_sim_t *s, double *out, uint32_t *n_xt);
 
 /* Timing: run n_steps and report wall milliseconds (HIP events on the sim
  * stream) and the summed duration of the dominant particle kernel
- * (electron particle kernel) with its launch count. */
+ * (electron particle kernel) with its launch count.  ms_kernel NULL: with
+ * graphs (fused path, n_t a multiple of 4*N_SUB) the window replays
+ * captured graphs only -- whole periods and the partial periods it starts
+ * or ends with, captured before the window opens; otherwise eager steps
+ * bracketed by events. */
 int pic_sim_time_steps(pic_sim_t *s, uint32_t n_steps, int measure, float *ms_total,
                        float *ms_kernel, uint32_t *kernel_launches,
                        unsigned long long *particle_steps);

@@ -255,7 +255,7 @@ def test_timed_window_equals_plain_run(ca, orc):
     E, I = synthetic_state(100_000, 110_000, seed=52)
     cs, se0, si0 = zero_tables(orc, po)
     sims = []
-    for _ in range(3):
+    for _ in range(4):
         s = make_sim(ca, p, 200_000, 1, graph=True)
         s.set_tables(cs, se0, si0)
         s.upload(0, *E)
@@ -267,6 +267,11 @@ def test_timed_window_equals_plain_run(ca, orc):
     # step that is not on a period boundary
     sims[2].run(3)
     k = sims[2].time_steps(877, kernel_events=True)
+    # timed windows that start and end inside periods: captured partial
+    # periods (the head of one, whole ones, the tail of another)
+    sims[3].run(3)
+    sims[3].time_steps(150)
+    sims[3].time_steps(727)
     assert r["particle_steps"] > 0 and r["ms_total"] > 0
     assert k["kernel_launches"] == 877 and 0 < k["ms_kernel"] < k["ms_total"]
     for extra in (0, 80):
@@ -276,7 +281,7 @@ def test_timed_window_equals_plain_run(ca, orc):
         ce = [s.counts() for s in sims]
         b = sort_rows(*sims[1].download(0))
         pb = sims[1].grid()[0]
-        for other in (0, 2):
+        for other in (0, 2, 3):
             assert abs(ce[other][0] - ce[1][0]) <= 2 and abs(ce[other][1] - ce[1][1]) <= 2, ce
             a = sort_rows(*sims[other].download(0))
             m = min(len(a), len(b))
@@ -294,9 +299,10 @@ def test_deferred_rng_write_back_equals_every_step(ca, orc):
     with the default deferred electron RNG write-back and with a write-back
     every step (rng_defer = 1).  Without ionisation the slot contents are
     deterministic, so after pic_sim_run (eager steps, a captured period,
-    eager steps) and pic_sim_time_steps with kernel events (eager steps that
-    end off a write-back boundary) every slot -- particles and RNG words --
-    must be equal (up to 1e-4 of the slots, for the last bit that the density
+    eager steps), a timed window inside one period (a captured partial
+    period that ends off a write-back boundary) and pic_sim_time_steps with
+    kernel events (eager steps) every slot -- particles and RNG words -- must
+    be equal (up to 1e-4 of the slots, for the last bit that the density
     flush's floating-point atomics may move in an f32 field value)."""
     p = ca.params(weight=2.0e3, precision=2)
     po = orc.params(weight=2.0e3)
@@ -310,8 +316,9 @@ def test_deferred_rng_write_back_equals_every_step(ca, orc):
             s.set_time(1)
             s.upload(0, *E)
             s.upload(1, *I)
-            print(f"pic_sim rng_defer={defer}: 170 steps run, 37 timed", flush=True)
+            print(f"pic_sim rng_defer={defer}: 170 steps run, 27 + 37 timed", flush=True)
             s.run(170, measure=True)
+            s.time_steps(27)
             s.time_steps(37, kernel_events=True)
             s.synchronize()
             s.check()
