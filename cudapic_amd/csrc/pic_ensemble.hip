@@ -58,6 +58,11 @@ struct pic_ensemble {
     bool need_prologue = true;
     bool use_graph = true;
     uint32_t compact_every = 800;  // steps between compaction passes (a multiple of 4*N_SUB)
+    // deferred electron RNG write-back: steps per run (divides 4*N_SUB) and
+    // the draws the stored words of live slots are behind at the next step
+    // (0 between calls and at every period start)
+    uint32_t rng_defer = 8;
+    uint32_t e_lag = 0;
     // instance groups [grp[g], grp[g+1]) and the periods a group runs at a time
     std::vector<int> grp;
     uint32_t group_periods = 5;
@@ -181,15 +186,27 @@ EnsCompactJob compact_job(pic_ensemble *s, int g0, int g1, bool electrons, float
     return j;
 }
 
+// The electrons' RNG write-back lag after the step at s->t, given the lag
+// s->e_lag it starts with: runs of rng_defer steps aligned to multiples of
+// rng_defer in the cycle; a compacting step (the compaction pass moves
+// particles to slots whose streams they continue), the last step of the
+// cycle and the last step of a call write the words back.
+uint32_t next_lag(const pic_ensemble *s, bool last) {
+    const uint32_t t = s->t;
+    if (last || t % s->compact_every == 0 || (t + 1u) % s->rng_defer == 0 || t + 1u == (uint32_t)s->p0.n_t) return 0;
+    return s->e_lag + 1u;
+}
+
 // One ensemble step of the instances [g0, g1).  Particles are updated in
 // place (lost ones leave holes, newborn electrons and ions are appended
 // after the watermarks); every compact_every steps a compaction pass after
 // the step packs each species into the other array of its pair.  Electron
 // watermarks alternate between two counter words every step: the step
 // reads one, the Poisson kernel copies it into the other, newborns are
-// appended there.  The roles and the host clock are advanced by
-// end_step once every group has had the step.
-int step_range(pic_ensemble *s, bool measure, int g0, int g1) {
+// appended there.  The electrons' RNG words start s->e_lag draws behind and
+// end lag_out behind.  The roles, the lag and the host clock are advanced
+// by end_step once every group has had the step.
+int step_range(pic_ensemble *s, bool measure, int g0, int g1, uint32_t lag_out) {
     hipStream_t st = s->stream;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
@@ -273,13 +290,16 @@ int step_range(pic_ensemble *s, bool measure, int g0, int g1) {
     fe.time = s->clock + g0;
     fe.zero = (uint32_t *)dens_at(s, s->e_dens[s->cd], g0);
     fe.zero_words = (uint32_t)((size_t)(g1 - g0) * s->inst_stride * (s->f64 ? 2 : 1));
+    fe.rng_lag = s->e_lag;
+    fe.rng_next_lag = lag_out;
     TRY(launch_inplace_ens(fe, s->sh, st));
     if (comp_e) TRY(launch_compact_ens(compact_job(s, g0, g1, true, s->e[s->ce], s->e[1 - s->ce], out_e, EC_C), st));
     return PIC_OK;
 }
 
-// the roles and the host clock after a step
-void end_step(pic_ensemble *s) {
+// the roles, the RNG write-back lag and the host clock after a step
+void end_step(pic_ensemble *s, uint32_t lag_out) {
+    s->e_lag = lag_out;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
     const bool comp_e = (t % s->compact_every) == 0;
@@ -294,10 +314,11 @@ void end_step(pic_ensemble *s) {
 }
 
 // one step of every instance, group by group (each group's kernels advance
-// its own device clock)
-int step(pic_ensemble *s, bool measure) {
-    for (size_t g = 0; g + 1 < s->grp.size(); ++g) TRY(step_range(s, measure, s->grp[g], s->grp[g + 1]));
-    end_step(s);
+// its own device clock); `last`: the call's last step
+int step(pic_ensemble *s, bool measure, bool last) {
+    const uint32_t lag_out = next_lag(s, last);
+    for (size_t g = 0; g + 1 < s->grp.size(); ++g) TRY(step_range(s, measure, s->grp[g], s->grp[g + 1], lag_out));
+    end_step(s, lag_out);
     return PIC_OK;
 }
 
@@ -344,12 +365,18 @@ int graph_for(pic_ensemble *s, bool measure, int g, pic_ensemble::GraphSlot **ou
     const int period = 4 * s->p0.n_sub;
     const uint32_t t0 = s->t;
     const unsigned long long st0 = s->steps;
+    if (s->e_lag) {
+        set_error("pic_ens: internal: a period starts behind on RNG words");
+        return PIC_ERR_ARG;
+    }
     hipGraph_t gr = nullptr;
     PIC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = PIC_OK;
     for (int i = 0; i < period && rc == PIC_OK; ++i) {
-        rc = step_range(s, measure, s->grp[g], s->grp[g + 1]);
-        end_step(s);
+        // the period's last step writes the words back (rng_defer divides it)
+        const uint32_t lag_out = next_lag(s, false);
+        rc = step_range(s, measure, s->grp[g], s->grp[g + 1], lag_out);
+        end_step(s, lag_out);
     }
     hipError_t e = hipStreamEndCapture(s->stream, &gr);
     // capture does not execute: remember the roles the period leaves, then
@@ -358,6 +385,7 @@ int graph_for(pic_ensemble *s, bool measure, int g, pic_ensemble::GraphSlot **ou
     get_roles(s, after);
     s->t = t0;
     s->steps = st0;
+    s->e_lag = 0;
     set_roles(s, key + 2);
     if (rc != PIC_OK) {
         if (gr) (void)hipGraphDestroy(gr);
@@ -457,7 +485,7 @@ int run(pic_ensemble *s, uint32_t n, bool measure, bool dry = false) {
             n -= block * period;
             continue;
         }
-        TRY(step(s, measure));
+        TRY(step(s, measure, n == 1));
         --n;
     }
     return PIC_OK;
@@ -565,6 +593,12 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
         if (s->compact_every % period) {
             delete s;
             set_error("pic_ens_create: compact_every must be a multiple of 4*n_sub");
+            return PIC_ERR_ARG;
+        }
+        s->rng_defer = c->rng_defer ? c->rng_defer : default_rng_defer(s->p0.n_sub);
+        if (s->rng_defer > kMaxRngLag || period % s->rng_defer) {
+            delete s;
+            set_error("pic_ens_create: rng_defer must divide 4*n_sub and be at most 64");
             return PIC_ERR_ARG;
         }
     }

@@ -880,6 +880,9 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
     const float dt = ELECTRON ? j.k.dt_e : j.k.dt_i;
     float *const X = j.x, *const VX = j.vx, *const VY = j.vy, *const VZ = j.vz;
     const int rep = blockIdx.x % A.reps;
+    // deferred RNG write-back (electrons), as in the single-discharge kernel
+    const uint32_t lag = ELECTRON ? j.rng_lag : 0u;
+    const uint32_t lag_out = ELECTRON ? j.rng_next_lag : 0u;
 
     // instance of a tile: the largest i with s_tiles[i] <= tile (instances
     // without tiles are skipped); tiles only grow, so the search walks on
@@ -945,6 +948,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                                      __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]),
                                      s);
                 }
+                xo_back(s, lag_out);
                 rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
                 VX[slot] = w;
                 VY[slot] = y;
@@ -975,8 +979,9 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         k.normal_range = in.k.normal_range;
         if (ELECTRON) {
             uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
-            app = Appends{X,    VX,    VY,    VZ,    c + j.c_out, in.e_off, in.e_cap,
-                          j.ix, j.ivx, j.ivy, j.ivz, c + j.c_ion, in.i_off, in.i_cap};
+            app = Appends{X,      VX,     VY,     VZ,     c + j.c_out, in.e_off, in.e_cap,
+                          j.ix,   j.ivx,  j.ivy,  j.ivz,  c + j.c_ion, in.i_off, in.i_cap,
+                          j.rng0, j.rng1, j.rng2, j.rng3, lag_out};
             i_dens_g = (Acc *)j.ion_dens + (size_t)i * j.inst_stride + rep * A.rep_stride;
         }
         cur = i;
@@ -1011,6 +1016,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         for (int u = 0; u < PPT; ++u) {
             valid[u] = u * blockDim.x < avail;
             if (!valid[u]) x[u] = -1.0f;
+        }
+        // a live slot's stream catches up with the draws it was spared
+        if (lag) {
+#pragma unroll
+            for (int u = 0; u < PPT; ++u)
+                if (x[u] >= 0.0f) xo_skip(st[u], lag);
         }
         float nx[PPT], spd[PPT], sig[PPT];
         float ax[PPT], ay[PPT], az[PPT];
@@ -1051,7 +1062,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
             if (valid[u]) {
                 X[i] = in[u] ? nx[u] : -1.0f;
                 VX[i] = vx[u];
-                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
+                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
             }
             if (!in[u] && x[u] >= 0.0f) {
                 const bool pw = nx[u] < 0.0f;
@@ -1100,6 +1112,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                         else
                             kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
                                              ay[u], az[u], spd[u], s);
+                        xo_back(s, lag_out);
                         rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
                         VX[i] = w;
                         VY[i] = y;
@@ -1235,6 +1248,11 @@ int launch_inplace_ens(const EnsStepJob &j, const Shape &sh0, hipStream_t s) {
     Shape sh = resolve(sh0);
     if (!particle_block_ok(sh.block)) {
         set_error("ensemble: workgroup size must be 128, 256 or 512");
+        return PIC_ERR_ARG;
+    }
+    if ((j.rng_lag || j.rng_next_lag) &&
+        (!j.electrons || j.rng_lag >= kMaxRngLag || (j.rng_next_lag != 0 && j.rng_next_lag != j.rng_lag + 1))) {
+        set_error("ensemble: deferred RNG write-back is for electron steps, next lag 0 or lag + 1");
         return PIC_ERR_ARG;
     }
     EnsArgs a;

@@ -329,6 +329,8 @@ struct EnsStepJob {
     int n_t;
     uint32_t *zero;                        // cleared while the kernel runs (optional)
     uint32_t zero_words;
+    // deferred RNG write-back of the electrons, as FusedJob (0 and 0 for ions)
+    uint32_t rng_lag, rng_next_lag;
 };
 int launch_inplace_ens(const EnsStepJob &j, const Shape &sh, hipStream_t s);
 

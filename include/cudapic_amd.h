@@ -456,6 +456,9 @@ typedef struct pic_ensemble_config {
     uint32_t group_slots, group_periods, group_streams;
     /* lanes per workgroup of the particle kernel: 128, 256 or 512 (0 = 256) */
     int32_t block_size;
+    /* steps per run of deferred electron RNG write-backs, as in
+     * pic_sim_config_t (divides 4*n_sub, at most 64; 1 = every step; 0 = 8) */
+    uint32_t rng_defer;
 } pic_ensemble_config_t;
 
 typedef struct pic_ensemble pic_ensemble_t;

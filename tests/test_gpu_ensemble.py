@@ -187,6 +187,51 @@ def test_ensemble_timed_window_equals_plain_run(ca):
         e.close()
 
 
+def test_ensemble_deferred_rng_write_back_equals_every_step(ca):
+    """The ensemble with the default deferred electron RNG write-back and
+    with a write-back every step (rng_defer = 1): cold electrons (1 eV) and
+    a 0.01 V drive, so electrons collide elastically but cannot ionise and
+    every slot's contents are deterministic.  From t = 1 (no compaction in
+    the window): eager steps, a captured period per instance group, eager
+    steps, then a timed window ending off a write-back boundary.  Every
+    slot -- particles and RNG words -- of every instance must be equal (up to
+    1e-4 of the slots, for the last bit that the density flush's
+    floating-point atomics may move in an f32 field value)."""
+    specs = [(0.01, 2.0e3, 60_000), (0.01, 4.0e3, 30_000)]
+    ps = [ca.params(v0=v, weight=w, precision=2) for v, w, _ in specs]
+    caps = [int(n * 1.5) for *_, n in specs]
+    runs = []
+    for defer in (1, 8):
+        e = ca.Ensemble(ps, caps, group_slots=1, group_periods=1, rng_defer=defer)
+        try:
+            e.set_time(1)
+            for i, (*_, n) in enumerate(specs):
+                E, I = synthetic_state(n, int(n * 1.05), seed=27 + i, te_ev=1.0)
+                e.upload(i, 0, *E)
+                e.upload(i, 1, *I)
+            print(f"ensemble rng_defer={defer}: 170 steps run, 27 timed", flush=True)
+            e.run(170)
+            e.time_steps(27)
+            e.synchronize()
+            e.check()
+            runs.append([(e.download(i, 0), e.rng_download(i, 0, caps[i]), e.download(i, 1),
+                          e.rng_download(i, 1, caps[i]), e.measurements(i)[2]) for i in range(len(specs))])
+        finally:
+            e.close()
+    for i in range(len(specs)):
+        ea, rea, ia, ria, ca_ = runs[0][i]
+        eb, reb, ib, rib, cb = runs[1][i]
+        assert ca_[4] > 1000, ca_   # electron collisions happened
+        for a, b, ra, rb in ((ea, eb, rea, reb), (ia, ib, ria, rib)):
+            assert [len(u) for u in a] == [len(v) for v in b]
+            n = len(a[0])
+            differ = np.zeros(len(ra[0]), bool)
+            for w in range(4):
+                differ[:n] |= np.asarray(a[w]).view(np.uint32) != np.asarray(b[w]).view(np.uint32)
+                differ |= ra[w] != rb[w]
+            assert differ.sum() <= max(2, n // 10_000), (i, differ.sum(), n)
+
+
 def test_ensemble_with_an_empty_instance(ca):
     """An instance without particles between two populated ones, each a
     group of its own on the graph path and all in one group eagerly: the
