@@ -16,6 +16,8 @@ values move with the device, the particle count and the grid size:
   replicas      global density replicas the workgroup flushes spread over
                 (8, 16, 32)
   segments      segments per species (8, 16, 32; the single-discharge path)
+  rng_defer     steps per run of deferred electron RNG write-backs (8, 4,
+                16: fewer stores against more catch-up draws per load)
 
 The tuner measures real steps: for each candidate it builds the simulation,
 uploads the same starting state, runs a warm-up (which captures the step
@@ -47,6 +49,7 @@ SIM_SPACE = {
     "replicas": [8, 16, 32],
     "segments": [0, 16, 32, 8],
     "ppt": [4, 2],
+    "rng_defer": [8, 4, 16],
 }
 ENSEMBLE_SPACE = {
     "block_size": [256, 128, 512],
@@ -57,6 +60,7 @@ ENSEMBLE_SPACE = {
     # electron slots per group of instances run a block at a time
     # (0xFFFFFFFF: one group of all)
     "group_slots": [9_000_000, 6_000_000, 12_000_000, 0xFFFFFFFF],
+    "rng_defer": [8, 4, 16],
 }
 
 
@@ -170,12 +174,13 @@ def _grid(cfg, cus):
 def sim_kwargs(cfg, cus):
     return {"block_size": cfg["block_size"], "grid_blocks": _grid(cfg, cus), "ppt_inplace": cfg["ppt_inplace"],
             "lds_replicas": cfg["lds_replicas"], "replicas": cfg["replicas"], "segments": cfg["segments"],
-            "ppt": cfg["ppt"]}
+            "ppt": cfg["ppt"], "rng_defer": cfg.get("rng_defer", 0)}
 
 
 def ensemble_kwargs(cfg, cus):
     return {"block_size": cfg["block_size"], "grid_blocks": _grid(cfg, cus), "ppt": cfg["ppt"],
-            "lds_replicas": cfg["lds_replicas"], "replicas": cfg["replicas"], "group_slots": cfg["group_slots"]}
+            "lds_replicas": cfg["lds_replicas"], "replicas": cfg["replicas"], "group_slots": cfg["group_slots"],
+            "rng_defer": cfg.get("rng_defer", 0)}
 
 
 def tune_ensemble(params_list, capacities, states, steps=80, warmup=80, passes=2, use_cache=True, log=None):
