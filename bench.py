@@ -94,6 +94,9 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the streaming-copy calibration kernel on MIB MiB, 20 times")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth cross-checks")
+    ap.add_argument("--long-cycles", type=int, default=50,
+                    help="RF cycles of the sustained run after the headline window (the configuration's "
+                         "50-cycle timing; 0 skips it; skipped with --no-extras)")
     return ap.parse_args()
 
 
@@ -287,6 +290,20 @@ def main():
         roofline["rng_defer"] = defer
         roofline["compulsory_bytes_per_unit"] = round(compulsory, 2)
         roofline["compulsory_frac"] = round(compulsory * units / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    # the same discharge continued for long_cycles RF cycles in one timed
+    # window (graphs): sustained throughput and how far the counts drift
+    long_run = None
+    if not args.no_extras and args.long_cycles > 0:
+        ne0, ni0 = sim.counts()
+        n_long = args.long_cycles * p.n_t
+        lt = sim.time_steps(n_long)
+        sim.check()
+        ne1, ni1 = sim.counts()
+        long_run = {"cycles": args.long_cycles, "steps": n_long,
+                    "value": round(lt["particle_steps"] / (lt["ms_total"] * 1e-3), 1),
+                    "unit": "particle-steps/s", "ms_per_step": round(lt["ms_total"] / n_long, 5),
+                    "wall_s": round(lt["ms_total"] * 1e-3, 2), "n_electrons": [ne0, ne1], "n_ions": [ni0, ni1]}
+        print(f"bench: {args.long_cycles} RF cycles in {lt['ms_total'] * 1e-3:.1f} s", file=sys.stderr, flush=True)
     sim.close()
     if roofline is not None and not args.no_copy:
         bw = copy_bandwidth(1024)
@@ -327,7 +344,7 @@ def main():
         "cpu_reference_edupic": cpu_edupic,
     }
     if not args.no_extras:
-        sim.close()
+        out["long_run"] = long_run
         out["ensemble"] = ens_fig
         out["adjoint"] = _extra(adjoint_figure, args.adjoint_steps)
         out["drop_in"] = drop_in
