@@ -300,6 +300,14 @@ __device__ __forceinline__ ff operator/(ff a, ff b) {
     return q + ff{q3, 0.0f};
 }
 
+// A lane's value from another lane of the wave by DPP (row_shr:n within
+// rows of 16 lanes, row_bcast:15 / :31 across rows); lanes with no source
+// read 0.  CTRL is the DPP control word, ROWMASK the rows that receive.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xF, true);
+}
+
 template <typename T> struct Num;
 template <> struct Num<float> {
     __device__ static float from(double d) { return (float)d; }
@@ -307,6 +315,8 @@ template <> struct Num<float> {
     __device__ static float zero() { return 0.0f; }
     __device__ static float to_f(float v) { return v; }
     __device__ static float shfl_up(float v, int d) { return __shfl_up(v, d, kWave); }
+    template <int C, int M>
+    __device__ static float dpp(float v) { return __int_as_float(dpp_i<C, M>(__float_as_int(v))); }
 };
 template <> struct Num<double> {
     __device__ static double from(double d) { return d; }
@@ -314,6 +324,12 @@ template <> struct Num<double> {
     __device__ static double zero() { return 0.0; }
     __device__ static float to_f(double v) { return (float)v; }
     __device__ static double shfl_up(double v, int d) { return __shfl_up(v, d, kWave); }
+    template <int C, int M>
+    __device__ static double dpp(double v) {
+        const long long b = __double_as_longlong(v);
+        const int lo = dpp_i<C, M>((int)(uint32_t)b), hi = dpp_i<C, M>((int)(b >> 32));
+        return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+    }
 };
 template <> struct Num<ff> {
     __device__ static ff from(double d) {
@@ -326,33 +342,40 @@ template <> struct Num<ff> {
     __device__ static ff shfl_up(ff v, int d) {
         return ff{__shfl_up(v.hi, d, kWave), __shfl_up(v.lo, d, kWave)};
     }
+    template <int C, int M>
+    __device__ static ff dpp(ff v) { return ff{Num<float>::dpp<C, M>(v.hi), Num<float>::dpp<C, M>(v.lo)}; }
 };
 
-// Inclusive scan over the 1024 lanes of the workgroup (16 waves): shuffle
-// scan inside each wave, then the 16 wave totals through LDS.
+// Inclusive scan over the 64 lanes of a wave on DPP moves: prefixes within
+// each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's total into row 1
+// and row 2's into row 3 (row_bcast:15), then lanes 0-31's total into rows 2
+// and 3 (row_bcast:31).
+template <typename T>
+__device__ __forceinline__ T wave_scan_incl(T v) {
+    v = v + Num<T>::template dpp<0x111, 0xF>(v);
+    v = v + Num<T>::template dpp<0x112, 0xF>(v);
+    v = v + Num<T>::template dpp<0x114, 0xF>(v);
+    v = v + Num<T>::template dpp<0x118, 0xF>(v);
+    v = v + Num<T>::template dpp<0x142, 0xA>(v);
+    v = v + Num<T>::template dpp<0x143, 0xC>(v);
+    return v;
+}
+
+// Inclusive scan over the lanes of the workgroup (up to 16 waves): DPP scan
+// inside each wave, then every wave adds the totals of the waves before it,
+// read from LDS (the last barrier lets the next scan reuse wtot).
 template <typename T>
 __device__ __forceinline__ T block_scan_incl(T v, T *wtot) {
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
-    const int nw = blockDim.x / kWave;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        T o = Num<T>::shfl_up(v, off);
-        if (lane >= off) v = o + v;
-    }
+    v = wave_scan_incl(v);
     if (lane == kWave - 1) wtot[wave] = v;
     __syncthreads();
-    if (wave == 0) {
-        T w = lane < nw ? wtot[lane] : Num<T>::zero();
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            T o = Num<T>::shfl_up(w, off);
-            if (lane >= off) w = o + w;
-        }
-        if (lane < nw) wtot[lane] = w;
+    if (wave > 0) {
+        T base = wtot[0];
+        for (int w = 1; w < wave; ++w) base = base + wtot[w];
+        v = base + v;
     }
-    __syncthreads();
-    if (wave > 0) v = wtot[wave - 1] + v;
     __syncthreads();
     return v;
 }
@@ -405,20 +428,18 @@ struct PoissonArgs {
 };
 
 // Node k of one density summed over the global replicas, in replica order.
-// The replica count is 1 or a multiple of 8; the loads of up to 16 replicas
-// are independent, so they cost one memory latency (a group of 8 also loads
-// replica reps-1 eight more times, from cache, to keep one code path).
+// The replica count is 1 or a multiple of 8; the loads of a group of 8
+// replicas are independent, so a group costs one memory latency.
 template <typename D>
 __device__ __forceinline__ double sum_replicas(const D *d, int reps, int stride, int k) {
     if (reps == 1) return (double)d[k];
     double sum = 0.0;
-    for (int r0 = 0; r0 < reps; r0 += 16) {
-        D v[16];
+    for (int r0 = 0; r0 < reps; r0 += 8) {
+        D v[8];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = d[min(r0 + j, reps - 1) * stride + k];
+        for (int j = 0; j < 8; ++j) v[j] = d[(r0 + j) * stride + k];
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
-            if (r0 + j < reps) sum += (double)v[j];
+        for (int j = 0; j < 8; ++j) sum += (double)v[j];
     }
     return sum;
 }
