@@ -29,6 +29,7 @@ struct KPhys {
     float factor_e, factor_i, dt_e, dt_i;
     float normal_range, f1, f2, half_me_over_e, half_mu_over_e, half_mi_over_e;
     float e_charge, me, exc_th, ion_th, de_cs;
+    float inv_de_cs;  // 1 / de_cs rounded to nearest (energy_index's reciprocal)
     float min_x, max_x, de_eepf, eepf_kick;
     // multiplies the total cross-section table: 1 for the per-discharge
     // tables (sigma_tot * n_gas precomputed), the gas density for a shared
@@ -158,10 +159,22 @@ __device__ __forceinline__ float field_at(const float *ef, float x, float inv_dx
 }
 
 // ---------------------------------------------------------------- collisions
-__device__ __forceinline__ int energy_index(float energy, float de_cs) {
-    const float c = energy / de_cs + 0.5f;
-    const uint32_t idx = (uint32_t)c;
-    return idx > (uint32_t)(kCsRanges - 1) ? kCsRanges - 1 : (int)idx;
+// The cross-section bin of an energy, min(CS_RANGES - 1, (int)(energy /
+// DE_CS + 0.5)) as main.rs:1002-1005 computes it with an IEEE division.  The
+// quotient is formed from the rounded reciprocal y = RN(1/DE_CS) with one
+// fused correction (Markstein's method): q = RN(e*y), r = fma(-q, DE_CS, e)
+// (exact), q' = RN(q + r*y), which is the correctly rounded quotient
+// whenever it neither underflows nor overflows -- three operations instead of
+// the division's ten.  A tiny energy gives bin 0 either way, and the clamp is
+// written so that an overflowed (NaN) quotient lands in the last bin, as the
+// division's infinity does; the bins are equal to the division's for every
+// non-negative float (checked exhaustively on the CPU).
+__device__ __forceinline__ int energy_index(float energy, const KPhys &k) {
+    float q = energy * k.inv_de_cs;
+    const float r = __builtin_fmaf(-q, k.de_cs, energy);
+    q = __builtin_fmaf(r, k.inv_de_cs, q);
+    const float c = q + 0.5f;
+    return c < (float)(kCsRanges - 1) ? (int)(uint32_t)c : kCsRanges - 1;
 }
 
 struct Frame {
@@ -207,8 +220,11 @@ enum CollKind { kNone = 0, kElastic = 1, kExcitation = 2, kIonisation = 3 };
 __device__ __forceinline__ int e_index(const KPhys &k, float vx, float vy, float vz, float &velocity) {
     const float v2 = vx * vx + vy * vy + vz * vz;
     const float energy = k.half_me_over_e * v2;
-    velocity = __builtin_sqrtf(v2);
-    return energy_index(energy, k.de_cs);
+    // the hardware square root (1 ulp), as the reference's sqrt.approx.f32
+    // (main.rs:1007, 695-706): the speed only scales the collision
+    // frequency, where an ulp moves the collision probability by ~1e-10
+    velocity = __builtin_amdgcn_sqrtf(v2);
+    return energy_index(energy, k);
 }
 
 __device__ __forceinline__ bool e_test(const KPhys &k, float stot, float velocity, Xo &s) {
@@ -291,7 +307,7 @@ __device__ __forceinline__ int i_index(const KPhys &k, float vx, float vy, float
     const float g2 = gx * gx + gy * gy + gz * gz;
     const float energy = k.half_mu_over_e * g2;
     g = __builtin_sqrtf(g2);
-    return energy_index(energy, k.de_cs);
+    return energy_index(energy, k);
 }
 
 __device__ __forceinline__ bool i_test(const KPhys &k, float stot, float g, Xo &s) {

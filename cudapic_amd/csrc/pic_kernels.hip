@@ -91,6 +91,7 @@ KPhys make_kphys(const pic_params_t &p) {
     k.exc_th = p.exc_th_f;
     k.ion_th = p.ion_th_f;
     k.de_cs = p.de_cs_f;
+    k.inv_de_cs = 1.0f / p.de_cs_f;
     k.min_x = p.min_x_f;
     k.max_x = p.max_x_f;
     k.de_eepf = p.de_eepf_f;
@@ -1741,7 +1742,7 @@ __global__ __launch_bounds__(256) void k_xt(XtJob j, KPhys k) {
             add(2, p, c1 * en);
             add(2, p + 1, c2 * en);
             if (ELECTRON) {
-                const float rate = j.sigma_ion[energy_index(en, k.de_cs)] * sqrtf(v2) * rate_f;
+                const float rate = j.sigma_ion[energy_index(en, k)] * sqrtf(v2) * rate_f;
                 add(3, p, c1 * rate);
                 add(3, p + 1, c2 * rate);
             }

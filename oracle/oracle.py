@@ -101,6 +101,7 @@ def _load():
         "or_collide_i_one": (C.c_int, [_f32p, _f32p, _f32p, _u32p, P]),
         "or_accumulate_eepf": (None, [_f32p, _f32p, _f32p, _f32p, _f32p, C.c_uint32, P, _u32p]),
         "or_step": (None, [C.POINTER(OrState), P, C.c_int, C.c_int]),
+        "or_bin_division_mismatches": (C.c_uint64, [C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -392,6 +392,40 @@ static inline int energy_index(float energy, const or_params *p) {
     return idx > CS_RANGES - 1 ? CS_RANGES - 1 : (int)idx;
 }
 
+/* Checker of an identity the GPU kernels rely on (not part of the
+ * reference): they form energy / DE_CS from the rounded reciprocal
+ * y = RN(1 / DE_CS) with one fused correction (Markstein's method:
+ * q = RN(e*y), r = fma(-q, DE_CS, e), q' = RN(q + r*y)), and clamp with a
+ * comparison that sends a NaN quotient to the last bin.  Returns how many
+ * energies with the bit patterns lo, lo + stride, ... <= hi get another bin
+ * than energy_index's IEEE division (main.rs:1002-1005); corrected = 0 drops
+ * the correction (the plain product e*y), to show that the check can fail. */
+uint64_t or_bin_division_mismatches(float de_cs, uint32_t lo, uint32_t hi, uint32_t stride, int corrected) {
+    or_params p;
+    memset(&p, 0, sizeof(p));
+    p.de_cs_f = de_cs;
+    const float y = 1.0f / de_cs;
+    uint64_t bad = 0;
+    for (uint64_t u = lo; u <= hi; u += stride) {
+        float e;
+        const uint32_t bits = (uint32_t)u;
+        memcpy(&e, &bits, sizeof(e));
+        float q = e * y;
+        if (corrected) {
+            const float r = fmaf(-q, de_cs, e);
+            q = fmaf(r, y, q);
+        }
+        const float c = q + 0.5f;
+        const int fast = c < (float)(CS_RANGES - 1) ? (int)(uint32_t)c : CS_RANGES - 1;
+        const float ce = e / de_cs + 0.5f;
+        /* the reference's cast saturates (Rust `as usize`); the C cast of an
+           infinity is undefined, so the exact bin is clamped first */
+        const int exact = ce < 4294967040.0f ? energy_index(e, &p) : CS_RANGES - 1;
+        bad += fast != exact;
+    }
+    return bad;
+}
+
 static inline void euler_angles(float gx, float gy, float gz, float *st, float *ct,
                                 float *sp, float *cp) {
     const float half_pi = (float)(0.5 * OR_PI);

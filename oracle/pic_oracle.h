@@ -99,6 +99,13 @@ void or_collide_i(const float *stot_i, const float *cs, uint32_t n_active,
 int or_collide_i_one(const float *stot_i, const float *cs, float v[3],
                      uint32_t s[4], const or_params *p);
 
+/* Energies (bit patterns lo..hi, step stride) whose cross-section bin from
+ * the reciprocal with one fused correction differs from the IEEE division's
+ * (main.rs:1002-1005): the check of the GPU kernels' bin formula (corrected = 0:
+ * without the correction). */
+uint64_t or_bin_division_mismatches(float de_cs, uint32_t lo, uint32_t hi, uint32_t stride,
+                                    int corrected);
+
 /* EEPF histogram of centre electrons (main.rs:1301-1339) */
 void or_accumulate_eepf(const float *x, const float *vx, const float *vy,
                         const float *vz, const float *efield, uint32_t n,

@@ -183,6 +183,25 @@ def test_ifed_counts_absorbed_ions(orc):
 
 
 # ---- collisions -------------------------------------------------------------
+def test_reciprocal_bin_formula_equals_division(orc):
+    """The GPU kernels take the cross-section bin of an energy from the rounded
+    reciprocal of DE_CS with one fused correction (Markstein's method) instead
+    of the reference's IEEE division (main.rs:1002-1005).  The bins are equal
+    for every float energy from 2^-14 eV to 2^11 eV (all 2.1e8 of them: the
+    table ends at 1000 eV) and for a sample of every other non-negative
+    pattern (zero, subnormals, huge energies, +inf); without the correction
+    step the same sample finds mismatches."""
+    f = orc.lib().or_bin_division_mismatches
+    de = float(np.float32(0.001))
+    lo = int(np.float32(2.0 ** -14).view(np.uint32))
+    hi = int(np.float32(2.0 ** 11).view(np.uint32))
+    assert f(de, lo, hi, 1, 1) == 0
+    assert f(de, 0, 0x7F800000, 9973, 1) == 0
+    for other in (0.002, 0.0005, 1.0 / 3.0):  # the identity does not hinge on DE_CS
+        assert f(float(np.float32(other)), 0, 0x7F800000, 997, 1) == 0
+    assert f(de, lo, hi, 4099, 0) > 0
+
+
 def _speed_for(ev, mass):
     return np.sqrt(2 * ev * E_CHARGE / mass)
 
