@@ -50,7 +50,8 @@ class PicSimConfig(C.Structure):
         ("seed_e", C.c_uint32 * 4), ("seed_i", C.c_uint32 * 4),
         *[(n, C.c_int32) for n in ("path", "use_graph", "block_size", "grid_blocks", "ppt",
                                     "replicas", "measure", "ppt_inplace", "lds_replicas")],
-        ("compact_every", C.c_uint32), ("rng_defer", C.c_uint32),
+        ("compact_every", C.c_uint32), ("rng_defer", C.c_uint32), ("cand_masks", C.c_int32),
+        ("cand_threshold", C.c_float),
     ]
 
 

@@ -227,11 +227,28 @@ __device__ __forceinline__ int e_index(const KPhys &k, float vx, float vy, float
     return energy_index(energy, k);
 }
 
-__device__ __forceinline__ bool e_test(const KPhys &k, float stot, float velocity, Xo &s) {
+// Collision probability of an electron in one step, 1 - exp(-nu dt_e) with
+// nu = sigma_tot n_gas |v| (main.rs:1007-1012); e_test compares the step's
+// draw with it.
+__device__ __forceinline__ float e_prob(const KPhys &k, float stot, float velocity) {
     const float nu = stot * k.stot_scale * velocity;
+    return 1.0f - expf(-nu * k.dt_e);
+}
+
+__device__ __forceinline__ bool e_test(const KPhys &k, float stot, float velocity, Xo &s) {
     const float rv = draw(s);
-    const float pc = 1.0f - expf(-nu * k.dt_e);
-    return rv < pc;
+    return rv < e_prob(k, stot, velocity);
+}
+
+// Collision-candidate mask of a slot for the lags [from, 8) of a deferred
+// write-back run: bit j is set when the draw the slot makes at entry lag j
+// (the (j - from)-th output from state s) is below thr.  A step whose bit is
+// clear and whose collision probability is at most thr cannot collide, so
+// it needs no RNG words.
+__device__ __forceinline__ uint32_t cand_mask(Xo s, uint32_t from, float thr) {
+    uint32_t m = 0;
+    for (uint32_t j = from; j < 8u; ++j) m |= (draw(s) < thr ? 1u : 0u) << j;
+    return m;
 }
 
 __device__ __forceinline__ int e_scatter(const KPhys &k, float c0, float c1, float c2, float &vx,

@@ -176,6 +176,7 @@ struct Appends {
     uint32_t i_base, i_cap;
     uint32_t *er0, *er1, *er2, *er3;
     uint32_t e_back;
+    uint8_t *e_mask;  // collision-candidate masks (null: none)
 };
 
 // Apply an electron collision's products: the new electron (into the LDS
@@ -211,6 +212,7 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
             xo_back(r, a.e_back);
             rng_store(a.er0, a.er1, a.er2, a.er3, es, r);
         }
+        if (a.e_mask) a.e_mask[es] = 0xFFu;  // loads its words on every step of the run
         deposit_fix(h, px, k, scale);
         if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
     } else {
@@ -410,8 +412,15 @@ enum CollField {
 constexpr int kCollFieldsE = CF_BIN + 1;
 constexpr int kCollFieldsI = CF_AZ + 1;
 
-template <bool ELECTRON, bool MEASURE, int PPT, typename Acc>
+// RM: collision-candidate masks of deferred electron runs.  0: none (every
+// live slot loads its RNG words); 1: masks read -- a live slot loads its
+// words at the start of its tile only when its bit for this step's lag is
+// set, and late (after the push) when its collision probability exceeds
+// the masks' threshold or when it leaves the gap; 2: a write-back step
+// that writes the next run's masks.
+template <bool ELECTRON, bool MEASURE, int PPT, typename Acc, int RM>
 __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
+    static_assert(RM == 0 || (ELECTRON && !MEASURE), "candidate masks: plain electron steps only");
     const FusedJob &j = A.j;
     const KPhys &k = A.k;
     extern __shared__ __align__(16) float sh[];
@@ -467,7 +476,9 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     const uint32_t lag_out = ELECTRON ? j.rng_next_lag : 0u;
     const Appends app{X,        VX,       VY,       VZ,       j.cnt_out + ko, ko * j.seg_size,     j.seg_size,
                       j.ix,     j.ivx,    j.ivy,    j.ivz,    j.ion_cnt + ko, ko * j.ion_seg_size, j.ion_seg_size,
-                      j.rng0,   j.rng1,   j.rng2,   j.rng3,   lag_out};
+                      j.rng0,   j.rng1,   j.rng2,   j.rng3,   lag_out,        RM ? j.mask : nullptr};
+    uint8_t *const MK = j.mask;
+    const float thr = j.coll_thr;
     uint32_t lost_pow = 0, lost_gnd = 0, ncoll = 0;
     bool overflow = false;
 
@@ -476,9 +487,11 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     // a tile of loads in flight behind its compute and gathers.  The loads
     // are unconditional (lanes past the segment's count read the segment's
     // first slot) and the count test is applied where the values are used,
-    // so that no wait for them is needed at the point of issue.
+    // so that no wait for them is needed at the point of issue.  With the
+    // masks read (RM 1) the tile brings the mask bytes instead of the RNG
+    // words, which only candidates load.
     auto load_tile = [&](uint32_t tile, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
-                         float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT]) {
+                         float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT], uint32_t (&lmk)[PPT]) {
         uint32_t safe = 0;
         avail = 0;
         s0 = 0;
@@ -501,15 +514,17 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             lvx[u] = VX[i];
             lvy[u] = VY[i];
             lvz[u] = VZ[i];
-            lst[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, i);
+            if (RM == 1) lmk[u] = MK[i];
+            else lst[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, i);
         }
     };
     float x[PPT], vx[PPT], vy[PPT], vz[PPT];
     Xo st[PPT];
+    uint32_t mk[PPT];
     uint32_t slot0 = 0, avail = 0;
     // workgroup b takes tiles b, b + G, b + 2G, ... (G = grid size)
     uint32_t tile = blockIdx.x;
-    load_tile(tile, slot0, avail, x, vx, vy, vz, st);
+    load_tile(tile, slot0, avail, x, vx, vy, vz, st, mk);
     while (tile < n_tiles) {
         const uint32_t tile_next = tile + gridDim.x;
         bool valid[PPT];
@@ -518,8 +533,19 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             valid[u] = u * blockDim.x < avail;
             if (!valid[u]) x[u] = -1.0f;  // past the segment's count
         }
-        // a live slot's stream catches up with the draws it was spared
-        if (lag) {
+        // a live slot's stream catches up with the draws it was spared; with
+        // the masks read only candidates load their words here (the catch-up
+        // follows the late loads after the push)
+        bool has[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            has[u] = x[u] >= 0.0f;
+            if (RM == 1) {
+                has[u] = has[u] && (lag >= 8u || ((mk[u] >> lag) & 1u));
+                if (has[u]) st[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
+            }
+        }
+        if (RM != 1 && lag) {
 #pragma unroll
             for (int u = 0; u < PPT; ++u)
                 if (x[u] >= 0.0f) xo_skip(st[u], lag);
@@ -554,8 +580,29 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
         for (int u = 0; u < PPT; ++u) sig[u] = j.stot[in[u] ? idx[u] : 0];
         float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
         Xo pst[PPT];
+        uint32_t pmk[PPT];
         uint32_t pslot0, pavail;
-        load_tile(tile_next, pslot0, pavail, px, pvx, pvy, pvz, pst);
+        load_tile(tile_next, pslot0, pavail, px, pvx, pvy, pvz, pst, pmk);
+        // masks read: a live slot that is no candidate loads its words now
+        // if it may collide above the threshold, or if it leaves the gap
+        // behind on its words (they are stored exact); then the loaded
+        // streams catch up
+        float pc[PPT];
+        if (RM == 1) {
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                pc[u] = in[u] ? e_prob(k, sig[u], spd[u]) : 0.0f;
+                if (!has[u] && x[u] >= 0.0f && (in[u] ? pc[u] > thr : lag != 0u)) {
+                    st[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
+                    has[u] = true;
+                }
+            }
+            if (lag) {
+#pragma unroll
+                for (int u = 0; u < PPT; ++u)
+                    if (has[u]) xo_skip(st[u], lag);
+            }
+        }
         // (3) collision tests, write-back, deposits; colliding particles
         // are listed; lost ones leave a hole.  Every slot of the tile writes
         // x and vx back, and on a step that leaves the RNG words exact
@@ -570,7 +617,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             const uint32_t i = slot0 + u * blockDim.x;
             bool hit = false;
             if (in[u]) {
-                hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
+                if (RM == 1) hit = has[u] && draw(st[u]) < pc[u];  // no candidate: no collision
+                else hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
                 deposit_fix(h, nx[u], k, scale);
                 if (MEASURE && ELECTRON && !hit) eepf_lds(aux, ef, nx[u], vx[u], vy[u], vz[u], k);
             }
@@ -579,6 +627,9 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                 VX[i] = vx[u];
                 if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
                     rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                // the next run's masks from the exact words (a colliding
+                // particle's are rewritten after its scattering)
+                if (RM == 2) MK[i] = (uint8_t)(in[u] && !hit ? cand_mask(st[u], 0u, thr) : 0xFFu);
             }
             if (!in[u] && x[u] >= 0.0f) {  // lost in this step
                 const bool pw = nx[u] < 0.0f;
@@ -633,6 +684,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                         else
                             kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
                                              ay[u], az[u], spd[u], s);
+                        if (RM) MK[i] = (uint8_t)cand_mask(s, lag_out, thr);
                         xo_back(s, lag_out);
                         rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
                         VX[i] = w;
@@ -655,7 +707,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             vx[u] = pvx[u];
             vy[u] = pvy[u];
             vz[u] = pvz[u];
-            st[u] = pst[u];
+            if (RM == 1) mk[u] = pmk[u];
+            else st[u] = pst[u];
         }
         tile = tile_next;
         if (++since_flush == cap && tile < n_tiles) {  // uniform over the workgroup
@@ -690,6 +743,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                                  __uint_as_float(q[CF_AX * kCollPerWg]), __uint_as_float(q[CF_AY * kCollPerWg]),
                                  __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]), s);
             }
+            // the rest of the run's masks from the state after the scattering
+            if (RM) MK[slot] = (uint8_t)cand_mask(s, lag_out, thr);
             xo_back(s, lag_out);
             rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
             VX[slot] = w;
@@ -731,11 +786,23 @@ int launch_count_live(const float *x, const uint32_t *cnt, int n_seg, uint32_t s
     return PIC_OK;
 }
 
+template <bool E, bool M, int PPT, int RM>
+static void launch_ip(const FusedArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
+    if (f64) k_inplace<E, M, PPT, double, RM><<<grid, block, lds, s>>>(a);
+    else k_inplace<E, M, PPT, float, RM><<<grid, block, lds, s>>>(a);
+}
+
 template <bool E, bool M, int PPT>
 static void launch_t(const FusedArgs &a, bool f64, bool inplace, int grid, int block, size_t lds, hipStream_t s) {
     if (inplace) {
-        if (f64) k_inplace<E, M, PPT, double><<<grid, block, lds, s>>>(a);
-        else k_inplace<E, M, PPT, float><<<grid, block, lds, s>>>(a);
+        const int rm = a.j.mask_read ? 1 : a.j.mask_write ? 2 : 0;
+        if constexpr (E && !M) {
+            if (rm == 1) launch_ip<E, M, PPT, 1>(a, f64, grid, block, lds, s);
+            else if (rm == 2) launch_ip<E, M, PPT, 2>(a, f64, grid, block, lds, s);
+            else launch_ip<E, M, PPT, 0>(a, f64, grid, block, lds, s);
+        } else {
+            launch_ip<E, M, PPT, 0>(a, f64, grid, block, lds, s);
+        }
     } else {
         if (f64) k_fused<E, M, PPT, double><<<grid, block, lds, s>>>(a);
         else k_fused<E, M, PPT, float><<<grid, block, lds, s>>>(a);
@@ -763,6 +830,13 @@ int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh0, hipStream_
     if (deferred && (!j.electrons || !j.inplace || j.rng_lag >= kMaxRngLag ||
                      (j.rng_next_lag != 0 && j.rng_next_lag != j.rng_lag + 1))) {
         set_error("fused: deferred RNG write-back is for in-place electron steps, next lag 0 or lag + 1");
+        return PIC_ERR_ARG;
+    }
+    if ((j.mask_read || j.mask_write) &&
+        (!j.mask || !j.electrons || !j.inplace || j.eepf || (j.mask_read && j.mask_write) ||
+         (j.mask_read && j.rng_next_lag == 0) || (j.mask_write && j.rng_next_lag != 0) || !(j.coll_thr >= 0.0f))) {
+        set_error("fused: candidate masks are read on deferred in-place electron steps and written on their "
+                  "write-back steps (no measurement)");
         return PIC_ERR_ARG;
     }
     Shape sh = resolve(sh0);

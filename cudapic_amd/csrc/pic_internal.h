@@ -31,6 +31,12 @@ int device_lds_per_cu();     // LDS bytes per compute unit
 int device_lds_per_block();  // LDS bytes one workgroup may allocate
 
 KPhys make_kphys(const pic_params_t &p);
+// Upper bound of an electron's collision probability in one step over the
+// energies of the table (the speed at the top of each energy bin; the
+// table's sigma_tot n_gas scaled by stot_scale): the threshold of the
+// collision-candidate masks.  A particle above it (beyond the table's last
+// bin) loads its RNG words on every step.
+float coll_threshold(const float *stot_e, double stot_scale, const pic_params_t &p);
 
 // Workgroup sizes the persistent particle kernels are built for (their
 // launch bounds allow up to kMaxParticleBlock lanes; the default is 256).
@@ -242,6 +248,19 @@ struct FusedJob {
     // RNG words; a colliding one and a newborn store their state rewound by
     // rng_next_lag draws.
     uint32_t rng_lag, rng_next_lag;
+    // Collision-candidate masks of a deferred run (in-place electron steps,
+    // not on measurement steps): one byte per slot, bit L set when the
+    // slot's draw at entry lag L of the run may fall below coll_thr (an
+    // upper bound of the collision probability; a step whose probability
+    // exceeds it loads the words regardless).  mask_read: the run's masks
+    // are valid, and a live slot whose bit for this step's lag is clear
+    // neither loads nor advances its RNG words.  mask_write (rng_next_lag
+    // == 0): every slot writes the masks of the next run from its exact
+    // words.  Colliding particles rewrite their masks from their new state,
+    // newborn electrons get all bits set.
+    uint8_t *mask;
+    bool mask_read, mask_write;
+    float coll_thr;
 };
 int launch_fused(const FusedJob &j, const KPhys &k, const Shape &sh, hipStream_t s);
 // Largest lag rng_lag / rng_next_lag may take.

@@ -100,6 +100,19 @@ KPhys make_kphys(const pic_params_t &p) {
     return k;
 }
 
+float coll_threshold(const float *stot_e, double stot_scale, const pic_params_t &p) {
+    const double e = 1.60217662e-19, me = 9.10938356e-31;
+    double pmax = 0.0;
+    for (int b = 0; b < PIC_CS_RANGES; ++b) {
+        const double v = std::sqrt(2.0 * e * ((double)b + 0.5) * (double)p.de_cs_f / me);
+        pmax = std::max(pmax, -std::expm1(-(double)stot_e[b] * stot_scale * v * (double)p.dt_e_f));
+    }
+    // a margin over the f32 rounding of the kernel's probability; the
+    // kernel compares its own probability with the threshold, so any value
+    // is correct and only the share of candidates depends on it
+    return (float)std::min(1.0, pmax * (1.0 + 1e-4) + 1e-7);
+}
+
 // LDS budget (bytes) one particle workgroup may take for its histograms
 // so that several workgroups stay resident per compute unit.
 static constexpr int kLdsBudget = 48 * 1024;

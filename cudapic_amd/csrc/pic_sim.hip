@@ -100,6 +100,13 @@ struct pic_sim {
     // the next step (0 between calls)
     uint32_t rng_defer = 8;
     uint32_t e_lag = 0;
+    // collision-candidate masks of the deferred runs (a byte per electron
+    // slot, null when off): mask_ok while the masks hold the current run's
+    // candidates (written by the last write-back step, cleared by anything
+    // else that changes the electrons or their tables)
+    uint8_t *e_mask = nullptr;
+    bool mask_ok = false;
+    float coll_thr = 1.0f;
     // captured runs of steps inside one 4*N_SUB-step period (whole periods,
     // and the partial ones a timed window starts or ends with), keyed by the
     // measure flag, whether the run starts with a compaction, the roles at
@@ -108,8 +115,9 @@ struct pic_sim {
     static constexpr int kGraphSlots = 16;
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
-        int key[9] = {};
+        int key[10] = {};
         int after[5] = {};
+        bool mask_after = false;
     } graphs[kGraphSlots];
     int graph_next = 0;  // slot the next capture replaces when all are taken
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -278,6 +286,7 @@ uint64_t dens_words32(pic_sim *s) { return (uint64_t)s->dens_words * (s->f64 ? 2
 int fused_prologue(pic_sim *s) {
     hipStream_t st = s->stream;
     Shape sh = s->sh;
+    s->mask_ok = false;  // the particles changed under the masks
     TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
     TRY(launch_fill(s->e_dens[1 - s->cd], 0u, dens_words32(s), st));  // the first step deposits here
     TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
@@ -327,6 +336,16 @@ int step_fused(pic_sim *s, bool measure, bool last, bool time_kernel, hipEvent_t
     if (comp_e && lag) {
         set_error("pic_sim: internal: a compacting step starts behind on RNG words");
         return PIC_ERR_ARG;
+    }
+    // candidate masks: read inside a run whose masks are valid, written by a
+    // write-back step whose next run has steps that can use them (the next
+    // step neither compacts nor writes back)
+    const bool masks_on = s->e_mask && !comp_e && !measure;
+    const bool mask_read = masks_on && lag_out != 0u && s->mask_ok;
+    bool mask_write = false;
+    if (masks_on && lag_out == 0u) {
+        const uint32_t t1 = (t + 1u) % (uint32_t)s->p.n_t;
+        mask_write = t1 % s->compact_every != 0 && (t1 + 1u) % s->rng_defer != 0 && t1 + 1u != (uint32_t)s->p.n_t;
     }
     // Poisson of this step: e density accumulated by the previous step,
     // ion density = the held one, or on ion steps the accumulated one
@@ -472,10 +491,17 @@ int step_fused(pic_sim *s, bool measure, bool last, bool time_kernel, hipEvent_t
     fe.zero_words = (uint32_t)dens_words32(s);
     fe.rng_lag = lag;
     fe.rng_next_lag = lag_out;
+    fe.mask = s->e_mask;
+    fe.mask_read = mask_read;
+    fe.mask_write = mask_write;
+    fe.coll_thr = s->coll_thr;
     if (time_kernel) PIC_HIP(hipEventRecord(e0, st));
     TRY(launch_fused(fe, s->k, s->sh, st));
     if (time_kernel) PIC_HIP(hipEventRecord(e1, st));
     s->e_lag = lag_out;
+    // a step that leaves a run's masks stale (a compaction, a measured step,
+    // a step inside an unmasked run) leaves them off until the next write
+    s->mask_ok = lag_out == 0u ? mask_write : mask_read;
     if (comp_e) s->ce ^= 1;
     s->pe ^= 1;
     s->cd ^= 1;
@@ -528,12 +554,13 @@ void set_roles(pic_sim *s, const int *r) {
 // the call).
 int graph_for(pic_sim *s, bool measure, uint32_t len, bool capture, pic_sim::GraphSlot **out) {
     const uint32_t period = 4u * (uint32_t)s->p.n_sub;
-    int key[9];
+    int key[10];
     key[0] = measure ? 1 : 0;
     key[1] = (s->t % s->compact_every) == 0 ? 1 : 0;
     get_roles(s, key + 2);
     key[7] = (int)(s->t % period);
     key[8] = (int)len;
+    key[9] = s->mask_ok ? 1 : 0;  // the run's first steps read the masks
     *out = nullptr;
     for (auto &g : s->graphs)
         if (g.exec && memcmp(g.key, key, sizeof(key)) == 0) {
@@ -556,9 +583,11 @@ int graph_for(pic_sim *s, bool measure, uint32_t len, bool capture, pic_sim::Gra
     // restore the host mirror (a run starts and ends with exact RNG words)
     int after[5];
     get_roles(s, after);
+    const bool mask_after = s->mask_ok;
     s->t = t0;
     s->steps = steps0;
     s->e_lag = 0;
+    s->mask_ok = key[9] != 0;
     set_roles(s, key + 2);
     if (rc != PIC_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -583,6 +612,7 @@ int graph_for(pic_sim *s, bool measure, uint32_t len, bool capture, pic_sim::Gra
     slot->exec = ex;
     memcpy(slot->key, key, sizeof(key));
     memcpy(slot->after, after, sizeof(after));
+    slot->mask_after = mask_after;
     *out = slot;
     return PIC_OK;
 }
@@ -602,6 +632,7 @@ int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
     get_roles(s, roles0);
     const uint32_t t0 = s->t;
     const unsigned long long steps0 = s->steps;
+    const bool mask0 = s->mask_ok;
     int rc = PIC_OK;
     while (n > 0 && rc == PIC_OK) {
         if (graphs && s->e_lag == 0) {
@@ -617,6 +648,7 @@ int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
                     }
                 }
                 set_roles(s, g->after);
+                s->mask_ok = g->mask_after;
                 s->t = (s->t + len) % (uint32_t)s->p.n_t;
                 s->steps += len;
                 n -= len;
@@ -632,6 +664,7 @@ int run_steps(pic_sim *s, uint32_t n, bool measure, bool dry = false) {
         s->t = t0;
         s->steps = steps0;
         s->e_lag = 0;
+        s->mask_ok = mask0;
     }
     return rc;
 }
@@ -650,6 +683,7 @@ int free_all(pic_sim *s) {
     (void)hipFree(s->xt);
     (void)hipFree(s->xt_count);
     (void)hipFree(s->xt_stage);
+    (void)hipFree(s->e_mask);
     void *bufs[] = {s->cnt, s->pot, s->efield, s->grid_e, s->grid_i, s->pot0_tab, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->cumul_e, s->cumul_i, s->eepf, s->ifed, s->counters, s->cs, s->stot_e, s->stot_i};
     for (void *b : bufs) (void)hipFree(b);
@@ -709,6 +743,11 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         if (s->rng_defer > kMaxRngLag || period % s->rng_defer) {
             delete s;
             set_error("pic_sim_create: rng_defer must divide 4*n_sub and be at most 64");
+            return PIC_ERR_ARG;
+        }
+        if (c->cand_masks < -1 || c->cand_masks > 1 || !(c->cand_threshold >= 0.0f)) {
+            delete s;
+            set_error("pic_sim_create: cand_masks must be -1 (off), 0 or 1 (on), cand_threshold >= 0");
             return PIC_ERR_ARG;
         }
     }
@@ -792,6 +831,7 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
     if ((rc = dalloc(&s->cs, (size_t)PIC_N_CS * PIC_CS_RANGES))) return fail(rc);
     if ((rc = dalloc(&s->stot_e, PIC_CS_RANGES))) return fail(rc);
     if ((rc = dalloc(&s->stot_i, PIC_CS_RANGES))) return fail(rc);
+    if (c->path == PIC_PATH_FUSED && c->cand_masks >= 0 && (rc = dalloc(&s->e_mask, s->slots))) return fail(rc);
     for (int i = 0; i < 2; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) return fail(hip_fail(hipGetLastError(), "hipEventCreate"));
     // tables (main.rs:1649, 1663) and RNG streams (main.rs:1666-1675)
@@ -822,6 +862,8 @@ int pic_sim_set_tables(pic_sim_t *s, const float *cs, const float *stot_e, const
     PIC_HIP(hipMemcpy(s->cs, cs, sizeof(float) * PIC_N_CS * PIC_CS_RANGES, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(s->stot_e, stot_e, sizeof(float) * PIC_CS_RANGES, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(s->stot_i, stot_i, sizeof(float) * PIC_CS_RANGES, hipMemcpyHostToDevice));
+    s->coll_thr = s->cfg.cand_threshold > 0.0f ? s->cfg.cand_threshold : coll_threshold(stot_e, 1.0, s->p);
+    s->mask_ok = false;
     return PIC_OK;
 }
 
@@ -915,6 +957,7 @@ int pic_sim_rng_upload(pic_sim_t *s, int species_id, const uint32_t *w0, const u
     uint32_t **r = species_id == PIC_ELECTRONS ? s->e_rng : s->i_rng;
     const uint32_t *src[4] = {w0, w1, w2, w3};
     for (int q = 0; q < 4; ++q) PIC_HIP(hipMemcpy(r[q], src[q], 4ull * n, hipMemcpyHostToDevice));
+    if (species_id == PIC_ELECTRONS) s->mask_ok = false;
     return PIC_OK;
 }
 

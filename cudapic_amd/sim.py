@@ -13,7 +13,8 @@ from ._abi import (PIC_ELECTRONS, PIC_IONS, PIC_N_EEPF, PIC_N_IFED, PIC_PATH_FUS
 class Sim:
     def __init__(self, p, capacity=120_000, path=PIC_PATH_FUSED, use_graph=True, seed_e=None,
                  seed_i=None, segments=0, block_size=0, grid_blocks=0, ppt=0, replicas=0,
-                 ppt_inplace=0, lds_replicas=0, compact_every=0, rng_defer=0):
+                 ppt_inplace=0, lds_replicas=0, compact_every=0, rng_defer=0, cand_masks=0,
+                 cand_threshold=0.0):
         cfg = PicSimConfig()
         lib().pic_sim_config_default(C.byref(cfg))
         cfg.params = p
@@ -30,6 +31,8 @@ class Sim:
         cfg.lds_replicas = lds_replicas
         cfg.compact_every = int(compact_every)
         cfg.rng_defer = int(rng_defer)
+        cfg.cand_masks = int(cand_masks)
+        cfg.cand_threshold = float(cand_threshold)
         self.p = p
         self.cfg = cfg
         self.capacity = int(capacity)

@@ -230,6 +230,18 @@ typedef struct pic_sim_config {
     uint32_t rng_defer;        /* fused path: steps per run of deferred electron
                                   RNG write-backs (as pic_step_config_t; the
                                   words are exact whenever a call returns) */
+    int32_t cand_masks;        /* fused path: collision-candidate masks of the
+                                  deferred runs (a byte per electron slot: the
+                                  steps of the run whose draw can fall below
+                                  the largest collision probability; the other
+                                  steps of a live slot read no RNG words):
+                                  0 or 1 = on, -1 = off.  The draws and their
+                                  outcomes are the same either way. */
+    float cand_threshold;      /* the masks' probability threshold: 0 = the
+                                  largest collision probability of the tables'
+                                  energies; a lower value makes particles above
+                                  it load their words on every step, a higher
+                                  one makes more steps candidates */
 } pic_sim_config_t;
 
 typedef struct pic_sim pic_sim_t;

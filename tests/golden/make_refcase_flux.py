@@ -13,12 +13,13 @@ eduPIC resets its counters in every process, so each run's info.txt is one
 batch.  eduPIC seeds its Mersenne twister from std::random_device
 (C/eduPIC.cc:164-165), so the chains are independent.
 
-    python tests/golden/make_refcase_flux.py [--chains 6] [--workdir DIR]
+    python tests/golden/make_refcase_flux.py [--chains 6] [--workdir DIR] [--append]
 
 Writes tests/golden/refcase_flux.json: per quantity the batch values (run
 by run: the first BATCHES belong to the first chain), and their mean and
 standard error over all batches.  Takes ~45 min on 6 cores of a 2.7 GHz
-Xeon.
+Xeon.  --append adds the new chains to the chains already in the output file
+(the fixture holds 14: 6 made first, 8 appended).
 """
 from __future__ import annotations
 
@@ -81,6 +82,7 @@ def main():
     ap.add_argument("--chains", type=int, default=6)
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--out", default=os.path.join(HERE, "refcase_flux.json"))
+    ap.add_argument("--append", action="store_true", help="add the chains to those already in --out")
     a = ap.parse_args()
     binary = os.path.join(ROOT, "oracle", "_ref", "edupic")
     if not os.path.exists(binary):
@@ -102,10 +104,21 @@ def main():
     batches = [b for r in results for b in r]
     if len(batches) != a.chains * BATCHES:
         raise SystemExit(f"only {len(batches)} of {a.chains * BATCHES} batches finished")
+    chains = a.chains
+    if a.append:
+        with open(a.out) as f:
+            old = json.load(f)
+        if old["relax_cycles"] != RELAX or old["cycles_per_batch"] != CYCLES or \
+                old["batches"] != old["chains"] * BATCHES:
+            raise SystemExit(f"{a.out} was made with other run lengths")
+        prev = [dict(zip(LABELS.values(), vals))
+                for vals in zip(*(old["quantities"][k]["batches"] for k in LABELS.values()))]
+        batches = prev + batches
+        chains += old["chains"]
     out = {
         "source": "eduPIC 1.0 (C/eduPIC.cc) built with g++ -O3 -march=x86-64-v3, info.txt of each run",
         "start": "cudapic_amd/data/refcase_state.npz as picdata.bin",
-        "relax_cycles": RELAX, "cycles_per_batch": CYCLES, "batches": len(batches), "chains": a.chains,
+        "relax_cycles": RELAX, "cycles_per_batch": CYCLES, "batches": len(batches), "chains": chains,
         "quantities": {},
     }
     for key in LABELS.values():

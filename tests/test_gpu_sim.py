@@ -339,6 +339,60 @@ def test_deferred_rng_write_back_equals_every_step(ca, orc):
         assert differ.sum() <= max(2, n // 10_000), (differ.sum(), n)
 
 
+def test_candidate_masks_equal_plain_runs(ca, orc):
+    """Collision-candidate masks (a live slot reads no RNG words on a step
+    of a deferred run whose draw cannot fall below the masks' threshold)
+    change no draw and no outcome.  Without ionisation the slot contents are
+    deterministic, so the simulation object with the masks at the default
+    threshold, at a threshold below most electrons' collision probability
+    (nearly every in-gap electron loads its words late, after the push), at
+    a threshold that makes a third of the steps candidates, and with the
+    masks off must leave every slot -- particles and RNG words -- as a run
+    that writes the words back every step, after eager steps, captured
+    periods, measured steps (which run without masks) and timed windows (up
+    to 1e-4 of the slots, for the last bit the density flush's
+    floating-point atomics may move in an f32 field value)."""
+    p = ca.params(weight=2.0e3, precision=2)
+    po = orc.params(weight=2.0e3)
+    E, I = synthetic_state(200_000, 210_000, seed=97, te_ev=6.0)
+    tables = no_ionisation_tables(orc, po)
+    out = []
+    variants = ((1, -1, 0.0), (8, 0, 0.0), (8, 1, 0.002), (8, 1, 0.05), (8, -1, 0.0))
+    for defer, masks, thr in variants:
+        s = ca.Sim(p, capacity=300_000, rng_defer=defer, cand_masks=masks, cand_threshold=thr)
+        try:
+            s.set_tables(*tables)
+            s.set_time(3)
+            s.upload(0, *E)
+            s.upload(1, *I)
+            print(f"pic_sim rng_defer={defer} cand_masks={masks} threshold={thr}: 330 steps", flush=True)
+            s.run(170)
+            s.run(45, measure=True)
+            s.run(51)
+            s.time_steps(27)
+            s.time_steps(37, kernel_events=True)
+            s.synchronize()
+            s.check()
+            out.append((s.counts(), s.download(0) + s.rng_download(0), s.download(1) + s.rng_download(1),
+                        s.measurements()["counters"]))
+        finally:
+            s.close()
+    nb, eb, ib, cb = out[0]
+    assert cb["e_collisions"] > 50_000 and cb["i_collisions"] > 0, cb
+    for (na, ea, ia, ca_), v in zip(out[1:], variants[1:]):
+        assert na == nb, v
+        assert abs(int(ca_["e_collisions"]) - int(cb["e_collisions"])) <= max(2, cb["e_collisions"] // 10_000), \
+            (v, ca_, cb)
+        for a, b in ((ea, eb), (ia, ib)):
+            n = len(a[0])
+            assert [len(u) for u in a] == [len(w) for w in b]
+            differ = np.zeros(len(a[4]), bool)
+            for w in range(4):
+                differ[:n] |= np.asarray(a[w]).view(np.uint32) != np.asarray(b[w]).view(np.uint32)
+                differ |= a[4 + w] != b[4 + w]
+            assert differ.sum() <= max(2, n // 10_000), (v, differ.sum(), n)
+
+
 def test_checkpoint_round_trip(ca, orc, tmp_path):
     p = ca.params(weight=2.0e3)
     E, I = synthetic_state(50_000, 52_000, seed=61)
