@@ -83,6 +83,23 @@ __device__ __forceinline__ void flush_fix(const uint32_t *hist, int lds_reps, in
     }
 }
 
+// flush_fix by the threads t0, t0 + stride, ... (t = threadIdx.x - first):
+// part of the workgroup flushes while the rest does other work.
+template <typename Acc>
+__device__ __forceinline__ void flush_fix_part(const uint32_t *hist, int lds_reps, int n_g, float scale, float wf,
+                                               Acc *out, int t, int stride) {
+    const Acc unit = (Acc)wf / (Acc)scale;
+    for (int q = t; q < n_g; q += stride) {
+        uint64_t sum = 0;
+        for (int r = 0; r < lds_reps; ++r) sum += hist[r * n_g + q];
+        if (sum) {
+            Acc v = (Acc)sum * unit;
+            if (q == 0 || q == n_g - 1) v *= (Acc)2;
+            unsafeAtomicAdd(&out[q], v);
+        }
+    }
+}
+
 // This workgroup's share of a buffer the particle kernels clear while they
 // run.
 __device__ __forceinline__ void clear_slice(uint32_t *buf, uint32_t words) {
@@ -90,6 +107,29 @@ __device__ __forceinline__ void clear_slice(uint32_t *buf, uint32_t words) {
     const uint32_t per = (words + gridDim.x - 1) / gridDim.x;
     const uint32_t w0 = blockIdx.x * per, w1 = min(words, w0 + per);
     for (uint32_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) buf[w] = 0u;
+}
+
+// Live count of every input segment (clamped: a counter that overflowed
+// its segment counts dropped particles) into s_cnt[K] and the inclusive
+// prefix of their tile counts into s_tiles[0..K] (s_tiles[0] = 0), by wave 0
+// in registers (K <= 64): one memory latency and a few lane shuffles instead
+// of a serial loop over LDS.  The caller's barrier publishes them.
+__device__ __forceinline__ void segment_tiles(const uint32_t *cnt_in, int K, uint32_t seg_size, uint32_t tile_sz,
+                                              uint32_t *s_cnt, uint32_t *s_tiles) {
+    if (threadIdx.x >= (unsigned)kWave) return;
+    const int q = (int)threadIdx.x;
+    const uint32_t c = q < K ? min(cnt_in[q], seg_size) : 0u;
+    uint32_t incl = (c + tile_sz - 1) / tile_sz;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, kWave);
+        if (q >= off) incl += v;
+    }
+    if (q < K) {
+        s_cnt[q] = c;
+        s_tiles[q + 1] = incl;
+    }
+    if (q == 0) s_tiles[0] = 0u;
 }
 
 // Wall losses and collisions of the workgroup: summed in LDS, then one
@@ -180,13 +220,14 @@ struct Appends {
 };
 
 // Apply an electron collision's products: the new electron (into the LDS
-// density and EEPF) and the new ion (into the global ion density).
-// Wave-aggregated slot reservation; `kind` per lane.
+// density, or the global one when e_dens_g is given, and the EEPF) and the
+// new ion (into the global ion density).  Wave-aggregated slot reservation;
+// `kind` per lane.
 template <bool MEASURE, typename Acc>
 __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhys &k, int kind, float px,
                                                     const float *se, const float *si, uint32_t *h, float scale,
                                                     Acc *i_dens_g, uint32_t *eepf_h, const float *ef,
-                                                    bool &overflow) {
+                                                    bool &overflow, Acc *e_dens_g = nullptr) {
     const uint64_t sp = ballot64(kind == kIonisation);
     if (!sp) return;
     const int leader = __builtin_ctzll(sp);
@@ -213,7 +254,8 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
             rng_store(a.er0, a.er1, a.er2, a.er3, es, r);
         }
         if (a.e_mask) a.e_mask[es] = 0xFFu;  // loads its words on every step of the run
-        deposit_fix(h, px, k, scale);
+        if (e_dens_g) deposit_global<Acc>(e_dens_g, px, k);
+        else deposit_fix(h, px, k, scale);
         if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
     } else {
         overflow = true;
@@ -252,26 +294,11 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
     uint32_t *aux = hist + A.lds_reps * n_g;  // EEPF (electrons) or IFED (ions)
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     const uint32_t tile_sz = blockDim.x * PPT;
+    segment_tiles(j.cnt_in, K, j.seg_size, tile_sz, s_cnt, s_tiles);
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
     clear_slice(j.zero, j.zero_words);
-    // live count of every input segment (clamped: a counter that overflowed
-    // its segment counts dropped particles) and its tile count
-    for (int q = threadIdx.x; q < K; q += blockDim.x) {
-        const uint32_t c = min(j.cnt_in[q], j.seg_size);
-        s_cnt[q] = c;
-        s_tiles[q + 1] = (c + tile_sz - 1) / tile_sz;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        s_tiles[0] = 0;
-        for (int q = 0; q < K; ++q) {
-            acc += s_tiles[q + 1];
-            s_tiles[q + 1] = acc;
-        }
-    }
     __syncthreads();
     const int wave = threadIdx.x / kWave;
     uint32_t *h = hist + (wave % A.lds_reps) * n_g;
@@ -434,25 +461,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     uint32_t *list = aux + aux_n;  // [NF][kCollPerWg]
     const uint32_t tile_sz = blockDim.x * PPT;
+    segment_tiles(j.cnt_in, K, j.seg_size, tile_sz, s_cnt, s_tiles);
     for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
     clear_slice(j.zero, j.zero_words);
-    for (int q = threadIdx.x; q < K; q += blockDim.x) {
-        const uint32_t c = min(j.cnt_in[q], j.seg_size);
-        s_cnt[q] = c;
-        s_tiles[q + 1] = (c + tile_sz - 1) / tile_sz;
-    }
     if (threadIdx.x == 0) s_ncoll = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        s_tiles[0] = 0;
-        for (int q = 0; q < K; ++q) {
-            acc += s_tiles[q + 1];
-            s_tiles[q + 1] = acc;
-        }
-    }
     __syncthreads();
     const int wave = threadIdx.x / kWave;
     uint32_t *h = hist + (wave % A.lds_reps) * n_g;
@@ -720,9 +734,19 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             __syncthreads();
         }
     }
-    // (4) scatter the listed collisions, 64 per wave pass
+    // (4) scatter the listed collisions, 64 per wave pass.  When they fit
+    // the first waves, the other waves flush the density histograms
+    // meanwhile and the scattering waves deposit newborn electrons straight
+    // into the global density replica.
     __syncthreads();
     const uint32_t n_list = min(s_ncoll, kCollPerWg);
+    const uint32_t n_waves = blockDim.x / kWave;
+    const uint32_t scat_waves = min(n_waves, (n_list + kWave - 1) / kWave);
+    const bool early_flush = scat_waves < n_waves && n_tiles > blockIdx.x;  // uniform
+    Acc *const dens_g = (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride;
+    if (early_flush && (uint32_t)wave >= scat_waves)
+        flush_fix_part<Acc>(hist, A.lds_reps, n_g, scale, k.wf, dens_g, (int)(threadIdx.x - scat_waves * kWave),
+                            (int)(blockDim.x - scat_waves * kWave));
     for (uint32_t b0 = (uint32_t)wave * kWave; b0 < n_list; b0 += blockDim.x) {
         const uint32_t e = b0 + lane_id();
         int kind = kNone;
@@ -753,13 +777,14 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             ++ncoll;
         }
         if (ELECTRON)
-            ionisation_products<MEASURE, Acc>(app, k, kind, px, se, si, h, scale, i_dens_g, aux, ef, overflow);
+            ionisation_products<MEASURE, Acc>(app, k, kind, px, se, si, h, scale, i_dens_g, aux, ef, overflow,
+                                              early_flush ? dens_g : nullptr);
     }
     add_diagnostics<ELECTRON>(
         j.counters ? j.counters + (size_t)(blockIdx.x % kCounterLanes) * kCounterWords : nullptr, j.overflow,
         lost_pow, lost_gnd, ncoll, overflow);
     if (!__syncthreads_or(n_tiles > blockIdx.x)) return;
-    flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, (Acc *)j.dens + (blockIdx.x % A.reps) * A.rep_stride);
+    if (!early_flush) flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf, dens_g);
     uint32_t *aux_out = ELECTRON ? j.eepf : j.ifed;
     if (aux_out)
         for (int q2 = threadIdx.x; q2 < aux_n; q2 += blockDim.x)
