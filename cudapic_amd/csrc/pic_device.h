@@ -20,6 +20,7 @@ constexpr float kPiF = 3.14159265358979f;
 constexpr float kTwoPiF = 6.28318530717959f;
 constexpr float kHalfPiF = 1.57079632679490f;
 constexpr float kLn2F = 0.693147180559945f;
+constexpr float kLog2eF = 1.4426950408889634f;  // log2(e) (main.rs:97)
 
 // Physics factors the particle kernels need, all in f32 as the reference
 // casts them (Rust-CUDA/src/main.rs:80-109).
@@ -227,12 +228,19 @@ __device__ __forceinline__ int e_index(const KPhys &k, float vx, float vy, float
     return energy_index(energy, k);
 }
 
+// 1 - exp(-a) as the reference evaluates it (main.rs:1012, 779-781): the
+// exponential as the hardware base-2 exponential of a * log2(e) (v_exp_f32,
+// the counterpart of the reference's ex2.approx.f32).
+__device__ __forceinline__ float one_minus_exp_neg(float a) {
+    return 1.0f - __builtin_amdgcn_exp2f(-a * kLog2eF);
+}
+
 // Collision probability of an electron in one step, 1 - exp(-nu dt_e) with
 // nu = sigma_tot n_gas |v| (main.rs:1007-1012); e_test compares the step's
 // draw with it.
 __device__ __forceinline__ float e_prob(const KPhys &k, float stot, float velocity) {
     const float nu = stot * k.stot_scale * velocity;
-    return 1.0f - expf(-nu * k.dt_e);
+    return one_minus_exp_neg(nu * k.dt_e);
 }
 
 __device__ __forceinline__ bool e_test(const KPhys &k, float stot, float velocity, Xo &s) {
@@ -330,7 +338,7 @@ __device__ __forceinline__ int i_index(const KPhys &k, float vx, float vy, float
 __device__ __forceinline__ bool i_test(const KPhys &k, float stot, float g, Xo &s) {
     const float nu = stot * k.stot_scale * g;
     const float rv = draw(s);
-    const float pc = 1.0f - expf(-nu * k.dt_i);
+    const float pc = one_minus_exp_neg(nu * k.dt_i);
     return rv < pc;
 }
 
