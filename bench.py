@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--lds-replicas", type=int, default=0)
     ap.add_argument("--rng-defer", type=int, default=0,
                     help="steps per run of deferred electron RNG write-backs (0 = 8; 1 = every step)")
+    ap.add_argument("--cand-masks", type=int, default=0, choices=(-1, 0, 1),
+                    help="collision-candidate masks of the deferred runs (0 or 1 = on, -1 = off)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick grid, particles per thread and replication with cudapic_amd.autotune "
                          "(cached per device and workload; the search runs before the warm-up)")
@@ -236,7 +238,7 @@ def main():
     print(f"bench: start: {start}, {len(E[0])} e + {len(I[0])} i", file=sys.stderr, flush=True)
     shape = dict(grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas, segments=args.segments,
                  ppt_inplace=args.ppt_inplace, lds_replicas=args.lds_replicas, block_size=args.block_size,
-                 rng_defer=args.rng_defer)
+                 rng_defer=args.rng_defer, cand_masks=args.cand_masks)
     tuned = None
     if args.autotune and path == ca.PIC_PATH_FUSED:
         from cudapic_amd import autotune
@@ -247,6 +249,7 @@ def main():
     sim.upload(ca.PIC_ELECTRONS, *E)
     sim.upload(ca.PIC_IONS, *I)
     del E, I
+    sim_thr = sim.cand_threshold()
     sim.run(args.warmup)
     sim.synchronize()
     sim.check()
@@ -282,12 +285,22 @@ def main():
                               "push+collide+deposit, timed with HIP events on the sim stream",
                     "kernel_avg_us": round(avg_ms * 1e3, 2), "electrons_per_launch": int(units),
                     "bytes_per_unit": ALG_BYTES}
-        # what the in-place electron kernel must move per electron: read x,
-        # v and the RNG words (32 B), write x, vx (8 B) and the RNG words
-        # once per run of deferred write-backs
+        # what the in-place electron kernel must move per electron: read x
+        # and v, write x and vx (24 B); the RNG words read and written once
+        # per run of deferred write-backs (32 B / R) and, with the
+        # collision-candidate masks, a mask byte read on the run's other
+        # steps, written once, and the words read only by the candidates
+        # (a share thr of those steps); without masks the words are read
+        # every step (40 + 16 / R B)
         defer = shape.get("rng_defer") or 8
-        compulsory = 40.0 + 16.0 / defer
+        thr, masks_on = sim_thr
+        if masks_on and defer > 1:
+            compulsory = 24.0 + 33.0 / defer + (defer - 1) / defer * (1.0 + 16.0 * thr)
+        else:
+            compulsory = 40.0 + 16.0 / defer
         roofline["rng_defer"] = defer
+        roofline["cand_masks"] = masks_on
+        roofline["cand_threshold"] = round(thr, 5)
         roofline["compulsory_bytes_per_unit"] = round(compulsory, 2)
         roofline["compulsory_frac"] = round(compulsory * units / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     # the same discharge continued for long_cycles RF cycles in one timed

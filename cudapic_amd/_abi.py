@@ -140,6 +140,7 @@ SIGNATURES = {
     "pic_sim_rng_upload": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_rng_download": (I32, [P, I32, P, P, P, P, U32]),
     "pic_sim_set_tables": (I32, [P, P, P, P]),
+    "pic_sim_cand_threshold": (I32, [P, P, P]),
     "pic_sim_counts": (I32, [P, C.POINTER(U32), C.POINTER(U32)]),
     "pic_sim_time": (I32, [P, C.POINTER(U32), C.POINTER(C.c_ulonglong)]),
     "pic_sim_set_time": (I32, [P, U32]),

@@ -867,6 +867,13 @@ int pic_sim_set_tables(pic_sim_t *s, const float *cs, const float *stot_e, const
     return PIC_OK;
 }
 
+int pic_sim_cand_threshold(pic_sim_t *s, float *threshold, int *masks_on) {
+    if (!s) return PIC_ERR_ARG;
+    if (threshold) *threshold = s->coll_thr;
+    if (masks_on) *masks_on = s->e_mask != nullptr;
+    return PIC_OK;
+}
+
 int pic_sim_upload(pic_sim_t *s, int species_id, const float *x, const float *vx, const float *vy,
                    const float *vz, uint32_t n) {
     if (!s || (species_id != PIC_ELECTRONS && species_id != PIC_IONS) || n > s->cap ||

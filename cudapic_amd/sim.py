@@ -84,6 +84,12 @@ class Sim:
         a = [np.ascontiguousarray(v, np.float32) for v in (cs, stot_e, stot_i)]
         call("pic_sim_set_tables", self.h, *(v.ctypes.data for v in a))
 
+    def cand_threshold(self):
+        """(threshold, masks on) of the collision-candidate masks."""
+        t, on = C.c_float(), C.c_int()
+        call("pic_sim_cand_threshold", self.h, C.byref(t), C.byref(on))
+        return t.value, bool(on.value)
+
     def counts(self):
         ne, ni = C.c_uint32(), C.c_uint32()
         call("pic_sim_counts", self.h, C.byref(ne), C.byref(ni))

@@ -276,6 +276,11 @@ int pic_sim_rng_download(pic_sim_t *s, int species, uint32_t *w0, uint32_t *w1,
                          uint32_t *w2, uint32_t *w3, uint32_t n);
 /* Replace the cross-section tables (host arrays as pic_cross_sections). */
 int pic_sim_set_tables(pic_sim_t *s, const float *cs, const float *stot_e, const float *stot_i);
+/* The collision-candidate masks' probability threshold in use (the
+ * configured one, or the largest collision probability of the current
+ * tables' energies) and whether the masks are on (no counterpart in the
+ * reference: its collision test reads every slot's words every step). */
+int pic_sim_cand_threshold(pic_sim_t *s, float *threshold, int *masks_on);
 
 /* Live counts (synchronises the sim stream; the reference's checkpoint read,
  * main.rs:1810-1816). */
