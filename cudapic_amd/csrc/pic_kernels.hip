@@ -604,22 +604,59 @@ __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     __shared__ int s_ion;
     __shared__ uint32_t s_t;
     const int tid = threadIdx.x;
-    // the clock, the step's pot0, the counters to copy and the densities are
-    // loaded together
+    // the densities, the clock, the step's pot0 and the counters to copy are
+    // all requested before any of them is waited for.  One node per lane and
+    // one group of eight replicas (the default) is loaded here; other shapes
+    // go through load_node_densities.  The clock and pot0 are vector loads:
+    // as scalar-cache loads the compiler would wait for each in turn.
+    const bool one_group = a.g.per == 1 && a.g.reps == 8;
+    const bool my_node = one_group && tid < a.g.n_g;
+    D ve[8], vi[8];
+    double is = 0.0;
+    if (my_node) {
+        const D *ed = (const D *)a.e_dens, *id = (const D *)a.i_dens;
+        const int st = a.g.rep_stride;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ve[j] = ed[j * st + tid];
+        if (a.i_sum) {
+            is = a.i_sum[tid];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) vi[j] = id[j * st + tid];
+        }
+    }
     uint32_t t = 0;
     double p0_in = 0.0;
     if (tid == 0 && a.t_value >= 0) {
         t = (uint32_t)a.t_value;
-        if (a.pot0_tab) p0_in = a.pot0_tab[t];
+        if (a.pot0_tab) p0_in = __hip_atomic_load(a.pot0_tab + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (tid == 0 && a.time) {
-        t = *a.time;
-        if (a.pot0_in) p0_in = *a.pot0_in;
+        t = __hip_atomic_load(a.time, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a.pot0_in) p0_in = __hip_atomic_load(a.pot0_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     uint32_t cnt_copy = 0, cnt_copy_ion = 0;
     if (a.copy_cnt_dst && tid < a.copy_cnt_words) cnt_copy = a.copy_cnt_src[tid];
-    if (a.copy_cnt_ion_dst && tid == 0) cnt_copy_ion = *a.copy_cnt_ion_src;
+    if (a.copy_cnt_ion_dst && tid == 0)
+        cnt_copy_ion = __hip_atomic_load(a.copy_cnt_ion_src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double dne[kPoissonMaxPer], dni[kPoissonMaxPer];
-    load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, a.i_sum, dne, dni);
+    if (one_group) {
+        // sum_replicas' order
+#pragma unroll
+        for (int u = 0; u < kPoissonMaxPer; ++u) dne[u] = dni[u] = 0.0;
+        if (my_node) {
+            double se = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) se += (double)ve[j];
+            if (!a.i_sum) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) is += (double)vi[j];
+            }
+            dne[0] = se;
+            dni[0] = is;
+        }
+    } else {
+        load_node_densities<D>(a.g, (const D *)a.e_dens, (const D *)a.i_dens, a.i_sum, dne, dni);
+    }
     double p0_next = 0.0;
     if (tid == 0) {
         double p0 = a.pot0;
