@@ -272,6 +272,20 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
     }
 }
 
+// Wave priority from the tiles a workgroup has left, its current one
+// included: 3 down to 0 over its last four.  The workgroups that share a
+// compute unit otherwise progress oldest first, so the unit's youngest one
+// would run its last tiles alone, at a quarter of the unit's occupancy;
+// lowering the priority of the workgroups ahead lets the others catch up
+// and the unit's workgroups end together.
+__device__ __forceinline__ void tile_priority(uint32_t left) {
+    left = __builtin_amdgcn_readfirstlane(left);
+    if (left >= 4u) __builtin_amdgcn_s_setprio(3);
+    else if (left == 3u) __builtin_amdgcn_s_setprio(2);
+    else if (left == 2u) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 // Segmented particle arrays: a species lives in K segments of seg_size
 // slots, each dense from its start with its own live counter.  The kernel
 // cuts every input segment into tiles (an LDS prefix of per-segment tile
@@ -316,6 +330,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
     bool overflow = false;
     int buf = 0;
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x, buf ^= 1) {
+        tile_priority((n_tiles - tile + gridDim.x - 1) / gridDim.x);
         // segment of this tile: largest q with s_tiles[q] <= tile
         int lo = 0, hi = K - 1;
         while (lo < hi) {
@@ -541,6 +556,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     load_tile(tile, slot0, avail, x, vx, vy, vz, st, mk);
     while (tile < n_tiles) {
         const uint32_t tile_next = tile + gridDim.x;
+        tile_priority((n_tiles - tile + gridDim.x - 1) / gridDim.x);
         bool valid[PPT];
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
@@ -1104,6 +1120,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
     ti = __builtin_amdgcn_readfirstlane(ti);
     load_tile(tile, ti, slot0, avail, x, vx, vy, vz, st);
     while (tile < t_end) {
+        tile_priority(C ? 4u : t_end - tile);
         if (ti != cur) {
             if (cur >= 0) finish();
             enter(ti);
