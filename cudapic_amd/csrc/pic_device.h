@@ -92,6 +92,20 @@ __device__ __forceinline__ float u32_to_real(uint32_t x) {
 
 __device__ __forceinline__ float draw(Xo &s) { return u32_to_real(xo_next(s)); }
 
+// Element i of a particle array as its base plus a 32-bit byte offset.
+// The offset i * sizeof(T) is formed in 32 bits (for arrays of at most
+// 2^30 elements of up to 4 bytes), so an access takes the global
+// instructions' scalar-base form: one vector offset, shared by the arrays
+// indexed alike, instead of 64-bit vector address arithmetic per array.
+template <typename T>
+__device__ __forceinline__ T &el(T *base, uint32_t i) {
+    return *(T *)((char *)base + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T &el(const T *base, uint32_t i) {
+    return *(const T *)((const char *)base + i * (uint32_t)sizeof(T));
+}
+
 __device__ __forceinline__ Xo rng_load(const uint32_t *__restrict__ r0, const uint32_t *__restrict__ r1,
                                        const uint32_t *__restrict__ r2, const uint32_t *__restrict__ r3,
                                        uint32_t i) {
@@ -105,6 +119,23 @@ __device__ __forceinline__ void rng_store(uint32_t *__restrict__ r0, uint32_t *_
     r1[i] = s.s1;
     r2[i] = s.s2;
     r3[i] = s.s3;
+}
+
+// The same through el(): for arrays of at most 2^30 slots (the fused
+// kernels, whose callers are checked against that bound).
+__device__ __forceinline__ Xo rng_load32(const uint32_t *__restrict__ r0, const uint32_t *__restrict__ r1,
+                                         const uint32_t *__restrict__ r2, const uint32_t *__restrict__ r3,
+                                         uint32_t i) {
+    return Xo{el(r0, i), el(r1, i), el(r2, i), el(r3, i)};
+}
+
+__device__ __forceinline__ void rng_store32(uint32_t *__restrict__ r0, uint32_t *__restrict__ r1,
+                                            uint32_t *__restrict__ r2, uint32_t *__restrict__ r3,
+                                            uint32_t i, const Xo &s) {
+    el(r0, i) = s.s0;
+    el(r1, i) = s.s1;
+    el(r2, i) = s.s2;
+    el(r3, i) = s.s3;
 }
 
 // Three N(0, sigma^2) samples from four uniforms (main.rs:908-969): two

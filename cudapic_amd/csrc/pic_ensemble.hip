@@ -632,9 +632,9 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
         in.e_cap = in.i_cap = cap;
         tot += ((uint64_t)cap + 255u) / 256u * 256u;
     }
-    if (tot > 0xffffffffull) {
+    if (tot > kMaxSlots) {
         delete s;
-        set_error("pic_ens_create: total capacity above 2^32");
+        set_error("pic_ens_create: total capacity above 2^30 slots per species");
         return PIC_ERR_ARG;
     }
     s->e_total = s->i_total = (uint32_t)tot;

@@ -241,19 +241,19 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
     bi = __shfl(bi, leader, kWave);
     if (kind != kIonisation) return;
     const uint32_t o = mbcnt64(sp);
-    const uint32_t el = be + o, il = bi + o;
-    if (el < a.e_cap) {
-        const uint32_t es = a.e_base + el;
-        a.ex[es] = px;
-        a.evx[es] = se[0];
-        a.evy[es] = se[1];
-        a.evz[es] = se[2];
+    const uint32_t eo = be + o, il = bi + o;
+    if (eo < a.e_cap) {
+        const uint32_t es = a.e_base + eo;
+        el(a.ex, es) = px;
+        el(a.evx, es) = se[0];
+        el(a.evy, es) = se[1];
+        el(a.evz, es) = se[2];
         if (a.e_back) {
-            Xo r = rng_load(a.er0, a.er1, a.er2, a.er3, es);
+            Xo r = rng_load32(a.er0, a.er1, a.er2, a.er3, es);
             xo_back(r, a.e_back);
-            rng_store(a.er0, a.er1, a.er2, a.er3, es, r);
+            rng_store32(a.er0, a.er1, a.er2, a.er3, es, r);
         }
-        if (a.e_mask) a.e_mask[es] = 0xFFu;  // loads its words on every step of the run
+        if (a.e_mask) el(a.e_mask, es) = 0xFFu;  // loads its words on every step of the run
         if (e_dens_g) deposit_global<Acc>(e_dens_g, px, k);
         else deposit_fix(h, px, k, scale);
         if (MEASURE) eepf_lds(eepf_h, ef, px, se[0], se[1], se[2], k);
@@ -262,10 +262,10 @@ __device__ __forceinline__ void ionisation_products(const Appends &a, const KPhy
     }
     if (il < a.i_cap) {
         const uint32_t is = a.i_base + il;
-        a.ix[is] = px;
-        a.ivx[is] = si[0];
-        a.ivy[is] = si[1];
-        a.ivz[is] = si[2];
+        el(a.ix, is) = px;
+        el(a.ivx, is) = si[0];
+        el(a.ivy, is) = si[1];
+        el(a.ivz, is) = si[2];
         deposit_global<Acc>(i_dens_g, px, k);
     } else {
         overflow = true;
@@ -349,21 +349,21 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
             const uint32_t v = base + u * blockDim.x;
             keep[u] = false;
             const uint32_t i = in_base + v;
-            const float x = v < seg_n ? j.sx[i] : -1.0f;
+            const float x = v < seg_n ? el(j.sx, i) : -1.0f;
             if (x >= 0.0f) {  // x < 0 marks a hole left by an in-place step
                 const float e = field_at(ef, x, k.inv_dx, n_g);
-                nvx[u] = __builtin_fmaf(factor, e, j.svx[i]);
+                nvx[u] = __builtin_fmaf(factor, e, el(j.svx, i));
                 nx[u] = __builtin_fmaf(nvx[u], dt, x);
                 keep[u] = nx[u] >= 0.0f && nx[u] <= k.gap;
                 if (keep[u]) {
-                    vy[u] = j.svy[i];
-                    vz[u] = j.svz[i];
+                    vy[u] = el(j.svy, i);
+                    vz[u] = el(j.svz, i);
                 } else {
                     const bool pw = nx[u] < 0.0f;
                     lost_pow += pw;
                     lost_gnd += !pw;
                     if (!ELECTRON) {
-                        const float y = j.svy[i], z = j.svz[i];
+                        const float y = el(j.svy, i), z = el(j.svz, i);
                         const float en = k.half_mi_over_e * (nvx[u] * nvx[u] + y * y + z * z);
                         const int b = (int)en;
                         if (en >= 0.0f && b < kIfedBins) atomicAdd(&aux[(pw ? 0 : 1) * kIfedBins + b], 1u);
@@ -387,15 +387,15 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
                 const uint32_t local = wbase + off[u];
                 if (local < j.seg_size) {
                     const uint32_t slot = out_base + local;
-                    Xo s = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot);
+                    Xo s = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, slot);
                     float vx = nvx[u], y = vy[u], z = vz[u];
                     if (ELECTRON) kind = collide_electron(j.stot, j.cs, k, vx, y, z, s, se, si);
                     else kind = collide_ion(j.stot, j.cs, k, vx, y, z, s);
-                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
-                    j.dx[slot] = nx[u];
-                    j.dvx[slot] = vx;
-                    j.dvy[slot] = y;
-                    j.dvz[slot] = z;
+                    rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+                    el(j.dx, slot) = nx[u];
+                    el(j.dvx, slot) = vx;
+                    el(j.dvy, slot) = y;
+                    el(j.dvz, slot) = z;
                     deposit_fix(h, nx[u], k, scale);
                     if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], vx, y, z, k);
                     ncoll += kind != kNone;
@@ -539,12 +539,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t i = u * blockDim.x < avail ? s0 + u * blockDim.x : safe;
-            lx[u] = X[i];
-            lvx[u] = VX[i];
-            lvy[u] = VY[i];
-            lvz[u] = VZ[i];
-            if (RM == 1) lmk[u] = MK[i];
-            else lst[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, i);
+            lx[u] = el(X, i);
+            lvx[u] = el(VX, i);
+            lvy[u] = el(VY, i);
+            lvz[u] = el(VZ, i);
+            if (RM == 1) lmk[u] = el(MK, i);
+            else lst[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, i);
         }
     };
     float x[PPT], vx[PPT], vy[PPT], vz[PPT];
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             has[u] = x[u] >= 0.0f;
             if (RM == 1) {
                 has[u] = has[u] && (lag >= 8u || ((mk[u] >> lag) & 1u));
-                if (has[u]) st[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
+                if (has[u]) st[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
             }
         }
         if (RM != 1 && lag) {
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
         // shared line) so that it stays in flight while the next tile's
         // loads are issued
 #pragma unroll
-        for (int u = 0; u < PPT; ++u) sig[u] = j.stot[in[u] ? idx[u] : 0];
+        for (int u = 0; u < PPT; ++u) sig[u] = el(j.stot, in[u] ? (uint32_t)idx[u] : 0u);
         float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
         Xo pst[PPT];
         uint32_t pmk[PPT];
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             for (int u = 0; u < PPT; ++u) {
                 pc[u] = in[u] ? e_prob(k, sig[u], spd[u]) : 0.0f;
                 if (!has[u] && x[u] >= 0.0f && (in[u] ? pc[u] > thr : lag != 0u)) {
-                    st[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
+                    st[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
                     has[u] = true;
                 }
             }
@@ -653,13 +653,13 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                 if (MEASURE && ELECTRON && !hit) eepf_lds(aux, ef, nx[u], vx[u], vy[u], vz[u], k);
             }
             if (valid[u]) {
-                X[i] = in[u] ? nx[u] : -1.0f;
-                VX[i] = vx[u];
+                el(X, i) = in[u] ? nx[u] : -1.0f;
+                el(VX, i) = vx[u];
                 if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
-                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                    rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
                 // the next run's masks from the exact words (a colliding
                 // particle's are rewritten after its scattering)
-                if (RM == 2) MK[i] = (uint8_t)(in[u] && !hit ? cand_mask(st[u], 0u, thr) : 0xFFu);
+                if (RM == 2) el(MK, i) = (uint8_t)(in[u] && !hit ? cand_mask(st[u], 0u, thr) : 0xFFu);
             }
             if (!in[u] && x[u] >= 0.0f) {  // lost in this step
                 const bool pw = nx[u] < 0.0f;
@@ -714,12 +714,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                         else
                             kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
                                              ay[u], az[u], spd[u], s);
-                        if (RM) MK[i] = (uint8_t)cand_mask(s, lag_out, thr);
+                        if (RM) el(MK, i) = (uint8_t)cand_mask(s, lag_out, thr);
                         xo_back(s, lag_out);
-                        rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
-                        VX[i] = w;
-                        VY[i] = y;
-                        VZ[i] = z;
+                        rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
+                        el(VX, i) = w;
+                        el(VY, i) = y;
+                        el(VZ, i) = z;
                         ++ncoll;
                         if (MEASURE && ELECTRON) eepf_lds(aux, ef, nx[u], w, y, z, k);
                     }
@@ -784,12 +784,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                                  __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]), s);
             }
             // the rest of the run's masks from the state after the scattering
-            if (RM) MK[slot] = (uint8_t)cand_mask(s, lag_out, thr);
+            if (RM) el(MK, slot) = (uint8_t)cand_mask(s, lag_out, thr);
             xo_back(s, lag_out);
-            rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
-            VX[slot] = w;
-            VY[slot] = y;
-            VZ[slot] = z;
+            rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+            el(VX, slot) = w;
+            el(VY, slot) = y;
+            el(VZ, slot) = z;
             ++ncoll;
         }
         if (ELECTRON)
@@ -1022,11 +1022,11 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t g = u * blockDim.x < avail ? s0 + u * blockDim.x : safe;
-            lx[u] = X[g];
-            lvx[u] = VX[g];
-            lvy[u] = VY[g];
-            lvz[u] = VZ[g];
-            lst[u] = rng_load(j.rng0, j.rng1, j.rng2, j.rng3, g);
+            lx[u] = el(X, g);
+            lvx[u] = el(VX, g);
+            lvy[u] = el(VY, g);
+            lvz[u] = el(VZ, g);
+            lst[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, g);
         }
     };
 
@@ -1064,10 +1064,10 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                                      s);
                 }
                 xo_back(s, lag_out);
-                rng_store(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
-                VX[slot] = w;
-                VY[slot] = y;
-                VZ[slot] = z;
+                rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
+                el(VX, slot) = w;
+                el(VY, slot) = y;
+                el(VZ, slot) = z;
                 ++ncoll;
             }
             if (ELECTRON)
@@ -1162,7 +1162,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
             }
         }
 #pragma unroll
-        for (int u = 0; u < PPT; ++u) sig[u] = j.stot[in[u] ? idx[u] : 0];
+        for (int u = 0; u < PPT; ++u) sig[u] = el(j.stot, in[u] ? (uint32_t)idx[u] : 0u);
         float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
         Xo pst[PPT];
         uint32_t pslot0, pavail;
@@ -1176,10 +1176,10 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                 deposit_fix(h, nx[u], k, scale);
             }
             if (valid[u]) {
-                X[i] = in[u] ? nx[u] : -1.0f;
-                VX[i] = vx[u];
+                el(X, i) = in[u] ? nx[u] : -1.0f;
+                el(VX, i) = vx[u];
                 if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
-                    rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                    rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
             }
             if (!in[u] && x[u] >= 0.0f) {
                 const bool pw = nx[u] < 0.0f;
@@ -1229,10 +1229,10 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                             kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
                                              ay[u], az[u], spd[u], s);
                         xo_back(s, lag_out);
-                        rng_store(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
-                        VX[i] = w;
-                        VY[i] = y;
-                        VZ[i] = z;
+                        rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
+                        el(VX, i) = w;
+                        el(VY, i) = y;
+                        el(VZ, i) = z;
                         ++ncoll;
                     }
                     if (ELECTRON)

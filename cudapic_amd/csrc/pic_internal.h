@@ -205,6 +205,9 @@ int launch_eepf(const float *x, const float *vx, const float *vy, const float *v
 // the next step's density (plus EEPF in measurement steps).  A species is
 // stored as n_seg segments of seg_size slots, each dense with its own count.
 constexpr int kMaxSegments = 64;
+// Slots per species (and per ensemble) on the fused path: its kernels
+// address the particle arrays by 32-bit byte offsets.
+constexpr uint32_t kMaxSlots = 1u << 30;
 // Measurement counters (pic_sim_read_measurements order) are kept in
 // kCounterLanes copies of kCounterWords each; workgroup b of a particle
 // kernel adds into copy b % kCounterLanes, so that thousands of workgroups

@@ -716,6 +716,10 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         set_error("pic_sim_create: bad argument");
         return PIC_ERR_ARG;
     }
+    if (c->capacity > kMaxSlots - 64u * kMaxSegments) {
+        set_error("pic_sim_create: capacity above 2^30 - 4096 slots per species");
+        return PIC_ERR_ARG;
+    }
     if (c->params.dt_e <= 0.0 || c->params.n_g < 3 || c->params.n_g > 4096) {
         set_error("pic_sim_create: params not derived (call pic_params_derive) or n_g out of [3, 4096]");
         return PIC_ERR_ARG;

@@ -159,6 +159,10 @@ int check_state(const pic_state_t *s, int flags) {
         set_error("pic_step: a required buffer of the state is null or its capacity is 0");
         return PIC_ERR_ARG;
     }
+    if (s->capacity > kMaxSlots) {
+        set_error("pic_step: capacity above 2^30 slots per species");
+        return PIC_ERR_ARG;
+    }
     if ((s->e_density == nullptr) != (s->i_density == nullptr)) {
         set_error("pic_step: give both e_density and i_density or neither");
         return PIC_ERR_ARG;

@@ -202,7 +202,8 @@ enum pic_path {
 
 typedef struct pic_sim_config {
     pic_params_t params;       /* derived */
-    uint32_t capacity;         /* MAX_PARTICLES per species (main.rs:79) */
+    uint32_t capacity;         /* MAX_PARTICLES per species (main.rs:79), at most
+                                  2^30 - 4096 */
     uint32_t segments;         /* fused path: segments per species, each with its own
                                   live counter (0 = one per 128 Ki slots of capacity,
                                   1 to 32); the reference path uses 1 */
@@ -367,7 +368,7 @@ typedef struct pic_state {
     float *tmp_i_x, *tmp_i_vx, *tmp_i_vy, *tmp_i_vz;
     double *cumul_e_density, *cumul_i_density;        /* PIC_STEP_MEASURE: n_g each */
     uint32_t *eepf_counts;                            /* PIC_STEP_MEASURE: PIC_N_EEPF */
-    uint32_t capacity;                                /* MAX_PARTICLES */
+    uint32_t capacity;                                /* MAX_PARTICLES, at most 2^30 */
     uint32_t pad;
 } pic_state_t;
 
@@ -451,7 +452,8 @@ int pic_step_lagged(pic_ctx_t *ctx, pic_state_t *s, uint32_t t, int flags, uint3
 typedef struct pic_ensemble_config {
     int32_t n_instances;          /* 1..256 */
     const pic_params_t *params;   /* [n_instances], derived */
-    const uint32_t *capacity;     /* [n_instances] slots per species */
+    const uint32_t *capacity;     /* [n_instances] slots per species (at most 2^30 in
+                                     all, each rounded up to 256) */
     uint32_t seed_e[4], seed_i[4];
     /* use_graph: replay 4*N_SUB steps from a hipGraph; grid_blocks (0 = 1024
      * lanes per compute unit), ppt (particles per thread per tile, 1, 2 or 4; 0 =

@@ -129,3 +129,33 @@ def test_unsupported_workgroup_sizes_are_rejected_before_the_gpu():
     ec = _abi.PicEnsembleConfig(n_instances=1, params=p, capacity=caps, block_size=192)
     assert lib.pic_ens_create(C.byref(ec), C.byref(h)) == _abi.PIC_ERR_ARG
     assert b"block_size" in lib.pic_last_error()
+
+
+def test_capacities_beyond_the_fused_kernels_offsets_are_rejected_before_the_gpu():
+    """The fused kernels address particle arrays by 32-bit byte offsets, so a
+    species holds at most 2^30 slots: pic_sim_create (2^30 - 4096 before
+    segment rounding), the caller-owned step and the ensemble's total all
+    refuse more before any HIP call."""
+    import cudapic_amd as ca
+    from cudapic_amd import _abi
+    lib = _abi.lib()
+    h = C.c_void_p()
+    cfg = _abi.PicSimConfig()
+    lib.pic_sim_config_default(C.byref(cfg))
+    cfg.capacity = (1 << 30) - 4095
+    assert lib.pic_sim_create(C.byref(cfg), C.byref(h)) == _abi.PIC_ERR_ARG
+    assert b"2^30" in lib.pic_last_error()
+    st = _abi.PicState()
+    for name, _ in st._fields_:
+        if name not in ("e_density", "i_density", "cumul_e_density", "cumul_i_density", "eepf_counts",
+                        "capacity", "pad"):
+            setattr(st, name, 0x1000)  # never dereferenced: the check comes first
+    st.capacity = (1 << 30) + 1
+    p = ca.params()
+    assert lib.pic_step_noalloc(C.byref(p), C.byref(st), 0, 0, None) == _abi.PIC_ERR_ARG
+    assert b"2^30" in lib.pic_last_error()
+    pp = (_abi.PicParams * 2)(ca.params(), ca.params())
+    caps = (C.c_uint32 * 2)(1 << 29, (1 << 29) + 1)
+    ec = _abi.PicEnsembleConfig(n_instances=2, params=pp, capacity=caps)
+    assert lib.pic_ens_create(C.byref(ec), C.byref(h)) == _abi.PIC_ERR_ARG
+    assert b"2^30" in lib.pic_last_error()
