@@ -109,16 +109,45 @@ __device__ __forceinline__ void clear_slice(uint32_t *buf, uint32_t words) {
     for (uint32_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) buf[w] = 0u;
 }
 
-// Live count of every input segment (clamped: a counter that overflowed
-// its segment counts dropped particles) into s_cnt[K] and the inclusive
-// prefix of their tile counts into s_tiles[0..K] (s_tiles[0] = 0), by wave 0
-// in registers (K <= 64): one memory latency and a few lane shuffles instead
-// of a serial loop over LDS.  The caller's barrier publishes them.
-__device__ __forceinline__ void segment_tiles(const uint32_t *cnt_in, int K, uint32_t seg_size, uint32_t tile_sz,
+// The field into LDS in two halves: field_issue loads this lane's first
+// kFieldPre nodes into registers (issued with the other set-up loads, not
+// waited for), field_to_lds stores them and copies any nodes beyond.
+constexpr int kFieldPre = 2;
+__device__ __forceinline__ void field_issue(const float *efield, int n_g, float (&v)[kFieldPre]) {
+#pragma unroll
+    for (int r = 0; r < kFieldPre; ++r) {
+        const int q = (int)threadIdx.x + r * (int)blockDim.x;
+        v[r] = efield[min(q, n_g - 1)];  // unconditional: no branch holds a wait
+    }
+}
+__device__ __forceinline__ void field_to_lds(const float *efield, int n_g, const float (&v)[kFieldPre], float *ef) {
+#pragma unroll
+    for (int r = 0; r < kFieldPre; ++r) {
+        const int q = (int)threadIdx.x + r * (int)blockDim.x;
+        if (q < n_g) ef[q] = v[r];
+    }
+    for (int q = (int)threadIdx.x + kFieldPre * (int)blockDim.x; q < n_g; q += blockDim.x) ef[q] = efield[q];
+}
+
+// Live count of input segment q = threadIdx.x < K (K <= 64: lanes of wave
+// 0), 0 for the other lanes; the caller issues this load with its other
+// set-up loads, before any of them is waited for.
+__device__ __forceinline__ uint32_t segment_count(const uint32_t *cnt_in, int K) {
+    // unconditional (lanes past K reread the last count), so that no branch
+    // holds the wait for it
+    return cnt_in[min(threadIdx.x, (unsigned)K - 1u)];
+}
+
+// The segments' live counts (segment_count; clamped: a counter that
+// overflowed its segment counts dropped particles) into s_cnt[K] and the
+// inclusive prefix of their tile counts into s_tiles[0..K] (s_tiles[0] = 0),
+// by wave 0 in registers: a few lane shuffles instead of a serial loop over
+// LDS.  The caller's barrier publishes them.
+__device__ __forceinline__ void segment_tiles(uint32_t cnt, int K, uint32_t seg_size, uint32_t tile_sz,
                                               uint32_t *s_cnt, uint32_t *s_tiles) {
     if (threadIdx.x >= (unsigned)kWave) return;
     const int q = (int)threadIdx.x;
-    const uint32_t c = q < K ? min(cnt_in[q], seg_size) : 0u;
+    const uint32_t c = q < K ? min(cnt, seg_size) : 0u;
     uint32_t incl = (c + tile_sz - 1) / tile_sz;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
@@ -308,8 +337,11 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_fused(FusedArgs A) {
     uint32_t *aux = hist + A.lds_reps * n_g;  // EEPF (electrons) or IFED (ions)
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     const uint32_t tile_sz = blockDim.x * PPT;
-    segment_tiles(j.cnt_in, K, j.seg_size, tile_sz, s_cnt, s_tiles);
-    for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
+    const uint32_t seg_cnt = segment_count(j.cnt_in, K);
+    float efv[kFieldPre];
+    field_issue(j.efield, n_g, efv);
+    segment_tiles(seg_cnt, K, j.seg_size, tile_sz, s_cnt, s_tiles);
+    field_to_lds(j.efield, n_g, efv, ef);
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
     clear_slice(j.zero, j.zero_words);
@@ -476,8 +508,11 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
     const int aux_n = ELECTRON ? (MEASURE ? kEepfBins : 0) : 2 * kIfedBins;
     uint32_t *list = aux + aux_n;  // [NF][kCollPerWg]
     const uint32_t tile_sz = blockDim.x * PPT;
-    segment_tiles(j.cnt_in, K, j.seg_size, tile_sz, s_cnt, s_tiles);
-    for (int q = threadIdx.x; q < n_g; q += blockDim.x) ef[q] = j.efield[q];
+    const uint32_t seg_cnt = segment_count(j.cnt_in, K);
+    float efv[kFieldPre];
+    field_issue(j.efield, n_g, efv);
+    segment_tiles(seg_cnt, K, j.seg_size, tile_sz, s_cnt, s_tiles);
+    field_to_lds(j.efield, n_g, efv, ef);
     for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
     for (int q = threadIdx.x; q < aux_n; q += blockDim.x) aux[q] = 0u;
     clear_slice(j.zero, j.zero_words);
