@@ -648,6 +648,16 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
         uint32_t pmk[PPT];
         uint32_t pslot0, pavail;
         load_tile(tile_next, pslot0, pavail, px, pvx, pvy, pvz, pst, pmk);
+        // the deposits and the x / vx write-back do not depend on the
+        // collision test: they go first, while the gather is in flight
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            if (in[u]) deposit_fix(h, nx[u], k, scale);
+            if (valid[u]) {
+                el(X, slot0 + u * blockDim.x) = in[u] ? nx[u] : -1.0f;
+                el(VX, slot0 + u * blockDim.x) = vx[u];
+            }
+        }
         // masks read: a live slot that is no candidate loads its words now
         // if it may collide above the threshold, or if it leaves the gap
         // behind on its words (they are stored exact); then the loaded
@@ -668,10 +678,11 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
                     if (has[u]) xo_skip(st[u], lag);
             }
         }
-        // (3) collision tests, write-back, deposits; colliding particles
-        // are listed; lost ones leave a hole.  Every slot of the tile writes
-        // x and vx back, and on a step that leaves the RNG words exact
-        // (lag_out == 0) also its RNG words (a colliding particle's values
+        // (3) collision tests and the RNG write-back; colliding particles
+        // are listed; lost ones leave a hole.  Every slot of the tile has
+        // written x and vx back (above), and on a step that leaves the RNG
+        // words exact (lag_out == 0) also writes its RNG words (a colliding
+        // particle's values
         // are overwritten by its scattering after the last tile, holes and
         // lost particles write their unchanged RNG words): stores that skip
         // lanes leave partly written lines, which cost a read-modify-write
@@ -684,12 +695,9 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace(FusedArgs A) {
             if (in[u]) {
                 if (RM == 1) hit = has[u] && draw(st[u]) < pc[u];  // no candidate: no collision
                 else hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
-                deposit_fix(h, nx[u], k, scale);
                 if (MEASURE && ELECTRON && !hit) eepf_lds(aux, ef, nx[u], vx[u], vy[u], vz[u], k);
             }
             if (valid[u]) {
-                el(X, i) = in[u] ? nx[u] : -1.0f;
-                el(VX, i) = vx[u];
                 if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
                     rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
                 // the next run's masks from the exact words (a colliding
@@ -1202,17 +1210,22 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         Xo pst[PPT];
         uint32_t pslot0, pavail;
         load_tile(tile_next, ti_next, pslot0, pavail, px, pvx, pvy, pvz, pst);
+        // deposits and the x / vx write-back first, while the gather is in
+        // flight (as in the single-discharge kernel)
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            if (in[u]) deposit_fix(h, nx[u], k, scale);
+            if (valid[u]) {
+                el(X, slot0 + u * blockDim.x) = in[u] ? nx[u] : -1.0f;
+                el(VX, slot0 + u * blockDim.x) = vx[u];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t i = slot0 + u * blockDim.x;
             bool hit = false;
-            if (in[u]) {
-                hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
-                deposit_fix(h, nx[u], k, scale);
-            }
+            if (in[u]) hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
             if (valid[u]) {
-                el(X, i) = in[u] ? nx[u] : -1.0f;
-                el(VX, i) = vx[u];
                 if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
                     rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
             }
