@@ -453,9 +453,14 @@ def adjoint_figure(steps):
         "where": "tests/test_gpu_adjoint.py",
         "particles": "200k e + 210k i (and 20k + 21k)", "window_steps": 400, "checkpoint": 200,
         "against": "FP64 CPU adjoint on the GPU's event tape; FP64 central differences of the frozen-event model",
-        "h_V": [1e-3, 1e-2], "tolerance": "1 % relative per coefficient",
-        "measured": "200k: 0.004-0.18 % at h = 1e-3 V, 0.01-0.09 % at h = 1e-2 V (4 coefficients); "
-                    "20k: within 0.06 % at h = 1e-3 V, up to 1.3 % off at h = 1e-2 V",
+        "h_V": [1e-3, 1e-2],
+        "tolerance": "h = 1e-3 V: 1 % relative per coefficient; h = 1e-2 V: 1 % for the four coefficients "
+                     "as a vector, 3 % per coefficient",
+        "measured": "200k: 0.02-0.1 % at h = 1e-3 V; at h = 1e-2 V 0.4-1.5 % per coefficient (0.5 % as a "
+                    "vector) on the current tape, 0.01-0.09 % on an earlier one; 20k: within 0.06 % at "
+                    "h = 1e-3 V, up to 1.3 % off at h = 1e-2 V",
+        "config_target_met": "no: the configuration's 1 % per coefficient at h = 1e-2 V fails on some tapes "
+                             "(sin_1 1.5 % off); the frozen-event model is only piecewise smooth at that step",
         "not_checked": "the configuration's 4M-particle 8000-step window: beyond ~500 steps the frozen-event "
                        "model is ill-conditioned, so that window is measured for cost only"}
     return out
