@@ -64,7 +64,7 @@ class PicEnsembleConfig(C.Structure):
         *[(n, C.c_int32) for n in ("use_graph", "grid_blocks", "ppt", "replicas", "lds_replicas")],
         ("compact_every", C.c_uint32),
         ("group_slots", C.c_uint32), ("group_periods", C.c_uint32), ("group_streams", C.c_uint32),
-        ("block_size", C.c_int32), ("rng_defer", C.c_uint32),
+        ("block_size", C.c_int32), ("rng_defer", C.c_uint32), ("cand_masks", C.c_int32),
     ]
 
 

@@ -63,6 +63,12 @@ struct pic_ensemble {
     // (0 between calls and at every period start)
     uint32_t rng_defer = 8;
     uint32_t e_lag = 0;
+    // collision-candidate masks of the deferred runs (a byte per pooled
+    // electron slot, null when off): mask_ok while they hold the current
+    // run's candidates (written by the last write-back step; a compaction,
+    // a measured step, an upload or a new clock leaves them stale)
+    uint8_t *e_mask = nullptr;
+    bool mask_ok = false;
     // instance groups [grp[g], grp[g+1]) and the periods a group runs at a time
     std::vector<int> grp;
     uint32_t group_periods = 5;
@@ -73,11 +79,13 @@ struct pic_ensemble {
     hipEvent_t fork = nullptr;
     // captured 4*N_SUB-step graphs of one group, keyed by the group, the
     // measure flag, whether the period starts with a compaction, and the
-    // roles at its start; each remembers the roles it leaves behind
+    // roles at its start (the mask state among them); each remembers the
+    // roles it leaves behind
+    static constexpr int kRoles = 7;
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
-        int key[9] = {};
-        int after[6] = {};
+        int key[kRoles + 3] = {};
+        int after[kRoles] = {};
     };
     std::vector<GraphSlot> graphs;
     int graph_next = 0;
@@ -139,6 +147,7 @@ void *dens_at(const pic_ensemble *s, void *b, int g0) {
 
 int prologue(pic_ensemble *s) {
     hipStream_t st = s->stream;
+    s->mask_ok = false;
     TRY(launch_fill(s->e_dens[s->cd], 0u, dens_words32(s), st));
     TRY(launch_fill(s->e_dens[1 - s->cd], 0u, dens_words32(s), st));
     TRY(launch_fill(s->i_dens[0], 0u, dens_words32(s), st));
@@ -195,6 +204,21 @@ uint32_t next_lag(const pic_ensemble *s, bool last) {
     const uint32_t t = s->t;
     if (last || t % s->compact_every == 0 || (t + 1u) % s->rng_defer == 0 || t + 1u == (uint32_t)s->p0.n_t) return 0;
     return s->e_lag + 1u;
+}
+
+// Candidate masks of the step at s->t that ends lag_out behind: read inside
+// a deferred run whose masks are valid, written by a write-back step whose
+// next run has steps that can use them (the next step neither compacts nor
+// writes back).  No masks on compacting or measured steps.
+void mask_mode(const pic_ensemble *s, bool measure, uint32_t lag_out, bool *read, bool *write) {
+    const uint32_t t = s->t;
+    const bool on = s->e_mask && t % s->compact_every != 0 && !measure;
+    *read = on && lag_out != 0u && s->mask_ok;
+    *write = false;
+    if (on && lag_out == 0u) {
+        const uint32_t t1 = (t + 1u) % (uint32_t)s->p0.n_t;
+        *write = t1 % s->compact_every != 0 && (t1 + 1u) % s->rng_defer != 0 && t1 + 1u != (uint32_t)s->p0.n_t;
+    }
 }
 
 // One ensemble step of the instances [g0, g1).  Particles are updated in
@@ -292,13 +316,19 @@ int step_range(pic_ensemble *s, bool measure, int g0, int g1, uint32_t lag_out) 
     fe.zero_words = (uint32_t)((size_t)(g1 - g0) * s->inst_stride * (s->f64 ? 2 : 1));
     fe.rng_lag = s->e_lag;
     fe.rng_next_lag = lag_out;
+    fe.mask = s->e_mask;
+    mask_mode(s, measure, lag_out, &fe.mask_read, &fe.mask_write);
     TRY(launch_inplace_ens(fe, s->sh, st));
     if (comp_e) TRY(launch_compact_ens(compact_job(s, g0, g1, true, s->e[s->ce], s->e[1 - s->ce], out_e, EC_C), st));
     return PIC_OK;
 }
 
-// the roles, the RNG write-back lag and the host clock after a step
-void end_step(pic_ensemble *s, uint32_t lag_out) {
+// the roles, the RNG write-back lag, the mask state and the host clock
+// after a step
+void end_step(pic_ensemble *s, bool measure, uint32_t lag_out) {
+    bool mread, mwrite;
+    mask_mode(s, measure, lag_out, &mread, &mwrite);
+    s->mask_ok = lag_out == 0u ? mwrite : mread;
     s->e_lag = lag_out;
     const uint32_t t = s->t;
     const bool ion = (t % (uint32_t)s->p0.n_sub) == 0;
@@ -318,7 +348,7 @@ void end_step(pic_ensemble *s, uint32_t lag_out) {
 int step(pic_ensemble *s, bool measure, bool last) {
     const uint32_t lag_out = next_lag(s, last);
     for (size_t g = 0; g + 1 < s->grp.size(); ++g) TRY(step_range(s, measure, s->grp[g], s->grp[g + 1], lag_out));
-    end_step(s, lag_out);
+    end_step(s, measure, lag_out);
     return PIC_OK;
 }
 
@@ -337,6 +367,7 @@ void get_roles(const pic_ensemble *s, int *r) {
     r[3] = s->hi;
     r[4] = s->pe;
     r[5] = s->ac;
+    r[6] = s->mask_ok ? 1 : 0;
 }
 
 void set_roles(pic_ensemble *s, const int *r) {
@@ -346,17 +377,18 @@ void set_roles(pic_ensemble *s, const int *r) {
     s->hi = r[3];
     s->pe = r[4];
     s->ac = r[5];
+    s->mask_ok = r[6] != 0;
 }
 
 // The graph of group g's 4*N_SUB steps starting now (t a multiple of
 // 4*N_SUB), from the cache or captured; a period holds at most one
 // compaction (at its first step).
 int graph_for(pic_ensemble *s, bool measure, int g, pic_ensemble::GraphSlot **out) {
-    int key[9];
+    int key[pic_ensemble::kRoles + 3];
     key[0] = measure ? 1 : 0;
     key[1] = (s->t % s->compact_every) == 0 ? 1 : 0;
     get_roles(s, key + 2);
-    key[8] = g;
+    key[pic_ensemble::kRoles + 2] = g;
     for (auto &gs : s->graphs)
         if (gs.exec && memcmp(gs.key, key, sizeof(key)) == 0) {
             *out = &gs;
@@ -376,12 +408,12 @@ int graph_for(pic_ensemble *s, bool measure, int g, pic_ensemble::GraphSlot **ou
         // the period's last step writes the words back (rng_defer divides it)
         const uint32_t lag_out = next_lag(s, false);
         rc = step_range(s, measure, s->grp[g], s->grp[g + 1], lag_out);
-        end_step(s, lag_out);
+        end_step(s, measure, lag_out);
     }
     hipError_t e = hipStreamEndCapture(s->stream, &gr);
     // capture does not execute: remember the roles the period leaves, then
     // restore the host mirror
-    int after[6];
+    int after[pic_ensemble::kRoles];
     get_roles(s, after);
     s->t = t0;
     s->steps = st0;
@@ -424,7 +456,7 @@ int run(pic_ensemble *s, uint32_t n, bool measure, bool dry = false) {
     const uint32_t period = 4u * (uint32_t)s->p0.n_sub;
     const bool graphs = s->use_graph && (uint32_t)s->p0.n_t % period == 0;
     if (dry) {
-        int roles0[6];
+        int roles0[pic_ensemble::kRoles];
         get_roles(s, roles0);
         const uint32_t t0 = s->t;
         int rc = PIC_OK;
@@ -447,7 +479,7 @@ int run(pic_ensemble *s, uint32_t n, bool measure, bool dry = false) {
             const uint32_t block = std::min(s->group_periods, n / period);
             const int n_groups = (int)s->grp.size() - 1;
             const int ns = std::min((int)s->work.size(), n_groups);
-            int roles0[6], roles1[6];
+            int roles0[pic_ensemble::kRoles], roles1[pic_ensemble::kRoles];
             get_roles(s, roles0);
             const uint32_t t0 = s->t;
             const unsigned long long st0 = s->steps;
@@ -536,6 +568,7 @@ void free_all(pic_ensemble *s) {
         (void)hipFree(s->e_rng[c]);
         (void)hipFree(s->i_rng[c]);
     }
+    (void)hipFree(s->e_mask);
     void *bufs[] = {s->inst_d, s->cnt, s->clock, s->live, s->e_dens[0], s->e_dens[1], s->i_dens[0], s->i_dens[1],
                     s->pot, s->efield, s->cumul_e, s->cumul_i, s->i_sum, s->pot0, s->trig,
                     s->counters, s->cs, s->stot_e, s->stot_i};
@@ -593,6 +626,11 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
         if (s->compact_every % period) {
             delete s;
             set_error("pic_ens_create: compact_every must be a multiple of 4*n_sub");
+            return PIC_ERR_ARG;
+        }
+        if (c->cand_masks < -1 || c->cand_masks > 1) {
+            delete s;
+            set_error("pic_ens_create: cand_masks must be -1 (off), 0 or 1 (on)");
             return PIC_ERR_ARG;
         }
         s->rng_defer = c->rng_defer ? c->rng_defer : default_rng_defer(s->p0.n_sub);
@@ -700,6 +738,7 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
         if ((rc = dalloc(&s->e_rng[k], s->e_total))) return fail(rc);
         if ((rc = dalloc(&s->i_rng[k], s->i_total))) return fail(rc);
     }
+    if (c->cand_masks >= 0 && (rc = dalloc(&s->e_mask, s->e_total))) return fail(rc);
     if ((rc = dalloc(&s->inst_d, s->E))) return fail(rc);
     if ((rc = dalloc(&s->cnt, (size_t)s->E * EC_WORDS))) return fail(rc);
     if ((rc = dalloc(&s->clock, (size_t)s->E))) return fail(rc);
@@ -722,17 +761,20 @@ int pic_ens_create(const pic_ensemble_config_t *c, pic_ensemble_t **out) {
     if ((rc = dalloc(&s->stot_i, PIC_CS_RANGES))) return fail(rc);
     for (int i = 0; i < 2; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) return fail(hip_fail(hipGetLastError(), "hipEventCreate"));
-    if (hipMemcpy(s->inst_d, s->inst_h.data(), sizeof(EnsInst) * s->E, hipMemcpyHostToDevice) != hipSuccess)
-        return fail(hip_fail(hipGetLastError(), "upload instances"));
     {
-        // summed cross sections (gas density 1): scaled per instance
+        // summed cross sections (gas density 1): scaled per instance; each
+        // instance's mask threshold is its largest collision probability
         std::vector<float> cs((size_t)PIC_N_CS * PIC_CS_RANGES), se(PIC_CS_RANGES), si(PIC_CS_RANGES);
         if ((rc = pic_cross_sections(1.0, cs.data(), se.data(), si.data()))) return fail(rc);
         if (hipMemcpy(s->cs, cs.data(), cs.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s->stot_e, se.data(), se.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s->stot_i, si.data(), si.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return fail(hip_fail(hipGetLastError(), "upload tables"));
+        for (int i = 0; i < s->E; ++i)
+            s->inst_h[i].coll_thr = coll_threshold(se.data(), (double)s->inst_h[i].k.stot_scale, s->params[i]);
     }
+    if (hipMemcpy(s->inst_d, s->inst_h.data(), sizeof(EnsInst) * s->E, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "upload instances"));
     if ((rc = pic_seed_streams(c->seed_e, s->e_total, s->e_rng[0], s->e_rng[1], s->e_rng[2], s->e_rng[3],
                                s->stream)))
         return fail(rc);

@@ -986,8 +986,13 @@ struct EnsArgs {
     int chunk;
 };
 
-template <bool ELECTRON, int PPT, typename Acc>
+// RM: collision-candidate masks of the deferred electron runs, as in the
+// single-discharge kernel (0 none, 1 read, 2 written by a write-back step);
+// the threshold is the instance's own (its gas density scales the
+// probabilities).
+template <bool ELECTRON, int PPT, typename Acc, int RM>
 __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
+    static_assert(RM == 0 || ELECTRON, "candidate masks: electron steps only");
     const EnsStepJob &j = A.j;
     extern __shared__ __align__(16) float sh[];
     __shared__ uint32_t s_tiles[kMaxEnsemble + 1], s_cnt[kMaxEnsemble], s_off[kMaxEnsemble];
@@ -1041,6 +1046,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
     // deferred RNG write-back (electrons), as in the single-discharge kernel
     const uint32_t lag = ELECTRON ? j.rng_lag : 0u;
     const uint32_t lag_out = ELECTRON ? j.rng_next_lag : 0u;
+    uint8_t *const MK = j.mask;
+    float thr = 1.0f;  // the current instance's mask threshold
 
     // instance of a tile: the largest i with s_tiles[i] <= tile (instances
     // without tiles are skipped); tiles only grow, so the search walks on
@@ -1052,7 +1059,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
     // loads of one tile (unconditional: lanes past the count read the
     // instance's first slot; the count test is applied at use)
     auto load_tile = [&](uint32_t tile, int i, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
-                         float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT]) {
+                         float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT], uint32_t (&lmk)[PPT]) {
         uint32_t safe = 0;
         avail = 0;
         s0 = 0;
@@ -1069,7 +1076,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
             lvx[u] = el(VX, g);
             lvy[u] = el(VY, g);
             lvz[u] = el(VZ, g);
-            lst[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, g);
+            if (RM == 1) lmk[u] = el(MK, g);
+            else lst[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, g);
         }
     };
 
@@ -1106,6 +1114,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                                      __uint_as_float(q[CF_AZ * kCollPerWg]), __uint_as_float(q[CF_G * kCollPerWg]),
                                      s);
                 }
+                if (RM) el(MK, slot) = (uint8_t)cand_mask(s, lag_out, thr);
                 xo_back(s, lag_out);
                 rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, slot, s);
                 el(VX, slot) = w;
@@ -1135,11 +1144,12 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         k.wf = in.k.wf;
         k.stot_scale = in.k.stot_scale;
         k.normal_range = in.k.normal_range;
+        thr = in.coll_thr;
         if (ELECTRON) {
             uint32_t *c = j.cnt + (size_t)i * EC_WORDS;
             app = Appends{X,      VX,     VY,     VZ,     c + j.c_out, in.e_off, in.e_cap,
                           j.ix,   j.ivx,  j.ivy,  j.ivz,  c + j.c_ion, in.i_off, in.i_cap,
-                          j.rng0, j.rng1, j.rng2, j.rng3, lag_out};
+                          j.rng0, j.rng1, j.rng2, j.rng3, lag_out, RM ? MK : nullptr};
             i_dens_g = (Acc *)j.ion_dens + (size_t)i * j.inst_stride + rep * A.rep_stride;
         }
         cur = i;
@@ -1148,6 +1158,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
 
     float x[PPT], vx[PPT], vy[PPT], vz[PPT];
     Xo st[PPT];
+    uint32_t mk[PPT];
     uint32_t slot0 = 0, avail = 0;
     uint32_t tile = t_begin;
     int ti = 0;
@@ -1161,7 +1172,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         ti = inst_of(tile, lo);
     }
     ti = __builtin_amdgcn_readfirstlane(ti);
-    load_tile(tile, ti, slot0, avail, x, vx, vy, vz, st);
+    load_tile(tile, ti, slot0, avail, x, vx, vy, vz, st, mk);
     while (tile < t_end) {
         tile_priority(C ? 4u : t_end - tile);
         if (ti != cur) {
@@ -1176,8 +1187,18 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
             valid[u] = u * blockDim.x < avail;
             if (!valid[u]) x[u] = -1.0f;
         }
-        // a live slot's stream catches up with the draws it was spared
-        if (lag) {
+        // a live slot's stream catches up with the draws it was spared;
+        // with the masks read only candidates load their words here
+        bool has[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            has[u] = x[u] >= 0.0f;
+            if (RM == 1) {
+                has[u] = has[u] && (lag >= 8u || ((mk[u] >> lag) & 1u));
+                if (has[u]) st[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
+            }
+        }
+        if (RM != 1 && lag) {
 #pragma unroll
             for (int u = 0; u < PPT; ++u)
                 if (x[u] >= 0.0f) xo_skip(st[u], lag);
@@ -1208,8 +1229,9 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         for (int u = 0; u < PPT; ++u) sig[u] = el(j.stot, in[u] ? (uint32_t)idx[u] : 0u);
         float px[PPT], pvx[PPT], pvy[PPT], pvz[PPT];
         Xo pst[PPT];
+        uint32_t pmk[PPT];
         uint32_t pslot0, pavail;
-        load_tile(tile_next, ti_next, pslot0, pavail, px, pvx, pvy, pvz, pst);
+        load_tile(tile_next, ti_next, pslot0, pavail, px, pvx, pvy, pvz, pst, pmk);
         // deposits and the x / vx write-back first, while the gather is in
         // flight (as in the single-discharge kernel)
 #pragma unroll
@@ -1220,14 +1242,39 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                 el(VX, slot0 + u * blockDim.x) = vx[u];
             }
         }
+        // masks read: a live slot that is no candidate loads its words now
+        // if it may collide above the threshold or leaves the gap behind on
+        // its words; then the loaded streams catch up
+        float pc[PPT];
+        if (RM == 1) {
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                pc[u] = in[u] ? e_prob(k, sig[u], spd[u]) : 0.0f;
+                if (!has[u] && x[u] >= 0.0f && (in[u] ? pc[u] > thr : lag != 0u)) {
+                    st[u] = rng_load32(j.rng0, j.rng1, j.rng2, j.rng3, slot0 + u * blockDim.x);
+                    has[u] = true;
+                }
+            }
+            if (lag) {
+#pragma unroll
+                for (int u = 0; u < PPT; ++u)
+                    if (has[u]) xo_skip(st[u], lag);
+            }
+        }
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const uint32_t i = slot0 + u * blockDim.x;
             bool hit = false;
-            if (in[u]) hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
+            if (in[u]) {
+                if (RM == 1) hit = has[u] && draw(st[u]) < pc[u];  // no candidate: no collision
+                else hit = ELECTRON ? e_test(k, sig[u], spd[u], st[u]) : i_test(k, sig[u], spd[u], st[u]);
+            }
             if (valid[u]) {
                 if (lag_out == 0u || (lag && !in[u] && x[u] >= 0.0f))
                     rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, st[u]);
+                // the next run's masks from the exact words (a colliding
+                // particle's are rewritten after its scattering)
+                if (RM == 2) el(MK, i) = (uint8_t)(in[u] && !hit ? cand_mask(st[u], 0u, thr) : 0xFFu);
             }
             if (!in[u] && x[u] >= 0.0f) {
                 const bool pw = nx[u] < 0.0f;
@@ -1276,6 +1323,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
                         else
                             kind = i_scatter(k, j.cs[3 * kCsRanges + b], j.cs[4 * kCsRanges + b], w, y, z, ax[u],
                                              ay[u], az[u], spd[u], s);
+                        if (RM) el(MK, i) = (uint8_t)cand_mask(s, lag_out, thr);
                         xo_back(s, lag_out);
                         rng_store32(j.rng0, j.rng1, j.rng2, j.rng3, i, s);
                         el(VX, i) = w;
@@ -1297,7 +1345,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
             vx[u] = pvx[u];
             vy[u] = pvy[u];
             vz[u] = pvz[u];
-            st[u] = pst[u];
+            if (RM == 1) mk[u] = pmk[u];
+            else st[u] = pst[u];
         }
         tile = tile_next;
         ti = ti_next;
@@ -1398,10 +1447,22 @@ int launch_count_live_ens(const float *x, const EnsInst *inst, int n_inst, const
     return PIC_OK;
 }
 
+template <bool E, int PPT, int RM>
+static void launch_ens_rm(const EnsArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
+    if (f64) k_inplace_ens<E, PPT, double, RM><<<grid, block, lds, s>>>(a);
+    else k_inplace_ens<E, PPT, float, RM><<<grid, block, lds, s>>>(a);
+}
+
 template <bool E, int PPT>
 static void launch_ens_t(const EnsArgs &a, bool f64, int grid, int block, size_t lds, hipStream_t s) {
-    if (f64) k_inplace_ens<E, PPT, double><<<grid, block, lds, s>>>(a);
-    else k_inplace_ens<E, PPT, float><<<grid, block, lds, s>>>(a);
+    const int rm = a.j.mask_read ? 1 : a.j.mask_write ? 2 : 0;
+    if constexpr (E) {
+        if (rm == 1) launch_ens_rm<E, PPT, 1>(a, f64, grid, block, lds, s);
+        else if (rm == 2) launch_ens_rm<E, PPT, 2>(a, f64, grid, block, lds, s);
+        else launch_ens_rm<E, PPT, 0>(a, f64, grid, block, lds, s);
+    } else {
+        launch_ens_rm<E, PPT, 0>(a, f64, grid, block, lds, s);
+    }
 }
 
 int launch_inplace_ens(const EnsStepJob &j, const Shape &sh0, hipStream_t s) {
@@ -1417,6 +1478,13 @@ int launch_inplace_ens(const EnsStepJob &j, const Shape &sh0, hipStream_t s) {
     if ((j.rng_lag || j.rng_next_lag) &&
         (!j.electrons || j.rng_lag >= kMaxRngLag || (j.rng_next_lag != 0 && j.rng_next_lag != j.rng_lag + 1))) {
         set_error("ensemble: deferred RNG write-back is for electron steps, next lag 0 or lag + 1");
+        return PIC_ERR_ARG;
+    }
+    if ((j.mask_read || j.mask_write) && (!j.mask || !j.electrons || (j.mask_read && j.mask_write) ||
+                                          (j.mask_read && j.rng_next_lag == 0) ||
+                                          (j.mask_write && j.rng_next_lag != 0))) {
+        set_error("ensemble: candidate masks are read on deferred electron steps and written on their "
+                  "write-back steps");
         return PIC_ERR_ARG;
     }
     EnsArgs a;

@@ -282,6 +282,7 @@ int launch_count_live(const float *x, const uint32_t *cnt, int n_seg, uint32_t s
 struct EnsInst {
     KPhys k;
     double v0;
+    float coll_thr;         // collision-candidate mask threshold (the instance's largest probability)
     uint32_t e_off, e_cap;  // electrons (and their RNG slots)
     uint32_t i_off, i_cap;  // ions (and their RNG slots)
 };
@@ -353,6 +354,11 @@ struct EnsStepJob {
     uint32_t zero_words;
     // deferred RNG write-back of the electrons, as FusedJob (0 and 0 for ions)
     uint32_t rng_lag, rng_next_lag;
+    // collision-candidate masks of the pooled electron slots (null: none),
+    // read on the steps of a deferred run or written by its write-back step;
+    // thresholds per instance (EnsInst::coll_thr)
+    uint8_t *mask;
+    bool mask_read, mask_write;
 };
 int launch_inplace_ens(const EnsStepJob &j, const Shape &sh, hipStream_t s);
 

@@ -56,6 +56,8 @@ def main():
                          "4294967295 = one group)")
     ap.add_argument("--group-periods", type=int, default=0, help="4*N_SUB-step periods per block (0 = 5)")
     ap.add_argument("--group-streams", type=int, default=0, help="streams the groups are dealt to (0 = 1)")
+    ap.add_argument("--cand-masks", type=int, default=0, choices=[-1, 0, 1],
+                    help="collision-candidate masks of the deferred electron runs (-1 off, 0 or 1 on)")
     ap.add_argument("--one-group", action="store_true",
                     help="also time the ensemble as one group of all instances (one launch per stage "
                          "and step over every instance, no temporal blocking)")
@@ -74,7 +76,7 @@ def main():
     def ensemble(group_slots):
         ens = ca.Ensemble(ps, caps, grid_blocks=args.grid, ppt=args.ppt, replicas=args.replicas,
                           group_slots=group_slots, group_periods=args.group_periods,
-                          group_streams=args.group_streams)
+                          group_streams=args.group_streams, cand_masks=args.cand_masks)
         for i, (E, I) in enumerate(states):
             ens.upload(i, 0, *E)
             ens.upload(i, 1, *I)

@@ -14,7 +14,7 @@ class Ensemble:
     def __init__(self, params_list, capacities, seed_e=(0x12345678, 0x11112222, 0x22223333, 0x33334444),
                  seed_i=(0x44445555, 0x55556666, 0x66667777, 0x77778888), use_graph=True, grid_blocks=0,
                  ppt=0, replicas=0, lds_replicas=0, compact_every=0, group_slots=0, group_periods=0,
-                 group_streams=0, block_size=0, rng_defer=0):
+                 group_streams=0, block_size=0, rng_defer=0, cand_masks=0):
         n = len(params_list)
         self._params = (PicParams * n)(*params_list)
         self._caps = (C.c_uint32 * n)(*[int(c) for c in capacities])
@@ -32,6 +32,7 @@ class Ensemble:
         cfg.group_streams = int(group_streams)
         cfg.block_size = int(block_size)
         cfg.rng_defer = int(rng_defer)
+        cfg.cand_masks = int(cand_masks)
         self.n = n
         self.n_g = params_list[0].n_g
         h = C.c_void_p()

@@ -478,6 +478,11 @@ typedef struct pic_ensemble_config {
     /* steps per run of deferred electron RNG write-backs, as in
      * pic_sim_config_t (divides 4*n_sub, at most 64; 1 = every step; 0 = 8) */
     uint32_t rng_defer;
+    /* collision-candidate masks of the deferred runs, as in pic_sim_config_t
+     * (0 or 1 = on, -1 = off), each instance with its own threshold (its
+     * largest collision probability); the draws and outcomes are the same
+     * either way */
+    int32_t cand_masks;
 } pic_ensemble_config_t;
 
 typedef struct pic_ensemble pic_ensemble_t;

@@ -232,6 +232,50 @@ def test_ensemble_deferred_rng_write_back_equals_every_step(ca):
             assert differ.sum() <= max(2, n // 10_000), (i, differ.sum(), n)
 
 
+def test_ensemble_candidate_masks_equal_plain_runs(ca):
+    """The ensemble with the collision-candidate masks (the default) and
+    without them (cand_masks = -1), both with the deferred RNG write-back:
+    two instances at 3 Pa and 30 Pa, so that their mask thresholds (each
+    instance's largest collision probability) differ by ten, cold electrons
+    (1 eV) and a 0.01 V drive as in the test above, so that every slot's
+    contents are deterministic.  From t = 1: eager steps, captured periods,
+    eager steps, then a timed window; every slot of every instance --
+    particles and RNG words -- must be equal (the same slack as above)."""
+    specs = [(3.0, 2.0e3, 60_000), (30.0, 4.0e3, 30_000)]
+    ps = [ca.params(v0=0.01, pressure=p, weight=w, precision=2) for p, w, _ in specs]
+    caps = [int(n * 1.5) for *_, n in specs]
+    runs = []
+    for masks in (-1, 0):
+        e = ca.Ensemble(ps, caps, group_slots=1, group_periods=1, cand_masks=masks)
+        try:
+            e.set_time(1)
+            for i, (*_, n) in enumerate(specs):
+                E, I = synthetic_state(n, int(n * 1.05), seed=31 + i, te_ev=1.0)
+                e.upload(i, 0, *E)
+                e.upload(i, 1, *I)
+            print(f"ensemble cand_masks={masks}: 250 steps run, 27 timed", flush=True)
+            e.run(250)
+            e.time_steps(27)
+            e.synchronize()
+            e.check()
+            runs.append([(e.download(i, 0), e.rng_download(i, 0, caps[i]), e.download(i, 1),
+                          e.rng_download(i, 1, caps[i]), e.measurements(i)[2]) for i in range(len(specs))])
+        finally:
+            e.close()
+    for i in range(len(specs)):
+        ea, rea, ia, ria, ca_ = runs[0][i]
+        eb, reb, ib, rib, cb = runs[1][i]
+        assert ca_[4] > 1000 and abs(int(ca_[4]) - int(cb[4])) <= 2, (ca_, cb)  # the same collisions
+        for a, b, ra, rb in ((ea, eb, rea, reb), (ia, ib, ria, rib)):
+            assert [len(u) for u in a] == [len(v) for v in b]
+            n = len(a[0])
+            differ = np.zeros(len(ra[0]), bool)
+            for w in range(4):
+                differ[:n] |= np.asarray(a[w]).view(np.uint32) != np.asarray(b[w]).view(np.uint32)
+                differ |= ra[w] != rb[w]
+            assert differ.sum() <= max(2, n // 10_000), (i, differ.sum(), n)
+
+
 def test_ensemble_with_an_empty_instance(ca):
     """An instance without particles between two populated ones, each a
     group of its own on the graph path and all in one group eagerly: the
