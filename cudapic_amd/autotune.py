@@ -15,7 +15,7 @@ values move with the device, the particle count and the grid size:
                 a workgroup share them round-robin)
   replicas      global density replicas the workgroup flushes spread over
                 (8, 16, 32)
-  segments      segments per species (8, 16, 32; the single-discharge path)
+  segments      segments per species (1, 8, 32; the single-discharge path)
   rng_defer     steps per run of deferred electron RNG write-backs (8, 4,
                 16: fewer stores against more catch-up draws per load)
 
@@ -47,7 +47,7 @@ SIM_SPACE = {
     "ppt_inplace": [2, 1, 4],
     "lds_replicas": [0, 2, 1],
     "replicas": [8, 16, 32],
-    "segments": [0, 16, 32, 8],
+    "segments": [0, 8, 32],
     "ppt": [4, 2],
     "rng_defer": [8, 4, 16],
 }

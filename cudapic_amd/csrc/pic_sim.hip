@@ -41,8 +41,6 @@ enum Cnt {
 // compaction sends each tile's survivors to one segment, so a segment must
 // hold its share of tiles plus one tile, which small capacities cannot
 // afford with many segments.
-constexpr int kDefaultSegments = 32;
-constexpr uint32_t kSlotsPerDefaultSegment = 131072;
 constexpr int kMeasCounters = kCounterWords;
 constexpr uint32_t kCkptMagic = 0x43504943u;  // "CIPC"
 constexpr uint32_t kCkptVersion = 2;
@@ -756,13 +754,14 @@ int pic_sim_create(const pic_sim_config_t *c, pic_sim_t **out) {
         }
     }
     s->cap = c->capacity;
-    // the fused path keeps each species in segments with their own counters;
-    // the reference path in one (its stage kernels see a plain array)
-    s->nseg = c->path == PIC_PATH_FUSED
-                  ? (c->segments > 0 ? (int)c->segments
-                                     : (int)std::max<uint32_t>(1u, std::min<uint32_t>(kDefaultSegments,
-                                                                                c->capacity / kSlotsPerDefaultSegment)))
-                  : 1;
+    // the fused path may keep each species in segments with their own
+    // counters; the reference path in one (its stage kernels see a plain
+    // array).  One by default: the in-place kernels map each tile to its
+    // segment by a search over the segments' tile prefix, which costs more
+    // (step 40.6 us with 32 segments, 38.6 us with one, at 3.8 M electrons)
+    // than the spread of the compaction's slot reservations over several
+    // counters saves once every compact_every steps
+    s->nseg = c->path == PIC_PATH_FUSED ? (c->segments > 0 ? (int)c->segments : 1) : 1;
     if (s->nseg > kMaxSegments) {
         delete s;
         set_error("pic_sim_create: at most 64 segments");

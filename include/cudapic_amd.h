@@ -205,8 +205,8 @@ typedef struct pic_sim_config {
     uint32_t capacity;         /* MAX_PARTICLES per species (main.rs:79), at most
                                   2^30 - 4096 */
     uint32_t segments;         /* fused path: segments per species, each with its own
-                                  live counter (0 = one per 128 Ki slots of capacity,
-                                  1 to 32); the reference path uses 1 */
+                                  live counter (0 = 1, at most 64); the reference
+                                  path uses 1 */
     uint32_t seed_e[4];        /* master xoshiro states (main.rs:1667, 1671) */
     uint32_t seed_i[4];
     int32_t path;              /* enum pic_path */

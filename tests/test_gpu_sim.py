@@ -44,8 +44,8 @@ def zero_tables(orc, po):
 # fused path; on the fused path 1 updates electrons in place (no ion step)
 # and 20 (= N_SUB) updates both species in place
 STARTS = [(0, 0), (1, 0), (1, 1), (1, 20)]
-# fused-path segment counts: the default for the test capacities (a few) and
-# the 32 of a multi-million-particle run
+# fused-path segment counts: the default (one) and 32, the most the
+# autotuner tries
 SEGMENTS = [0, 32]
 # (path, t0, segments); the reference path keeps one segment
 CASES = [(path, t0, seg) for path, t0 in STARTS for seg in SEGMENTS if path or not seg]
