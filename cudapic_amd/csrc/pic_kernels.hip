@@ -377,8 +377,10 @@ __device__ __forceinline__ T wave_scan_incl(T v) {
 
 // Inclusive scan over the lanes of the workgroup (up to 16 waves): DPP scan
 // inside each wave, then every wave adds the totals of the waves before it,
-// read from LDS (the last barrier lets the next scan reuse wtot).
-template <typename T>
+// read from LDS (wtot: 16 entries).  REUSE: a last barrier lets the next
+// scan reuse wtot (a caller that gives each scan its own wtot goes without
+// it).
+template <typename T, bool REUSE = true>
 __device__ __forceinline__ T block_scan_incl(T v, T *wtot) {
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
@@ -390,12 +392,13 @@ __device__ __forceinline__ T block_scan_incl(T v, T *wtot) {
         for (int w = 1; w < wave; ++w) base = base + wtot[w];
         v = base + v;
     }
-    __syncthreads();
+    if (REUSE) __syncthreads();
     return v;
 }
 
 constexpr int kPoissonThreads = 1024;
 constexpr int kPoissonMaxPer = 4;  // n_g <= 4096
+constexpr int kPoissonWtot = 33;   // LDS entries of poisson_solve beyond the n_g potentials
 
 // Nodes per lane and lanes of a Poisson workgroup: as few waves as cover the
 // grid (their barriers and the inter-wave scans are the solve's latency).
@@ -503,7 +506,8 @@ __device__ __forceinline__ double pot0_from_trig(double v0, double c, double s) 
 //   G_i = sum_{j<=i} j f_j,  r_i = -G_i / ((i+1) i),  R = scan(r),
 //   pot_i = i (R_{N-2} - R_i + r_i),
 // then E by central differences with the electrode formulas of
-// C/eduPIC.cc:491-493.  pot_s holds n_g values of T, wtot 17.
+// C/eduPIC.cc:491-493.  pot_s holds n_g values of T, wtot kPoissonWtot
+// (the two scans' wave totals, then R_{N-2}).
 // The densities of the lane's nodes come in dne / dni (load_node_densities).
 template <typename T>
 __device__ void poisson_solve(const PoissonGeom &pg, const double *dne, const double *dni, double pot0_d, bool ion,
@@ -537,7 +541,7 @@ __device__ void poisson_solve(const PoissonGeom &pg, const double *dne, const do
             g[u] = run;  // chunk-local inclusive prefix
         }
     }
-    T excl = block_scan_incl<T>(run, wtot) - run;
+    T excl = block_scan_incl<T, false>(run, wtot) - run;
     T r[kPoissonMaxPer], R[kPoissonMaxPer];
     run = Num<T>::zero();
 #pragma unroll
@@ -554,13 +558,13 @@ __device__ void poisson_solve(const PoissonGeom &pg, const double *dne, const do
             R[u] = run;
         }
     }
-    excl = block_scan_incl<T>(run, wtot) - run;
+    excl = block_scan_incl<T, false>(run, wtot + 16) - run;
     // R_{N-2}
 #pragma unroll
     for (int u = 0; u < kPoissonMaxPer; ++u)
-        if (u < per && k0 + u == n_g - 2) wtot[16] = excl + R[u];
+        if (u < per && k0 + u == n_g - 2) wtot[32] = excl + R[u];
     __syncthreads();
-    const T r_total = wtot[16];
+    const T r_total = wtot[32];
 #pragma unroll
     for (int u = 0; u < kPoissonMaxPer; ++u) {
         const int k = k0 + u;
@@ -599,7 +603,7 @@ template <typename T, typename D>
 __global__ __launch_bounds__(kPoissonThreads) void k_poisson(PoissonArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     T *pot_s = reinterpret_cast<T *>(smem);  // n_g
-    T *wtot = pot_s + a.g.n_g;               // 17
+    T *wtot = pot_s + a.g.n_g;               // kPoissonWtot
     __shared__ double s_pot0;
     __shared__ int s_ion;
     __shared__ uint32_t s_t;
@@ -824,17 +828,17 @@ int launch_poisson(const PoissonJob &j, hipStream_t s) {
     }
     switch (j.precision) {
     case PIC_PREC_FP32: {
-        size_t lds = (j.n_g + 17) * sizeof(float);
+        size_t lds = (j.n_g + kPoissonWtot) * sizeof(float);
         k_poisson<float, float><<<1, poisson_threads(j.n_g), lds, s>>>(a);
         break;
     }
     case PIC_PREC_COMP: {
-        size_t lds = (j.n_g + 17) * sizeof(ff);
+        size_t lds = (j.n_g + kPoissonWtot) * sizeof(ff);
         k_poisson<ff, float><<<1, poisson_threads(j.n_g), lds, s>>>(a);
         break;
     }
     case PIC_PREC_FP64: {
-        size_t lds = (j.n_g + 17) * sizeof(double);
+        size_t lds = (j.n_g + kPoissonWtot) * sizeof(double);
         k_poisson<double, double><<<1, poisson_threads(j.n_g), lds, s>>>(a);
         break;
     }
@@ -1076,15 +1080,15 @@ int launch_poisson_ens(const EnsPoissonJob &j, hipStream_t s) {
     switch (j.precision) {
     case PIC_PREC_FP32:
         a.zero_words = j.inst_stride;
-        k_poisson_ens<float, float><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + 17) * sizeof(float), s>>>(a);
+        k_poisson_ens<float, float><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + kPoissonWtot) * sizeof(float), s>>>(a);
         break;
     case PIC_PREC_COMP:
         a.zero_words = j.inst_stride;
-        k_poisson_ens<ff, float><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + 17) * sizeof(ff), s>>>(a);
+        k_poisson_ens<ff, float><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + kPoissonWtot) * sizeof(ff), s>>>(a);
         break;
     default:
         a.zero_words = 2 * j.inst_stride;
-        k_poisson_ens<double, double><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + 17) * sizeof(double), s>>>(a);
+        k_poisson_ens<double, double><<<j.n_inst, poisson_threads(j.g.n_g), (j.g.n_g + kPoissonWtot) * sizeof(double), s>>>(a);
         break;
     }
     PIC_LAUNCHED("k_poisson_ens");
