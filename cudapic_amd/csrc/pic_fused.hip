@@ -1058,16 +1058,27 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
     };
     // loads of one tile (unconditional: lanes past the count read the
     // instance's first slot; the count test is applied at use)
-    auto load_tile = [&](uint32_t tile, int i, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
+    // li_*: first tile, end tile, first slot and live count of the instance
+    // of the last tile loaded, kept in scalar registers so that a tile of
+    // the same instance (the common case: contiguous ranges) needs no LDS
+    // lookup before its loads are issued
+    uint32_t li_t0 = 0, li_t1 = 0, li_off = 0, li_cnt = 0;
+    auto inst_regs = [&](int i) {
+        li_t0 = __builtin_amdgcn_readfirstlane(s_tiles[i]);
+        li_t1 = __builtin_amdgcn_readfirstlane(s_tiles[i + 1]);
+        li_off = __builtin_amdgcn_readfirstlane(s_off[i]);
+        li_cnt = __builtin_amdgcn_readfirstlane(s_cnt[i]);
+    };
+    auto load_tile = [&](uint32_t tile, uint32_t &s0, uint32_t &avail, float (&lx)[PPT], float (&lvx)[PPT],
                          float (&lvy)[PPT], float (&lvz)[PPT], Xo (&lst)[PPT], uint32_t (&lmk)[PPT]) {
         uint32_t safe = 0;
         avail = 0;
         s0 = 0;
         if (tile < t_end) {
-            safe = s_off[i];
-            const uint32_t local0 = (tile - s_tiles[i]) * tile_sz + threadIdx.x;
+            safe = li_off;
+            const uint32_t local0 = (tile - li_t0) * tile_sz + threadIdx.x;
             s0 = safe + local0;
-            avail = s_cnt[i] > local0 ? s_cnt[i] - local0 : 0u;
+            avail = li_cnt > local0 ? li_cnt - local0 : 0u;
         }
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
@@ -1172,7 +1183,8 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         ti = inst_of(tile, lo);
     }
     ti = __builtin_amdgcn_readfirstlane(ti);
-    load_tile(tile, ti, slot0, avail, x, vx, vy, vz, st, mk);
+    inst_regs(ti);
+    load_tile(tile, slot0, avail, x, vx, vy, vz, st, mk);
     while (tile < t_end) {
         tile_priority(C ? 4u : t_end - tile);
         if (ti != cur) {
@@ -1180,7 +1192,13 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
             enter(ti);
         }
         const uint32_t tile_next = next_tile(tile);
-        const int ti_next = __builtin_amdgcn_readfirstlane(tile_next < t_end ? inst_of(tile_next, ti) : ti);
+        // the next tile's instance: the current one unless the tile is past
+        // its end (li_* describe instance ti here)
+        int ti_next = ti;
+        if (tile_next < t_end && tile_next >= li_t1) {  // uniform
+            ti_next = __builtin_amdgcn_readfirstlane(inst_of(tile_next, ti));
+            inst_regs(ti_next);
+        }
         bool valid[PPT];
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
@@ -1231,7 +1249,7 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         Xo pst[PPT];
         uint32_t pmk[PPT];
         uint32_t pslot0, pavail;
-        load_tile(tile_next, ti_next, pslot0, pavail, px, pvx, pvy, pvz, pst, pmk);
+        load_tile(tile_next, pslot0, pavail, px, pvx, pvy, pvz, pst, pmk);
         // deposits and the x / vx write-back first, while the gather is in
         // flight (as in the single-discharge kernel)
 #pragma unroll
