@@ -1369,12 +1369,10 @@ __global__ __launch_bounds__(kMaxParticleBlock) void k_inplace_ens(EnsArgs A) {
         tile = tile_next;
         ti = ti_next;
         if (++since_flush == cap && tile < t_end && ti == cur) {  // uniform over the workgroup
-            since_flush = 0;
-            __syncthreads();
-            flush_fix<Acc>(hist, A.lds_reps, n_g, scale, k.wf,
-                           (Acc *)j.dens + (size_t)cur * j.inst_stride + rep * A.rep_stride);
-            __syncthreads();
-            for (int q = threadIdx.x; q < A.lds_reps * n_g; q += blockDim.x) hist[q] = 0u;
+            // flush the histograms and scatter the listed collisions, so
+            // that a long tile range does not fill the list (its overflow is
+            // scattered in place, a divergent wave at a time)
+            finish();
             __syncthreads();
         }
     }

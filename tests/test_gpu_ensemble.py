@@ -30,13 +30,17 @@ def _source_slots(orc, po, ef, parts, name):
 # place, 0 both species in place followed by the compaction pass; the
 # instances stepped as one group, or each as a group of its own (every
 # per-instance array then addressed from the group's first instance)
+# grid 8: eight particle workgroups, so that each takes ~45 tiles and
+# flushes its histograms and scatters its collision list several times
+# within an instance
+@pytest.mark.parametrize("grid", [0, 8], ids=["grid-default", "grid-8"])
 @pytest.mark.parametrize("grouped", [False, True], ids=["one-group", "group-per-instance"])
 @pytest.mark.parametrize("t0", [0, 1, 20])
-def test_ensemble_step_per_instance_replay(ca, orc, t0, grouped):
+def test_ensemble_step_per_instance_replay(ca, orc, t0, grouped, grid):
     specs = [(250.0, 10.0, 2.0e3, 60_000), (120.0, 4.0, 5.0e3, 30_000), (380.0, 25.0, 1.0e3, 90_000)]
     ps = [ca.params(v0=v, pressure=pr, weight=w, precision=2) for v, pr, w, _ in specs]
     caps = [int(n * 1.3) for *_, n in specs]
-    ens = ca.Ensemble(ps, caps, use_graph=False, group_slots=1 if grouped else 0xFFFFFFFF)
+    ens = ca.Ensemble(ps, caps, use_graph=False, group_slots=1 if grouped else 0xFFFFFFFF, grid_blocks=grid)
     ens.set_time(t0)
     states = []
     for i, (v, pr, w, n) in enumerate(specs):
